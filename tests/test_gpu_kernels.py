@@ -1,0 +1,358 @@
+"""GPU parity of each HIP entry point against the float64 oracle (same seeded inputs).
+
+Integer/index outputs must match exactly; floating-point tolerances are written per test.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    from vtem_amd._lib import require_gpu
+    require_gpu()
+    return torch
+
+
+def _banner(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def test_mfma_layouts(torch_mod):
+    """Accumulator/operand lane layout of every MFMA form the kernels use."""
+    torch = torch_mod
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    rng = np.random.default_rng(0)
+    for kind, (K, dt) in enumerate([(4, torch.float64), (4, torch.float32), (32, torch.float32),
+                                    (32, torch.float32)]):
+        _banner(f"mfma probe kind={kind}")
+        A = rng.integers(-4, 5, (16, K)).astype(np.float64)
+        B = rng.integers(-4, 5, (K, 16)).astype(np.float64)
+        a = torch.tensor(A, dtype=dt, device="cuda")
+        b = torch.tensor(B, dtype=dt, device="cuda")
+        out = torch.zeros((16, 16), dtype=dt, device="cuda")
+        lib.vt_mfma_probe(kind, ptr(a), ptr(b), ptr(out), stream_ptr())
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy().astype(np.float64), A @ B)
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(64, 64, 16, False, False), (100, 37, 53, False, True),
+                                         (17, 129, 200, True, False), (5, 3, 1, True, True)])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_gemm(torch_mod, M, N, K, ta, tb, dtype):
+    torch = torch_mod
+    from vtem_amd.block import gemm
+    _banner(f"gemm {dtype} M={M} N={N} K={K} ta={ta} tb={tb}")
+    rng = np.random.default_rng(M * N + K)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    C0 = rng.standard_normal((M, N))
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    a = torch.tensor(A, dtype=tdt, device="cuda")
+    b = torch.tensor(B, dtype=tdt, device="cuda")
+    c = torch.tensor(C0, dtype=tdt, device="cuda")
+    sam, sak = (1, M) if ta else (K, 1)
+    sbk, sbn = (1, K) if tb else (N, 1)
+    gemm(M, N, K, 0.5, a, sam, sak, b, sbk, sbn, 2.0, c, N)
+    ref = 0.5 * ((A.T if ta else A) @ (B.T if tb else B)) + 2.0 * C0
+    tol = 1e-12 if dtype == "f64" else 2e-4
+    err = np.abs(c.cpu().numpy() - ref).max() / max(1.0, np.abs(ref).max())
+    assert err < tol, err
+
+
+def test_rf_bank(torch_mod, oracle):
+    torch = torch_mod
+    from vtem_amd import synthetic as syn
+    from vtem_amd.block import rf_bank
+    for name, bank, P in [("c0", syn.bank_config0(), 32), ("dyadic64", syn.bank_dyadic(64., 64.), 64)]:
+        _banner(f"rf_bank {name}")
+        ref = oracle.rf_bank(bank, P, P).reshape(bank.num_neurons, -1)
+        got = rf_bank(bank, P, P).cpu().numpy()
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err < 1e-13, err
+
+
+def _c0_inputs(oracle):
+    from vtem_amd import synthetic as syn
+    vid = syn.make_video(32, 32, 1.0, 100.0, 5, 5, 4, seed=0)
+    bank = syn.bank_config0()
+    iafp = syn.iaf_params(bank.num_neurons, 2.0, 1.0, 0.04, 0.0, np.random.default_rng(1))
+    RF = oracle.rf_bank(bank, 32, 32)
+    v = oracle.filter_video(vid.frames, RF)
+    t = np.arange(10001) * 1e-4
+    vu = oracle.upsample(v, 4, 1.0, 100.0, t)
+    return vid, bank, iafp, RF, v, vu
+
+
+@pytest.mark.parametrize("root_mode", [0, 1])
+def test_iaf_bitexact(torch_mod, oracle, root_mode):
+    """Same dt-rate input -> identical spike counts and bit-identical spike times."""
+    torch = torch_mod
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    _banner(f"iaf bitexact root_mode={root_mode}")
+    vid, bank, iafp, RF, v, vu = _c0_inputs(oracle)
+    lists, cnt, v0o, tco, buf = oracle.iaf_encode(vu, 1e-4, iafp.kappa, iafp.bias, iafp.delta,
+                                                  iafp.v0, root_mode=root_mode, max_spike=200)
+    N = bank.num_neurons
+    d = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    inp = d(vu)
+    spikes = torch.zeros((200, N), dtype=torch.float64, device="cuda")
+    counts = torch.zeros(N, dtype=torch.int32, device="cuda")
+    v0 = d(iafp.v0)
+    tc = torch.zeros(N, dtype=torch.float64, device="cuda")
+    kap, bia, dlt = d(iafp.kappa), d(iafp.bias), d(iafp.delta)
+    lib.vt_iaf_encode(ptr(inp), 0, N, N, vu.shape[0], ptr(spikes), N, 200, ptr(v0), ptr(kap),
+                      ptr(bia), ptr(dlt), ptr(tc), ptr(counts), 1e-4, root_mode, stream_ptr())
+    torch.cuda.synchronize()
+    gc = counts.cpu().numpy()
+    np.testing.assert_array_equal(gc, cnt)
+    gs = spikes.cpu().numpy()
+    for j in range(N):
+        np.testing.assert_array_equal(gs[:gc[j], j], lists[j])
+    np.testing.assert_array_equal(v0.cpu().numpy(), v0o)
+    np.testing.assert_array_equal(tc.cpu().numpy(), tco)
+
+
+def test_encode_trig_c0(torch_mod, oracle):
+    """GPU filter + trig upsampling + IAF vs oracle: counts identical, times within 1e-9 s."""
+    torch = torch_mod
+    from vtem_amd.basis import Geometry
+    from vtem_amd.block import Encoder, dev
+    _banner("encode c0 (filter + trig IAF)")
+    vid, bank, iafp, RF, v, vu = _c0_inputs(oracle)
+    lists, cnt, *_ = oracle.iaf_encode(vu, 1e-4, iafp.kappa, iafp.bias, iafp.delta, iafp.v0)
+    geom = Geometry(32, 32, 5, 5, 4, 1.0)
+    enc = Encoder(geom, bank)
+    V = dev(vid.frames.reshape(100, -1))
+    gv = enc.filter(V).cpu().numpy()
+    assert np.abs(gv - v).max() < 1e-12 * max(1, np.abs(v).max())
+    st = enc.encode(V, iafp)
+    torch.cuda.synchronize()
+    gl = st.to_lists()
+    np.testing.assert_array_equal(st.counts.cpu().numpy(), cnt)
+    err = max(np.abs(gl[j] - lists[j]).max() for j in range(len(lists)) if cnt[j])
+    assert err < 1e-9, err
+
+
+def _c0_decode_inputs(oracle):
+    vid, bank, iafp, RF, v, vu = _c0_inputs(oracle)
+    lists, cnt, *_ = oracle.iaf_encode(vu, 1e-4, iafp.kappa, iafp.bias, iafp.delta, iafp.v0)
+    return vid, bank, iafp, RF, lists, cnt
+
+
+def _spike_buffer(torch, lists):
+    N = len(lists)
+    m = max(len(s) for s in lists) + 2
+    buf = np.zeros((m, N))
+    for j, s in enumerate(lists):
+        buf[:len(s), j] = s
+    from vtem_amd.block import SpikeTrains
+    return SpikeTrains(torch.tensor(buf, dtype=torch.float64, device="cuda"),
+                       torch.tensor([len(s) for s in lists], dtype=torch.int32, device="cuda"))
+
+
+def test_intervals_q_moments(torch_mod, oracle):
+    torch = torch_mod
+    from vtem_amd.basis import Geometry
+    from vtem_amd.block import Decoder, spatial_coeffs, rf_bank
+    _banner("intervals / q / T_j")
+    vid, bank, iafp, RF, lists, cnt = _c0_decode_inputs(oracle)
+    geom = Geometry(32, 32, 5, 5, 4, 1.0)
+    S = spatial_coeffs(rf_bank(bank, 32, 32), geom)
+    dec = Decoder(geom, S, iafp.kappa, iafp.delta, iafp.bias)
+    st = _spike_buffer(torch, lists)
+    for window in [(0.0, 1.0), (0.25, 0.5)]:
+        tk1, tk2, nid, off = dec.intervals(st, window[0], window[1] - window[0])
+        o1, o2, oi, oc = oracle.select_intervals(lists, window)
+        n = int(off[-1].item())
+        assert n == o1.size
+        np.testing.assert_array_equal(nid[:n].cpu().numpy(), oi)
+        np.testing.assert_array_equal(tk1[:n].cpu().numpy(), o1)
+        np.testing.assert_array_equal(tk2[:n].cpu().numpy(), o2)
+        np.testing.assert_array_equal(np.diff(off.cpu().numpy()), oc)
+    tk1, tk2, nid, off = dec.intervals(st, 0.0, 1.0)
+    q, Tm, tv = dec.moments(tk1, tk2, nid, off)
+    o1, o2, oi, oc = oracle.select_intervals(lists, (0.0, 1.0))
+    oq = oracle.compute_q(o1, o2, oi, iafp.kappa, iafp.delta, iafp.bias, np.ones(256))
+    n = o1.size
+    np.testing.assert_allclose(q[:n].cpu().numpy(), oq, rtol=0, atol=1e-15)
+    al = oracle.temporal_factors(o1, o2, 4, 2 * np.pi)
+    oT, ot = oracle.neuron_T(al, oq, oi, 256)
+    eT = np.abs(Tm.cpu().numpy() - oT).max() / np.abs(oT).max()
+    et = np.abs(tv.cpu().numpy() - ot).max() / np.abs(ot).max()
+    assert eT < 1e-13 and et < 1e-13, (eT, et)
+
+
+@pytest.mark.parametrize("cfg", ["c0", "c1small"])
+def test_gram_f64(torch_mod, oracle, cfg):
+    """Kronecker Gram on FP64 MFMA vs oracle: relative Frobenius error < 1e-13 (lower triangle)."""
+    torch = torch_mod
+    from vtem_amd.block import dev
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    _banner(f"gram f64 {cfg}")
+    rng = np.random.default_rng(3)
+    N, Ds, Dt = (256, 121, 9) if cfg == "c0" else (300, 289, 17)
+    S = rng.standard_normal((N, Ds))
+    al = rng.standard_normal((4000, Dt)) * 0.01
+    nid = np.sort(rng.integers(0, N, 4000)).astype(np.int32)
+    Tm, _ = oracle.neuron_T(al, np.zeros(4000), nid, N)
+    Aref = oracle.gram_kron(S, Tm)
+    D = Ds * Dt
+    A = torch.zeros((D, D), dtype=torch.float64, device="cuda")
+    lib.vt_gram_f64(ptr(dev(S)), Ds, ptr(dev(Tm)), N, Ds, Dt, ptr(A), D, stream_ptr())
+    got = np.tril(A.cpu().numpy())
+    ref = np.tril(Aref)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err < 1e-13, err
+
+
+@pytest.mark.parametrize("D", [1, 63, 64, 65, 200, 1089])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_potrf_potrs(torch_mod, D, prec):
+    torch = torch_mod
+    from vtem_amd.block import mat_descs
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    _banner(f"potrf/potrs {prec} D={D}")
+    rng = np.random.default_rng(D)
+    G = rng.standard_normal((D, D + 5))
+    A = G @ G.T / D + np.eye(D)
+    b = rng.standard_normal(D)
+    tdt = torch.float64 if prec == "f64" else torch.float32
+    a = torch.tensor(A, dtype=tdt, device="cuda")
+    nblk = (D + 63) // 64
+    inv = torch.zeros(nblk * 4096, dtype=tdt, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    mats = mat_descs([(a.data_ptr(), D, D)], "cuda")
+    x = torch.tensor(b, dtype=torch.float64, device="cuda")
+    f = lib.vt_potrf_f64 if prec == "f64" else lib.vt_potrf_f32
+    g = lib.vt_potrs_f64 if prec == "f64" else lib.vt_potrs_f32
+    f(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(info), stream_ptr())
+    g(ptr(mats), 1, ptr(inv), inv.numel(), ptr(x), D, stream_ptr())
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    L = np.tril(a.cpu().numpy().astype(np.float64))
+    Lref = np.linalg.cholesky(A)
+    tol = 1e-12 if prec == "f64" else 1e-5
+    assert np.abs(L - Lref).max() / np.abs(Lref).max() < tol
+    xr = np.linalg.solve(A, b)
+    assert np.abs(x.cpu().numpy() - xr).max() / np.abs(xr).max() < (1e-11 if prec == "f64" else 1e-4)
+
+
+def test_potrf_batched_ragged(torch_mod):
+    """One launch sequence factors matrices of different sizes (serving form)."""
+    torch = torch_mod
+    from vtem_amd.block import mat_descs
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    _banner("potrf batched ragged")
+    sizes = [45, 130, 64, 300, 7]
+    rng = np.random.default_rng(5)
+    mats, As, bs = [], [], []
+    maxD = max(sizes)
+    for D in sizes:
+        G = rng.standard_normal((D, D))
+        A = G @ G.T / D + np.eye(D)
+        As.append(A)
+        mats.append(torch.tensor(A, dtype=torch.float64, device="cuda"))
+        bs.append(rng.standard_normal(D))
+    nblk = (maxD + 63) // 64
+    inv = torch.zeros(len(sizes) * nblk * 4096, dtype=torch.float64, device="cuda")
+    info = torch.zeros(len(sizes), dtype=torch.int32, device="cuda")
+    desc = mat_descs([(m.data_ptr(), D, D) for m, D in zip(mats, sizes)], "cuda")
+    x = torch.zeros(len(sizes) * maxD, dtype=torch.float64, device="cuda")
+    for i, b in enumerate(bs):
+        x[i * maxD:i * maxD + len(b)] = torch.tensor(b, device="cuda")
+    lib.vt_potrf_f64(ptr(desc), len(sizes), maxD, ptr(inv), nblk * 4096, ptr(info), stream_ptr())
+    lib.vt_potrs_f64(ptr(desc), len(sizes), ptr(inv), nblk * 4096, ptr(x), maxD, stream_ptr())
+    torch.cuda.synchronize()
+    assert (info.cpu().numpy() == 0).all()
+    xs = x.cpu().numpy()
+    for i, (A, b) in enumerate(zip(As, bs)):
+        xr = np.linalg.solve(A, b)
+        got = xs[i * maxD:i * maxD + len(b)]
+        assert np.abs(got - xr).max() / np.abs(xr).max() < 1e-11
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_decode_c0_end_to_end(torch_mod, oracle, precision):
+    """Config-0 decode on the GPU vs the oracle: SNR within 0.1 dB, coefficients close."""
+    torch = torch_mod
+    from vtem_amd.basis import Geometry
+    from vtem_amd.block import Decoder, spatial_coeffs, rf_bank
+    _banner(f"decode c0 end to end ({precision})")
+    vid, bank, iafp, RF, lists, cnt = _c0_decode_inputs(oracle)
+    geom = Geometry(32, 32, 5, 5, 4, 1.0)
+    # oracle
+    o1, o2, oi, oc = oracle.select_intervals(lists, (0.0, 1.0))
+    oq = oracle.compute_q(o1, o2, oi, iafp.kappa, iafp.delta, iafp.bias, np.ones(256))
+    x, y = oracle._grid(32, 32, 32.0, 32.0)
+    Bs = oracle.spatial_basis(x, y, oracle.spatial_modes(5, 5), 2 * np.pi / 32, 2 * np.pi / 32)
+    S = oracle.spatial_coeffs(RF, Bs)
+    al = oracle.temporal_factors(o1, o2, 4, 2 * np.pi)
+    oT, ot = oracle.neuron_T(al, oq, oi, 256)
+    A = oracle.gram_kron(S, oT)
+    r = oracle.rhs_kron(S, ot)
+    lam = oracle.tikhonov_lambda(A, o1.size)
+    X = oracle.solve_tikhonov(A, r, lam)
+    urec = oracle.synthesize(X, Bs, oracle.temporal_basis(np.arange(100) / 100, 4, 2 * np.pi))
+    snr_o = oracle.snr_db(vid.frames, urec)
+    # gpu
+    Sg = spatial_coeffs(rf_bank(bank, 32, 32), geom)
+    assert np.abs(Sg.cpu().numpy() - S).max() / np.abs(S).max() < 1e-13
+    dec = Decoder(geom, Sg, iafp.kappa, iafp.delta, iafp.bias)
+    res = dec.decode(_spike_buffer(torch, lists), precision=precision, refine_iters=6)
+    torch.cuda.synchronize()
+    assert res.info is None or int(res.info.item()) == 0
+    lam_g = res.lam.cpu().numpy()
+    assert abs(lam_g[0] - lam) / lam < 1e-12
+    fr = res.frames.cpu().numpy().reshape(100, 32, 32)
+    snr_g = oracle.snr_db(vid.frames, fr)
+    print(f"SNR oracle {snr_o:.4f} dB  gpu {snr_g:.4f} dB  ({precision})")
+    assert abs(snr_g - snr_o) < 0.1
+    assert oracle.snr_db(urec, fr) > 60.0
+
+
+def test_spike_space_reference_path(torch_mod, oracle):
+    """Reference spike-space G (closed form), RNN and reconstruction on the GPU vs oracle."""
+    torch = torch_mod
+    from vtem_amd.block import dev
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    _banner("spike-space G / rnn / reconstruct")
+    vid, bank, iafp, RF, lists, cnt = _c0_decode_inputs(oracle)
+    sub = list(range(0, 256, 16))       # 16 neurons -> n ~ 800 intervals
+    ls = [lists[j] for j in sub]
+    o1, o2, oi, oc = oracle.select_intervals(ls, (0.0, 1.0))
+    n = o1.size
+    norm = np.ones(len(sub))
+    RF4 = RF[sub]
+    Ds_ = oracle.compute_Ds_fft(RF4, 1.0, 1.0, 5, 5, "ellipse")
+    Dsw = oracle.compute_Dsw(Ds_, norm)
+    G = oracle.compute_G_spike(o1, o2, oi, Dsw, 2 * np.pi * 4, 4)
+    Gg = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    lib.vt_G_spike(ptr(Gg), n, ptr(dev(o1)), ptr(dev(o2)), 2 * np.pi * 4, 4, ptr(dev(Dsw)),
+                   len(sub), ptr(dev(oi, torch.int32)), n, stream_ptr())
+    e = np.linalg.norm(Gg.cpu().numpy() - G) / np.linalg.norm(G)
+    assert e < 1e-13, e
+    q = oracle.compute_q(o1, o2, oi, iafp.kappa[sub], iafp.delta[sub], iafp.bias[sub], norm)
+    c = oracle.rnn3(G, q, dt=1e-6, alpha=5000, steps=200)
+    cg = torch.zeros(n, dtype=torch.float64, device="cuda")
+    w1 = torch.empty(n, dtype=torch.float64, device="cuda")
+    w2 = torch.empty(n, dtype=torch.float64, device="cuda")
+    lib.vt_rnn_spike(ptr(cg), ptr(Gg), n, ptr(dev(q)), n, 1e-6 * 5000, 200, ptr(w1), ptr(w2),
+                     stream_ptr())
+    e = np.abs(cg.cpu().numpy() - c).max() / np.abs(c).max()
+    assert e < 1e-12, e
+    psi = oracle.compute_psi(Ds_, 32, 32, 32.0, 32.0, 2 * np.pi / 32, 2 * np.pi / 32)
+    t = np.arange(0, 1.0, 0.01)
+    u = oracle.reconstruct_spike(c, o1, o2, oi, norm, psi, t, 4, 2 * np.pi)
+    ug = torch.zeros((t.size, 1024), dtype=torch.float64, device="cuda")
+    lib.vt_reconstruct_spike(ptr(ug), 1024, ptr(dev(psi.reshape(len(sub), -1))), 1024,
+                             ptr(dev(o1)), ptr(dev(o2)), ptr(cg), ptr(dev(t)), t.size,
+                             ptr(dev(oi, torch.int32)), ptr(dev(norm)), 4, 2 * np.pi, n, 1024,
+                             stream_ptr())
+    e = np.abs(ug.cpu().numpy().reshape(u.shape) - u).max() / np.abs(u).max()
+    assert e < 1e-11, e

@@ -1,0 +1,354 @@
+"""Pins of the float64 oracle: closed forms, invariants and independent formulations.
+
+The reference ships no tests, golden vectors or fixtures (its demos only write videos
+for visual inspection), so every check here comes from the mathematics of the method.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from vtem_amd import synthetic as syn
+
+
+# --------------------------------------------------------------------------------------
+# Receptive fields (vtem/vrf.py:1246-1286, :1161-1188)
+# --------------------------------------------------------------------------------------
+
+def test_gabor_centre_values_and_symmetry(oracle):
+    P = 33
+    h_alpha = np.array([3.0, 3.0])
+    # centre on the pixel (16, 16): x = 16 - 16.5 = -0.5, y = -(16 - 16.5) = 0.5
+    x0 = np.array([-0.5, -0.5])
+    y0 = np.array([0.5, 0.5])
+    th = np.array([0.3, 0.3])
+    ab = np.array([0, 1])
+    RF = oracle.rf_gabor(h_alpha, th, x0, y0, ab, 2.0, P, P, float(P), float(P))
+    a = 1 / 3.0
+    assert RF[0, 16, 16] == pytest.approx(a / math.sqrt(2 * math.pi), rel=1e-15)
+    assert abs(RF[1, 16, 16]) < 1e-17
+    # even: symmetric under (x,y) -> -(x,y) about the centre; odd: antisymmetric
+    assert np.allclose(RF[0], RF[0][::-1, ::-1], atol=1e-15)
+    assert np.allclose(RF[1], -RF[1][::-1, ::-1], atol=1e-15)
+    # rotation by pi/2 of an untilted Gabor equals the transposed grid (square grid)
+    R0 = oracle.rf_gabor(np.array([3.0]), np.array([0.0]), x0[:1], y0[:1], ab[:1], 2.0, P, P, P * 1., P * 1.)
+    R1 = oracle.rf_gabor(np.array([3.0]), np.array([math.pi / 2]), x0[:1], y0[:1], ab[:1], 2.0, P, P, P * 1., P * 1.)
+    assert np.allclose(R1[0], np.rot90(R0[0], k=1), atol=1e-14)
+
+
+def test_gabor_integral_closed_form(oracle):
+    """Even Gabor: integral = sqrt(8 pi) exp(-KAPPA^2/2) h_alpha (Gaussian Fourier transform)."""
+    P, S = 512, 160.0
+    d = S / P
+    h_alpha, kap = 4.0, 1.0
+    RF = oracle.rf_gabor(np.array([h_alpha]), np.array([0.7]), np.array([0.0]), np.array([0.0]),
+                         np.array([0]), kap, P, P, S, S)
+    val = RF.sum() * d * d
+    ref = math.sqrt(8 * math.pi) * math.exp(-kap ** 2 / 2) * h_alpha
+    assert val == pytest.approx(ref, rel=1e-10)
+
+
+def test_dog_integral_closed_form(oracle):
+    """DoG: integral = 0.1 h_alpha (the 0.9 surround weight of vtem/vrf.py:1180)."""
+    P, S = 400, 80.0
+    d = S / P
+    RF = oracle.rf_dog(np.array([1.5]), np.array([0.0]), np.array([0.0]), 2.0, 3.2, P, P, S, S)
+    assert RF.sum() * d * d == pytest.approx(0.1 * 1.5, rel=1e-10)
+    # radial symmetry
+    assert np.allclose(RF[0], RF[0].T, atol=1e-16)
+
+
+# --------------------------------------------------------------------------------------
+# IAF (vtem/ensemble_encode.py:486-545)
+# --------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("root_mode", [0, 1])
+def test_iaf_constant_input(oracle, root_mode):
+    """Constant input c: t_k = kappa (k delta - v0) / (c + b)."""
+    T, N, dt = 2001, 3, 1e-3
+    c = np.array([0.3, -0.5, 1.0])
+    b = np.array([1.0, 1.0, 2.0])
+    kap = np.array([1.0, 2.0, 0.5])
+    dl = np.array([0.05, 0.02, 0.1])
+    v0 = np.array([0.01, 0.0, 0.07])
+    u = np.tile(c, (T, 1))
+    lists, cnt, *_ = oracle.iaf_encode(u, dt, kap, b, dl, v0, root_mode=root_mode)
+    for j in range(N):
+        k = np.arange(1, cnt[j] + 1)
+        ref = kap[j] * (k * dl[j] - v0[j]) / (c[j] + b[j])
+        assert cnt[j] == int(np.floor(((T - 1) * dt * (c[j] + b[j]) / kap[j] + v0[j]) / dl[j]))
+        np.testing.assert_allclose(lists[j], ref, rtol=0, atol=1e-12)
+
+
+def test_iaf_linear_ramp(oracle):
+    """Linear input (exact under linear interpolation): quadratic closed form."""
+    T, dt = 5001, 1e-4
+    t = np.arange(T) * dt
+    c0, c1, b, kap, dl, v0 = -0.4, 3.0, 1.0, 1.0, 0.01, 0.004
+    u = (c0 + c1 * t)[:, None]
+    lists, cnt, *_ = oracle.iaf_encode(u, dt, [kap], [b], [dl], [v0])
+    k = np.arange(1, cnt[0] + 1)
+    A2, B1 = c1 / (2 * kap), (c0 + b) / kap
+    ref = (-B1 + np.sqrt(B1 * B1 + 4 * A2 * (k * dl - v0))) / (2 * A2)
+    np.testing.assert_allclose(lists[0], ref, rtol=0, atol=1e-11)
+
+
+def test_iaf_falling_branch_and_t_transform(oracle):
+    """Input that goes negative exercises the falling branch; every inter-spike interval
+    satisfies the t-transform  int_{tk}^{tk+1} (u+b)/kappa dt = delta  exactly for the
+    linear interpolant of the samples."""
+    rng = np.random.default_rng(7)
+    T, dt = 4001, 1e-4
+    t = np.arange(T) * dt
+    u = np.stack([1.4 * np.sin(2 * np.pi * 3 * t + p) for p in rng.uniform(0, 6, 4)], 1)
+    b, kap, dl = np.ones(4), np.array([1.0, 0.5, 1.0, 2.0]), np.array([0.004, 0.002, 0.01, 0.003])
+    lists, cnt, *_ = oracle.iaf_encode(u, dt, kap, b, dl, np.zeros(4))
+    assert (cnt > 5).all()
+
+    def integral(j, a, bb):   # exact integral of the linear interpolant of (u+b)/kappa
+        y = (u[:, j] + b[j]) / kap[j]
+        def F(x):
+            i = min(int(x // dt), T - 2)
+            s = x - i * dt
+            cum = np.concatenate([[0], np.cumsum((y[:-1] + y[1:]) / 2 * dt)])
+            return cum[i] + y[i] * s + (y[i + 1] - y[i]) / dt * s * s / 2
+        return F(bb) - F(a)
+    for j in range(4):
+        s = lists[j]
+        for k in range(len(s) - 1):
+            assert integral(j, s[k], s[k + 1]) == pytest.approx(dl[j], abs=1e-12)
+    # falling branch actually used: some samples are below -b
+    assert (u + b < 0).any()
+
+
+def test_iaf_overflow_raises(oracle):
+    u = np.full((100, 1), 5.0)
+    with pytest.raises(ValueError):
+        oracle.iaf_encode(u, 1e-2, [1.0], [1.0], [0.01], [0.0], max_spike=4)
+
+
+def test_iaf_rt_zero_sigma_equals_deterministic(oracle):
+    rng = np.random.default_rng(3)
+    T = 3001
+    u = rng.standard_normal((T, 5)).cumsum(0) * 0.01
+    kap, b, dl = np.ones(5), np.full(5, 1.5), np.full(5, 0.02)
+    l1, c1, *_ = oracle.iaf_encode(u, 1e-4, kap, b, dl, np.zeros(5))
+    rnd = rng.standard_normal((500, 5))
+    l2, c2, *_ = oracle.iaf_encode_rt(u, 1e-4, kap, b, dl, np.zeros(5), dl, np.zeros(5), rnd)
+    np.testing.assert_array_equal(c1, c2)
+    for a, bb in zip(l1, l2):
+        np.testing.assert_array_equal(a, bb)
+
+
+# --------------------------------------------------------------------------------------
+# Upsampler, intervals
+# --------------------------------------------------------------------------------------
+
+def test_trig_upsampler_exact(oracle):
+    rng = np.random.default_rng(1)
+    Lt, T, fps = 4, 1.0, 100.0
+    Cr = rng.standard_normal((2 * Lt + 1, 3))
+    W = 2 * np.pi / T
+    tf = np.arange(100) / fps
+    v = oracle.temporal_basis(tf, Lt, W) @ Cr
+    t = np.linspace(0, 1, 1001)
+    got = oracle.upsample(v, Lt, T, fps, t)
+    np.testing.assert_allclose(got, oracle.temporal_basis(t, Lt, W) @ Cr, rtol=0, atol=1e-13)
+
+
+def test_select_intervals_handmade(oracle):
+    lists = [np.array([0.05, 0.2, 0.35, 0.6, 0.9]), np.array([]), np.array([0.5]),
+             np.array([1.2, 1.5])]
+    tk1, tk2, nid, cnt = oracle.select_intervals(lists, (0.3, 0.8))
+    # neuron 0: in (0, 0.5) after shift -> 0.05(0.35), 0.3(0.6); keep one before/after
+    np.testing.assert_allclose(tk1, [0.2 - 0.3, 0.35 - 0.3, 0.6 - 0.3])
+    np.testing.assert_allclose(tk2, [0.35 - 0.3, 0.6 - 0.3, 0.9 - 0.3])
+    np.testing.assert_array_equal(nid, [0, 0, 0])
+    np.testing.assert_array_equal(cnt, [3, 0, 0, 0])
+
+
+# --------------------------------------------------------------------------------------
+# Spatial coefficients, Gram, solves
+# --------------------------------------------------------------------------------------
+
+def _small_setup(oracle, seed=0, N=12, P=16, L=3, Lt=2, nspk=10):
+    rng = np.random.default_rng(seed)
+    bank = syn.Bank(np.r_[np.zeros(N - 2, np.int32), np.ones(2, np.int32)],
+                    rng.uniform(1.5, 3, N), rng.uniform(0, np.pi, N), rng.uniform(-5, 5, N),
+                    rng.uniform(-5, 5, N), rng.integers(0, 2, N).astype(np.int32), 2.0)
+    RF = oracle.rf_bank(bank, P, P)
+    lists = [np.sort(rng.uniform(0, 1, nspk)) for _ in range(N)]
+    return rng, bank, RF, lists
+
+
+def test_Ds_fft_equals_direct_dft_and_real_basis(oracle):
+    rng, bank, RF, _ = _small_setup(oracle)
+    P, M = 16, 3
+    Ds = oracle.compute_Ds_fft(RF, 1.0, 1.0, M, M, "rect")
+    x, y = oracle._grid(P, P, P * 1., P * 1.)
+    Om = 2 * np.pi / P
+    for my in range(-M, M + 1):
+        for mx in range(-M, M + 1):
+            e = np.exp(-1j * (mx * Om * x[None, :] + my * Om * y[:, None]))
+            direct = (RF * e[None]).sum(axis=(1, 2))
+            np.testing.assert_allclose(Ds[:, my + M, mx + M], direct, atol=1e-13)
+    # real-basis coefficients S relate to Ds: S_0 = Ds(0), S_c = sqrt2 Re Ds, S_s = -sqrt2 Im Ds
+    md = oracle.spatial_modes(M, M)
+    S = oracle.spatial_coeffs(RF, oracle.spatial_basis(x, y, md, Om, Om))
+    np.testing.assert_allclose(S[:, 0], Ds[:, M, M].real, atol=1e-13)
+    for i, (mx, my) in enumerate(md):
+        np.testing.assert_allclose(S[:, 1 + 2 * i], math.sqrt(2) * Ds[:, my + M, mx + M].real, atol=1e-13)
+        np.testing.assert_allclose(S[:, 2 + 2 * i], -math.sqrt(2) * Ds[:, my + M, mx + M].imag, atol=1e-13)
+    # Dsw = Re(Ds Ds^H) = S S^T (norm 1)
+    np.testing.assert_allclose(oracle.compute_Dsw(Ds, np.ones(len(RF))), S @ S.T, atol=1e-12)
+    # psi (vrf.py:1373-1433) = sum_ps S_ps basis_ps
+    psi = oracle.compute_psi(Ds, P, P, P * 1., P * 1., Om, Om, "rect")
+    B = oracle.spatial_basis(x, y, md, Om, Om)
+    np.testing.assert_allclose(psi, np.einsum("jp,yxp->jyx", S, B), atol=1e-13)
+
+
+def test_ellipse_mask_modes(oracle):
+    md = oracle.spatial_modes(5, 5, "ellipse")
+    # the reference keeps 81 of 121 modes at L = 5: (81 - 1) / 2 = 40 half-set modes
+    assert len(md) == 40
+
+
+def test_G_three_ways(oracle):
+    """Spike-space G: reference closed form == Phi Phi^T == numerical quadrature."""
+    rng, bank, RF, lists = _small_setup(oracle)
+    P, M, Lt = 16, 3, 2
+    tk1, tk2, nid, cnt = oracle.select_intervals(lists, (0.0, 1.0))
+    Ds = oracle.compute_Ds_fft(RF, 1.0, 1.0, M, M, "rect")
+    norm = rng.uniform(0.5, 1.0, len(RF))
+    Dsw = oracle.compute_Dsw(Ds, norm)
+    Wt, Mt = 2 * np.pi * Lt, Lt
+    G = oracle.compute_G_spike(tk1, tk2, nid, Dsw, Wt, Mt)
+    x, y = oracle._grid(P, P, P * 1., P * 1.)
+    S = oracle.spatial_coeffs(RF, oracle.spatial_basis(x, y, oracle.spatial_modes(M, M), 2 * np.pi / P, 2 * np.pi / P))
+    al = oracle.temporal_factors(tk1, tk2, Lt, Wt / Mt)
+    Phi = oracle.phi_matrix(S, al, nid, norm)
+    np.testing.assert_allclose(Phi @ Phi.T, G, rtol=0, atol=1e-12 * np.abs(G).max())
+    # quadrature of the temporal kernel on a few pairs: Gt(k,l) = int int K(t,s), K = sum_m e^{im W (t-s)}
+    for (k, l) in [(0, 1), (3, 3), (5, 20)]:
+        ts = np.linspace(tk1[k], tk2[k], 801)
+        ss = np.linspace(tk1[l], tk2[l], 801)
+        K = np.ones((ts.size, ss.size))
+        for m in range(1, Mt + 1):
+            K += 2 * np.cos(m * (Wt / Mt) * (ts[:, None] - ss[None, :]))
+        w1 = np.full(ts.size, ts[1] - ts[0]); w1[[0, -1]] /= 2
+        w2 = np.full(ss.size, ss[1] - ss[0]); w2[[0, -1]] /= 2
+        Gt = w1 @ K @ w2
+        ref = G[k, l] / Dsw[nid[k], nid[l]]
+        assert Gt == pytest.approx(ref, rel=1e-5)
+
+
+def test_dspace_gram_kron_and_trace(oracle):
+    rng, bank, RF, lists = _small_setup(oracle, seed=2, nspk=40)
+    P, M, Lt = 16, 3, 2
+    tk1, tk2, nid, _ = oracle.select_intervals(lists, (0.0, 1.0))
+    x, y = oracle._grid(P, P, P * 1., P * 1.)
+    S = oracle.spatial_coeffs(RF, oracle.spatial_basis(x, y, oracle.spatial_modes(M, M), 2 * np.pi / P, 2 * np.pi / P))
+    norm = rng.uniform(0.5, 1.0, len(RF))
+    al = oracle.temporal_factors(tk1, tk2, Lt, 2 * np.pi)
+    q = rng.standard_normal(tk1.size)
+    Tm, tv = oracle.neuron_T(al, q, nid, len(RF), norm)
+    A = oracle.gram_kron(S, Tm)
+    Phi = oracle.phi_matrix(S, al, nid, norm)
+    np.testing.assert_allclose(A, Phi.T @ Phi, atol=1e-12 * np.abs(A).max())
+    np.testing.assert_allclose(oracle.rhs_kron(S, tv), Phi.T @ q, atol=1e-12)
+    assert np.trace(A) == pytest.approx(np.trace(Phi @ Phi.T), rel=1e-12)
+    # temporal factors: quadrature of the basis
+    k = 4
+    ts = np.linspace(tk1[k], tk2[k], 2001)
+    w = np.full(ts.size, ts[1] - ts[0]); w[[0, -1]] /= 2
+    np.testing.assert_allclose(al[k], w @ oracle.temporal_basis(ts, Lt, 2 * np.pi), atol=1e-9)
+
+
+def test_push_through_and_truncated_pinv(oracle):
+    """Spike-space solutions mapped by Phi^T equal the D-space solutions."""
+    rng, bank, RF, lists = _small_setup(oracle, seed=4, nspk=15)
+    P, M, Lt = 16, 2, 2
+    tk1, tk2, nid, _ = oracle.select_intervals(lists, (0.0, 1.0))
+    x, y = oracle._grid(P, P, P * 1., P * 1.)
+    S = oracle.spatial_coeffs(RF, oracle.spatial_basis(x, y, oracle.spatial_modes(M, M), 2 * np.pi / P, 2 * np.pi / P))
+    al = oracle.temporal_factors(tk1, tk2, Lt, 2 * np.pi)
+    Phi = oracle.phi_matrix(S, al, nid)
+    G = Phi @ Phi.T
+    q = rng.standard_normal(tk1.size)
+    A = Phi.T @ Phi
+    r = Phi.T @ q
+    lam = oracle.tikhonov_lambda(A, tk1.size, 1e-3)
+    c = np.linalg.solve(oracle.add_diag(G, lam), q)
+    np.testing.assert_allclose(Phi.T @ c, oracle.solve_tikhonov(A, r, lam), rtol=1e-8, atol=1e-10)
+    # truncated pseudoinverse: same nonzero spectrum, same rcond threshold
+    cp = oracle.solve_eq_sym(G, q, rcond=1e-8)
+    w, V = np.linalg.eigh(A)
+    keep = np.abs(w) > 1e-8 * np.abs(w).max()
+    Xp = V[:, keep] @ ((V[:, keep].T @ r) / w[keep])
+    np.testing.assert_allclose(Phi.T @ cp, Xp, rtol=1e-6, atol=1e-8 * np.abs(Xp).max())
+    # solve_eq_sym equals numpy's pinv with the same relative cutoff
+    np.testing.assert_allclose(cp, np.linalg.pinv(G, rcond=1e-8, hermitian=True) @ q,
+                               atol=1e-8 * np.abs(cp).max())
+
+
+def test_rnn_closed_form_and_dspace_iterates(oracle):
+    rng = np.random.default_rng(9)
+    n, D = 30, 12
+    Phi = rng.standard_normal((n, D))
+    G = Phi @ Phi.T
+    q = rng.standard_normal(n)
+    eta = 0.5 / np.linalg.eigvalsh(G).max()
+    steps = 40
+    c = oracle.rnn3(G, q, dt=eta, alpha=1.0, steps=steps)
+    w, V = np.linalg.eigh(G)
+    s = steps + 1
+    f = np.where(np.abs(w) > 1e-12, (1 - (1 - eta * w) ** s) / np.where(np.abs(w) > 1e-12, w, 1), s * eta)
+    np.testing.assert_allclose(c, V @ (f * (V.T @ q)), atol=1e-10)
+    X = oracle.rnn_dspace(Phi.T @ Phi, Phi.T @ q, eta, steps)
+    np.testing.assert_allclose(Phi.T @ c, X, atol=1e-10)
+
+
+def test_reconstruction_recovers_true_coefficients(oracle):
+    """Exact measurements of a video in the space, n >> D, lam -> 0: X = X_true."""
+    rng, bank, RF, lists = _small_setup(oracle, seed=5, N=40, nspk=30)
+    P, M, Lt = 16, 2, 2
+    x, y = oracle._grid(P, P, P * 1., P * 1.)
+    md = oracle.spatial_modes(M, M)
+    B = oracle.spatial_basis(x, y, md, 2 * np.pi / P, 2 * np.pi / P)
+    S = oracle.spatial_coeffs(RF, B)
+    Ds, Dt = S.shape[1], 2 * Lt + 1
+    Xt = rng.standard_normal(Ds * Dt)
+    tk1, tk2, nid, _ = oracle.select_intervals(lists, (0.0, 1.0))
+    al = oracle.temporal_factors(tk1, tk2, Lt, 2 * np.pi)
+    q = oracle.phi_matrix(S, al, nid) @ Xt
+    Tm, tv = oracle.neuron_T(al, q, nid, len(RF))
+    X = oracle.solve_tikhonov(oracle.gram_kron(S, Tm), oracle.rhs_kron(S, tv), 1e-14)
+    np.testing.assert_allclose(X, Xt, rtol=0, atol=1e-6)
+    # the reference's reconstruction formula (psi, sin series) equals the separable synthesis
+    Ds_c = oracle.compute_Ds_fft(RF, 1.0, 1.0, M, M, "rect")
+    psi = oracle.compute_psi(Ds_c, P, P, P * 1., P * 1., 2 * np.pi / P, 2 * np.pi / P, "rect")
+    c = rng.standard_normal(tk1.size)
+    t = np.arange(0, 1, 0.05)
+    u1 = oracle.reconstruct_spike(c, tk1, tk2, nid, np.ones(len(RF)), psi, t, Lt, 2 * np.pi)
+    Xc = oracle.phi_matrix(S, al, nid).T @ c
+    u2 = oracle.synthesize(Xc, B, oracle.temporal_basis(t, Lt, 2 * np.pi))
+    np.testing.assert_allclose(u1, u2, atol=1e-11 * np.abs(u1).max())
+
+
+def test_stitch_windows_partition_of_unity(oracle):
+    for L in [1, 4, 13]:
+        w = oracle.hann_ramp(L)
+        np.testing.assert_allclose(w, np.hanning(2 * L)[:L], atol=1e-15)
+        a = oracle.tile_window(40, 0, L)
+        b = oracle.tile_window(40, L, 0)
+        ov = a[-L:] + b[:L]
+        np.testing.assert_allclose(ov, 1.0, atol=1e-15)
+
+
+def test_video_generator_hermitian_and_scaled():
+    vid = syn.make_video(16, 16, 1.0, 50.0, 3, 3, 2, seed=3)
+    assert np.abs(vid.frames).max() == pytest.approx(1.0)
+    c = vid.coeffs
+    np.testing.assert_allclose(c, np.conj(c[::-1, ::-1, ::-1]), atol=1e-15)
+    assert abs(c[2, 3, 3].imag) < 1e-15

@@ -1,0 +1,351 @@
+"""GPU pipeline for one block (a video window decoded in its own Dirichlet space).
+
+Everything stays resident in HBM: RF bank, spatial coefficients S, spike buffer,
+intervals, per-neuron temporal moments T_j, the D x D Gram and its factor.  Host code
+only sizes buffers and launches the C-ABI entries of libvtem_hip.so.
+
+Encode  (vtem/vtem.py:176-200 -> vrf.filter :264-292 -> IAF_encode.encode
+        vtem/ensemble_encode.py:106-168):
+    RF (N x P)  ->  v = dx dy V RF^T (frames x N)  ->  temporal coefficients
+    C = Theta_f^T v / frames (Dt x N)  ->  IAF with on-the-fly trigonometric
+    interpolation to dt  ->  spike buffer (max_spike x N) + counts.
+Decode  (vtem/vtdm.py:630-652 -> dirichlet.compute_q/compute_G, linalg.solve_eq_sym,
+        dirichlet.reconstruct), in coefficient space:
+    intervals (CSR by neuron) -> q -> T_j, t_j -> A = sum_j S_j S_j^T kron T_j,
+    r = sum_j S_j kron t_j -> lam = lam_rel tr(A)/n -> Cholesky (A + lam I) X = r
+    -> frames = B_s X Theta_out^T.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Optional
+
+import numpy as np
+import torch
+
+from ._lib import lib, ptr, stream_ptr
+from .basis import Geometry
+from .synthetic import Bank, IAFParams
+
+F64 = torch.float64
+I32 = torch.int32
+
+MAT_DTYPE = np.dtype([("A", np.uint64), ("lda", np.int64), ("D", np.int32), ("pad", np.int32)])
+
+
+def mat_descs(items, device) -> torch.Tensor:
+    """Device array of {A, lda, D} descriptors for the batched Cholesky entries."""
+    arr = np.zeros(len(items), MAT_DTYPE)
+    for i, (A, lda, D) in enumerate(items):
+        arr[i] = (int(A), int(lda), int(D), 0)
+    return torch.from_numpy(arr.view(np.uint8).copy()).to(device)
+
+
+def dev(a, dtype=F64, device="cuda"):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(device)
+
+
+# --------------------------------------------------------------------------------------
+# Kernels wrapped with torch buffers
+# --------------------------------------------------------------------------------------
+
+def gemm(M, N, K, alpha, A, sam, sak, B, sbk, sbn, beta, C, ldc, batch=1, bsa=0, bsb=0,
+         bsc=0, stream=None):
+    f = lib.vt_gemm_f64 if C.dtype == F64 else lib.vt_gemm_f32
+    f(M, N, K, alpha, ptr(A), sam, sak, bsa, ptr(B), sbk, sbn, bsb, beta, ptr(C), ldc, bsc,
+      batch, stream_ptr(stream))
+
+
+def rf_bank(bank: Bank, Px: int, Py: int, dx: float = 1.0, dy: float = 1.0, dtype=F64,
+            device="cuda", x_start=None, y_start=None, out=None) -> torch.Tensor:
+    """(N, Py*Px) receptive fields of a mixed Gabor/DoG bank (vtem/vrf.py:1197-1291)."""
+    N = bank.num_neurons
+    Sx, Sy = Px * dx, Py * dy
+    x_start = -Sx / 2 if x_start is None else x_start
+    y_start = -Sy / 2 if y_start is None else y_start
+    if out is None:
+        out = torch.empty((N, Px * Py), dtype=dtype, device=device)
+    kind = dev(bank.kind, I32, device)
+    ainv = dev(1.0 / bank.h_alpha, F64, device)
+    th = dev(bank.h_l, F64, device)
+    x0 = dev(bank.h_x0, F64, device)
+    y0 = dev(bank.h_y0, F64, device)
+    ab = dev(bank.h_ab, I32, device)
+    lib.vt_rf_bank(ptr(out), 0 if dtype == F64 else 1, out.stride(0), N, ptr(kind), ptr(ainv),
+                   ptr(th), ptr(x0), ptr(y0), ptr(ab), float(bank.KAPPA),
+                   float(bank.sigma_center) ** 2, float(bank.sigma_surround) ** 2, Px, Py, Sx, Sy,
+                   x_start, y_start, stream_ptr())
+    return out
+
+
+def spatial_coeffs(RF: torch.Tensor, geom: Geometry, Bs: torch.Tensor = None) -> torch.Tensor:
+    """S = dx dy RF B_s  (N x Ds): the real form of the reference's Ds (vtem/vrf.py:333-377)."""
+    N, P = RF.shape
+    if Bs is None:
+        Bs = dev(geom.spatial(), F64, RF.device)
+    Ds = Bs.shape[1]
+    S = torch.empty((N, Ds), dtype=F64, device=RF.device)
+    gemm(N, Ds, P, geom.dx * geom.dy, RF, RF.stride(0), 1, Bs, Ds, 1, 0.0, S, Ds)
+    return S
+
+
+@dataclasses.dataclass
+class SpikeTrains:
+    """Device spike buffer as the encoder writes it (vtem/ensemble_encode.py:146):
+    spikes (max_spike x N), neuron-contiguous; counts (N) int32."""
+    spikes: torch.Tensor
+    counts: torch.Tensor
+
+    @property
+    def num_neurons(self):
+        return self.spikes.shape[1]
+
+    def to_lists(self):
+        """Host list of per-neuron arrays (vtem/ensemble_encode.py:335-340)."""
+        c = self.counts.cpu().numpy()
+        s = self.spikes.cpu().numpy()
+        return [s[:c[j], j].copy() for j in range(c.size)]
+
+
+class Encoder:
+    """Filter + IAF encoder of one block on the GPU."""
+
+    def __init__(self, geom: Geometry, bank: Bank, device="cuda", filter_dtype=F64):
+        self.geom = geom
+        self.bank = bank
+        self.device = device
+        self.filter_dtype = filter_dtype
+        self.RF = rf_bank(bank, geom.Px, geom.Py, geom.dx, geom.dy, filter_dtype, device)
+        self.theta_f = dev(geom.temporal(geom.frame_times()), filter_dtype, device)  # (F, Dt)
+        self.theta_iaf = dev(geom.temporal(geom.iaf_times()), F64, device)          # (T, Dt)
+
+    def filter(self, V: torch.Tensor) -> torch.Tensor:
+        """v = dx dy V RF^T, V (frames x P) -> (frames x N) (vtem/vrf.py:264-292)."""
+        g = self.geom
+        T, P = V.shape
+        N = self.RF.shape[0]
+        out = torch.empty((T, N), dtype=self.filter_dtype, device=self.device)
+        gemm(T, N, P, g.dx * g.dy, V, V.stride(0), 1, self.RF, 1, self.RF.stride(0), 0.0, out, N)
+        return out
+
+    def trig_coeffs(self, v: torch.Tensor) -> torch.Tensor:
+        """C = Theta_f^T v / frames  (Dt x N), exact for trig polynomials of order Lt."""
+        g = self.geom
+        F_, N = v.shape
+        Dt = g.Dt
+        C = torch.empty((Dt, N), dtype=self.filter_dtype, device=self.device)
+        gemm(Dt, N, F_, 1.0 / F_, self.theta_f, 1, Dt, v, N, 1, 0.0, C, N)
+        return C.to(F64) if C.dtype != F64 else C
+
+    def max_spikes(self, C: torch.Tensor, iaf: IAFParams) -> int:
+        """Spike buffer rows: rate bound (sum|C| sqrt2 + b)/(kappa delta) over the window."""
+        bound = (C.abs().sum(0) * math.sqrt(2.0)).cpu().numpy()
+        rate = (bound + iaf.bias) / (iaf.kappa * iaf.delta)
+        return int(math.ceil(self.geom.duration * float(rate.max()) * 1.05)) + 8
+
+    def encode(self, V: torch.Tensor, iaf: IAFParams, root_mode: int = 0,
+               max_spike: Optional[int] = None):
+        g = self.geom
+        v = self.filter(V)
+        C = self.trig_coeffs(v)
+        return self.encode_coeffs(C, iaf, root_mode, max_spike)
+
+    def encode_coeffs(self, C: torch.Tensor, iaf: IAFParams, root_mode: int = 0,
+                      max_spike: Optional[int] = None):
+        g = self.geom
+        N = C.shape[1]
+        if max_spike is None:
+            max_spike = self.max_spikes(C, iaf)
+        spikes = torch.empty((max_spike, N), dtype=F64, device=self.device)
+        counts = torch.empty(N, dtype=I32, device=self.device)
+        v0 = dev(iaf.v0, F64, self.device)
+        tc = torch.zeros(N, dtype=F64, device=self.device)
+        kap, bia, dlt = (dev(a, F64, self.device) for a in (iaf.kappa, iaf.bias, iaf.delta))
+        lib.vt_iaf_encode_trig(ptr(C), C.stride(0), ptr(self.theta_iaf), g.Dt, N, g.iaf_samples,
+                               ptr(spikes), N, max_spike, ptr(v0), ptr(kap), ptr(bia), ptr(dlt),
+                               ptr(tc), ptr(counts), g.dt, root_mode, stream_ptr())
+        self.last_state = (v0, tc)
+        return SpikeTrains(spikes, counts)
+
+
+# --------------------------------------------------------------------------------------
+# Decoder
+# --------------------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class DecodeResult:
+    frames: torch.Tensor          # (T_out, Py*Px) float64
+    X: torch.Tensor               # (D,) coefficients, index ps*Dt + pt
+    lam: torch.Tensor             # (2,) [lambda, trace(A)]
+    n: torch.Tensor               # number of intervals (device int)
+    info: Optional[torch.Tensor] = None
+    resid: Optional[torch.Tensor] = None
+
+
+class Decoder:
+    """Coefficient-space decoder for one block geometry and neuron set.
+
+    S (N x Ds) is the decoder's prepared spatial data (the reference's VTDM_prep output,
+    vtem/vtdm.py:311-447, kept resident instead of written to dsw.h5/dirich.h5).
+    """
+
+    def __init__(self, geom: Geometry, S: torch.Tensor, kappa, delta, bias, norm=None,
+                 device="cuda"):
+        self.geom = geom
+        self.device = device
+        self.S = S.contiguous()
+        N = S.shape[0]
+        self.N = N
+        self.kappa = dev(kappa, F64, device)
+        self.delta = dev(delta, F64, device)
+        self.bias = dev(bias, F64, device)
+        self.norm = dev(np.ones(N) if norm is None else norm, F64, device)
+        self.Bs = dev(geom.spatial(), F64, device)
+        self.theta_out = dev(geom.temporal(geom.frame_times()), F64, device)
+
+    # -- spikes -> intervals -> moments ------------------------------------------------
+    def intervals(self, st: SpikeTrains, t0: float = 0.0, Tw: float = None):
+        N = self.N
+        Tw = self.geom.duration if Tw is None else Tw
+        cap = st.spikes.shape[0] * N
+        ws = torch.empty(2 * N, dtype=I32, device=self.device)
+        offsets = torch.empty(N + 1, dtype=I32, device=self.device)
+        tk1 = torch.empty(cap, dtype=F64, device=self.device)
+        tk2 = torch.empty(cap, dtype=F64, device=self.device)
+        nid = torch.empty(cap, dtype=I32, device=self.device)
+        lib.vt_intervals(ptr(st.spikes), st.spikes.stride(0), st.spikes.shape[0], ptr(st.counts),
+                         N, t0, Tw, ptr(ws),
+                         ptr(offsets), ptr(tk1), ptr(tk2), ptr(nid), cap, stream_ptr())
+        return tk1, tk2, nid, offsets
+
+    def moments(self, tk1, tk2, nid, offsets):
+        g = self.geom
+        N = self.N
+        cap = tk1.shape[0]
+        n_dev = offsets[N:]
+        q = torch.empty(cap, dtype=F64, device=self.device)
+        lib.vt_q(ptr(q), ptr(tk1), ptr(tk2), ptr(nid), ptr(self.kappa), ptr(self.delta),
+                 ptr(self.bias), ptr(self.norm), ptr(n_dev), cap, stream_ptr())
+        Tm = torch.empty((N, g.Dt, g.Dt), dtype=F64, device=self.device)
+        tv = torch.empty((N, g.Dt), dtype=F64, device=self.device)
+        lib.vt_neuron_T(ptr(Tm), ptr(tv), ptr(tk1), ptr(tk2), ptr(q), ptr(self.norm), ptr(offsets),
+                        N, g.Lt, g.W, stream_ptr())
+        return q, Tm, tv
+
+    # -- Gram, rhs, structured operator --------------------------------------------------
+    def gram(self, Tm, out=None):
+        g = self.geom
+        D = g.D
+        A = torch.empty((D, D), dtype=F64, device=self.device) if out is None else out
+        lib.vt_gram_f64(ptr(self.S), self.S.stride(0), ptr(Tm), self.N, g.Ds, g.Dt, ptr(A),
+                        A.stride(0), stream_ptr())
+        return A
+
+    def rhs(self, tv):
+        g = self.geom
+        r = torch.empty(g.D, dtype=F64, device=self.device)
+        gemm(g.Ds, g.Dt, self.N, 1.0, self.S, 1, self.S.stride(0), tv, g.Dt, 1, 0.0, r, g.Dt)
+        return r
+
+    def apply_A(self, Tm, X, out=None):
+        """out = A X through the Kronecker structure: S^T (T_j (S X)), 4 N D flops."""
+        g = self.geom
+        N, Ds, Dt = self.N, g.Ds, g.Dt
+        Y = torch.empty((N, Dt), dtype=F64, device=self.device)
+        gemm(N, Dt, Ds, 1.0, self.S, self.S.stride(0), 1, X, Dt, 1, 0.0, Y, Dt)
+        Z = torch.empty_like(Y)
+        lib.vt_tj_apply(ptr(Z), ptr(Tm), ptr(Y), N, Dt, stream_ptr())
+        out = torch.empty(g.D, dtype=F64, device=self.device) if out is None else out
+        gemm(Ds, Dt, N, 1.0, self.S, 1, self.S.stride(0), Z, Dt, 1, 0.0, out, Dt)
+        return out
+
+    def synthesize(self, X, out=None):
+        """frames (T_out x P) = Theta_out Xm^T B_s^T."""
+        g = self.geom
+        T = self.theta_out.shape[0]
+        Y = torch.empty((T, g.Ds), dtype=F64, device=self.device)
+        gemm(T, g.Ds, g.Dt, 1.0, self.theta_out, g.Dt, 1, X, 1, g.Dt, 0.0, Y, g.Ds)
+        out = torch.empty((T, g.P), dtype=F64, device=self.device) if out is None else out
+        gemm(T, g.P, g.Ds, 1.0, Y, g.Ds, 1, self.Bs, 1, g.Ds, 0.0, out, g.P)
+        return out
+
+    # -- solves ---------------------------------------------------------------------------
+    def solve_tikhonov(self, A, r, n_dev, lam_rel=1e-6, precision="fp64", refine_iters=4):
+        """(A + lam I) X = r with lam = lam_rel tr(A)/n.
+
+        precision "fp64": FP64 Cholesky.  "mixed": FP32 Cholesky of the Gram as
+        preconditioner plus FP64 iterative refinement whose residual uses the exact
+        FP64 operator through the Kronecker structure (needs self._Tm).
+        """
+        g = self.geom
+        D = g.D
+        lam = torch.empty(2, dtype=F64, device=self.device)
+        lib.vt_trace_lambda(ptr(A), 0, A.stride(0), D, ptr(n_dev), lam_rel, ptr(lam), stream_ptr())
+        nblk = (D + 63) // 64
+        info = torch.zeros(1, dtype=I32, device=self.device)
+        X = r.clone()
+        if precision == "fp64":
+            lib.vt_diag_add(ptr(A), 0, A.stride(0), D, 0.0, ptr(lam), stream_ptr())
+            inv = torch.empty(nblk * 64 * 64, dtype=F64, device=self.device)
+            mats = mat_descs([(A.data_ptr(), A.stride(0), D)], self.device)
+            lib.vt_potrf_f64(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(info), stream_ptr())
+            lib.vt_potrs_f64(ptr(mats), 1, ptr(inv), inv.numel(), ptr(X), D, stream_ptr())
+            return X, lam, info, None
+        # mixed
+        A32 = torch.empty((D, D), dtype=torch.float32, device=self.device)
+        lib.vt_lower_to_f32(ptr(A32), D, ptr(A), A.stride(0), D, ptr(lam), stream_ptr())
+        inv = torch.empty(nblk * 64 * 64, dtype=torch.float32, device=self.device)
+        mats = mat_descs([(A32.data_ptr(), D, D)], self.device)
+        lib.vt_potrf_f32(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(info), stream_ptr())
+        lib.vt_potrs_f32(ptr(mats), 1, ptr(inv), inv.numel(), ptr(X), D, stream_ptr())
+        res = torch.empty(D, dtype=F64, device=self.device)
+        AX = torch.empty(D, dtype=F64, device=self.device)
+        nrm = torch.zeros(refine_iters + 1, dtype=F64, device=self.device)
+        lib.vt_sumsq(ptr(r), D, ptr(nrm[0:]), stream_ptr())
+        for it in range(refine_iters):
+            self.apply_A(self._Tm, X, out=AX)
+            lib.vt_residual(ptr(res), ptr(r), ptr(AX), ptr(X), ptr(lam), D, stream_ptr())
+            lib.vt_sumsq(ptr(res), D, ptr(nrm[it + 1:]), stream_ptr())
+            lib.vt_potrs_f32(ptr(mats), 1, ptr(inv), inv.numel(), ptr(res), D, stream_ptr())
+            lib.vt_axpbypcz(ptr(X), 1.0, ptr(X), 1.0, ptr(res), 0.0, 0, D, stream_ptr())
+        return X, lam, info, nrm
+
+    def solve_rnn(self, Tm, r, lam, steps: int, eta: float):
+        """X <- X + eta (r - (A + lam I) X), steps+1 iterations from 0 (the D-space image of
+        vtem/neural_network.py:78-82), with A applied through its Kronecker structure."""
+        g = self.geom
+        X = torch.zeros(g.D, dtype=F64, device=self.device)
+        AX = torch.empty_like(X)
+        res = torch.empty_like(X)
+        for _ in range(steps + 1):
+            self.apply_A(Tm, X, out=AX)
+            lib.vt_residual(ptr(res), ptr(r), ptr(AX), ptr(X), ptr(lam), g.D, stream_ptr())
+            lib.vt_axpbypcz(ptr(X), 1.0, ptr(X), eta, ptr(res), 0.0, 0, g.D, stream_ptr())
+        return X
+
+    # -- full decode ----------------------------------------------------------------------
+    def decode(self, st: SpikeTrains, t0: float = 0.0, method: str = "tikhonov",
+               precision: str = "fp64", lam_rel: float = 1e-6, rnn_steps: int = 500,
+               rnn_eta: Optional[float] = None, refine_iters: int = 4) -> DecodeResult:
+        tk1, tk2, nid, offsets = self.intervals(st, t0)
+        q, Tm, tv = self.moments(tk1, tk2, nid, offsets)
+        self._Tm = Tm
+        n_dev = offsets[self.N:]
+        r = self.rhs(tv)
+        A = self.gram(Tm)
+        if method == "tikhonov":
+            X, lam, info, nrm = self.solve_tikhonov(A, r, n_dev, lam_rel, precision, refine_iters)
+        elif method == "rnn":
+            lam = torch.empty(2, dtype=F64, device=self.device)
+            lib.vt_trace_lambda(ptr(A), 0, A.stride(0), self.geom.D, ptr(n_dev), lam_rel,
+                                ptr(lam), stream_ptr())
+            if rnn_eta is None:
+                raise ValueError("rnn_eta is required for the RNN decoder")
+            X = self.solve_rnn(Tm, r, lam, rnn_steps, rnn_eta)
+            info, nrm = None, None
+        else:
+            raise ValueError(f"unknown method {method}")
+        frames = self.synthesize(X)
+        return DecodeResult(frames, X, lam, n_dev, info, nrm)

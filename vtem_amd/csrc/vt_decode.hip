@@ -1,0 +1,476 @@
+// Decode side, element-wise and segmented-reduction kernels.
+//
+// Coefficient space (D-space) factors, index conventions:
+//   temporal basis  Theta_0 = 1, Theta_{2m-1} = sqrt2 cos(m W t), Theta_{2m} = sqrt2 sin(m W t)
+//   alpha[k, pt]    = integral of Theta_pt over [tk1_k, tk2_k]
+//   T_j[pt, pt']    = sum_{k in j} norm_j^2 alpha[k,pt] alpha[k,pt']
+//   t_j[pt]         = sum_{k in j} norm_j alpha[k,pt] q_k
+// Spike-space kernels restate vtem/dirichlet.py (compute_q :204-222, G :272-330,
+// diag_add :334-350, reconstruct :354-420) and the RNN update of
+// vtem/neural_network.py:102-128.
+#include "vt_common.hpp"
+
+#define SQRT2 1.4142135623730950488
+
+// q_k = (kappa_j delta_j - b_j (tk2 - tk1)) norm_j   (dirichlet.py:216)
+__global__ void q_kernel(double *__restrict__ q, const double *__restrict__ tk1,
+                         const double *__restrict__ tk2, const int *__restrict__ nid,
+                         const double *__restrict__ kappa, const double *__restrict__ delta,
+                         const double *__restrict__ bias, const double *__restrict__ norm,
+                         const int *__restrict__ n_dev, int n_host)
+{
+    const int n = n_dev ? *n_dev : n_host;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int j = nid[i];
+        q[i] = (kappa[j] * delta[j] - bias[j] * (tk2[i] - tk1[i])) * norm[j];
+    }
+}
+
+VT_EXPORT int vt_q(double *q, const double *tk1, const double *tk2, const int *neuron_ind,
+                   const double *kappa, const double *delta, const double *bias,
+                   const double *norm, const int *n_dev, int n, void *stream)
+{
+    VT_CHECK_ARG(q && tk1 && tk2 && neuron_ind && kappa && delta && bias && norm && n >= 0);
+    if (n == 0) return VT_OK;
+    q_kernel<<<vt_blocks(n, 256) < 2048 ? vt_blocks(n, 256) : 2048, 256, 0, (hipStream_t)stream>>>(
+        q, tk1, tk2, neuron_ind, kappa, delta, bias, norm, n_dev, n);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// alpha for one interval into a[0..Dt)
+__device__ __forceinline__ void temporal_factor(double t1, double t2, int Lt, double W, double w,
+                                                double *a)
+{
+    a[0] = (t2 - t1) * w;
+    const double mid = 0.5 * (t1 + t2), half = 0.5 * (t2 - t1);
+    for (int m = 1; m <= Lt; ++m) {
+        const double om = m * W;
+        double s = 2.0 * SQRT2 * sin(om * half) / om;
+        double sm, cm;
+        sincos(om * mid, &sm, &cm);
+        a[2 * m - 1] = s * cm * w;
+        a[2 * m] = s * sm * w;
+    }
+}
+
+__global__ void temporal_factor_kernel(double *__restrict__ alpha, const double *__restrict__ tk1,
+                                       const double *__restrict__ tk2, int Lt, double W,
+                                       const int *__restrict__ n_dev, int n_host)
+{
+    const int n = n_dev ? *n_dev : n_host;
+    const int Dt = 2 * Lt + 1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        temporal_factor(tk1[i], tk2[i], Lt, W, 1.0, alpha + (long long)i * Dt);
+}
+
+VT_EXPORT int vt_temporal_factors(double *alpha, const double *tk1, const double *tk2, int Lt,
+                                  double W, const int *n_dev, int n, void *stream)
+{
+    VT_CHECK_ARG(alpha && tk1 && tk2 && Lt >= 0 && Lt <= 32 && n >= 0 && W > 0);
+    if (n == 0) return VT_OK;
+    temporal_factor_kernel<<<vt_blocks(n, 256) < 2048 ? vt_blocks(n, 256) : 2048, 256, 0,
+                             (hipStream_t)stream>>>(alpha, tk1, tk2, Lt, W, n_dev, n);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// One 64-lane workgroup per neuron: intervals in chunks of 64, alpha staged in LDS, each
+// lane owns a strided set of (pt, pt') entries of T_j.  Batched over independent problems
+// through per-problem descriptors (offsets into shared arrays).
+struct TjDesc {
+    int neuron_base;      // first neuron of this problem in the global neuron numbering
+    int num_neurons;
+    long long t_base;     // offset of this problem's intervals (tk1/tk2/q arrays)
+    int Lt;
+    double W;
+    int T_base;           // offset (in neurons) into T/t outputs
+};
+
+__global__ void __launch_bounds__(64)
+neuron_T_kernel(double *__restrict__ Tm, double *__restrict__ tv, const double *__restrict__ tk1,
+                const double *__restrict__ tk2, const double *__restrict__ q,
+                const double *__restrict__ norm, const int *__restrict__ offsets,
+                const TjDesc *__restrict__ descs, TjDesc d0)
+{
+    __shared__ double al[64][66];
+    __shared__ double qs[64];
+    // locate problem: blockIdx.y = problem, blockIdx.x = neuron within problem
+    const TjDesc d = descs ? descs[blockIdx.y] : d0;
+    const int jl = blockIdx.x;
+    if (jl >= d.num_neurons) return;
+    const int j = d.neuron_base + jl;
+    const int Dt = 2 * d.Lt + 1;
+    const int lane = threadIdx.x;
+    const int k0 = offsets[j], k1 = offsets[j + 1];
+    const double w = norm ? norm[j] : 1.0;
+    double accT[5] = {0, 0, 0, 0, 0};   // entries lane, lane+64, ... < Dt*Dt (Dt <= 17)
+    double acct = 0.0;
+    const int DT2 = Dt * Dt;
+    for (int base = k0; base < k1; base += 64) {
+        const int k = base + lane;
+        if (k < k1) {
+            const long long kk = d.t_base + k;
+            temporal_factor(tk1[kk], tk2[kk], d.Lt, d.W, w, &al[lane][0]);
+            qs[lane] = q[kk];
+        } else {
+            for (int p = 0; p < Dt; ++p) al[lane][p] = 0.0;
+            qs[lane] = 0.0;
+        }
+        __syncthreads();
+        const int cnt = min(64, k1 - base);
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            int idx = lane + 64 * e;
+            if (idx < DT2) {
+                int p = idx / Dt, pp = idx % Dt;
+                double s = 0.0;
+                for (int r = 0; r < cnt; ++r) s += al[r][p] * al[r][pp];
+                accT[e] += s;
+            }
+        }
+        if (lane < Dt) {
+            double s = 0.0;
+            for (int r = 0; r < cnt; ++r) s += al[r][lane] * qs[r];
+            acct += s;
+        }
+        __syncthreads();
+    }
+    const long long ob = (long long)(d.T_base + jl);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+        int idx = lane + 64 * e;
+        if (idx < DT2) Tm[ob * Dt * Dt + idx] = accT[e];
+    }
+    if (lane < Dt) tv[ob * Dt + lane] = acct;
+}
+
+// Single problem: N neurons with CSR offsets (N+1), intervals tk1/tk2/q.
+VT_EXPORT int vt_neuron_T(double *Tm, double *tv, const double *tk1, const double *tk2,
+                          const double *q, const double *norm, const int *offsets, int N,
+                          int Lt, double W, void *stream)
+{
+    VT_CHECK_ARG(Tm && tv && tk1 && tk2 && q && offsets && N >= 1 && Lt >= 0 && Lt <= 8 && W > 0);
+    TjDesc d{0, N, 0, Lt, W, 0};
+    hipStream_t s = (hipStream_t)stream;
+    neuron_T_kernel<<<dim3(N, 1), 64, 0, s>>>(Tm, tv, tk1, tk2, q, norm, offsets, nullptr, d);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// Batched: descs (device) has n_desc problems; neurons numbered globally, offsets global
+// (offsets[j] indexes the shared interval arrays directly, so t_base = 0 for all).
+VT_EXPORT int vt_neuron_T_batched(double *Tm, double *tv, const double *tk1, const double *tk2,
+                                  const double *q, const double *norm, const int *offsets,
+                                  const void *descs_dev, int n_desc, int max_neurons,
+                                  void *stream)
+{
+    VT_CHECK_ARG(Tm && tv && tk1 && tk2 && q && offsets && descs_dev && n_desc >= 1 &&
+                 max_neurons >= 1);
+    TjDesc d0{0, 0, 0, 0, 1.0, 0};
+    neuron_T_kernel<<<dim3(max_neurons, n_desc), 64, 0, (hipStream_t)stream>>>(
+        Tm, tv, tk1, tk2, q, norm, offsets, (const TjDesc *)descs_dev, d0);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Structured operator helpers: Z[j, :] = T_j Y[j, :]  (T_j symmetric Dt x Dt)
+// ------------------------------------------------------------------------------------
+__global__ void tj_apply_kernel(double *__restrict__ Z, const double *__restrict__ Tm,
+                                const double *__restrict__ Y, int N, int Dt)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)N * Dt) return;
+    const long long j = idx / Dt;
+    const int p = (int)(idx % Dt);
+    const double *T = Tm + j * Dt * Dt + (long long)p * Dt;
+    const double *y = Y + j * Dt;
+    double s = 0.0;
+    for (int pp = 0; pp < Dt; ++pp) s += T[pp] * y[pp];
+    Z[idx] = s;
+}
+
+VT_EXPORT int vt_tj_apply(double *Z, const double *Tm, const double *Y, int N, int Dt,
+                          void *stream)
+{
+    VT_CHECK_ARG(Z && Tm && Y && N >= 0 && Dt >= 1);
+    if (N == 0) return VT_OK;
+    tj_apply_kernel<<<vt_blocks((long long)N * Dt, 256), 256, 0, (hipStream_t)stream>>>(Z, Tm, Y,
+                                                                                      N, Dt);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// out = a*x + b*y + c*z (element-wise, fp64), used for residuals and RNN updates.
+__global__ void axpbypcz_kernel(double *__restrict__ out, double a, const double *__restrict__ x,
+                                double b, const double *__restrict__ y, double c,
+                                const double *__restrict__ z, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        double v = a * x[i];
+        if (y) v += b * y[i];
+        if (z) v += c * z[i];
+        out[i] = v;
+    }
+}
+
+VT_EXPORT int vt_axpbypcz(double *out, double a, const double *x, double b, const double *y,
+                          double c, const double *z, long long n, void *stream)
+{
+    VT_CHECK_ARG(out && x && n >= 0);
+    if (n == 0) return VT_OK;
+    unsigned g = vt_blocks(n, 256);
+    axpbypcz_kernel<<<g < 4096 ? g : 4096, 256, 0, (hipStream_t)stream>>>(out, a, x, b, y, c, z, n);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Tikhonov lambda on device: lam = lam_rel * trace(A) / n, n read from offsets[N].
+// Batched by descriptor arrays of (A pointer, lda, D, offsets pointer index).
+// ------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+trace_lambda_kernel(const T *A, long long lda, int D, const int *n_ptr, double lam_rel,
+                    double *lam_out)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < D; i += 256) s += (double)A[(long long)i * lda + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int n = *n_ptr;
+        lam_out[0] = n > 0 ? lam_rel * red[0] / (double)n : 0.0;
+        lam_out[1] = red[0];
+    }
+}
+
+VT_EXPORT int vt_trace_lambda(const void *A, int dtype, long long lda, int D, const int *n_dev,
+                              double lam_rel, double *lam_out, void *stream)
+{
+    VT_CHECK_ARG(A && n_dev && lam_out && D >= 1 && lda >= D);
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == 0)
+        trace_lambda_kernel<double><<<1, 256, 0, s>>>((const double *)A, lda, D, n_dev, lam_rel, lam_out);
+    else
+        trace_lambda_kernel<float><<<1, 256, 0, s>>>((const float *)A, lda, D, n_dev, lam_rel, lam_out);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Spike-space reference kernels
+// ------------------------------------------------------------------------------------
+
+// G_kl = Gt(k,l) * Dsw[j(k), j(l)]  (dirichlet.py:274-285, :316-323).  One row per block.
+__device__ __forceinline__ double G_dirichlet_time(double tk1, double tk2, double tl1, double tl2,
+                                                   int M, double WM)
+{
+    double sum = 0;
+    for (int m = 1; m <= M; ++m)
+        sum += (cos(m * WM * (tk2 - tl2)) - cos(m * WM * (tk2 - tl1)) -
+                cos(m * WM * (tk1 - tl2)) + cos(m * WM * (tk1 - tl1))) / (m * m);
+    sum = sum * 2 / (WM * WM) + (tk2 - tk1) * (tl2 - tl1);
+    return sum;
+}
+
+__global__ void __launch_bounds__(256)
+G_spike_kernel(double *__restrict__ G, long long ldg, const double *__restrict__ tk1,
+               const double *__restrict__ tk2, double Wt, int Mt, const double *__restrict__ Dsw,
+               long long ldw, const int *__restrict__ nid, int n)
+{
+    const int row = blockIdx.y;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n || col >= n) return;
+    const double a1 = tk1[row], a2 = tk2[row];
+    const int j1 = nid[row];
+    G[(long long)row * ldg + col] =
+        G_dirichlet_time(a1, a2, tk1[col], tk2[col], Mt, Wt / Mt) * Dsw[(long long)j1 * ldw + nid[col]];
+}
+
+VT_EXPORT int vt_G_spike(double *G, long long ldg, const double *tk1, const double *tk2, double Wt,
+                         int Mt, const double *Dsw, long long ldw, const int *neuron_ind, int n,
+                         void *stream)
+{
+    VT_CHECK_ARG(G && tk1 && tk2 && Dsw && neuron_ind && n >= 1 && ldg >= n && Mt >= 1);
+    VT_CHECK_ARG(n <= 65535);
+    G_spike_kernel<<<dim3(vt_blocks(n, 256), n), 256, 0, (hipStream_t)stream>>>(G, ldg, tk1, tk2, Wt,
+                                                                              Mt, Dsw, ldw,
+                                                                              neuron_ind, n);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+template <typename T>
+__global__ void diag_add_kernel(T *A, long long lda, int D, double lam, const double *lam_dev)
+{
+    const double l = lam_dev ? lam_dev[0] : lam;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < D; i += gridDim.x * blockDim.x)
+        A[(long long)i * lda + i] += (T)l;
+}
+
+VT_EXPORT int vt_diag_add(void *A, int dtype, long long lda, int D, double lam,
+                          const double *lam_dev, void *stream)
+{
+    VT_CHECK_ARG(A && D >= 0 && lda >= D);
+    if (D == 0) return VT_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == 0)
+        diag_add_kernel<double><<<vt_blocks(D, 256), 256, 0, s>>>((double *)A, lda, D, lam, lam_dev);
+    else
+        diag_add_kernel<float><<<vt_blocks(D, 256), 256, 0, s>>>((float *)A, lda, D, lam, lam_dev);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// u(t,p) = sum_k c_k norm_j psi_j(p) [2/WM sum_m (sin mWM(t-tk1) - sin mWM(t-tk2))/m + tk2-tk1]
+// (dirichlet.py:356-416).  Intervals staged 256 at a time in LDS; the temporal factor of
+// interval k at time t is computed once per block and shared by the block's pixels.
+__global__ void __launch_bounds__(256)
+reconstruct_spike_kernel(double *__restrict__ u, long long ldu, const double *__restrict__ psi,
+                         long long ldpsi, const double *__restrict__ tk1,
+                         const double *__restrict__ tk2, const double *__restrict__ c,
+                         const double *__restrict__ times, const int *__restrict__ nid,
+                         const double *__restrict__ norm, int M, double WM, int n, int P)
+{
+    __shared__ double wk[256];
+    __shared__ int jk[256];
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const double t = times[blockIdx.y];
+    double acc = 0.0;
+    for (int base = 0; base < n; base += 256) {
+        const int k = base + threadIdx.x;
+        if (k < n) {
+            double a1 = tk1[k], a2 = tk2[k];
+            double phi = 0;
+            for (int m = 1; m <= M; ++m)
+                phi += (sin(m * WM * (t - a1)) - sin(m * WM * (t - a2))) / m;
+            int j = nid[k];
+            wk[threadIdx.x] = c[k] * (phi * 2 / WM + a2 - a1) * norm[j];
+            jk[threadIdx.x] = j;
+        }
+        __syncthreads();
+        const int cnt = min(256, n - base);
+        if (pix < P)
+            for (int r = 0; r < cnt; ++r) acc += wk[r] * psi[(long long)jk[r] * ldpsi + pix];
+        __syncthreads();
+    }
+    if (pix < P) u[(long long)blockIdx.y * ldu + pix] = acc;
+}
+
+VT_EXPORT int vt_reconstruct_spike(double *u, long long ldu, const double *psi, long long ldpsi,
+                                   const double *tk1, const double *tk2, const double *c,
+                                   const double *times, int T, const int *neuron_ind,
+                                   const double *norm, int Mt, double WM, int n, int P,
+                                   void *stream)
+{
+    VT_CHECK_ARG(u && psi && tk1 && tk2 && c && times && neuron_ind && norm && T >= 1 && n >= 0 &&
+                 P >= 1 && ldu >= P && ldpsi >= P && T <= 65535);
+    reconstruct_spike_kernel<<<dim3(vt_blocks(P, 256), T), 256, 0, (hipStream_t)stream>>>(
+        u, ldu, psi, ldpsi, tk1, tk2, c, times, neuron_ind, norm, Mt, WM, n, P);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// Symmetric mat-vec y = G c for the RNN (one wave per row, fp64).
+__global__ void __launch_bounds__(256)
+gemv_kernel(double *__restrict__ y, const double *__restrict__ G, long long ldg,
+            const double *__restrict__ x, int n)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double *g = G + (long long)row * ldg;
+    double s = 0.0;
+    for (int i = lane; i < n; i += 64) s += g[i] * x[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) y[row] = s;
+}
+
+// c += eta (q - Gc); y = q - Gc  (neural_network.py:111-121, groupsize 1)
+__global__ void rnn_update_kernel(double *__restrict__ c, double eta, const double *__restrict__ q,
+                                  const double *__restrict__ Gc, double *__restrict__ y, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double dy = q[i] - Gc[i];
+        c[i] += eta * dy;
+        y[i] = dy;
+    }
+}
+
+VT_EXPORT int vt_gemv(double *y, const double *G, long long ldg, const double *x, int n, void *stream)
+{
+    VT_CHECK_ARG(y && G && x && n >= 0 && ldg >= n);
+    if (n == 0) return VT_OK;
+    gemv_kernel<<<vt_blocks(n, 4), 256, 0, (hipStream_t)stream>>>(y, G, ldg, x, n);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// steps+1 iterations of {Gc = G c; c += eta (q - Gc)} on the stream (rnn3's loop).
+VT_EXPORT int vt_rnn_spike(double *c, const double *G, long long ldg, const double *q, int n,
+                           double eta, int steps, double *work_Gc, double *work_y, void *stream)
+{
+    VT_CHECK_ARG(c && G && q && work_Gc && work_y && n >= 1 && steps >= 0 && ldg >= n);
+    hipStream_t s = (hipStream_t)stream;
+    for (int i = 0; i <= steps; ++i) {
+        gemv_kernel<<<vt_blocks(n, 4), 256, 0, s>>>(work_Gc, G, ldg, c, n);
+        unsigned g = vt_blocks(n, 256);
+        rnn_update_kernel<<<g < 1024 ? g : 1024, 256, 0, s>>>(c, eta, q, work_Gc, work_y, n);
+    }
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// res = r - AX - lam[0] * X  (lam on device, no host round trip)
+__global__ void residual_kernel(double *__restrict__ res, const double *__restrict__ r,
+                                const double *__restrict__ AX, const double *__restrict__ X,
+                                const double *__restrict__ lam, long long n)
+{
+    const double l = lam ? lam[0] : 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        res[i] = r[i] - AX[i] - l * X[i];
+}
+
+VT_EXPORT int vt_residual(double *res, const double *r, const double *AX, const double *X,
+                          const double *lam_dev, long long n, void *stream)
+{
+    VT_CHECK_ARG(res && r && AX && X && n >= 0);
+    if (n == 0) return VT_OK;
+    unsigned g = vt_blocks(n, 256);
+    residual_kernel<<<g < 1024 ? g : 1024, 256, 0, (hipStream_t)stream>>>(res, r, AX, X, lam_dev, n);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// out[slot] = sum x^2 over n doubles (one workgroup; small vectors)
+__global__ void __launch_bounds__(1024) sumsq_kernel(const double *__restrict__ x, long long n,
+                                                    double *__restrict__ out)
+{
+    __shared__ double red[1024];
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 1024) s += x[i] * x[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+VT_EXPORT int vt_sumsq(const double *x, long long n, double *out, void *stream)
+{
+    VT_CHECK_ARG(x && out && n >= 0);
+    sumsq_kernel<<<1, 1024, 0, (hipStream_t)stream>>>(x, n, out);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
