@@ -1,0 +1,213 @@
+/*
+ * vtem_hip.h -- C-ABI of libvtem_hip.so, the MI355X (gfx950) implementation of the video
+ * time-encoding / time-decoding path of bionet/vtem.
+ *
+ * Conventions (all entries):
+ *   - pointers are device addresses (hipMalloc / torch tensors) unless stated;
+ *   - matrices are row-major with an explicit leading dimension in elements (the
+ *     reference's PitchArray `ld`, vtem/utils/parray.py:136-157);
+ *   - dtype selectors: 0 = float64, 1 = float32;
+ *   - `stream` is a hipStream_t (void*); 0 is the legacy default stream;
+ *   - every entry only enqueues work on `stream` (no host synchronisation) and returns
+ *     an int status: 0 OK, > 0 hipError_t, < 0 argument error (-1 invalid argument,
+ *     -2 unsupported, -3 spike buffer overflow, -4 not positive definite, -5 beyond a
+ *     documented size limit);
+ *   - callers own every buffer; workspaces are passed in explicitly.
+ *
+ * Coefficient-space conventions: real spatial basis index ps (0: constant; 1+2i, 2+2i:
+ * sqrt2 cos / sqrt2 sin of half-set mode i), real temporal basis index pt (0: constant;
+ * 2m-1, 2m: sqrt2 cos / sqrt2 sin of m*W*t); D-space index d = ps*Dt + pt.
+ */
+#ifndef VTEM_HIP_H
+#define VTEM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- receptive fields -- */
+
+/* Mixed Gabor/DoG bank on a Px x Py grid -> out (N x Px*Py, row stride ld).
+ * kind[j]: 0 Gabor, 1 DoG.  alpha_inv = 1/h_alpha.
+ * Replaces vrf_gabor/vrf_cs.generate_visual_receptive_fields (vtem/vrf.py:296-331,
+ * :876-913) and the gabor_Kernel / cs_Kernel they launch (vtem/vrf.py:1197-1291,
+ * :1120-1193). */
+int vt_rf_bank(void *out, int dtype, long long ld, int N, const int *kind,
+               const double *alpha_inv, const double *theta, const double *x0,
+               const double *y0, const int *ab, double kappa, double sigma_c2,
+               double sigma_s2, int Px, int Py, double Sx, double Sy, double x_start,
+               double y_start, void *stream);
+/* Gabor-only bank: gabor_Kernel (vtem/vrf.py:1197-1291). */
+int vt_rf_gabor(void *out, int dtype, long long ld, int N, const double *alpha_inv,
+                const double *theta, const double *x0, const double *y0, const int *ab,
+                double kappa, int Px, int Py, double Sx, double Sy, double x_start,
+                double y_start, void *stream);
+/* DoG-only bank: cs_Kernel (vtem/vrf.py:1120-1193). */
+int vt_rf_dog(void *out, int dtype, long long ld, int N, const double *alpha_inv,
+              const double *x0, const double *y0, double sigma_c2, double sigma_s2, int Px,
+              int Py, double Sx, double Sy, double x_start, double y_start, void *stream);
+
+/* ---------------------------------------------------------------- dense algebra ----- */
+
+/* C[b] = alpha * A[b](m,k) B[b](k,n) + beta * C[b]; A(m,k) = A[m*sam + k*sak],
+ * B(k,n) = B[k*sbk + n*sbn], C[m*ldc + n]; batch strides bsa/bsb/bsc.
+ * Replaces the cuBLAS GEMMs of the path: filtering (vtem/vrf.py:291, :871), the
+ * spatial Gram Zgemm (vtem/vrf.py:529, :1112; vtem/dirichlet.py:109-112) and
+ * vtem/utils/linalg.py dot (:32-338). */
+int vt_gemm_f64(int M, int N, int K, double alpha, const double *A, long long sam,
+                long long sak, long long bsa, const double *B, long long sbk, long long sbn,
+                long long bsb, double beta, double *C, long long ldc, long long bsc, int batch,
+                void *stream);
+int vt_gemm_f32(int M, int N, int K, float alpha, const float *A, long long sam, long long sak,
+                long long bsa, const float *B, long long sbk, long long sbn, long long bsb,
+                float beta, float *C, long long ldc, long long bsc, int batch, void *stream);
+/* C[m,n] = alpha sum_k A[k*lda+m] B[k*ldb+n] + beta C[m,n] for narrow outputs (Nc <= 32)
+ * and long K: split over 16 K-chunks into workspace (16*M*Nc doubles), then summed in a
+ * fixed order (deterministic).  Used for S^T t (the D-space right
+ * hand side, replacing the reference's per-interval q vector use in
+ * vtem/utils/linalg.py:623). */
+int vt_gemm_tn_small(int M, int Nc, int K, double alpha, const double *A, long long lda,
+                     const double *B, long long ldb, double beta, double *C, long long ldc,
+                     double *workspace, void *stream);
+/* One MFMA of a given kind (0 fp64 16x16x4, 1 fp32 16x16x4, 2 bf16 16x16x32,
+ * 3 f16 16x16x32) on 16xK / Kx16 inputs -> 16x16 out.  Test hook for the lane layouts;
+ * no reference counterpart. */
+int vt_mfma_probe(int kind, const void *A, const void *B, void *out, void *stream);
+
+/* ---------------------------------------------------------------- IAF encoder ------- */
+
+/* Linear IAF population encoder, one lane per neuron.  input (size x N, row stride
+ * input_ld, dtype in_dtype); spikes (max_spike x N, row stride spike_ld) fp64; v0 and
+ * time_count carried in/out; spike_count out (a count > max_spike - 1 means overflow,
+ * as the reference's clamp, and the host raises).  root_mode 0 = the reference's
+ * crossing formula, 1 = the cancellation-free equivalent.
+ * Replaces IAF_encode.encode and its ensemble_encode kernel
+ * (vtem/ensemble_encode.py:106-168, :438-552). */
+int vt_iaf_encode(const void *input, int in_dtype, long long input_ld, int N, int size,
+                  double *spikes, long long spike_ld, int max_spike, double *v0,
+                  const double *kappa, const double *bias, const double *delta,
+                  double *time_count, int *spike_count, double dt, int root_mode,
+                  void *stream);
+/* Same encoder with the input given as temporal trigonometric coefficients coef
+ * (Dt x N, row stride coef_ld) and the basis table basis (size x Dt): the dt-rate input
+ * is evaluated in registers.  Replaces the frame-rate input of VTEM_IAF
+ * (vtem/vtem.py:176-200) with exact interpolation to dt; no reference counterpart for
+ * the interpolation itself. */
+int vt_iaf_encode_trig(const double *coef, long long coef_ld, const double *basis, int Dt,
+                       int N, int size, double *spikes, long long spike_ld, int max_spike,
+                       double *v0, const double *kappa, const double *bias,
+                       const double *delta, double *time_count, int *spike_count, double dt,
+                       int root_mode, void *stream);
+/* Random-threshold IAF: spikes holds N(0,1) draws on entry; delta_value carried in/out.
+ * Replaces IAF_encode_rt.encode (vtem/ensemble_encode.py:265-329, kernel :555-674). */
+int vt_iaf_encode_rt(const double *input, long long input_ld, int N, int size, double *spikes,
+                     long long spike_ld, int max_spike, double *v0, const double *kappa,
+                     const double *bias, const double *delta_mean, double *time_count,
+                     int *spike_count, double dt, double *delta_value, const double *sigma,
+                     int root_mode, void *stream);
+
+/* ---------------------------------------------------------------- intervals, q ------- */
+
+/* Spike buffer -> intervals of a window [t0, t0+Tw] (times shifted by t0; per neuron
+ * the spikes in (0, Tw) plus one before and one after), CSR by neuron: offsets (N+1,
+ * offsets[N] = n on device), tk1, tk2, neuron_ind (capacity entries).  workspace: 2N
+ * ints.  Replaces vtem_storage_read.read_select_spikes (vtem/utils/spike_store.py:279-351). */
+int vt_intervals(const double *spikes, long long spike_ld, int max_rows,
+                 const int *spike_count, int N, double t0, double Tw, int *workspace,
+                 int *offsets, double *tk1, double *tk2, int *neuron_ind, int capacity,
+                 void *stream);
+/* offsets[0..n] = exclusive scan of in[0..n) (one workgroup). */
+int vt_exclusive_scan(const int *in, int n, int *offsets, void *stream);
+/* q_k = (kappa_j delta_j - b_j (tk2-tk1)) norm_j; n from n_dev when given.
+ * Replaces dirichlet.compute_q / compute_q_Kernel (vtem/dirichlet.py:55-60, :204-222). */
+int vt_q(double *q, const double *tk1, const double *tk2, const int *neuron_ind,
+         const double *kappa, const double *delta, const double *bias, const double *norm,
+         const int *n_dev, int n, void *stream);
+/* alpha (n x Dt): integrals of the temporal basis over each interval (Lt <= 32). */
+int vt_temporal_factors(double *alpha, const double *tk1, const double *tk2, int Lt, double W,
+                        const int *n_dev, int n, void *stream);
+/* Per-neuron temporal moments T_j = sum norm^2 alpha alpha^T (N x Dt x Dt) and
+ * t_j = sum norm alpha q (N x Dt), Lt <= 8.  Together with vt_gram_f64 this replaces
+ * dirichlet.compute_G (vtem/dirichlet.py:62-85, kernel :272-330). */
+int vt_neuron_T(double *Tm, double *tv, const double *tk1, const double *tk2, const double *q,
+                const double *norm, const int *offsets, int N, int Lt, double W, void *stream);
+/* Batched over independent problems described by a device descriptor array. */
+int vt_neuron_T_batched(double *Tm, double *tv, const double *tk1, const double *tk2,
+                        const double *q, const double *norm, const int *offsets,
+                        const void *descs_dev, int n_desc, int max_neurons, void *stream);
+
+/* ---------------------------------------------------------------- Gram / operator --- */
+
+/* A = sum_j (S_j S_j^T) kron T_j on FP64 MFMA, lower block triangle of A (D x D, D =
+ * Ds*Dt).  Replaces compute_Dsw + compute_G (vtem/vrf.py:502-536,
+ * vtem/dirichlet.py:62-85) through the push-through identity. */
+int vt_gram_f64(const double *S, long long lds, const double *Tm, int N, int Ds, int Dt,
+                double *A, long long lda, void *stream);
+/* Copy the lower triangle into the upper one. */
+int vt_symmetrize(void *A, int dtype, long long lda, int D, void *stream);
+/* Z[j,:] = T_j Y[j,:] (structured operator step). */
+int vt_tj_apply(double *Z, const double *Tm, const double *Y, int N, int Dt, void *stream);
+/* out = a x + b y + c z (y, z may be null). */
+int vt_axpbypcz(double *out, double a, const double *x, double b, const double *y, double c,
+                const double *z, long long n, void *stream);
+/* res = r - AX - lam_dev[0] X. */
+int vt_residual(double *res, const double *r, const double *AX, const double *X,
+                const double *lam_dev, long long n, void *stream);
+/* out[0] = sum x^2. */
+int vt_sumsq(const double *x, long long n, double *out, void *stream);
+/* lam_out[0] = lam_rel * trace(A) / n_dev[0], lam_out[1] = trace(A). */
+int vt_trace_lambda(const void *A, int dtype, long long lda, int D, const int *n_dev,
+                    double lam_rel, double *lam_out, void *stream);
+/* A += lam I (lam from lam_dev[0] when given).  Replaces diag_add_Kernel
+ * (vtem/dirichlet.py:83-85, :334-350). */
+int vt_diag_add(void *A, int dtype, long long lda, int D, double lam, const double *lam_dev,
+                void *stream);
+
+/* ---------------------------------------------------------------- solves ------------- */
+
+/* Blocked Cholesky (block 64) of a batch of matrices described by a device array of
+ * {T *A; long long lda; int D; int pad;} records; inv_ws receives the diagonal-block
+ * inverses (per matrix ceil(maxD/64)*4096 elements, inv_stride apart); info[b] = first
+ * failing pivot + 1 (caller zeroes it).  Replaces linalg.solve_eq_sym's factorisation
+ * step (CULA gesvd, vtem/utils/linalg.py:591-631) with the Tikhonov-regularised form. */
+int vt_potrf_f64(const void *mats_dev, int batch, int maxD, double *inv_ws,
+                 long long inv_stride, int *info, void *stream);
+int vt_potrf_f32(const void *mats_dev, int batch, int maxD, float *inv_ws,
+                 long long inv_stride, int *info, void *stream);
+/* x <- (L L^T)^-1 x for each matrix of the batch (x fp64, x_stride apart), one workgroup
+ * per 64-row block with flag-synchronised forward/backward sweeps.  sync: 2 +
+ * 2*batch*ceil(maxD/64) ints zeroed once by the caller and reused (sync[0] is an epoch
+ * bumped on device, sync[1] != 0 reports an expired bounded wait).  Replaces the U^H q /
+ * U qq products of solve_eq_sym (vtem/utils/linalg.py:623-630). */
+int vt_potrs_f64(const void *mats_dev, int batch, int maxD, const double *inv_ws,
+                 long long inv_stride, double *x, long long x_stride, int *sync, void *stream);
+int vt_potrs_f32(const void *mats_dev, int batch, int maxD, const float *inv_ws,
+                 long long inv_stride, double *x, long long x_stride, int *sync, void *stream);
+/* fp32 copy of the lower triangle of an fp64 matrix, lam_dev[0] added to the diagonal. */
+int vt_lower_to_f32(float *dst, long long ldd, const double *src, long long lds, int D,
+                    const double *lam_dev, void *stream);
+
+/* ---------------------------------------------------------------- spike-space path --- */
+
+/* G (n x n) by the reference's separable closed form.  Replaces compute_G_Kernel
+ * (vtem/dirichlet.py:272-330). */
+int vt_G_spike(double *G, long long ldg, const double *tk1, const double *tk2, double Wt,
+               int Mt, const double *Dsw, long long ldw, const int *neuron_ind, int n,
+               void *stream);
+/* y = G x (one wave per row). */
+int vt_gemv(double *y, const double *G, long long ldg, const double *x, int n, void *stream);
+/* steps+1 iterations of c += eta (q - G c).  Replaces neural_network.rnn3
+ * (vtem/neural_network.py:15-99, update kernel :102-128). */
+int vt_rnn_spike(double *c, const double *G, long long ldg, const double *q, int n,
+                 double eta, int steps, double *work_Gc, double *work_y, void *stream);
+/* u(t,p) for T output times.  Replaces dirichlet.reconstruct / reconstruct_Kernel
+ * (vtem/dirichlet.py:165-189, :354-420). */
+int vt_reconstruct_spike(double *u, long long ldu, const double *psi, long long ldpsi,
+                         const double *tk1, const double *tk2, const double *c,
+                         const double *times, int T, const int *neuron_ind,
+                         const double *norm, int Mt, double WM, int n, int P, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VTEM_HIP_H */

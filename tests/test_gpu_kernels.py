@@ -65,6 +65,23 @@ def test_gemm(torch_mod, M, N, K, ta, tb, dtype):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("M,Nc,K", [(289, 17, 2048), (5, 1, 3), (40, 32, 999)])
+def test_gemm_tn_small(torch_mod, M, Nc, K):
+    torch = torch_mod
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    _banner(f"gemm_tn_small M={M} Nc={Nc} K={K}")
+    rng = np.random.default_rng(M + K)
+    A = rng.standard_normal((K, M + 3))
+    B = rng.standard_normal((K, Nc))
+    C0 = rng.standard_normal((M, Nc))
+    a, b, c = (torch.tensor(v, device="cuda") for v in (A, B, C0))
+    ws = torch.empty(16 * M * Nc, dtype=torch.float64, device="cuda")
+    lib.vt_gemm_tn_small(M, Nc, K, 2.0, ptr(a), M + 3, ptr(b), Nc, -1.0, ptr(c), Nc, ptr(ws),
+                         stream_ptr())
+    ref = 2.0 * A[:, :M].T @ B - C0
+    assert np.abs(c.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-13
+
+
 def test_rf_bank(torch_mod, oracle):
     torch = torch_mod
     from vtem_amd import synthetic as syn
@@ -204,7 +221,8 @@ def test_gram_f64(torch_mod, oracle, cfg):
     Aref = oracle.gram_kron(S, Tm)
     D = Ds * Dt
     A = torch.zeros((D, D), dtype=torch.float64, device="cuda")
-    lib.vt_gram_f64(ptr(dev(S)), Ds, ptr(dev(Tm)), N, Ds, Dt, ptr(A), D, stream_ptr())
+    S_d, T_d = dev(S), dev(Tm)      # keep the buffers alive until the launch is enqueued
+    lib.vt_gram_f64(ptr(S_d), Ds, ptr(T_d), N, Ds, Dt, ptr(A), D, stream_ptr())
     got = np.tril(A.cpu().numpy())
     ref = np.tril(Aref)
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
@@ -231,10 +249,16 @@ def test_potrf_potrs(torch_mod, D, prec):
     x = torch.tensor(b, dtype=torch.float64, device="cuda")
     f = lib.vt_potrf_f64 if prec == "f64" else lib.vt_potrf_f32
     g = lib.vt_potrs_f64 if prec == "f64" else lib.vt_potrs_f32
+    sync = torch.zeros(2 + 2 * nblk, dtype=torch.int32, device="cuda")
     f(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(info), stream_ptr())
-    g(ptr(mats), 1, ptr(inv), inv.numel(), ptr(x), D, stream_ptr())
+    g(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(x), D, ptr(sync), stream_ptr())
+    # a second solve reuses the flag workspace (epoch advanced on device)
+    x2 = torch.tensor(b, dtype=torch.float64, device="cuda")
+    g(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(x2), D, ptr(sync), stream_ptr())
     torch.cuda.synchronize()
     assert int(info.item()) == 0
+    assert int(sync[1].item()) == 0 and int(sync[0].item()) == 2
+    np.testing.assert_array_equal(x.cpu().numpy(), x2.cpu().numpy())
     L = np.tril(a.cpu().numpy().astype(np.float64))
     Lref = np.linalg.cholesky(A)
     tol = 1e-12 if prec == "f64" else 1e-5
@@ -267,9 +291,12 @@ def test_potrf_batched_ragged(torch_mod):
     for i, b in enumerate(bs):
         x[i * maxD:i * maxD + len(b)] = torch.tensor(b, device="cuda")
     lib.vt_potrf_f64(ptr(desc), len(sizes), maxD, ptr(inv), nblk * 4096, ptr(info), stream_ptr())
-    lib.vt_potrs_f64(ptr(desc), len(sizes), ptr(inv), nblk * 4096, ptr(x), maxD, stream_ptr())
+    sync = torch.zeros(2 + 2 * len(sizes) * nblk, dtype=torch.int32, device="cuda")
+    lib.vt_potrs_f64(ptr(desc), len(sizes), maxD, ptr(inv), nblk * 4096, ptr(x), maxD, ptr(sync),
+                     stream_ptr())
     torch.cuda.synchronize()
     assert (info.cpu().numpy() == 0).all()
+    assert int(sync[1].item()) == 0
     xs = x.cpu().numpy()
     for i, (A, b) in enumerate(zip(As, bs)):
         xr = np.linalg.solve(A, b)
@@ -333,8 +360,9 @@ def test_spike_space_reference_path(torch_mod, oracle):
     Dsw = oracle.compute_Dsw(Ds_, norm)
     G = oracle.compute_G_spike(o1, o2, oi, Dsw, 2 * np.pi * 4, 4)
     Gg = torch.zeros((n, n), dtype=torch.float64, device="cuda")
-    lib.vt_G_spike(ptr(Gg), n, ptr(dev(o1)), ptr(dev(o2)), 2 * np.pi * 4, 4, ptr(dev(Dsw)),
-                   len(sub), ptr(dev(oi, torch.int32)), n, stream_ptr())
+    d1, d2, dw, di = dev(o1), dev(o2), dev(Dsw), dev(oi, torch.int32)
+    lib.vt_G_spike(ptr(Gg), n, ptr(d1), ptr(d2), 2 * np.pi * 4, 4, ptr(dw), len(sub), ptr(di), n,
+                   stream_ptr())
     e = np.linalg.norm(Gg.cpu().numpy() - G) / np.linalg.norm(G)
     assert e < 1e-13, e
     q = oracle.compute_q(o1, o2, oi, iafp.kappa[sub], iafp.delta[sub], iafp.bias[sub], norm)
@@ -342,7 +370,8 @@ def test_spike_space_reference_path(torch_mod, oracle):
     cg = torch.zeros(n, dtype=torch.float64, device="cuda")
     w1 = torch.empty(n, dtype=torch.float64, device="cuda")
     w2 = torch.empty(n, dtype=torch.float64, device="cuda")
-    lib.vt_rnn_spike(ptr(cg), ptr(Gg), n, ptr(dev(q)), n, 1e-6 * 5000, 200, ptr(w1), ptr(w2),
+    dq = dev(q)
+    lib.vt_rnn_spike(ptr(cg), ptr(Gg), n, ptr(dq), n, 1e-6 * 5000, 200, ptr(w1), ptr(w2),
                      stream_ptr())
     e = np.abs(cg.cpu().numpy() - c).max() / np.abs(c).max()
     assert e < 1e-12, e
@@ -350,9 +379,8 @@ def test_spike_space_reference_path(torch_mod, oracle):
     t = np.arange(0, 1.0, 0.01)
     u = oracle.reconstruct_spike(c, o1, o2, oi, norm, psi, t, 4, 2 * np.pi)
     ug = torch.zeros((t.size, 1024), dtype=torch.float64, device="cuda")
-    lib.vt_reconstruct_spike(ptr(ug), 1024, ptr(dev(psi.reshape(len(sub), -1))), 1024,
-                             ptr(dev(o1)), ptr(dev(o2)), ptr(cg), ptr(dev(t)), t.size,
-                             ptr(dev(oi, torch.int32)), ptr(dev(norm)), 4, 2 * np.pi, n, 1024,
-                             stream_ptr())
+    dpsi, dt_, dn = dev(psi.reshape(len(sub), -1)), dev(t), dev(norm)
+    lib.vt_reconstruct_spike(ptr(ug), 1024, ptr(dpsi), 1024, ptr(d1), ptr(d2), ptr(cg), ptr(dt_),
+                             t.size, ptr(di), ptr(dn), 4, 2 * np.pi, n, 1024, stream_ptr())
     e = np.abs(ug.cpu().numpy().reshape(u.shape) - u).max() / np.abs(u).max()
     assert e < 1e-11, e

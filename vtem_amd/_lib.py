@@ -37,6 +37,7 @@ SIGNATURES = {
     "vt_gemm_f64": [I, I, I, D, P, L, L, L, P, L, L, L, D, P, L, L, I, P],
     "vt_gemm_f32": [I, I, I, F, P, L, L, L, P, L, L, L, F, P, L, L, I, P],
     "vt_mfma_probe": [I, P, P, P, P],
+    "vt_gemm_tn_small": [I, I, I, D, P, L, P, L, D, P, L, P, P],
     # decode
     "vt_q": [P, P, P, P, P, P, P, P, P, I, P],
     "vt_temporal_factors": [P, P, P, I, D, P, I, P],
@@ -57,8 +58,8 @@ SIGNATURES = {
     "vt_symmetrize": [P, I, L, I, P],
     "vt_potrf_f64": [P, I, I, P, L, P, P],
     "vt_potrf_f32": [P, I, I, P, L, P, P],
-    "vt_potrs_f64": [P, I, P, L, P, L, P],
-    "vt_potrs_f32": [P, I, P, L, P, L, P],
+    "vt_potrs_f64": [P, I, I, P, L, P, L, P, P],
+    "vt_potrs_f32": [P, I, I, P, L, P, L, P, P],
     "vt_lower_to_f32": [P, L, P, L, I, P, P],
 }
 

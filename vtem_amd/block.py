@@ -204,6 +204,7 @@ class Decoder:
         self.norm = dev(np.ones(N) if norm is None else norm, F64, device)
         self.Bs = dev(geom.spatial(), F64, device)
         self.theta_out = dev(geom.temporal(geom.frame_times()), F64, device)
+        self._tn_ws = torch.empty(16 * geom.Ds * geom.Dt, dtype=F64, device=device)
 
     # -- spikes -> intervals -> moments ------------------------------------------------
     def intervals(self, st: SpikeTrains, t0: float = 0.0, Tw: float = None):
@@ -246,7 +247,8 @@ class Decoder:
     def rhs(self, tv):
         g = self.geom
         r = torch.empty(g.D, dtype=F64, device=self.device)
-        gemm(g.Ds, g.Dt, self.N, 1.0, self.S, 1, self.S.stride(0), tv, g.Dt, 1, 0.0, r, g.Dt)
+        lib.vt_gemm_tn_small(g.Ds, g.Dt, self.N, 1.0, ptr(self.S), self.S.stride(0), ptr(tv), g.Dt,
+                             0.0, ptr(r), g.Dt, ptr(self._tn_ws), stream_ptr())
         return r
 
     def apply_A(self, Tm, X, out=None):
@@ -258,7 +260,8 @@ class Decoder:
         Z = torch.empty_like(Y)
         lib.vt_tj_apply(ptr(Z), ptr(Tm), ptr(Y), N, Dt, stream_ptr())
         out = torch.empty(g.D, dtype=F64, device=self.device) if out is None else out
-        gemm(Ds, Dt, N, 1.0, self.S, 1, self.S.stride(0), Z, Dt, 1, 0.0, out, Dt)
+        lib.vt_gemm_tn_small(Ds, Dt, N, 1.0, ptr(self.S), self.S.stride(0), ptr(Z), Dt, 0.0,
+                             ptr(out), Dt, ptr(self._tn_ws), stream_ptr())
         return out
 
     def synthesize(self, X, out=None):
@@ -272,6 +275,14 @@ class Decoder:
         return out
 
     # -- solves ---------------------------------------------------------------------------
+    def _sync_ws(self, batch, maxD):
+        """Flag workspace of the triangular solves (zeroed once, reused across calls)."""
+        need = 2 + 2 * batch * ((maxD + 63) // 64)
+        ws = getattr(self, "_sync", None)
+        if ws is None or ws.numel() < need:
+            ws = torch.zeros(need, dtype=I32, device=self.device)
+            self._sync = ws
+        return ws
     def solve_tikhonov(self, A, r, n_dev, lam_rel=1e-6, precision="fp64", refine_iters=4):
         """(A + lam I) X = r with lam = lam_rel tr(A)/n.
 
@@ -285,13 +296,15 @@ class Decoder:
         lib.vt_trace_lambda(ptr(A), 0, A.stride(0), D, ptr(n_dev), lam_rel, ptr(lam), stream_ptr())
         nblk = (D + 63) // 64
         info = torch.zeros(1, dtype=I32, device=self.device)
+        sync = self._sync_ws(1, D)
         X = r.clone()
         if precision == "fp64":
             lib.vt_diag_add(ptr(A), 0, A.stride(0), D, 0.0, ptr(lam), stream_ptr())
             inv = torch.empty(nblk * 64 * 64, dtype=F64, device=self.device)
             mats = mat_descs([(A.data_ptr(), A.stride(0), D)], self.device)
             lib.vt_potrf_f64(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(info), stream_ptr())
-            lib.vt_potrs_f64(ptr(mats), 1, ptr(inv), inv.numel(), ptr(X), D, stream_ptr())
+            lib.vt_potrs_f64(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(X), D, ptr(sync),
+                             stream_ptr())
             return X, lam, info, None
         # mixed
         A32 = torch.empty((D, D), dtype=torch.float32, device=self.device)
@@ -299,7 +312,7 @@ class Decoder:
         inv = torch.empty(nblk * 64 * 64, dtype=torch.float32, device=self.device)
         mats = mat_descs([(A32.data_ptr(), D, D)], self.device)
         lib.vt_potrf_f32(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(info), stream_ptr())
-        lib.vt_potrs_f32(ptr(mats), 1, ptr(inv), inv.numel(), ptr(X), D, stream_ptr())
+        lib.vt_potrs_f32(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(X), D, ptr(sync), stream_ptr())
         res = torch.empty(D, dtype=F64, device=self.device)
         AX = torch.empty(D, dtype=F64, device=self.device)
         nrm = torch.zeros(refine_iters + 1, dtype=F64, device=self.device)
@@ -308,7 +321,8 @@ class Decoder:
             self.apply_A(self._Tm, X, out=AX)
             lib.vt_residual(ptr(res), ptr(r), ptr(AX), ptr(X), ptr(lam), D, stream_ptr())
             lib.vt_sumsq(ptr(res), D, ptr(nrm[it + 1:]), stream_ptr())
-            lib.vt_potrs_f32(ptr(mats), 1, ptr(inv), inv.numel(), ptr(res), D, stream_ptr())
+            lib.vt_potrs_f32(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(res), D, ptr(sync),
+                             stream_ptr())
             lib.vt_axpbypcz(ptr(X), 1.0, ptr(X), 1.0, ptr(res), 0.0, 0, D, stream_ptr())
         return X, lam, info, nrm
 

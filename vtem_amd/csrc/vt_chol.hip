@@ -38,65 +38,99 @@ template <> struct Mfma16<float> {
 };
 
 // ------------------------------------------------------------------------------------
-// diag: factor + invert the 64x64 diagonal block (in LDS, 256 threads)
+// diag: factor + invert the 64x64 diagonal block.  256 threads; thread (r, q) keeps the
+// 16 entries [16q, 16q+16) of row r in registers (static indices after unrolling).  At
+// column step c the threads owning column c publish it to LDS, everybody reads the
+// pivot and the scaled column back as broadcasts, and each thread applies at most 16
+// register FMAs:  L[r][rp] -= L[r][c] L[rp][c]  for c < rp <= r.  The inverse
+// X = L^-1 uses the same layout, right-looking over rows:
+//   X[k,:] /= L[k][k];   X[r,:] -= L[r][k] X[k,:]  (r > k).
 // ------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256)
 potrf_diag_kernel(const VtMat<T> *__restrict__ mats, int kb, T *__restrict__ inv_ws,
                   long long inv_stride, int *__restrict__ info)
 {
-    __shared__ T L[NB][NB + 1];
-    __shared__ T X[NB][NB + 1];
+    __shared__ T Ls[NB][NB + 1];
+    __shared__ T col[NB];
+    __shared__ T xrow[NB];
     const VtMat<T> m = mats[blockIdx.x];
     if (kb >= m.D) return;
     const int nb = min(NB, m.D - kb);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6, c0 = q * 16;
     T *Ad = m.A + (long long)kb * m.lda + kb;
-    for (int e = tid; e < NB * NB; e += 256) {
-        int r = e / NB, c = e % NB;
-        L[r][c] = (r < nb && c < nb && c <= r) ? Ad[(long long)r * m.lda + c] : (T)0;
-        X[r][c] = (r == c) ? (T)1 : (T)0;
+    T Lr[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int c = c0 + i;
+        T v = (T)0;
+        if (r < nb && c <= r) v = Ad[(long long)r * m.lda + c];
+        if (r >= nb && c == r) v = (T)1;      // identity rows pad a partial last block
+        Lr[i] = v;
     }
+    int bad = 0;
+    for (int c = 0; c < NB; ++c) {
+        const int cq = c >> 4, ci = c & 15;
+        if (q == cq) {
+            T v = (T)0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) if (i == ci) v = Lr[i];
+            col[r] = v;                         // A[r][c] after the previous updates
+        }
+        __syncthreads();
+        T d = col[c];
+        if (!(d > (T)0)) {
+            if (c < nb && !bad) bad = kb + c + 1;
+            d = (T)1e-30;    // keep going without NaNs; info reports the failure
+        }
+        const T sd = sqrt(d);
+        const T inv = (T)1 / sd;
+        const T lrc = (r > c) ? col[r] * inv : (T)0;   // L[r][c]
+        if (q == cq) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                if (i == ci) Lr[i] = (r == c) ? sd : ((r > c) ? lrc : Lr[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int rp = c0 + i;
+            if (rp > c && rp <= r) Lr[i] -= lrc * (col[rp] * inv);
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && bad && info && info[blockIdx.x] == 0) info[blockIdx.x] = bad;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Ls[r][c0 + i] = (c0 + i <= r) ? Lr[i] : (T)0;
     __syncthreads();
-    // right-looking unblocked factorisation
-    for (int c = 0; c < nb; ++c) {
-        if (tid == 0) {
-            T d = L[c][c];
-            if (!(d > (T)0)) {
-                if (info && info[blockIdx.x] == 0) info[blockIdx.x] = kb + c + 1;
-                d = (T)1e-300 > (T)0 ? (T)1e-30 : (T)1e-30;
-            }
-            L[c][c] = sqrt(d);
-        }
-        __syncthreads();
-        const T dc = L[c][c];
-        for (int r = c + 1 + tid; r < nb; r += 256) L[r][c] /= dc;
-        __syncthreads();
-        const int w = nb - c - 1;   // trailing size
-        for (int e = tid; e < w * w; e += 256) {
-            int r = c + 1 + e / w, rp = c + 1 + e % w;
-            if (rp <= r) L[r][rp] -= L[r][c] * L[rp][c];
-        }
-        __syncthreads();
-    }
-    // inverse: X = L^{-1} (forward substitution on the identity, columns in parallel)
-    for (int r = 0; r < nb; ++r) {
-        const T d = L[r][r];
-        for (int c = tid; c <= r; c += 256) X[r][c] /= d;
-        __syncthreads();
-        const int rows = nb - r - 1;
-        for (int e = tid; e < rows * (r + 1); e += 256) {
-            int rr = r + 1 + e / (r + 1), c = e % (r + 1);
-            X[rr][c] -= L[rr][r] * X[r][c];
-        }
-        __syncthreads();
-    }
-    // write back L (lower) and Linv
-    T *W = inv_ws + (long long)blockIdx.x * inv_stride + (long long)(kb / NB) * NB * NB;
     for (int e = tid; e < NB * NB; e += 256) {
-        int r = e / NB, c = e % NB;
-        if (r < nb && c <= r) Ad[(long long)r * m.lda + c] = L[r][c];
-        W[e] = (r < nb && c < nb) ? X[r][c] : (T)0;
+        const int rr = e >> 6, cc = e & 63;
+        if (rr < nb && cc <= rr) Ad[(long long)rr * m.lda + cc] = Ls[rr][cc];
+    }
+    // inverse: X[r][c0..c0+15] in registers (reuse Lr), starts as the identity
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Lr[i] = (r == c0 + i) ? (T)1 : (T)0;
+    for (int k = 0; k < NB; ++k) {
+        if (r == k) {
+            const T dk = (T)1 / Ls[k][k];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                Lr[i] *= dk;
+                xrow[c0 + i] = Lr[i];
+            }
+        }
+        __syncthreads();
+        if (r > k) {
+            const T lrk = Ls[r][k];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) Lr[i] -= lrk * xrow[c0 + i];
+        }
+        __syncthreads();
+    }
+    T *W = inv_ws + (long long)blockIdx.x * inv_stride + (long long)(kb / NB) * NB * NB;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int c = c0 + i;
+        W[r * NB + c] = (r < nb && c < nb) ? Lr[i] : (T)0;
     }
 }
 
@@ -140,7 +174,7 @@ potrf_panel_kernel(const VtMat<T> *__restrict__ mats, int kb, const T *__restric
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
             for (int q = 0; q < 4; ++q) {
-                int r = wm + i * 16 + mfma_row(lane, q), c = wn + j * 16 + mfma_col(lane);
+                int r = wm + i * 16 + acc_row<T>(lane, q), c = wn + j * 16 + mfma_col(lane);
                 if (r0 + r < m.D && c < nb) Ap[(long long)r * m.lda + c] = acc[i][j][q];
             }
 }
@@ -189,7 +223,7 @@ potrf_update_kernel(const VtMat<T> *__restrict__ mats, int kb)
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
             for (int q = 0; q < 4; ++q) {
-                int r = ri + wm + i * 16 + mfma_row(lane, q);
+                int r = ri + wm + i * 16 + acc_row<T>(lane, q);
                 int c = rj + wn + j * 16 + mfma_col(lane);
                 if (r < m.D && c < m.D && c <= r) m.A[(long long)r * m.lda + c] -= acc[i][j][q];
             }
@@ -230,102 +264,189 @@ VT_EXPORT int vt_potrf_f32(const void *mats_dev, int batch, int maxD, float *inv
 }
 
 // ------------------------------------------------------------------------------------
-// Triangular solves with the factor: x = L^-T L^-1 r (in place on x), one workgroup per
-// matrix (batch via the descriptor array).  The factor may be fp32 (preconditioner of the
-// mixed-precision path) while the vector is always fp64.
+// Triangular solves with the factor: x <- L^-T L^-1 x (x fp64; the factor may be fp32,
+// the preconditioner of the mixed-precision path).
+//
+// One workgroup per 64-row block and matrix, one launch per direction.  Workgroup b of
+// the forward sweep folds in L[b, j] y_j for j < b as soon as block j publishes y_j
+// (flag = epoch, release store at agent scope after a fence), then applies the stored
+// diagonal-block inverse and publishes y_b; the backward sweep mirrors it with the
+// bottom block first.  A workgroup only ever waits on lower workgroup IDs, which the
+// in-order dispatcher has already started, so the chain always drains; each wait is
+// additionally bounded (VT_SPIN_LIMIT polls) and sets the error word instead of hanging.
+//
+// sync workspace (ints, zeroed once by the caller): [0] epoch, [1] error, then
+// 2*batch*nblk flags.  The epoch is read on device and bumped by a one-thread kernel at
+// the end of each solve, so the sequence can be captured in a graph and replayed.
 // ------------------------------------------------------------------------------------
+#define VT_SPIN_LIMIT 20000000
+
+__device__ __forceinline__ bool wait_flag(const int *flag, int want, int *err)
+{
+    long it = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != want) {
+        if (++it > VT_SPIN_LIMIT) {
+            atomicExch(err, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
 template <typename T>
-__global__ void __launch_bounds__(1024)
-potrs_kernel(const VtMat<T> *__restrict__ mats, const T *__restrict__ inv_ws, long long inv_stride,
-             double *__restrict__ x, long long x_stride)
+__global__ void __launch_bounds__(256)
+trsv_fwd_kernel(const VtMat<T> *__restrict__ mats, const T *__restrict__ inv_ws,
+                long long inv_stride, double *__restrict__ x, long long x_stride, int *sync,
+                int nblk_max)
 {
-    __shared__ double yk[NB];
-    __shared__ double part[1024];
-    const VtMat<T> m = mats[blockIdx.x];
-    const int D = m.D;
-    const int tid = threadIdx.x;
-    double *xv = x + (long long)blockIdx.x * x_stride;
-    const T *Wb = inv_ws + (long long)blockIdx.x * inv_stride;
-    const int nblk = (D + NB - 1) / NB;
-    // forward: L y = r
-    for (int b = 0; b < nblk; ++b) {
-        const int kb = b * NB, nb = min(NB, D - kb);
-        const T *W = Wb + (long long)b * NB * NB;
-        // y_b = Linv_b * r_b : 16 threads per row
-        {
-            const int r = tid >> 4, l = tid & 15;
-            double s = 0.0;
-            if (r < nb)
-                for (int c = l; c <= r; c += 16) s += (double)W[r * NB + c] * xv[kb + c];
-            part[tid] = s;
-        }
+    __shared__ double yj[NB];
+    __shared__ double part[4][NB];
+    __shared__ double acc[NB];
+    __shared__ int ok;
+    const VtMat<T> m = mats[blockIdx.y];
+    const int b = blockIdx.x;
+    const int kb = b * NB;
+    if (kb >= m.D) return;
+    const int nb = min(NB, m.D - kb);
+    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
+    const int epoch = sync[0] + 1;
+    int *err = sync + 1;
+    int *flags = sync + 2 + (long long)blockIdx.y * nblk_max;
+    double *xv = x + (long long)blockIdx.y * x_stride;
+    if (tid < NB) acc[tid] = tid < nb ? xv[kb + tid] : 0.0;
+    if (tid == 0) ok = 1;
+    __syncthreads();
+    for (int j = 0; j < b; ++j) {
+        if (tid == 0 && !wait_flag(flags + j, epoch, err)) ok = 0;
         __syncthreads();
-        if (tid < nb) {
-            double s = 0.0;
-            for (int l = 0; l < 16; ++l) s += part[tid * 16 + l];
-            yk[tid] = s;
-        }
+        if (!ok) return;
+        if (tid < NB)
+            yj[tid] = __hip_atomic_load(xv + j * NB + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        if (tid < nb) xv[kb + tid] = yk[tid];
-        // r_i -= L[i, kb:kb+nb] . y_b  for i >= kb + nb   (16 lanes per row)
-        for (int i0 = kb + nb + (tid >> 4); i0 < D; i0 += 64) {
-            const int l = tid & 15;
-            const T *Lr = m.A + (long long)i0 * m.lda + kb;
-            double s = 0.0;
-            for (int c = l; c < nb; c += 16) s += (double)Lr[c] * yk[c];
-            for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
-            if (l == 0) xv[i0] -= s;
+        // part[q][r] = sum_{c in quarter q} L[kb+r, j*64+c] y_j[c]
+        double s = 0.0;
+        if (r < nb) {
+            const T *Lr = m.A + (long long)(kb + r) * m.lda + j * NB + q * 16;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) s += (double)Lr[c] * yj[q * 16 + c];
         }
+        part[q][r] = s;
+        __syncthreads();
+        if (tid < NB) acc[tid] -= part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
         __syncthreads();
     }
-    // backward: L^T x = y
-    for (int b = nblk - 1; b >= 0; --b) {
-        const int kb = b * NB, nb = min(NB, D - kb);
-        const T *W = Wb + (long long)b * NB * NB;
-        // x_b = Linv_b^T * y_b
-        {
-            const int r = tid >> 4, l = tid & 15;
-            double s = 0.0;
-            if (r < nb)
-                for (int c = r + l; c < nb; c += 16) s += (double)W[c * NB + r] * xv[kb + c];
-            part[tid] = s;
-        }
-        __syncthreads();
-        if (tid < nb) {
-            double s = 0.0;
-            for (int l = 0; l < 16; ++l) s += part[tid * 16 + l];
-            yk[tid] = s;
-        }
-        __syncthreads();
-        if (tid < nb) xv[kb + tid] = yk[tid];
-        // y_i -= sum_c L[kb+c, i] x_b[c]  for i < kb  (coalesced over i)
-        for (int i = tid; i < kb; i += 1024) {
-            double s = 0.0;
-            for (int c = 0; c < nb; ++c) s += (double)m.A[(long long)(kb + c) * m.lda + i] * yk[c];
-            xv[i] -= s;
-        }
-        __syncthreads();
+    // y_b = Linv_b acc
+    const T *W = inv_ws + (long long)blockIdx.y * inv_stride + (long long)b * NB * NB;
+    double s = 0.0;
+    if (r < nb)
+        for (int c = q * 16; c < q * 16 + 16; ++c) s += (double)W[r * NB + c] * acc[c];
+    part[q][r] = s;
+    __syncthreads();
+    if (tid < nb) {
+        double y = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+        __hip_atomic_store(xv + kb + tid, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-VT_EXPORT int vt_potrs_f64(const void *mats_dev, int batch, const double *inv_ws,
-                           long long inv_stride, double *x, long long x_stride, void *stream)
+template <typename T>
+__global__ void __launch_bounds__(256)
+trsv_bwd_kernel(const VtMat<T> *__restrict__ mats, const T *__restrict__ inv_ws,
+                long long inv_stride, double *__restrict__ x, long long x_stride, int *sync,
+                int nblk_max, int batch)
 {
-    VT_CHECK_ARG(mats_dev && batch >= 1 && inv_ws && x);
-    potrs_kernel<double><<<batch, 1024, 0, (hipStream_t)stream>>>(
-        (const VtMat<double> *)mats_dev, inv_ws, inv_stride, x, x_stride);
+    __shared__ double xj[NB];
+    __shared__ double part[4][NB];
+    __shared__ double acc[NB];
+    __shared__ int ok;
+    const VtMat<T> m = mats[blockIdx.y];
+    const int nblk = (m.D + NB - 1) / NB;
+    if ((int)blockIdx.x >= nblk) return;
+    const int b = nblk - 1 - blockIdx.x;      // bottom block has the lowest ID
+    const int kb = b * NB;
+    const int nb = min(NB, m.D - kb);
+    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
+    const int epoch = sync[0] + 1;
+    int *err = sync + 1;
+    int *flags = sync + 2 + (long long)(batch + blockIdx.y) * nblk_max;
+    double *xv = x + (long long)blockIdx.y * x_stride;
+    if (tid < NB) acc[tid] = tid < nb ? xv[kb + tid] : 0.0;
+    if (tid == 0) ok = 1;
+    __syncthreads();
+    for (int j = nblk - 1; j > b; --j) {
+        const int nbj = min(NB, m.D - j * NB);
+        if (tid == 0 && !wait_flag(flags + j, epoch, err)) ok = 0;
+        __syncthreads();
+        if (!ok) return;
+        if (tid < NB)
+            xj[tid] = tid < nbj ? __hip_atomic_load(xv + j * NB + tid, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        __syncthreads();
+        // part[q][r] = sum_{c in quarter q} L[j*64+c, kb+r] x_j[c]   (coalesced over r)
+        double s = 0.0;
+        if (r < nb)
+            for (int c = q * 16; c < q * 16 + 16 && c < nbj; ++c)
+                s += (double)m.A[(long long)(j * NB + c) * m.lda + kb + r] * xj[c];
+        part[q][r] = s;
+        __syncthreads();
+        if (tid < NB) acc[tid] -= part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+        __syncthreads();
+    }
+    // x_b = Linv_b^T acc
+    const T *W = inv_ws + (long long)blockIdx.y * inv_stride + (long long)b * NB * NB;
+    double s = 0.0;
+    if (r < nb)
+        for (int c = q * 16; c < q * 16 + 16; ++c) s += (double)W[c * NB + r] * acc[c];
+    part[q][r] = s;
+    __syncthreads();
+    if (tid < nb) {
+        double v = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+        __hip_atomic_store(xv + kb + tid, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void epoch_bump_kernel(int *sync) { sync[0] += 1; }
+
+template <typename T>
+static int potrs_run(const void *mats_dev, int batch, int maxD, const T *inv_ws,
+                     long long inv_stride, double *x, long long x_stride, int *sync, hipStream_t s)
+{
+    VT_CHECK_ARG(mats_dev && batch >= 1 && maxD >= 1 && inv_ws && x && sync);
+    VT_CHECK_ARG(batch <= 65535);
+    const int nblk = (maxD + NB - 1) / NB;
+    const VtMat<T> *mats = (const VtMat<T> *)mats_dev;
+    trsv_fwd_kernel<T><<<dim3(nblk, batch), 256, 0, s>>>(mats, inv_ws, inv_stride, x, x_stride,
+                                                        sync, nblk);
+    trsv_bwd_kernel<T><<<dim3(nblk, batch), 256, 0, s>>>(mats, inv_ws, inv_stride, x, x_stride,
+                                                        sync, nblk, batch);
+    epoch_bump_kernel<<<1, 1, 0, s>>>(sync);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
 
-VT_EXPORT int vt_potrs_f32(const void *mats_dev, int batch, const float *inv_ws,
-                           long long inv_stride, double *x, long long x_stride, void *stream)
+// x <- (L L^T)^-1 x for each matrix.  sync: 2 + 2*batch*ceil(maxD/64) ints, zeroed once
+// by the caller and reused across calls; sync[1] != 0 after a solve means a bounded wait
+// expired (results invalid).
+VT_EXPORT int vt_potrs_f64(const void *mats_dev, int batch, int maxD, const double *inv_ws,
+                           long long inv_stride, double *x, long long x_stride, int *sync,
+                           void *stream)
 {
-    VT_CHECK_ARG(mats_dev && batch >= 1 && inv_ws && x);
-    potrs_kernel<float><<<batch, 1024, 0, (hipStream_t)stream>>>(
-        (const VtMat<float> *)mats_dev, inv_ws, inv_stride, x, x_stride);
-    VT_LAUNCH_CHECK();
-    return VT_OK;
+    return potrs_run<double>(mats_dev, batch, maxD, inv_ws, inv_stride, x, x_stride, sync,
+                             (hipStream_t)stream);
+}
+
+VT_EXPORT int vt_potrs_f32(const void *mats_dev, int batch, int maxD, const float *inv_ws,
+                           long long inv_stride, double *x, long long x_stride, int *sync,
+                           void *stream)
+{
+    return potrs_run<float>(mats_dev, batch, maxD, inv_ws, inv_stride, x, x_stride, sync,
+                            (hipStream_t)stream);
 }
 
 // fp64 -> fp32 copy of the lower triangle (with lam added to the diagonal), for the

@@ -44,8 +44,14 @@ static inline unsigned vt_blocks(long long n, int threads) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
-// MFMA accumulator layout for the 16x16 tiles used here (fp64 16x16x4, fp32 16x16x4,
-// bf16/f16 16x16x32): lane l, accumulator register i holds C[4*(l/16) + i][l % 16].
+// MFMA accumulator layouts for the 16x16 tiles used here, measured on gfx950 with a
+// lane-layout probe (vt_mfma_probe):
+//   fp32 16x16x4, bf16/f16 16x16x32: lane l, register i holds C[4*(l/16) + i][l % 16];
+//   fp64 16x16x4:                    lane l, register i holds C[(l/16) + 4*i][l % 16].
 // A operand: lane l supplies row l % 16 (k slice l/16); B operand: column l % 16.
 __device__ __forceinline__ int mfma_row(int lane, int i) { return 4 * (lane >> 4) + i; }
+__device__ __forceinline__ int mfma_row_f64(int lane, int i) { return (lane >> 4) + 4 * i; }
 __device__ __forceinline__ int mfma_col(int lane) { return lane & 15; }
+template <typename T> __device__ __forceinline__ int acc_row(int lane, int i);
+template <> __device__ __forceinline__ int acc_row<double>(int lane, int i) { return mfma_row_f64(lane, i); }
+template <> __device__ __forceinline__ int acc_row<float>(int lane, int i) { return mfma_row(lane, i); }

@@ -86,7 +86,7 @@ gemm_kernel(int M, int N, int K, T alpha, const T *__restrict__ A, long long sam
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int gm = m0 + wm + i * 16 + mfma_row(lane, r);
+                int gm = m0 + wm + i * 16 + acc_row<T>(lane, r);
                 int gn = n0 + wn + j * 16 + mfma_col(lane);
                 if (gm < M && gn < N) {
                     T v = alpha * acc[i][j][r];
@@ -148,7 +148,7 @@ __global__ void mfma_probe_kernel(int kind, const void *A, const void *B, void *
         c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[(lane & 15) * 4 + (lane >> 4)],
                                                  b[(lane >> 4) * 16 + (lane & 15)], c, 0, 0, 0);
         for (int r = 0; r < 4; ++r)
-            ((double *)out)[mfma_row(lane, r) * 16 + mfma_col(lane)] = c[r];
+            ((double *)out)[mfma_row_f64(lane, r) * 16 + mfma_col(lane)] = c[r];
     } else if (kind == 1) {
         const float *a = (const float *)A, *b = (const float *)B;
         v4f c = {0, 0, 0, 0};
@@ -184,6 +184,80 @@ VT_EXPORT int vt_mfma_probe(int kind, const void *A, const void *B, void *out, v
 {
     VT_CHECK_ARG(kind >= 0 && kind <= 3 && A && B && out);
     mfma_probe_kernel<<<1, 64, 0, (hipStream_t)stream>>>(kind, A, B, out);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Skinny transposed product for K-long reductions with a narrow output:
+//   C[m, n] = alpha * sum_k A[k*lda + m] * B[k*ldb + n] + beta * C[m, n],  n < Nc <= 32.
+// Used for r = S^T t and S^T Z (K = neurons, output Ds x Dt).  Pass 1: grid
+// (ceil(M/16), TN_KS); a workgroup owns 16 rows m and one K chunk, thread t reads
+// A[k, m0 + t%16] (128-byte rows) for its k-group and keeps Nc partial sums; the 16
+// k-groups are summed in LDS and written to workspace[ks][m][n].  Pass 2 sums the TN_KS
+// chunks in a fixed order, so results are deterministic (no atomics).
+// workspace: TN_KS * M * Nc doubles.
+// ------------------------------------------------------------------------------------
+#define TN_KS 16
+
+__global__ void __launch_bounds__(256)
+gemm_tn_small_partial(int M, int Nc, int K, const double *__restrict__ A, long long lda,
+                      const double *__restrict__ B, long long ldb, double *__restrict__ ws)
+{
+    __shared__ double red[16][16][33];
+    const int tid = threadIdx.x, ms = tid & 15, kg = tid >> 4;
+    const int m = blockIdx.x * 16 + ms;
+    const int chunk = (K + TN_KS - 1) / TN_KS;
+    const int kbeg = blockIdx.y * chunk, kend = min(K, kbeg + chunk);
+    double acc[32];
+#pragma unroll
+    for (int n = 0; n < 32; ++n) acc[n] = 0.0;
+#pragma unroll 4
+    for (int k = kbeg + kg; k < kend; k += 16) {
+        const double a = (m < M) ? A[(long long)k * lda + m] : 0.0;
+        const double *b = B + (long long)k * ldb;
+#pragma unroll
+        for (int n = 0; n < 32; ++n)
+            if (n < Nc) acc[n] += a * b[n];
+    }
+#pragma unroll
+    for (int n = 0; n < 32; ++n)
+        if (n < Nc) red[kg][ms][n] = acc[n];
+    __syncthreads();
+    for (int e = tid; e < 16 * Nc; e += 256) {
+        const int mm = e / Nc, n = e % Nc;
+        const int gm = blockIdx.x * 16 + mm;
+        if (gm < M) {
+            double s = 0.0;
+            for (int g = 0; g < 16; ++g) s += red[g][mm][n];
+            ws[((long long)blockIdx.y * M + gm) * Nc + n] = s;
+        }
+    }
+}
+
+__global__ void gemm_tn_small_reduce(int M, int Nc, double alpha, const double *__restrict__ ws,
+                                     double beta, double *__restrict__ C, long long ldc)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= M * Nc) return;
+    const int m = e / Nc, n = e % Nc;
+    double s = 0.0;
+    for (int k = 0; k < TN_KS; ++k) s += ws[((long long)k * M + m) * Nc + n];
+    double *p = C + (long long)m * ldc + n;
+    *p = (beta == 0.0) ? alpha * s : alpha * s + beta * *p;
+}
+
+VT_EXPORT int vt_gemm_tn_small(int M, int Nc, int K, double alpha, const double *A, long long lda,
+                               const double *B, long long ldb, double beta, double *C,
+                               long long ldc, double *workspace, void *stream)
+{
+    VT_CHECK_ARG(M >= 0 && Nc >= 1 && Nc <= 32 && K >= 0 && A && B && C && ldc >= Nc && workspace);
+    if (M == 0) return VT_OK;
+    hipStream_t s = (hipStream_t)stream;
+    gemm_tn_small_partial<<<dim3(vt_blocks(M, 16), TN_KS), 256, 0, s>>>(M, Nc, K, A, lda, B, ldb,
+                                                                      workspace);
+    gemm_tn_small_reduce<<<vt_blocks((long long)M * Nc, 256), 256, 0, s>>>(M, Nc, alpha, workspace,
+                                                                         beta, C, ldc);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }

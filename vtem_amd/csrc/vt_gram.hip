@@ -39,7 +39,6 @@ gram_f64_kernel(const double *__restrict__ S, long long lds, const double *__res
     while ((b1 + 1) * (b1 + 2) / 2 <= pid) ++b1;
     while (b1 * (b1 + 1) / 2 > pid) --b1;
     const int b2 = pid - b1 * (b1 + 1) / 2;
-    A += (long long)blockIdx.y * 0;   // batch dimension handled by the batched entry
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int DT2 = Dt * Dt;
@@ -84,7 +83,7 @@ gram_f64_kernel(const double *__restrict__ S, long long lds, const double *__res
         const int pt = col / Dt, ptp = col % Dt;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int rr = wave * 16 + mfma_row(lane, q);
+            const int rr = wave * 16 + mfma_row_f64(lane, q);
             const int ps = b1 * BP + (rr >> 3), psp = b2 * BP + (rr & 7);
             if (ps < Ds && psp < Ds)
                 A[(long long)(ps * Dt + pt) * lda + psp * Dt + ptp] = acc[c][q];
