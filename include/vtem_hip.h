@@ -143,6 +143,32 @@ int vt_neuron_T_batched(double *Tm, double *tv, const double *tk1, const double 
  * vtem/dirichlet.py:62-85) through the push-through identity. */
 int vt_gram_f64(const double *S, long long lds, const double *Tm, int N, int Ds, int Dt,
                 double *A, long long lda, void *stream);
+/* Split T (n = N*Dt^2 doubles) for the low-precision Gram: mode 0 -> hi/lo bf16 arrays;
+ * mode 1 -> fp16(T*sT) in hi, with power-of-two scales [sT, sW, 1/(sT sW)] computed on
+ * device from max|T| and max|S| (S: nS doubles) into `scales` (3 floats).  No reference
+ * counterpart (the reference has only fp32/fp64 paths). */
+int vt_gram_split_T(const double *Tm, long long n, void *hi, void *lo, int mode, const double *S,
+                    long long nS, float *scales, void *stream);
+/* Batched low-precision Gram (mode 0 split-BF16 3-product, mode 1 FP16) on the 16x16x32
+ * MFMAs with fp32 accumulation into fp32 A (lower block triangle).  descs: device array of
+ * {const double *S; const u16 *Th, *Tl; float *A; long long lds, lda; int N, Ds, Dt, pad;
+ * const float *scales;} (72 bytes); all problems share Dt.  Same role as vt_gram_f64. */
+int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int mode, void *stream);
+/* s2[j] = |S_j|^2 (once per neuron set). */
+int vt_row_sumsq(const double *S, long long lds, int N, int Ds, double *s2, void *stream);
+/* Exact tr(A) = sum_j s2_j tr(T_j) in fp64; lam_out = {lam_rel tr(A)/n, tr(A)}.  Replaces
+ * the trace needed for lambda = 1e-6 tr(G)/n without forming G or A. */
+int vt_kron_trace(const double *s2, const double *Tm, int N, int Dt, const int *n_dev,
+                  double lam_rel, double *lam_out, void *stream);
+/* Batched over problems b: N[b], s2 + s2_off[b], Tm + T_off[b]*Dt*Dt, n_dev[b*n_stride];
+ * lam_out[2b], lam_out[2b+1]. */
+int vt_kron_trace_batched(const double *s2, const double *Tm, const int *Nb,
+                          const long long *s2_off, const long long *T_off, int Dt,
+                          const int *n_dev, int n_stride, double lam_rel, double *lam_out,
+                          int batch, void *stream);
+/* Batched FP64 Gram: descs = {const double *S, *Tm; double *A; long long lds, lda; int N,
+ * Ds, Dt, pad;} per problem (56 bytes), common Dt. */
+int vt_gram_f64_batched(const void *descs, int batch, int max_Ds, int Dt, void *stream);
 /* Copy the lower triangle into the upper one. */
 int vt_symmetrize(void *A, int dtype, long long lda, int D, void *stream);
 /* Z[j,:] = T_j Y[j,:] (structured operator step). */
@@ -206,6 +232,37 @@ int vt_reconstruct_spike(double *u, long long ldu, const double *psi, long long 
                          const double *tk1, const double *tk2, const double *c,
                          const double *times, int T, const int *neuron_ind,
                          const double *norm, int Mt, double WM, int n, int P, void *stream);
+
+/* ---------------------------------------------------------------- batched stages ----- */
+/* Many independent blocks per launch (tiles/windows of a stitched video, served
+ * requests); the reference runs them one after another (vtem/vtdm.py:236-302, :630-692).
+ * Neurons are numbered globally across the batch. */
+
+/* Intervals of all blocks: spike_descs = {const double *spikes; const int *counts;
+ * long long ld; int max_rows, neuron_base; double t0, Tw;} per block (48 bytes);
+ * job_of[g] = block of global neuron g; offsets (Ntot+1) global CSR; neuron_ind global. */
+int vt_intervals_batched(const void *spike_descs, const int *job_of, int Ntot, int *workspace,
+                         int *offsets, double *tk1, double *tk2, int *neuron_ind, int capacity,
+                         void *stream);
+/* Grouped fp64 GEMM: descs = {const double *A, *B; double *C; long long sam, sak, sbk,
+ * sbn, ldc; int M, N, K, pad; double alpha, beta;} per problem (96 bytes). */
+int vt_gemm_grouped(const void *descs, int batch, int max_M, int max_N, void *stream);
+/* Batched skinny A^T B: descs = {const double *A, *B; double *C, *ws; long long lda, ldb,
+ * ldc; int M, K;} (64 bytes), ws: 16*M*Nc doubles per problem. */
+int vt_gemm_tn_batched(const void *descs, int batch, int max_M, int Nc, void *stream);
+/* res = r - AX - lam[2*elem_job[i]] X over a concatenated vector. */
+int vt_residual_batched(double *res, const double *r, const double *AX, const double *X,
+                        const double *lam, const int *elem_job, long long n, void *stream);
+/* out[b*out_stride] = sum_{seg[b] <= i < seg[b+1]} x_i^2. */
+int vt_seg_sumsq(const double *x, const long long *seg, int nseg, double *out, int out_stride,
+                 void *stream);
+/* A_b += lam[2b] I; descs = {void *A; long long lda; int D, pad;} (24 bytes). */
+int vt_diag_add_batched(const void *descs, int batch, int dtype, const double *lam, int max_D,
+                        void *stream);
+/* x += y. */
+int vt_add_inplace(double *x, const double *y, long long n, void *stream);
+/* out[b] = offsets[base[b+1]] - offsets[base[b]] (intervals per block). */
+int vt_seg_counts(const int *offsets, const int *base, int nseg, int *out, void *stream);
 
 #ifdef __cplusplus
 }

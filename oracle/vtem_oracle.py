@@ -492,13 +492,19 @@ def neuron_T(alpha: np.ndarray, q: np.ndarray, neuron_ind: np.ndarray, N: int, n
     return Tm, tv
 
 
-def gram_kron(S: np.ndarray, Tm: np.ndarray) -> np.ndarray:
-    """A = sum_j (S_j S_j^T) kron T_j  (D x D, index ps*Dt + pt)."""
+def gram_kron(S: np.ndarray, Tm: np.ndarray, chunk: int = 16) -> np.ndarray:
+    """A = sum_j (S_j S_j^T) kron T_j  (D x D, index ps*Dt + pt), built `chunk` spatial
+    rows at a time so the N x (chunk*Ds) product stays small."""
     N, Ds = S.shape
     Dt = Tm.shape[1]
-    P = (S[:, :, None] * S[:, None, :]).reshape(N, Ds * Ds)
-    A4 = (P.T @ Tm.reshape(N, Dt * Dt)).reshape(Ds, Ds, Dt, Dt)
-    return A4.transpose(0, 2, 1, 3).reshape(Ds * Dt, Ds * Dt)
+    T2 = Tm.reshape(N, Dt * Dt)
+    A = np.empty((Ds * Dt, Ds * Dt))
+    for p0 in range(0, Ds, chunk):
+        p1 = min(Ds, p0 + chunk)
+        P = (S[:, p0:p1, None] * S[:, None, :]).reshape(N, (p1 - p0) * Ds)
+        A4 = (P.T @ T2).reshape(p1 - p0, Ds, Dt, Dt)
+        A[p0 * Dt:p1 * Dt] = A4.transpose(0, 2, 1, 3).reshape((p1 - p0) * Dt, Ds * Dt)
+    return A
 
 
 def rhs_kron(S: np.ndarray, tv: np.ndarray) -> np.ndarray:

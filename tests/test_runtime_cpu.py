@@ -1,0 +1,70 @@
+"""Host logic of the serving runtime (no GPU): arrival process, footprints, precision
+policy, configuration laws."""
+
+import numpy as np
+import pytest
+
+from vtem_amd import synthetic as syn
+
+
+def test_poisson_arrivals_rate_and_order():
+    from vtem_amd.serve import poisson_arrivals
+    a = poisson_arrivals(20000, 50.0, np.random.default_rng(0))
+    assert a[0] == 0 and np.all(np.diff(a) >= 0)
+    assert abs((len(a) - 1) / a[-1] - 50.0) < 1.5
+    assert np.all(poisson_arrivals(5, float("inf"), np.random.default_rng(0)) == 0)
+
+
+class _G:
+    def __init__(self, Ds, Dt):
+        self.Ds, self.Dt = Ds, Dt
+
+
+class _Dec:
+    def __init__(self, Ds, Dt):
+        self.geom = _G(Ds, Dt)
+
+
+class _Job:
+    def __init__(self, Ds, Dt):
+        self.decoder = _Dec(Ds, Dt)
+
+
+def test_precision_policy_routes_failing_class_to_fp64():
+    from vtem_amd.serve import PrecisionPolicy
+    p = PrecisionPolicy("bf16x3", min_fail=2, rate=0.5)
+    bad, good = _Job(81, 5), _Job(289, 17)
+    assert p.choose(bad) == "bf16x3"
+    p.record(bad, "bf16x3", True)
+    assert p.choose(bad) == "bf16x3"          # one failure is not enough
+    p.record(bad, "bf16x3", True)
+    assert p.choose(bad) == "fp64"
+    for _ in range(5):
+        p.record(good, "bf16x3", False)
+    p.record(good, "bf16x3", True)
+    p.record(good, "bf16x3", True)
+    assert p.choose(good) == "bf16x3"         # 2 of 7 failures: keep low precision
+    p.record(bad, "fp64", True)               # fp64 outcomes do not count
+    assert p.stats[(81, 5)] == [2, 2]
+
+
+def test_serve_block_law():
+    for side in syn.SERVE_SIDES:
+        for dur in syn.SERVE_DURATIONS:
+            b = syn.serve_block(side, dur, 7)
+            L, Lt = round(side / 8), round(4 * dur)
+            assert (b.Lx, b.Lt) == (L, Lt)
+            D = (2 * L + 1) ** 2 * (2 * Lt + 1)
+            assert b.D == D and b.bank.num_neurons == round(2 * D / 9)
+            n_dog = int((b.bank.kind == 1).sum())
+            assert n_dog == round(b.bank.num_neurons / 8)
+            assert b.video.frames.shape == (int(round(dur * 100)), side, side)
+
+
+def test_dyadic_bank_counts():
+    b1 = syn.bank_dyadic(64.0, 64.0)
+    assert b1.num_neurons == 2048 and int((b1.kind == 0).sum()) == 1792
+    b2 = syn.bank_dyadic(128.0, 128.0)
+    assert b2.num_neurons == 8192 and int((b2.kind == 0).sum()) == 7168
+    b0 = syn.bank_config0()
+    assert b0.num_neurons == 256 and set(np.round(b0.h_l / (np.pi / 8)).astype(int)) == set(range(8))

@@ -55,12 +55,27 @@ SIGNATURES = {
     "vt_rnn_spike": [P, P, L, P, I, D, I, P, P, P],
     # gram / solve
     "vt_gram_f64": [P, L, P, I, I, I, P, L, P],
+    "vt_gram_f64_batched": [P, I, I, I, P],
+    "vt_gram_split_T": [P, L, P, P, I, P, L, P, P],
+    "vt_gram_lp": [P, I, I, I, I, P],
+    "vt_kron_trace": [P, P, I, I, P, D, P, P],
+    "vt_kron_trace_batched": [P, P, P, P, P, I, P, I, D, P, I, P],
+    "vt_row_sumsq": [P, L, I, I, P, P],
     "vt_symmetrize": [P, I, L, I, P],
     "vt_potrf_f64": [P, I, I, P, L, P, P],
     "vt_potrf_f32": [P, I, I, P, L, P, P],
     "vt_potrs_f64": [P, I, I, P, L, P, L, P, P],
     "vt_potrs_f32": [P, I, I, P, L, P, L, P, P],
     "vt_lower_to_f32": [P, L, P, L, I, P, P],
+    # batched
+    "vt_intervals_batched": [P, P, I, P, P, P, P, P, I, P],
+    "vt_gemm_grouped": [P, I, I, I, P],
+    "vt_gemm_tn_batched": [P, I, I, I, P],
+    "vt_residual_batched": [P, P, P, P, P, P, L, P],
+    "vt_seg_sumsq": [P, P, I, P, I, P],
+    "vt_diag_add_batched": [P, I, I, P, I, P],
+    "vt_add_inplace": [P, P, L, P],
+    "vt_seg_counts": [P, P, I, P, P],
 }
 
 

@@ -1,0 +1,384 @@
+"""Batched decode of many independent blocks (tiles / windows / served requests).
+
+The reference decodes stitched tiles and windows one after another
+(vtem/vtdm.py:236-302 spatial tiles, :630-692 temporal windows), each with its own
+VTDM_prep / compute_G / solve / reconstruct.  Here a group of blocks that share the
+temporal order Dt runs every stage as one launch (or two) with per-block descriptors:
+
+  intervals (global neuron numbering) -> q -> T_j, t_j -> r_b = S_b^T t_b -> lam_b
+  -> low-precision Gram A_b (split-BF16 x3 or FP16, FP32 accumulation)
+  -> A_b + lam_b I -> batched FP32 Cholesky -> batched triangular solves
+  -> FP64 iterative refinement through each block's exact Kronecker operator
+  -> frames_b = Theta_b X_b^T B_s,b^T.
+
+Blocks keep their own sizes (neurons, spatial order, spike counts, duration); matrices
+of different D share one launch sequence (a block that has finished its columns skips
+the step).  Each block's result is the same as decoding it alone: no reduction order
+depends on which other blocks are in the batch.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Any, Dict, List, Optional
+
+import numpy as np
+import torch
+
+from ._lib import lib, ptr, stream_ptr
+from .block import (F64, I32, Decoder, DecodeResult, SpikeTrains, gram_descs, mat_descs)
+
+SPIKE_DTYPE = np.dtype([("spikes", np.uint64), ("counts", np.uint64), ("ld", np.int64),
+                        ("max_rows", np.int32), ("neuron_base", np.int32), ("t0", np.float64),
+                        ("Tw", np.float64)])
+TJ_DTYPE = np.dtype([("neuron_base", np.int32), ("num_neurons", np.int32), ("t_base", np.int64),
+                     ("Lt", np.int32), ("pad0", np.int32), ("W", np.float64), ("T_base", np.int32),
+                     ("pad1", np.int32)])
+GEMM_DTYPE = np.dtype([("A", np.uint64), ("B", np.uint64), ("C", np.uint64), ("sam", np.int64),
+                       ("sak", np.int64), ("sbk", np.int64), ("sbn", np.int64), ("ldc", np.int64),
+                       ("M", np.int32), ("N", np.int32), ("K", np.int32), ("pad", np.int32),
+                       ("alpha", np.float64), ("beta", np.float64)])
+TN_DTYPE = np.dtype([("A", np.uint64), ("B", np.uint64), ("C", np.uint64), ("ws", np.uint64),
+                     ("lda", np.int64), ("ldb", np.int64), ("ldc", np.int64), ("M", np.int32),
+                     ("K", np.int32)])
+DIAG_DTYPE = np.dtype([("A", np.uint64), ("lda", np.int64), ("D", np.int32), ("pad", np.int32)])
+GRAM_LP_DTYPE = np.dtype([("S", np.uint64), ("Th", np.uint64), ("Tl", np.uint64), ("A", np.uint64),
+                          ("lds", np.int64), ("lda", np.int64), ("N", np.int32), ("Ds", np.int32),
+                          ("Dt", np.int32), ("pad", np.int32), ("scales", np.uint64)])
+GRAM64_DTYPE = np.dtype([("S", np.uint64), ("Tm", np.uint64), ("A", np.uint64), ("lds", np.int64),
+                         ("lda", np.int64), ("N", np.int32), ("Ds", np.int32), ("Dt", np.int32),
+                         ("pad", np.int32)])
+
+assert SPIKE_DTYPE.itemsize == 48 and TJ_DTYPE.itemsize == 40 and GEMM_DTYPE.itemsize == 96
+assert TN_DTYPE.itemsize == 64 and DIAG_DTYPE.itemsize == 24
+assert GRAM_LP_DTYPE.itemsize == 72 and GRAM64_DTYPE.itemsize == 56
+
+
+def _to_dev(arr: np.ndarray, device) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).copy()).to(device)
+
+
+@dataclasses.dataclass
+class BlockJob:
+    """One block to decode: a prepared Decoder (geometry, S, IAF parameters) and its
+    resident spike trains; window starts at t0 and lasts the decoder's duration."""
+    decoder: Decoder
+    spikes: SpikeTrains
+    t0: float = 0.0
+    tag: Any = None
+
+
+@dataclasses.dataclass
+class BatchHandle:
+    jobs: List[BlockJob]
+    results: List[Optional[DecodeResult]]
+    keep: list
+    checks: list
+    precisions: list = dataclasses.field(default_factory=list)
+
+
+PRECISIONS = ("fp64", "mixed", "bf16x3", "fp16")
+
+
+class BatchDecoder:
+    """Decodes lists of BlockJobs, grouped by (temporal order Dt, precision).
+
+    precision per block: "fp64" (FP64 Gram + FP64 Cholesky), "mixed" (FP64 Gram, FP32
+    Cholesky, FP64 refinement), "bf16x3" / "fp16" (low-precision Gram, FP32 Cholesky,
+    FP64 refinement).  Low-precision blocks that fail the guard are redone together as
+    one batched FP64 group."""
+
+    def __init__(self, device="cuda", precision: str = "bf16x3", lam_rel: float = 1e-6,
+                 refine_iters: int = 4, tol: float = 1e-10, verify: bool = True):
+        if precision not in PRECISIONS:
+            raise ValueError(f"batched precision must be one of {PRECISIONS}")
+        self.device = device
+        self.precision = precision
+        self.lam_rel = lam_rel
+        self.refine_iters = refine_iters
+        self.tol = tol
+        self.verify = verify
+        self.last_stats: Dict[str, Any] = {}
+        self.timers: Optional[Dict[str, list]] = None     # stage -> [(ev0, ev1, work)]
+
+    def instrument(self, on: bool = True):
+        """Record HIP events around the main stages (on the launch stream)."""
+        self.timers = {} if on else None
+
+    def _tic(self):
+        if self.timers is None:
+            return None
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    def _toc(self, name, e0, work):
+        if e0 is None:
+            return
+        e1 = torch.cuda.Event(enable_timing=True)
+        e1.record()
+        self.timers.setdefault(name, []).append((e0, e1, work))
+
+    def decode(self, jobs: List[BlockJob], precisions=None) -> List[DecodeResult]:
+        """Launch and finalize (synchronous on the current stream's results)."""
+        return self.finalize(self.launch(jobs, precisions))
+
+    def launch(self, jobs: List[BlockJob], precisions=None) -> "BatchHandle":
+        """Enqueue every stage of every group on the current stream; no host sync."""
+        if precisions is None:
+            precisions = [self.precision] * len(jobs)
+        h = BatchHandle(jobs, [None] * len(jobs), [], [])
+        h.precisions = list(precisions)
+        groups: Dict[tuple, List[int]] = {}
+        for i, (jb, pr) in enumerate(zip(jobs, precisions)):
+            if pr not in PRECISIONS:
+                raise ValueError(f"unknown precision {pr}")
+            groups.setdefault((jb.decoder.geom.Dt, pr), []).append(i)
+        for (Dt, pr), idx in sorted(groups.items()):
+            res, keep, checks = self._decode_group([jobs[i] for i in idx], Dt, pr)
+            for i, r in zip(idx, res):
+                h.results[i] = r
+            h.keep.append(keep)
+            h.checks.append((idx, checks, pr))
+        return h
+
+    def finalize(self, h: "BatchHandle") -> List[DecodeResult]:
+        """Guard: blocks whose low-precision factor failed or whose refinement did not
+        reach `tol` relative residual are redone as one batched FP64 group (flagged)."""
+        redo = []
+        failed_fp64 = 0
+        if self.verify:
+            for idx, (nrm, info), pr in h.checks:
+                hn = nrm.cpu().numpy()
+                inf = info.cpu().numpy()
+                for k, i in enumerate(idx):
+                    if pr == "fp64":
+                        failed_fp64 += int(inf[k] != 0)
+                        continue
+                    ok = inf[k] == 0 and np.isfinite(hn[k]).all() and \
+                        math.sqrt(hn[k, -1]) <= self.tol * math.sqrt(hn[k, 0])
+                    if not ok:
+                        redo.append(i)
+        if redo:
+            h2 = self.launch([h.jobs[i] for i in redo], ["fp64"] * len(redo))
+            for i, rr in zip(redo, h2.results):
+                rr.fallback = True
+                h.results[i] = rr
+            h.keep.append(h2.keep)
+        self.last_stats = {"groups": len(h.checks), "fallbacks": len(redo),
+                           "fp64_not_pd": failed_fp64, "fallback_index": redo}
+        torch.cuda.current_stream().synchronize()
+        h.keep = []
+        return h.results
+
+    # ----------------------------------------------------------------------------------
+    def _decode_group(self, jobs: List[BlockJob], Dt: int, prec: str):
+        dev = self.device
+        s = stream_ptr()
+        B = len(jobs)
+        decs = [j.decoder for j in jobs]
+        N = np.array([d.N for d in decs], np.int64)
+        Ds = np.array([d.geom.Ds for d in decs], np.int64)
+        D = Ds * Dt
+        nbase = np.concatenate([[0], np.cumsum(N)])
+        dbase = np.concatenate([[0], np.cumsum(D)])
+        Ntot, Dtot = int(nbase[-1]), int(dbase[-1])
+        # per-neuron parameters in global numbering
+        kappa = torch.cat([d.kappa for d in decs])
+        delta = torch.cat([d.delta for d in decs])
+        bias = torch.cat([d.bias for d in decs])
+        norm = torch.cat([d.norm for d in decs])
+        s2 = torch.cat([d.s2 for d in decs])
+        job_of = torch.from_numpy(np.repeat(np.arange(B, dtype=np.int32), N)).to(dev)
+        # ---- intervals -------------------------------------------------------------------
+        sd = np.zeros(B, SPIKE_DTYPE)
+        cap = 0
+        for b, jb in enumerate(jobs):
+            st = jb.spikes
+            sd[b] = (st.spikes.data_ptr(), st.counts.data_ptr(), st.spikes.stride(0),
+                     st.spikes.shape[0], nbase[b], jb.t0, jb.decoder.geom.duration)
+            cap += st.spikes.shape[0] * int(N[b])
+        sd_d = _to_dev(sd, dev)
+        ws = torch.empty(2 * Ntot, dtype=I32, device=dev)
+        offsets = torch.empty(Ntot + 1, dtype=I32, device=dev)
+        tk1 = torch.empty(cap, dtype=F64, device=dev)
+        tk2 = torch.empty(cap, dtype=F64, device=dev)
+        nid = torch.empty(cap, dtype=I32, device=dev)
+        lib.vt_intervals_batched(ptr(sd_d), ptr(job_of), Ntot, ptr(ws), ptr(offsets), ptr(tk1),
+                                 ptr(tk2), ptr(nid), cap, s)
+        base_d = torch.from_numpy(nbase.astype(np.int32)).to(dev)
+        n_b = torch.empty(B, dtype=I32, device=dev)
+        lib.vt_seg_counts(ptr(offsets), ptr(base_d), B, ptr(n_b), s)
+        # ---- q, T_j, t_j -----------------------------------------------------------------
+        q = torch.empty(cap, dtype=F64, device=dev)
+        lib.vt_q(ptr(q), ptr(tk1), ptr(tk2), ptr(nid), ptr(kappa), ptr(delta), ptr(bias), ptr(norm),
+                 ptr(offsets[Ntot:]), cap, s)
+        Tm = torch.empty((Ntot, Dt, Dt), dtype=F64, device=dev)
+        tv = torch.empty((Ntot, Dt), dtype=F64, device=dev)
+        tj = np.zeros(B, TJ_DTYPE)
+        for b, d in enumerate(decs):
+            tj[b] = (nbase[b], N[b], 0, d.geom.Lt, 0, d.geom.W, nbase[b], 0)
+        tj_d = _to_dev(tj, dev)
+        lib.vt_neuron_T_batched(ptr(Tm), ptr(tv), ptr(tk1), ptr(tk2), ptr(q), ptr(norm),
+                                ptr(offsets), ptr(tj_d), B, int(N.max()), s)
+        # ---- rhs, lambda -----------------------------------------------------------------
+        r = torch.empty(Dtot, dtype=F64, device=dev)
+        tn_ws = torch.empty(int(16 * Dt * Ds.sum()), dtype=F64, device=dev)
+        ws_base = np.concatenate([[0], np.cumsum(16 * Dt * Ds)])
+        tn_r = np.zeros(B, TN_DTYPE)
+        for b, d in enumerate(decs):
+            tn_r[b] = (d.S.data_ptr(), tv.data_ptr() + 8 * int(nbase[b]) * Dt,
+                       r.data_ptr() + 8 * int(dbase[b]), tn_ws.data_ptr() + 8 * int(ws_base[b]),
+                       d.S.stride(0), Dt, Dt, Ds[b], N[b])
+        tn_r_d = _to_dev(tn_r, dev)
+        lib.vt_gemm_tn_batched(ptr(tn_r_d), B, int(Ds.max()), Dt, s)
+        lam = torch.empty(2 * B, dtype=F64, device=dev)
+        s2off = torch.from_numpy(nbase[:-1].astype(np.int64)).to(dev)
+        N_d = torch.from_numpy(N.astype(np.int32)).to(dev)
+        lib.vt_kron_trace_batched(ptr(s2), ptr(Tm), ptr(N_d), ptr(s2off), ptr(s2off), Dt, ptr(n_b), 1,
+                                  self.lam_rel, ptr(lam), B, s)
+        # ---- Gram -------------------------------------------------------------------------
+        maxD = int(D.max())
+        nblk = (maxD + 63) // 64
+        gflops = float(np.sum(N * D * (D + 1)))
+        low = prec in ("bf16x3", "fp16")
+        f64_factor = prec == "fp64"
+        fdt = F64 if f64_factor else torch.float32
+        Apool = torch.empty(int((D * D).sum()), dtype=fdt, device=dev)
+        abase = np.concatenate([[0], np.cumsum(D * D)])
+        As = [Apool[int(abase[b]):int(abase[b + 1])].view(int(D[b]), int(D[b])) for b in range(B)]
+        extra_keep = []
+        if low:
+            mode = 0 if prec == "bf16x3" else 1
+            nT = Ntot * Dt * Dt
+            Th = torch.empty(nT, dtype=torch.int16, device=dev)
+            Tl = torch.empty(nT if mode == 0 else 1, dtype=torch.int16, device=dev)
+            scales = torch.ones(3, dtype=torch.float32, device=dev)
+            if mode == 0:
+                lib.vt_gram_split_T(ptr(Tm), nT, ptr(Th), ptr(Tl), 0, 0, 0, 0, s)
+            else:
+                S_all = torch.cat([d.S.reshape(-1) for d in decs])
+                extra_keep.append(S_all)
+                lib.vt_gram_split_T(ptr(Tm), nT, ptr(Th), ptr(Tl), 1, ptr(S_all), S_all.numel(),
+                                    ptr(scales), s)
+            gd = np.zeros(B, GRAM_LP_DTYPE)
+            for b, d in enumerate(decs):
+                off = int(nbase[b]) * Dt * Dt * 2
+                gd[b] = (d.S.data_ptr(), Th.data_ptr() + off, Tl.data_ptr() + (off if mode == 0 else 0),
+                         As[b].data_ptr(), d.S.stride(0), D[b], N[b], Ds[b], Dt, 0, scales.data_ptr())
+            gd_d = _to_dev(gd, dev)
+            e0 = self._tic()
+            lib.vt_gram_lp(ptr(gd_d), B, int(Ds.max()), Dt, mode, s)
+            self._toc("gram_lp", e0, gflops)
+            extra_keep += [Th, Tl, scales, gd_d]
+        else:
+            # FP64 Gram (into the factor buffer for fp64, into a staging buffer for mixed)
+            G64 = Apool if f64_factor else torch.empty(int((D * D).sum()), dtype=F64, device=dev)
+            Gs = [G64[int(abase[b]):int(abase[b + 1])].view(int(D[b]), int(D[b])) for b in range(B)]
+            g64 = np.zeros(B, GRAM64_DTYPE)
+            for b, d in enumerate(decs):
+                g64[b] = (d.S.data_ptr(), Tm.data_ptr() + 8 * int(nbase[b]) * Dt * Dt,
+                          Gs[b].data_ptr(), d.S.stride(0), D[b], N[b], Ds[b], Dt, 0)
+            g64_d = _to_dev(g64, dev)
+            e0 = self._tic()
+            lib.vt_gram_f64_batched(ptr(g64_d), B, int(Ds.max()), Dt, s)
+            self._toc("gram_f64", e0, gflops)
+            extra_keep += [G64, g64_d]
+            if prec == "mixed":
+                for b in range(B):
+                    lib.vt_lower_to_f32(ptr(As[b]), int(D[b]), ptr(Gs[b]), int(D[b]), int(D[b]),
+                                        ptr(lam[2 * b:2 * b + 1]), s)
+        if prec != "mixed":
+            dd = np.zeros(B, DIAG_DTYPE)
+            for b in range(B):
+                dd[b] = (As[b].data_ptr(), D[b], D[b], 0)
+            dd_d = _to_dev(dd, dev)
+            lib.vt_diag_add_batched(ptr(dd_d), B, 0 if f64_factor else 1, ptr(lam), maxD, s)
+            extra_keep.append(dd_d)
+        # ---- Cholesky + solves -------------------------------------------------------------
+        inv = torch.empty(B * nblk * 4096, dtype=fdt, device=dev)
+        info = torch.zeros(B, dtype=I32, device=dev)
+        mats = mat_descs([(As[b].data_ptr(), int(D[b]), int(D[b])) for b in range(B)], dev)
+        potrf = lib.vt_potrf_f64 if f64_factor else lib.vt_potrf_f32
+        potrs = lib.vt_potrs_f64 if f64_factor else lib.vt_potrs_f32
+        e0 = self._tic()
+        potrf(ptr(mats), B, maxD, ptr(inv), nblk * 4096, ptr(info), s)
+        self._toc("potrf_f64" if f64_factor else "potrf_f32", e0,
+                  float(np.sum(D.astype(np.float64) ** 3 / 3)))
+        sync = torch.zeros(2 + 2 * B * nblk, dtype=I32, device=dev)
+        # vectors in a padded (B x maxD) layout for the solves, concatenated elsewhere
+        Xp = torch.zeros((B, maxD), dtype=F64, device=dev)
+        seg = torch.from_numpy(dbase.astype(np.int64)).to(dev)
+        elem_job = torch.from_numpy(np.repeat(np.arange(B, dtype=np.int32), D)).to(dev)
+        pad_idx = torch.from_numpy(np.concatenate([b * maxD + np.arange(D[b]) for b in range(B)])).to(dev)
+
+        def to_padded(vec, out):
+            out.view(-1).index_copy_(0, pad_idx, vec)
+
+        def from_padded(padded):
+            return padded.view(-1).index_select(0, pad_idx)
+
+        e_ref = self._tic()
+        to_padded(r, Xp)
+        potrs(ptr(mats), B, maxD, ptr(inv), nblk * 4096, ptr(Xp), maxD, ptr(sync), s)
+        X = from_padded(Xp)
+        iters = 0 if f64_factor else self.refine_iters
+        # ---- refinement through the exact operators ----------------------------------------
+        Y = torch.empty((Ntot, Dt), dtype=F64, device=dev)
+        Z = torch.empty_like(Y)
+        AX = torch.empty(Dtot, dtype=F64, device=dev)
+        res = torch.empty(Dtot, dtype=F64, device=dev)
+        nrm = torch.zeros((B, iters + 2), dtype=F64, device=dev)
+        lib.vt_seg_sumsq(ptr(r), ptr(seg), B, ptr(nrm), iters + 2, s)
+        g1 = np.zeros(B, GEMM_DTYPE)
+        tn_a = np.zeros(B, TN_DTYPE)
+        for b, d in enumerate(decs):
+            g1[b] = (d.S.data_ptr(), X.data_ptr() + 8 * int(dbase[b]), Y.data_ptr() + 8 * int(nbase[b]) * Dt,
+                     d.S.stride(0), 1, Dt, 1, Dt, N[b], Dt, Ds[b], 0, 1.0, 0.0)
+            tn_a[b] = (d.S.data_ptr(), Z.data_ptr() + 8 * int(nbase[b]) * Dt,
+                       AX.data_ptr() + 8 * int(dbase[b]), tn_ws.data_ptr() + 8 * int(ws_base[b]),
+                       d.S.stride(0), Dt, Dt, Ds[b], N[b])
+        g1_d, tn_a_d = _to_dev(g1, dev), _to_dev(tn_a, dev)
+        resp = torch.zeros((B, maxD), dtype=F64, device=dev)
+        for it in range(iters + 1):
+            lib.vt_gemm_grouped(ptr(g1_d), B, int(N.max()), Dt, s)
+            lib.vt_tj_apply(ptr(Z), ptr(Tm), ptr(Y), Ntot, Dt, s)
+            lib.vt_gemm_tn_batched(ptr(tn_a_d), B, int(Ds.max()), Dt, s)
+            lib.vt_residual_batched(ptr(res), ptr(r), ptr(AX), ptr(X), ptr(lam), ptr(elem_job), Dtot, s)
+            nv = nrm[:, it + 1:]
+            lib.vt_seg_sumsq(ptr(res), ptr(seg), B, ptr(nv), iters + 2, s)
+            if it == iters:
+                break
+            to_padded(res, resp)
+            potrs(ptr(mats), B, maxD, ptr(inv), nblk * 4096, ptr(resp), maxD, ptr(sync), s)
+            lib.vt_add_inplace(ptr(X), ptr(from_padded(resp)), Dtot, s)
+        self._toc("solve_refine", e_ref, float(np.sum(D.astype(np.float64) ** 2)) * 2 * (iters + 1))
+        # ---- synthesis ---------------------------------------------------------------------
+        T_out = np.array([d.theta_out.shape[0] for d in decs], np.int64)
+        P = np.array([d.geom.P for d in decs], np.int64)
+        Y2 = torch.empty(int((T_out * Ds).sum()), dtype=F64, device=dev)
+        y2base = np.concatenate([[0], np.cumsum(T_out * Ds)])
+        frames = [torch.empty((int(T_out[b]), int(P[b])), dtype=F64, device=dev) for b in range(B)]
+        gs1 = np.zeros(B, GEMM_DTYPE)
+        gs2 = np.zeros(B, GEMM_DTYPE)
+        for b, d in enumerate(decs):
+            y2p = Y2.data_ptr() + 8 * int(y2base[b])
+            gs1[b] = (d.theta_out.data_ptr(), X.data_ptr() + 8 * int(dbase[b]), y2p, Dt, 1, 1, Dt, Ds[b],
+                      T_out[b], Ds[b], Dt, 0, 1.0, 0.0)
+            gs2[b] = (y2p, d.Bs.data_ptr(), frames[b].data_ptr(), Ds[b], 1, 1, Ds[b], P[b],
+                      T_out[b], P[b], Ds[b], 0, 1.0, 0.0)
+        gs1_d, gs2_d = _to_dev(gs1, dev), _to_dev(gs2, dev)
+        lib.vt_gemm_grouped(ptr(gs1_d), B, int(T_out.max()), int(Ds.max()), s)
+        lib.vt_gemm_grouped(ptr(gs2_d), B, int(T_out.max()), int(P.max()), s)
+        # every buffer referenced by enqueued work stays alive with the handle
+        keep = (N_d, sd_d, ws, offsets, tk1, tk2, nid, base_d, q, Tm, tv, tn_ws, tn_r_d, s2off,
+                extra_keep, Apool, inv, mats, sync, Xp, seg, elem_job, pad_idx,
+                      Y, Z, AX, res, g1_d, tn_a_d, resp, Y2, gs1_d, gs2_d, kappa, delta, bias, norm,
+                      s2, job_of, r)
+        results = []
+        for b in range(B):
+            Xb = X[int(dbase[b]):int(dbase[b + 1])]
+            results.append(DecodeResult(frames[b], Xb, lam[2 * b:2 * b + 2], n_b[b:b + 1],
+                                        info[b:b + 1], nrm[b], False))
+        return results, keep, (nrm, info)

@@ -43,6 +43,20 @@ def mat_descs(items, device) -> torch.Tensor:
     return torch.from_numpy(arr.view(np.uint8).copy()).to(device)
 
 
+GRAM_DTYPE = np.dtype([("S", np.uint64), ("Th", np.uint64), ("Tl", np.uint64), ("A", np.uint64),
+                       ("lds", np.int64), ("lda", np.int64), ("N", np.int32), ("Ds", np.int32),
+                       ("Dt", np.int32), ("pad", np.int32), ("scales", np.uint64)])
+
+
+def gram_descs(items, device) -> torch.Tensor:
+    """Device array of low-precision Gram problem descriptors (72 bytes each)."""
+    arr = np.zeros(len(items), GRAM_DTYPE)
+    for i, (S, Th, Tl, A, N, Ds, Dt, scales) in enumerate(items):
+        arr[i] = (S.data_ptr(), Th.data_ptr(), Tl.data_ptr(), A.data_ptr(), S.stride(0),
+                  A.stride(0), N, Ds, Dt, 0, scales.data_ptr())
+    return torch.from_numpy(arr.view(np.uint8).copy()).to(device)
+
+
 def dev(a, dtype=F64, device="cuda"):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(device)
 
@@ -182,6 +196,7 @@ class DecodeResult:
     n: torch.Tensor               # number of intervals (device int)
     info: Optional[torch.Tensor] = None
     resid: Optional[torch.Tensor] = None
+    fallback: bool = False
 
 
 class Decoder:
@@ -205,6 +220,8 @@ class Decoder:
         self.Bs = dev(geom.spatial(), F64, device)
         self.theta_out = dev(geom.temporal(geom.frame_times()), F64, device)
         self._tn_ws = torch.empty(16 * geom.Ds * geom.Dt, dtype=F64, device=device)
+        self.s2 = torch.empty(N, dtype=F64, device=device)
+        lib.vt_row_sumsq(ptr(self.S), self.S.stride(0), N, geom.Ds, ptr(self.s2), stream_ptr())
 
     # -- spikes -> intervals -> moments ------------------------------------------------
     def intervals(self, st: SpikeTrains, t0: float = 0.0, Tw: float = None):
@@ -283,47 +300,90 @@ class Decoder:
             ws = torch.zeros(need, dtype=I32, device=self.device)
             self._sync = ws
         return ws
-    def solve_tikhonov(self, A, r, n_dev, lam_rel=1e-6, precision="fp64", refine_iters=4):
-        """(A + lam I) X = r with lam = lam_rel tr(A)/n.
+    def lam_exact(self, Tm, n_dev, lam_rel):
+        """[lam, tr(A)] with tr(A) = sum_j |S_j|^2 tr(T_j) in FP64 (no Gram needed)."""
+        g = self.geom
+        lam = torch.empty(2, dtype=F64, device=self.device)
+        lib.vt_kron_trace(ptr(self.s2), ptr(Tm), self.N, g.Dt, ptr(n_dev), lam_rel, ptr(lam),
+                          stream_ptr())
+        return lam
 
-        precision "fp64": FP64 Cholesky.  "mixed": FP32 Cholesky of the Gram as
-        preconditioner plus FP64 iterative refinement whose residual uses the exact
-        FP64 operator through the Kronecker structure (needs self._Tm).
+    def gram_lp(self, Tm, mode: int, out=None):
+        """Low-precision Gram into FP32 (mode 0 split-BF16 x3, mode 1 FP16)."""
+        g = self.geom
+        D, N = g.D, self.N
+        n = N * g.Dt * g.Dt
+        hi = torch.empty(n, dtype=torch.int16, device=self.device)
+        lo = torch.empty(n if mode == 0 else 1, dtype=torch.int16, device=self.device)
+        scales = torch.ones(3, dtype=torch.float32, device=self.device)
+        lib.vt_gram_split_T(ptr(Tm), n, ptr(hi), ptr(lo), mode, ptr(self.S), self.S.numel(),
+                            ptr(scales), stream_ptr())
+        A = torch.empty((D, D), dtype=torch.float32, device=self.device) if out is None else out
+        desc = gram_descs([(self.S, hi, lo, A, N, g.Ds, g.Dt, scales)], self.device)
+        lib.vt_gram_lp(ptr(desc), 1, g.Ds, g.Dt, mode, stream_ptr())
+        self._keep = (hi, lo, scales, desc)   # alive until the next decode
+        return A
+
+    def solve_tikhonov(self, Tm, r, n_dev, lam_rel=1e-6, precision="fp64", refine_iters=4,
+                       A=None):
+        """(A + lam I) X = r, lam = lam_rel tr(A)/n, A = sum_j S_j S_j^T kron T_j.
+
+        "fp64"   : FP64 Gram, FP64 Cholesky (the plain path).
+        "mixed"  : FP64 Gram rounded to FP32, FP32 Cholesky, FP64 refinement.
+        "bf16x3" : split-BF16 3-product Gram (FP32 accumulation), FP32 Cholesky, FP64
+                   refinement.
+        "fp16"   : FP16 Gram (power-of-two scaled, FP32 accumulation), FP32 Cholesky, FP64
+                   refinement.
+        Refinement residuals use the exact FP64 operator through the Kronecker structure,
+        so the refined X solves the FP64 system; the residual history is returned.
         """
         g = self.geom
         D = g.D
-        lam = torch.empty(2, dtype=F64, device=self.device)
-        lib.vt_trace_lambda(ptr(A), 0, A.stride(0), D, ptr(n_dev), lam_rel, ptr(lam), stream_ptr())
         nblk = (D + 63) // 64
         info = torch.zeros(1, dtype=I32, device=self.device)
         sync = self._sync_ws(1, D)
         X = r.clone()
         if precision == "fp64":
+            A = self.gram(Tm) if A is None else A
+            lam = torch.empty(2, dtype=F64, device=self.device)
+            lib.vt_trace_lambda(ptr(A), 0, A.stride(0), D, ptr(n_dev), lam_rel, ptr(lam),
+                                stream_ptr())
             lib.vt_diag_add(ptr(A), 0, A.stride(0), D, 0.0, ptr(lam), stream_ptr())
             inv = torch.empty(nblk * 64 * 64, dtype=F64, device=self.device)
             mats = mat_descs([(A.data_ptr(), A.stride(0), D)], self.device)
             lib.vt_potrf_f64(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(info), stream_ptr())
             lib.vt_potrs_f64(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(X), D, ptr(sync),
                              stream_ptr())
+            self._keep_solve = (A, inv, mats)
             return X, lam, info, None
-        # mixed
-        A32 = torch.empty((D, D), dtype=torch.float32, device=self.device)
-        lib.vt_lower_to_f32(ptr(A32), D, ptr(A), A.stride(0), D, ptr(lam), stream_ptr())
+        lam = self.lam_exact(Tm, n_dev, lam_rel)
+        if precision == "mixed":
+            A64 = self.gram(Tm) if A is None else A
+            A32 = torch.empty((D, D), dtype=torch.float32, device=self.device)
+            lib.vt_lower_to_f32(ptr(A32), D, ptr(A64), A64.stride(0), D, ptr(lam), stream_ptr())
+        elif precision in ("bf16x3", "fp16"):
+            A32 = self.gram_lp(Tm, 0 if precision == "bf16x3" else 1)
+            lib.vt_diag_add(ptr(A32), 1, D, D, 0.0, ptr(lam), stream_ptr())
+        else:
+            raise ValueError(f"unknown precision {precision}")
         inv = torch.empty(nblk * 64 * 64, dtype=torch.float32, device=self.device)
         mats = mat_descs([(A32.data_ptr(), D, D)], self.device)
         lib.vt_potrf_f32(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(info), stream_ptr())
         lib.vt_potrs_f32(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(X), D, ptr(sync), stream_ptr())
         res = torch.empty(D, dtype=F64, device=self.device)
         AX = torch.empty(D, dtype=F64, device=self.device)
-        nrm = torch.zeros(refine_iters + 1, dtype=F64, device=self.device)
+        nrm = torch.zeros(refine_iters + 2, dtype=F64, device=self.device)
         lib.vt_sumsq(ptr(r), D, ptr(nrm[0:]), stream_ptr())
-        for it in range(refine_iters):
-            self.apply_A(self._Tm, X, out=AX)
+        for it in range(refine_iters + 1):
+            self.apply_A(Tm, X, out=AX)
             lib.vt_residual(ptr(res), ptr(r), ptr(AX), ptr(X), ptr(lam), D, stream_ptr())
             lib.vt_sumsq(ptr(res), D, ptr(nrm[it + 1:]), stream_ptr())
+            if it == refine_iters:
+                break
             lib.vt_potrs_f32(ptr(mats), 1, D, ptr(inv), inv.numel(), ptr(res), D, ptr(sync),
                              stream_ptr())
             lib.vt_axpbypcz(ptr(X), 1.0, ptr(X), 1.0, ptr(res), 0.0, 0, D, stream_ptr())
+        self._keep_solve = (A32, inv, mats, res, AX)
         return X, lam, info, nrm
 
     def solve_rnn(self, Tm, r, lam, steps: int, eta: float):
@@ -342,19 +402,28 @@ class Decoder:
     # -- full decode ----------------------------------------------------------------------
     def decode(self, st: SpikeTrains, t0: float = 0.0, method: str = "tikhonov",
                precision: str = "fp64", lam_rel: float = 1e-6, rnn_steps: int = 500,
-               rnn_eta: Optional[float] = None, refine_iters: int = 4) -> DecodeResult:
+               rnn_eta: Optional[float] = None, refine_iters: int = 4,
+               verify: bool = True, tol: float = 1e-10) -> DecodeResult:
+        """Decode one window.  For the low-precision forms with `verify`, a factorisation
+        that lost positive definiteness or a refinement that did not reach `tol` relative
+        residual is redone on the FP64 path (result.fallback = True); this costs one host
+        synchronisation."""
         tk1, tk2, nid, offsets = self.intervals(st, t0)
         q, Tm, tv = self.moments(tk1, tk2, nid, offsets)
-        self._Tm = Tm
         n_dev = offsets[self.N:]
         r = self.rhs(tv)
-        A = self.gram(Tm)
+        fallback = False
         if method == "tikhonov":
-            X, lam, info, nrm = self.solve_tikhonov(A, r, n_dev, lam_rel, precision, refine_iters)
+            X, lam, info, nrm = self.solve_tikhonov(Tm, r, n_dev, lam_rel, precision, refine_iters)
+            if verify and precision != "fp64":
+                h = nrm.cpu().numpy()
+                ok = int(info.item()) == 0 and np.isfinite(h).all() and \
+                    math.sqrt(h[-1]) <= tol * math.sqrt(h[0])
+                if not ok:
+                    X, lam, info, nrm = self.solve_tikhonov(Tm, r, n_dev, lam_rel, "fp64")
+                    fallback = True
         elif method == "rnn":
-            lam = torch.empty(2, dtype=F64, device=self.device)
-            lib.vt_trace_lambda(ptr(A), 0, A.stride(0), self.geom.D, ptr(n_dev), lam_rel,
-                                ptr(lam), stream_ptr())
+            lam = self.lam_exact(Tm, n_dev, lam_rel)
             if rnn_eta is None:
                 raise ValueError("rnn_eta is required for the RNN decoder")
             X = self.solve_rnn(Tm, r, lam, rnn_steps, rnn_eta)
@@ -362,4 +431,4 @@ class Decoder:
         else:
             raise ValueError(f"unknown method {method}")
         frames = self.synthesize(X)
-        return DecodeResult(frames, X, lam, n_dev, info, nrm)
+        return DecodeResult(frames, X, lam, n_dev, info, nrm, fallback)

@@ -474,3 +474,79 @@ VT_EXPORT int vt_sumsq(const double *x, long long n, double *out, void *stream)
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
+
+// Row norms s2[j] = |S_j|^2 (computed once per neuron set).
+__global__ void row_sumsq_kernel(const double *__restrict__ S, long long lds, int N, int Ds,
+                                 double *__restrict__ s2)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    double s = 0.0;
+    for (int p = 0; p < Ds; ++p) s += S[(long long)j * lds + p] * S[(long long)j * lds + p];
+    s2[j] = s;
+}
+
+VT_EXPORT int vt_row_sumsq(const double *S, long long lds, int N, int Ds, double *s2, void *stream)
+{
+    VT_CHECK_ARG(S && s2 && N >= 1 && Ds >= 1 && lds >= Ds);
+    row_sumsq_kernel<<<vt_blocks(N, 256), 256, 0, (hipStream_t)stream>>>(S, lds, N, Ds, s2);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// Exact trace of the Kronecker Gram without forming it: tr(A) = sum_j s2_j tr(T_j) with
+// s2_j = |S_j|^2.  lam_out[0] = lam_rel * tr(A) / n (n from n_dev), lam_out[1] = tr(A).
+// Batched: problem b uses s2 + s2_off[b], Tm + T_off[b] (in neurons), N[b], n_dev[b].
+__global__ void __launch_bounds__(256)
+kron_trace_kernel(const double *__restrict__ s2, const double *__restrict__ Tm, const int *Nb,
+                  const long long *s2_off, const long long *T_off, int N0, int Dt,
+                  const int *__restrict__ n_dev, int n_stride, double lam_rel,
+                  double *__restrict__ lam_out)
+{
+    __shared__ double red[256];
+    const int b = blockIdx.x;
+    const int N = Nb ? Nb[b] : N0;
+    const double *s = s2 + (s2_off ? s2_off[b] : 0);
+    const double *T = Tm + (T_off ? T_off[b] : 0) * (long long)Dt * Dt;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < N; j += 256) {
+        double tt = 0.0;
+        for (int p = 0; p < Dt; ++p) tt += T[((long long)j * Dt + p) * Dt + p];
+        acc += s[j] * tt;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int n = n_dev[(long long)b * n_stride];
+        lam_out[2 * b] = n > 0 ? lam_rel * red[0] / (double)n : 0.0;
+        lam_out[2 * b + 1] = red[0];
+    }
+}
+
+VT_EXPORT int vt_kron_trace(const double *s2, const double *Tm, int N, int Dt, const int *n_dev,
+                            double lam_rel, double *lam_out, void *stream)
+{
+    VT_CHECK_ARG(s2 && Tm && n_dev && lam_out && N >= 1 && Dt >= 1);
+    kron_trace_kernel<<<1, 256, 0, (hipStream_t)stream>>>(s2, Tm, nullptr, nullptr, nullptr, N, Dt,
+                                                         n_dev, 0, lam_rel, lam_out);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// Batched: per problem b: N[b] neurons at s2 + s2_off[b], T at Tm + T_off[b]*Dt*Dt,
+// n at n_dev[b * n_stride]; lam_out[2b..2b+1].  All arrays on device.
+VT_EXPORT int vt_kron_trace_batched(const double *s2, const double *Tm, const int *Nb,
+                                    const long long *s2_off, const long long *T_off, int Dt,
+                                    const int *n_dev, int n_stride, double lam_rel,
+                                    double *lam_out, int batch, void *stream)
+{
+    VT_CHECK_ARG(s2 && Tm && Nb && s2_off && T_off && n_dev && lam_out && batch >= 1 && Dt >= 1);
+    kron_trace_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(s2, Tm, Nb, s2_off, T_off, 0, Dt, n_dev,
+                                                             n_stride, lam_rel, lam_out);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}

@@ -22,10 +22,17 @@
 #define BP 8
 #define KC 16
 
+struct Gram64Desc {
+    const double *S;      // N x Ds (row stride lds)
+    const double *Tm;     // N x Dt^2
+    double *A;            // D x D (row stride lda)
+    long long lds, lda;
+    int N, Ds, Dt, pad;
+};
+
 template <int NCT>
 __global__ void __launch_bounds__(256)
-gram_f64_kernel(const double *__restrict__ S, long long lds, const double *__restrict__ Tm, int N,
-                int Ds, int Dt, double *__restrict__ A, long long lda, int nbp)
+gram_f64_kernel(const Gram64Desc *__restrict__ descs, Gram64Desc d0)
 {
     constexpr int NC = NCT * 16;
     constexpr int TLD = NC + 2;
@@ -33,8 +40,16 @@ gram_f64_kernel(const double *__restrict__ S, long long lds, const double *__res
     __shared__ double S2[KC][BP];
     __shared__ double Tt[KC][TLD];
 
+    const Gram64Desc d = descs ? descs[blockIdx.y] : d0;
+    const double *__restrict__ S = d.S;
+    const double *__restrict__ Tm = d.Tm;
+    double *__restrict__ A = d.A;
+    const long long lds = d.lds, lda = d.lda;
+    const int N = d.N, Ds = d.Ds, Dt = d.Dt;
+    const int nbp = (Ds + BP - 1) / BP;
     // block pair from the linear index: b1 >= b2
     const int pid = blockIdx.x;
+    if (pid >= nbp * (nbp + 1) / 2) return;
     int b1 = (int)((sqrt(8.0 * pid + 1.0) - 1.0) * 0.5);
     while ((b1 + 1) * (b1 + 2) / 2 <= pid) ++b1;
     while (b1 * (b1 + 1) / 2 > pid) --b1;
@@ -91,13 +106,18 @@ gram_f64_kernel(const double *__restrict__ S, long long lds, const double *__res
     }
 }
 
-template <int NCT>
-static void launch_gram_f64(const double *S, long long lds, const double *Tm, int N, int Ds,
-                            int Dt, double *A, long long lda, hipStream_t s)
+static int launch_gram_f64(const Gram64Desc *descs, const Gram64Desc &d0, int batch, int max_Ds,
+                           int Dt, hipStream_t s)
 {
-    const int nbp = (Ds + BP - 1) / BP;
-    const int npairs = nbp * (nbp + 1) / 2;
-    gram_f64_kernel<NCT><<<dim3(npairs, 1), 256, 0, s>>>(S, lds, Tm, N, Ds, Dt, A, lda, nbp);
+    const int nbp = (max_Ds + BP - 1) / BP;
+    dim3 grid(nbp * (nbp + 1) / 2, batch);
+    const int nct = (Dt * Dt + 15) / 16;
+    if (nct <= 2) gram_f64_kernel<2><<<grid, 256, 0, s>>>(descs, d0);
+    else if (nct <= 6) gram_f64_kernel<6><<<grid, 256, 0, s>>>(descs, d0);
+    else if (nct <= 19) gram_f64_kernel<19><<<grid, 256, 0, s>>>(descs, d0);
+    else return VT_ERANGE;
+    VT_LAUNCH_CHECK();
+    return VT_OK;
 }
 
 // A (D x D, row stride lda, D = Ds*Dt): lower block triangle written (see above).
@@ -106,14 +126,17 @@ VT_EXPORT int vt_gram_f64(const double *S, long long lds, const double *Tm, int 
 {
     VT_CHECK_ARG(S && Tm && A && N >= 1 && Ds >= 1 && Dt >= 1 && lds >= Ds &&
                  lda >= (long long)Ds * Dt);
-    hipStream_t s = (hipStream_t)stream;
-    const int nct = (Dt * Dt + 15) / 16;
-    if (nct <= 2) launch_gram_f64<2>(S, lds, Tm, N, Ds, Dt, A, lda, s);
-    else if (nct <= 6) launch_gram_f64<6>(S, lds, Tm, N, Ds, Dt, A, lda, s);
-    else if (nct <= 19) launch_gram_f64<19>(S, lds, Tm, N, Ds, Dt, A, lda, s);
-    else return VT_ERANGE;
-    VT_LAUNCH_CHECK();
-    return VT_OK;
+    Gram64Desc d0{S, Tm, A, lds, lda, N, Ds, Dt, 0};
+    return launch_gram_f64(nullptr, d0, 1, Ds, Dt, (hipStream_t)stream);
+}
+
+// Batched FP64 Gram: descs = device array of {S, Tm, A, lds, lda, N, Ds, Dt, pad}
+// (56 bytes); all problems share Dt.
+VT_EXPORT int vt_gram_f64_batched(const void *descs, int batch, int max_Ds, int Dt, void *stream)
+{
+    VT_CHECK_ARG(descs && batch >= 1 && batch <= 65535 && max_Ds >= 1 && Dt >= 1);
+    Gram64Desc d0{};
+    return launch_gram_f64((const Gram64Desc *)descs, d0, batch, max_Ds, Dt, (hipStream_t)stream);
 }
 
 // Mirror the lower triangle into the upper one (for callers that want the full matrix).
@@ -135,6 +158,216 @@ VT_EXPORT int vt_symmetrize(void *A, int dtype, long long lda, int D, void *stre
         symmetrize_kernel<double><<<g, b, 0, (hipStream_t)stream>>>((double *)A, lda, D);
     else
         symmetrize_kernel<float><<<g, b, 0, (hipStream_t)stream>>>((float *)A, lda, D);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Low-precision Gram on the gfx950 16x16x32 MFMAs, FP32 accumulation, FP32 output
+// (lower block triangle), batched over independent blocks by descriptor.
+//
+//   mode 0 (split-BF16, 3 products): x = hi + lo with hi = bf16(x), lo = bf16(x - hi);
+//          A += Whi Thi + Whi Tlo + Wlo Thi   (about 16 significant bits per operand);
+//   mode 1 (FP16, 1 product): W and T pre-scaled by powers of two (scales on device) so
+//          they sit in FP16's normal range; the product is unscaled in the epilogue.
+//
+// T_j is split once per decode (vt_gram_split_T) into two N x Dt^2 16-bit arrays; W =
+// S[j,ps] S[j,ps'] is formed in FP32 registers and split per k-step.  T is staged in LDS
+// transposed ([column][neuron]) so each lane's 8-neuron B operand is one 16-byte read.
+// ------------------------------------------------------------------------------------
+struct GramDesc {
+    const double *S;      // N x Ds (row stride lds)
+    const unsigned short *Th, *Tl;   // N x Dt^2 16-bit (Tl unused in mode 1)
+    float *A;             // D x D (row stride lda)
+    long long lds, lda;
+    int N, Ds, Dt, pad;
+    const float *scales;  // mode 1: [sT, sW, 1/(sT sW)]
+};
+
+#define KL 32
+
+template <int NCT, int MODE>
+__global__ void __launch_bounds__(256)
+gram_lp_kernel(const GramDesc *__restrict__ descs)
+{
+    constexpr int NC = NCT * 16;
+    constexpr int KP = KL + 8;      // padded k stride (16-bit elements)
+    __shared__ float S1[KL][BP];
+    __shared__ float S2[KL][BP];
+    __shared__ __attribute__((aligned(16))) unsigned short Th[NC][KP];
+    __shared__ __attribute__((aligned(16))) unsigned short Tl[MODE == 0 ? NC : 1][KP];
+
+    const GramDesc d = descs[blockIdx.y];
+    const int nbp = (d.Ds + BP - 1) / BP;
+    const int pid = blockIdx.x;
+    if (pid >= nbp * (nbp + 1) / 2) return;
+    int b1 = (int)((sqrt(8.0 * pid + 1.0) - 1.0) * 0.5);
+    while ((b1 + 1) * (b1 + 2) / 2 <= pid) ++b1;
+    while (b1 * (b1 + 1) / 2 > pid) --b1;
+    const int b2 = pid - b1 * (b1 + 1) / 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int Dt = d.Dt, DT2 = Dt * Dt, N = d.N;
+    const float sW = MODE == 1 ? d.scales[1] : 1.0f;
+
+    v4f acc[NCT];
+#pragma unroll
+    for (int c = 0; c < NCT; ++c) acc[c] = (v4f){0, 0, 0, 0};
+    const int r = wave * 16 + (lane & 15);
+    const int i1 = r >> 3, i2 = r & 7;
+    const int g = lane >> 4;
+
+    for (int j0 = 0; j0 < N; j0 += KL) {
+        for (int e = tid; e < KL * BP; e += 256) {
+            int kk = e / BP, i = e % BP;
+            int j = j0 + kk, p1 = b1 * BP + i, p2 = b2 * BP + i;
+            S1[kk][i] = (j < N && p1 < d.Ds) ? (float)d.S[(long long)j * d.lds + p1] : 0.f;
+            S2[kk][i] = (j < N && p2 < d.Ds) ? (float)d.S[(long long)j * d.lds + p2] : 0.f;
+        }
+        for (int e = tid; e < KL * NC; e += 256) {
+            int kk = e / NC, c = e % NC;
+            bool ok = (c < DT2 && j0 + kk < N);
+            long long gi = (long long)(j0 + kk) * DT2 + c;
+            Th[c][kk] = ok ? d.Th[gi] : (unsigned short)0;
+            if (MODE == 0) Tl[c][kk] = ok ? d.Tl[gi] : (unsigned short)0;
+        }
+        __syncthreads();
+        // A operand: W for row r, neurons 8g..8g+7 of this chunk
+        float w[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) w[i] = S1[8 * g + i][i1] * S2[8 * g + i][i2];
+        if (MODE == 0) {
+            v8bf wh, wl;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                __bf16 h = (__bf16)w[i];
+                wh[i] = h;
+                wl[i] = (__bf16)(w[i] - (float)h);
+            }
+#pragma unroll
+            for (int c = 0; c < NCT; ++c) {
+                const int cc = c * 16 + (lane & 15);
+                v8bf bh = *(const v8bf *)&Th[cc][8 * g];
+                v8bf bl = *(const v8bf *)&Tl[cc][8 * g];
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, bh, acc[c], 0, 0, 0);
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, bl, acc[c], 0, 0, 0);
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl, bh, acc[c], 0, 0, 0);
+            }
+        } else {
+            v8h wh;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) wh[i] = (_Float16)(w[i] * sW);
+#pragma unroll
+            for (int c = 0; c < NCT; ++c) {
+                const int cc = c * 16 + (lane & 15);
+                v8h bh = *(const v8h *)&Th[cc][8 * g];
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wh, bh, acc[c], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    const float unscale = MODE == 1 ? d.scales[2] : 1.0f;
+#pragma unroll
+    for (int c = 0; c < NCT; ++c) {
+        const int col = c * 16 + mfma_col(lane);
+        if (col >= DT2) continue;
+        const int pt = col / Dt, ptp = col % Dt;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rr = wave * 16 + mfma_row(lane, q);
+            const int ps = b1 * BP + (rr >> 3), psp = b2 * BP + (rr & 7);
+            if (ps < d.Ds && psp < d.Ds)
+                d.A[(long long)(ps * Dt + pt) * d.lda + psp * Dt + ptp] = acc[c][q] * unscale;
+        }
+    }
+}
+
+// mode 0: hi/lo bf16 split of T (n doubles).  mode 1: fp16 of T * scales[0].
+__global__ void split_T_kernel(const double *__restrict__ T, long long n, unsigned short *hi,
+                               unsigned short *lo, int mode, const float *scales)
+{
+    const float s = mode == 1 ? scales[0] : 1.0f;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        float t = (float)T[i];
+        if (mode == 0) {
+            __bf16 h = (__bf16)t;
+            __bf16 l = (__bf16)(t - (float)h);
+            hi[i] = __builtin_bit_cast(unsigned short, h);
+            lo[i] = __builtin_bit_cast(unsigned short, l);
+        } else {
+            _Float16 h = (_Float16)(t * s);
+            hi[i] = __builtin_bit_cast(unsigned short, h);
+        }
+    }
+}
+
+// FP16 scales from max|T| and max|S|: powers of two putting max|T| * sT and
+// max|S|^2 * sW near 2^14 (FP16 max 65504), stored as [sT, sW, 1/(sT sW)].
+__global__ void __launch_bounds__(1024)
+fp16_scales_kernel(const double *__restrict__ T, long long nT, const double *__restrict__ S,
+                   long long nS, float *scales)
+{
+    __shared__ double red[2][1024];
+    double mt = 0.0, ms = 0.0;
+    for (long long i = threadIdx.x; i < nT; i += 1024) mt = fmax(mt, fabs(T[i]));
+    for (long long i = threadIdx.x; i < nS; i += 1024) ms = fmax(ms, fabs(S[i]));
+    red[0][threadIdx.x] = mt;
+    red[1][threadIdx.x] = ms;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double tmax = red[0][0] > 0 ? red[0][0] : 1.0, smax = red[1][0] > 0 ? red[1][0] : 1.0;
+        double sT = exp2(floor(log2(16384.0 / tmax)));
+        double sW = exp2(floor(log2(16384.0 / (smax * smax))));
+        scales[0] = (float)sT;
+        scales[1] = (float)sW;
+        scales[2] = (float)(1.0 / (sT * sW));
+    }
+}
+
+// Split T (n = N*Dt^2 doubles) for the low-precision Gram.  mode 0: hi, lo (bf16);
+// mode 1: hi = fp16(T*sT) with scales computed here from T and S (nS doubles).
+VT_EXPORT int vt_gram_split_T(const double *Tm, long long n, void *hi, void *lo, int mode,
+                              const double *S, long long nS, float *scales, void *stream)
+{
+    VT_CHECK_ARG(Tm && hi && n >= 0 && (mode == 0 || mode == 1));
+    VT_CHECK_ARG(mode == 1 ? (S && scales && nS > 0) : (lo != nullptr));
+    hipStream_t s = (hipStream_t)stream;
+    if (mode == 1) fp16_scales_kernel<<<1, 1024, 0, s>>>(Tm, n, S, nS, scales);
+    unsigned g = vt_blocks(n, 256);
+    split_T_kernel<<<g < 4096 ? g : 4096, 256, 0, s>>>(Tm, n, (unsigned short *)hi,
+                                                      (unsigned short *)lo, mode, scales);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// Batched low-precision Gram.  descs: device array of GramDesc (80 bytes each); every
+// problem must have the same Dt (column tiling), max_Ds bounds the grid.
+VT_EXPORT int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int mode, void *stream)
+{
+    VT_CHECK_ARG(descs && batch >= 1 && batch <= 65535 && max_Ds >= 1 && Dt >= 1 &&
+                 (mode == 0 || mode == 1));
+    const int nbp = (max_Ds + BP - 1) / BP;
+    dim3 grid(nbp * (nbp + 1) / 2, batch);
+    const int nct = (Dt * Dt + 15) / 16;
+    hipStream_t s = (hipStream_t)stream;
+    const GramDesc *dd = (const GramDesc *)descs;
+#define LAUNCH_LP(NCTV)                                                                       \
+    do {                                                                                      \
+        if (mode == 0) gram_lp_kernel<NCTV, 0><<<grid, 256, 0, s>>>(dd);                      \
+        else gram_lp_kernel<NCTV, 1><<<grid, 256, 0, s>>>(dd);                                \
+    } while (0)
+    if (nct <= 2) LAUNCH_LP(2);
+    else if (nct <= 6) LAUNCH_LP(6);
+    else if (nct <= 19) LAUNCH_LP(19);
+    else return VT_ERANGE;
+#undef LAUNCH_LP
     VT_LAUNCH_CHECK();
     return VT_OK;
 }

@@ -1,0 +1,223 @@
+"""Serving runtime: continuous batching of decode requests on one GPU.
+
+Requests (video blocks of different sizes and durations, with resident spike trains and
+prepared decoders) arrive over time.  The server:
+
+* admits arrived requests into a ready queue;
+* whenever one of its streams is free, forms a batch from the head of the queue, bounded
+  by a request count and by an HBM budget (each request's in-flight footprint: its
+  D x D FP32 factor, block inverses, moments, intervals and output frames);
+* launches the whole batched decode on that stream without a host synchronisation, so
+  batches formed from requests that arrive while others are on the GPU run concurrently
+  on the device;
+* retires a batch when its completion event fires: runs the precision guard (FP64 redo
+  of any block whose low-precision solve missed tolerance), releases its memory and
+  stamps each request's completion time.
+
+Latency is arrival -> result resident in HBM.  The plain mode is the per-block FP64
+path: one request at a time, in arrival order, no batching.
+
+The reference has no serving layer; its decode drivers run one spike file at a time
+(vtem/vtdm.py:18-309).
+"""
+
+from __future__ import annotations
+
+import collections
+import dataclasses
+import math
+import time
+from typing import Any, Dict, List, Optional
+
+import numpy as np
+import torch
+
+from .batch import BatchDecoder, BlockJob
+from .block import DecodeResult
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float                 # seconds after the start of the run
+    job: BlockJob
+    frames: int                    # frames produced by the decode
+    footprint: int                 # HBM bytes while in flight
+    start: Optional[float] = None
+    done: Optional[float] = None
+    result: Optional[DecodeResult] = None
+    batch: Optional[int] = None
+
+
+def footprint_bytes(job: BlockJob) -> int:
+    """In-flight HBM of one decode: FP32 factor + block inverses + moments + intervals +
+    split moments + frames + vectors (the decoder's S and the spike trains are resident
+    before admission and not counted)."""
+    d = job.decoder
+    g = d.geom
+    D = g.D
+    nblk = (D + 63) // 64
+    cap = job.spikes.spikes.shape[0] * d.N
+    return int(4 * D * D + 4 * nblk * 4096 + 8 * d.N * g.Dt * (g.Dt + 1) * 2 + 4 * d.N * g.Dt * g.Dt
+               + 20 * cap + 8 * g.frames * g.P + 8 * g.frames * g.Ds + 64 * D)
+
+
+class PrecisionPolicy:
+    """Per request class (spatial order Ds, temporal order Dt): start with the
+    low-precision form and route the class to batched FP64 once its guard failures
+    dominate (>= `min_fail` failures and a failure rate above `rate`).  Blocks of a
+    class share bank density and duration, hence their conditioning."""
+
+    def __init__(self, low: str = "bf16x3", min_fail: int = 2, rate: float = 0.5):
+        self.low = low
+        self.min_fail = min_fail
+        self.rate = rate
+        self.stats: Dict[tuple, List[int]] = {}      # class -> [attempts, failures]
+
+    @staticmethod
+    def key(job: BlockJob) -> tuple:
+        g = job.decoder.geom
+        return (g.Ds, g.Dt)
+
+    def choose(self, job: BlockJob) -> str:
+        a, f = self.stats.get(self.key(job), (0, 0))
+        if f >= self.min_fail and f > self.rate * a:
+            return "fp64"
+        return self.low
+
+    def record(self, job: BlockJob, precision: str, failed: bool):
+        if precision == "fp64":
+            return
+        st = self.stats.setdefault(self.key(job), [0, 0])
+        st[0] += 1
+        st[1] += int(failed)
+
+
+class Server:
+    def __init__(self, hbm_budget: int = 64 << 30, max_batch: int = 32, streams: int = 2,
+                 precision: str = "bf16x3", verify: bool = True, adaptive: bool = True):
+        self.hbm_budget = int(hbm_budget)
+        self.max_batch = int(max_batch)
+        self.n_streams = int(streams)
+        self.precision = precision
+        self.verify = verify
+        self.policy = PrecisionPolicy(precision) if adaptive else None
+
+    # ----------------------------------------------------------------------------------
+    def serve(self, requests: List[Request], mode: str = "served",
+              clock=time.perf_counter, instrument: bool = False) -> Dict[str, Any]:
+        """Serve the requests; with `instrument`, stats["timers"] holds HIP events of the
+        batched stages (name -> [(start, end, algorithmic work)])."""
+        if mode == "plain":
+            return self._serve_plain(requests, clock)
+        for r in requests:
+            if r.footprint > self.hbm_budget:
+                raise ValueError(f"request {r.rid} needs {r.footprint} B, above the HBM budget")
+        bd = BatchDecoder(precision=self.precision, verify=self.verify)
+        bd.instrument(instrument)
+        streams = [torch.cuda.Stream() for _ in range(self.n_streams)]
+        busy = [None] * self.n_streams          # (event, handle, reqs, bytes)
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        ready: collections.deque = collections.deque()
+        in_use = 0
+        nbatch = 0
+        fallbacks = 0
+        torch.cuda.synchronize()
+        t0 = clock()
+        while pending or ready or any(b is not None for b in busy):
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                ready.append(pending.popleft())
+            # retire finished batches
+            for k, b in enumerate(busy):
+                if b is not None and b[0].query():
+                    ev, h, reqs, nbytes = b
+                    with torch.cuda.stream(streams[k]):
+                        res = bd.finalize(h)
+                    fallbacks += bd.last_stats.get("fallbacks", 0)
+                    if self.policy is not None:
+                        failed = set(bd.last_stats.get("fallback_index", []))
+                        for i, (r, pr) in enumerate(zip(reqs, h.precisions)):
+                            self.policy.record(r.job, pr, i in failed)
+                    tdone = clock() - t0
+                    for r, rr in zip(reqs, res):
+                        r.result, r.done = rr, tdone
+                    in_use -= nbytes
+                    busy[k] = None
+            # launch on a free stream
+            launched = False
+            for k in range(self.n_streams):
+                if busy[k] is not None or not ready:
+                    continue
+                batch, nbytes = [], 0
+                while ready and len(batch) < self.max_batch and \
+                        in_use + nbytes + ready[0].footprint <= self.hbm_budget:
+                    r = ready.popleft()
+                    batch.append(r)
+                    nbytes += r.footprint
+                if not batch:
+                    break
+                tstart = clock() - t0
+                precs = [self.policy.choose(r.job) if self.policy else self.precision
+                         for r in batch]
+                with torch.cuda.stream(streams[k]):
+                    h = bd.launch([r.job for r in batch], precs)
+                    ev = torch.cuda.Event()
+                    ev.record()
+                for r in batch:
+                    r.start, r.batch = tstart, nbatch
+                nbatch += 1
+                in_use += nbytes
+                busy[k] = (ev, h, batch, nbytes)
+                launched = True
+            if not launched:
+                # nothing to launch: wait for the next arrival or a completion
+                nxt = pending[0].arrival - (clock() - t0) if pending else 0.0
+                if any(b is not None for b in busy):
+                    time.sleep(min(max(nxt, 0.0), 2e-4))
+                elif nxt > 0:
+                    time.sleep(min(nxt, 5e-3))
+        torch.cuda.synchronize()
+        wall = clock() - t0
+        extra = {"batches": nbatch, "fallbacks": fallbacks}
+        if instrument:
+            extra["timers"] = bd.timers
+        return self._stats(requests, wall, extra)
+
+    def _serve_plain(self, requests, clock):
+        order = sorted(requests, key=lambda r: r.arrival)
+        torch.cuda.synchronize()
+        t0 = clock()
+        for r in order:
+            now = clock() - t0
+            if r.arrival > now:
+                time.sleep(r.arrival - now)
+            r.start = clock() - t0
+            jb = r.job
+            r.result = jb.decoder.decode(jb.spikes, jb.t0, precision="fp64")
+            torch.cuda.current_stream().synchronize()
+            r.done = clock() - t0
+        wall = clock() - t0
+        return self._stats(requests, wall, {"batches": len(order), "fallbacks": 0})
+
+    @staticmethod
+    def _stats(requests, wall, extra):
+        lat = np.array([r.done - r.arrival for r in requests]) * 1000.0
+        frames = sum(r.frames for r in requests)
+        first = min(r.arrival for r in requests)
+        span = max(r.done for r in requests) - first
+        out = {"requests": len(requests), "frames": frames, "wall_s": wall,
+               "frames_per_s": frames / span if span > 0 else float("nan"),
+               "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+               "mean_ms": float(lat.mean())}
+        out.update(extra)
+        return out
+
+
+def poisson_arrivals(n: int, rate: float, rng: np.random.Generator) -> np.ndarray:
+    """Arrival times of a Poisson process with `rate` requests/s (first at 0)."""
+    if rate <= 0 or math.isinf(rate):
+        return np.zeros(n)
+    gaps = rng.exponential(1.0 / rate, n)
+    gaps[0] = 0.0
+    return np.cumsum(gaps)

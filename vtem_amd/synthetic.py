@@ -283,3 +283,41 @@ def iaf_params(N: int, b: float, kappa: float, delta_mean: float, delta_std: flo
         delta = np.full(N, float(delta_mean))
     v0 = rng.uniform(0.0, 1.0, N) * delta
     return IAFParams(np.full(N, float(kappa)), np.full(N, float(b)), delta, v0, np.zeros(N))
+
+
+# --------------------------------------------------------------------------------------
+# Serving-form blocks: side in {32, 48, 64, 96} px, duration in {0.5, 1, 2} s
+# --------------------------------------------------------------------------------------
+
+SERVE_SIDES = (32, 48, 64, 96)
+SERVE_DURATIONS = (0.5, 1.0, 2.0)
+
+
+@dataclasses.dataclass
+class ServeBlock:
+    side: int
+    duration: float
+    Lx: int
+    Lt: int
+    video: Video
+    bank: Bank
+    iaf: IAFParams
+
+    @property
+    def D(self):
+        return (2 * self.Lx + 1) ** 2 * (2 * self.Lt + 1)
+
+
+def serve_block(side: int, duration: float, seed: int) -> ServeBlock:
+    """One request block: Lx = Ly = round(side/8), Lt = round(4*duration), N =
+    round(2*D/9) neurons from the config-1 law (Gabor:DoG = 7:1), IAF b=2.5, kappa=1,
+    delta ~ N(0.04, 0.004^2), video as the other configurations (seeded)."""
+    rng = np.random.default_rng(seed)
+    L = int(round(side / 8.0))
+    Lt = int(round(4.0 * duration))
+    D = (2 * L + 1) ** 2 * (2 * Lt + 1)
+    N = int(round(2.0 * D / 9.0))
+    vid = make_video(side, side, duration, 100.0, L, L, Lt, seed=int(rng.integers(1 << 30)))
+    bank = bank_random(float(side), float(side), N, rng)
+    iaf = iaf_params(N, 2.5, 1.0, 0.04, 0.004, rng)
+    return ServeBlock(side, duration, L, Lt, vid, bank, iaf)
