@@ -10,9 +10,13 @@ produces 100 fps frames over its duration.
 
 One step = one wave of R requests with Poisson arrivals at an offered rate above the
 server's capacity (default), decoded by
-  chosen: the served path (continuous batching on 2 streams, HBM-budget admission,
-          batched split-BF16 Gram + FP32 Cholesky + FP64 refinement, FP64 guard);
-  plain:  the per-block FP64 path (FP64 Gram + FP64 Cholesky), one request at a time.
+  chosen:     the served path (continuous batching on 2 streams, HBM-budget admission)
+              with the batched Kronecker-preconditioned CG solver: FP64 conjugate
+              gradients through each block's exact Kronecker operator, preconditioner
+              (S^T C S + lam I) kron Tbar, no Gram; FP64 guard on the true residual;
+  gram_bf16x3: the same server with batched split-BF16 Gram + FP32 Cholesky + FP64
+              refinement (FP64 for request classes whose FP32 factor fails);
+  plain:      the per-block FP64 path (FP64 Gram + FP64 Cholesky), one request at a time.
 `value` = chosen decoded frames/s (frames of the wave / time from first arrival to last
 result).  A latency sweep at sub-saturation rates gives p50/p99 (arrival -> result in
 HBM).  SNR of every request is checked against its plain FP64 decode (|dSNR| < 0.1 dB).
@@ -35,8 +39,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-PEAKS = {"gram_lp": (2500.0, "TFLOP/s", "mfma"), "potrf_f32": (157.3, "TFLOP/s", "mfma"),
-         "solve_refine": (157.3, "TFLOP/s", "mfma")}
+# stage -> (peak, unit, bound); the stage's recorded work is flops for "mfma" stages and
+# HBM bytes for "hbm" stages
+PEAKS = {"gram_lp": (2500.0, "TFLOP/s", "mfma"), "gram_f64": (78.6, "TFLOP/s", "mfma"),
+         "potrf_f32": (157.3, "TFLOP/s", "mfma"), "potrf_f64": (78.6, "TFLOP/s", "mfma"),
+         "solve_refine": (157.3, "TFLOP/s", "mfma"), "kpcg_setup": (78.6, "TFLOP/s", "mfma"),
+         "kpcg_iterate": (8000.0, "GB/s", "hbm")}
+WORK_NOTES = {"kpcg_iterate": "algorithmic HBM bytes per CG iteration: 8*(2*N*Ds + N*Dt^2 + 4*N*Dt "
+                              "+ Ds^2 + 12*D) per block (S twice, T_j, Y/Z, Mi, vectors)",
+              "gram_lp": "algorithmic flops N*D*(D+1) per block",
+              "potrf_f32": "algorithmic flops D^3/3 per block"}
 
 
 def parse():
@@ -48,6 +60,8 @@ def parse():
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--max-batch", type=int, default=32)
     p.add_argument("--streams", type=int, default=2)
+    p.add_argument("--form", default="kpcg", help="solver form of the chosen served path")
+    p.add_argument("--compare", default="bf16x3", help="second served form measured ('' = none)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-sweep", action="store_true", help="skip the latency sweep")
     p.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
@@ -135,39 +149,61 @@ def main():
     frames = sum(g.frames for _, g, _, _ in items)
     D = np.array([g.D for _, g, _, _ in items])
     zeros = np.zeros(R)                       # offered load above capacity
-    server = Server(max_batch=args.max_batch, streams=args.streams)
-    # warmup (allocator, code objects)
-    for _ in range(args.warmup):
-        server.serve(make_requests(items, zeros))
+
+    def run_form(form):
+        server = Server(max_batch=args.max_batch, streams=args.streams, precision=form,
+                        adaptive=form != "kpcg")
+        for _ in range(args.warmup):           # allocator, code objects, precision policy
+            server.serve(make_requests(items, zeros))
+        stats, reqs = [], None
+        for _ in range(args.steps):
+            reqs = make_requests(items, zeros)
+            torch.cuda.synchronize()
+            stats.append(server.serve(reqs))
+        wall = sum(st["wall_s"] for st in stats)
+        return server, stats, reqs, wall
+
     # plain (per-block FP64) on the same wave
     plain_reqs = make_requests(items, zeros)
+    Server().serve(plain_reqs, mode="plain")              # warm
+    plain_reqs = make_requests(items, zeros)
     plain = Server().serve(plain_reqs, mode="plain")
-    # chosen, timed steps; instrument the last one
-    stats = []
-    chosen_reqs = None
-    for k in range(args.steps):
-        chosen_reqs = make_requests(items, zeros)
-        torch.cuda.synchronize()
-        stats.append(server.serve(chosen_reqs))
-    wall = sum(s["wall_s"] for s in stats)
+    server, stats, chosen_reqs, wall = run_form(args.form)
     value = frames * args.steps / wall
-    # SNR of each request: chosen vs video, plain vs video
-    d_snr, snrs = [], []
-    for (blk, geom, job, _), rc, rp in zip(items, chosen_reqs, plain_reqs):
-        u = blk.video.frames.reshape(geom.frames, -1)
-        sc = snr_db(u, rc.result.frames.cpu().numpy())
-        sp = snr_db(u, rp.result.frames.cpu().numpy())
-        d_snr.append(abs(sc - sp))
-        snrs.append(sp)
+
+    def snr_diffs(reqs):
+        d = []
+        for (blk, geom, job, _), rc, rp in zip(items, reqs, plain_reqs):
+            u = blk.video.frames.reshape(geom.frames, -1)
+            d.append(abs(snr_db(u, rc.result.frames.cpu().numpy()) -
+                         snr_db(u, rp.result.frames.cpu().numpy())))
+        return d
+
+    d_snr = snr_diffs(chosen_reqs)
+    snrs = [snr_db(blk.video.frames.reshape(geom.frames, -1), rp.result.frames.cpu().numpy())
+            for (blk, geom, _, _), rp in zip(items, plain_reqs)]
+    compare = None
+    if args.compare:
+        _, cst, creqs, cwall = run_form(args.compare)
+        cd = snr_diffs(creqs)
+        compare = {"form": args.compare, "frames_per_s": round(frames * args.steps / cwall, 2),
+                   "p50_ms": round(cst[-1]["p50_ms"], 3), "p99_ms": round(cst[-1]["p99_ms"], 3),
+                   "fp64_fallbacks": cst[-1]["fallbacks"],
+                   "speedup_vs_plain": round(frames * args.steps / cwall / plain["frames_per_s"], 3),
+                   "snr_max_abs_diff_vs_plain_db": float(f"{max(cd):.3g}")}
     # roofline of the dominant stage (HIP events on the launch stream, one instrumented wave)
-    timers = server.serve(make_requests(items, zeros), instrument=True)["timers"]
+    # one stream, so that a stage's events bracket only its own kernels
+    iserver = Server(max_batch=args.max_batch, streams=1, precision=args.form,
+                     adaptive=args.form != "kpcg")
+    iserver.policy = server.policy
+    timers = iserver.serve(make_requests(items, zeros), instrument=True)["timers"]
     torch.cuda.synchronize()
     stage = {k: (sum(a.elapsed_time(b) for a, b, _ in v), sum(w for _, _, w in v), len(v))
              for k, v in timers.items()}
     dom = max(stage, key=lambda k: stage[k][0])
     t_ms, work, nlaunch = stage[dom]
     peak, unit, bound = PEAKS[dom]
-    achieved = work / (t_ms * 1e-3) / 1e12
+    achieved = work / (t_ms * 1e-3) / (1e9 if unit == "GB/s" else 1e12)
     line = {
         "metric": "decoded frames/s at fixed SNR (chosen vs plain form) + p50/p99 block latency",
         "value": round(value, 2),
@@ -179,25 +215,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3+f32+f64",
+        "dtype": "f64" if args.form == "kpcg" else "bf16x3+f32+f64",
         "data": "synthetic (seeded bandlimited videos, encoded on the GPU before timing)",
         "config": {"workload": "serving form: blocks {32,48,64,96}^2 px x {0.5,1,2} s, "
                                "N=round(2D/9), Poisson arrivals above capacity",
                    "requests_per_step": R, "frames_per_step": frames, "D_min": int(D.min()),
                    "D_max": int(D.max()), "max_batch": args.max_batch, "streams": args.streams},
-        "chosen": {"frames_per_s": round(value, 2),
+        "chosen": {"form": args.form, "frames_per_s": round(value, 2),
                    "p50_ms": round(stats[-1]["p50_ms"], 3), "p99_ms": round(stats[-1]["p99_ms"], 3),
                    "batches": stats[-1]["batches"], "fp64_fallbacks": stats[-1]["fallbacks"]},
         "plain": {"frames_per_s": round(plain["frames_per_s"], 2),
                   "p50_ms": round(plain["p50_ms"], 3), "p99_ms": round(plain["p99_ms"], 3)},
         "speedup_vs_plain": round(value / plain["frames_per_s"], 3),
+        "compare": compare,
         "snr": {"mean_db": round(float(np.mean(snrs)), 3), "min_db": round(float(np.min(snrs)), 3),
                 "max_abs_diff_vs_plain_db": float(f"{max(d_snr):.3g}")},
         "stage_ms": {k: round(v[0], 3) for k, v in stage.items()},
         "roofline": {"kernel": dom, "bound": bound, "achieved": round(achieved, 3), "peak": peak,
                      "unit": unit, "frac": round(achieved / peak, 5), "traffic": None,
                      "avg_ms": round(t_ms / nlaunch, 4),
-                     "work_note": "algorithmic flops: Gram sum N*D*(D+1), Cholesky sum D^3/3"},
+                     "work_note": WORK_NOTES.get(dom, "")},
     }
     if not args.no_sweep:
         cap_rps = R / (wall / args.steps)

@@ -259,10 +259,47 @@ int vt_seg_sumsq(const double *x, const long long *seg, int nseg, double *out, i
 /* A_b += lam[2b] I; descs = {void *A; long long lda; int D, pad;} (24 bytes). */
 int vt_diag_add_batched(const void *descs, int batch, int dtype, const double *lam, int max_D,
                         void *stream);
+/* Skinny grouped product, Nc <= 32: C_b (M x Nc) = A_b B_b with A_b(m,k) at
+ * A[m*sam + k*sak], B and C row-major; if T != 0 each row is then multiplied by
+ * T + m*tstride (C_m <- T_m C_m).  descs = {const double *A, *B; double *C; const double *T;
+ * long long sam, sak, ldb, ldc, tstride; int M, K;} (80 bytes).  The structured operator
+ * S^T (T_j (S X)) of the coefficient-space Gram (vtem/dirichlet.py:62-85 builds that Gram
+ * in spike space) is two such products. */
+int vt_gemm_skinny(const void *descs, int batch, int max_M, int Nc, void *stream);
 /* x += y. */
 int vt_add_inplace(double *x, const double *y, long long n, void *stream);
 /* out[b] = offsets[base[b+1]] - offsets[base[b]] (intervals per block). */
 int vt_seg_counts(const int *offsets, const int *base, int nseg, int *out, void *stream);
+
+/* ------------------------------------------- Kronecker-preconditioned CG (batched) -- */
+/* Matrix-free solve of (A + lam I) X = r with A = sum_j S_j S_j^T (x) T_j and the
+ * preconditioner (S^T C S + lam I) (x) Tbar, c_j = tr(T_j)/Dt, Tbar = sum_j T_j / sum_j c_j.
+ * It solves the same Tikhonov system as the Cholesky path and replaces, for that system,
+ * the reference's pseudoinverse solve (vtem/utils/linalg.py:591-631, called from
+ * vtem/vtdm.py:651) and its Gram build (vtem/dirichlet.py:62-85), which it never forms. */
+/* Tbar inverse per block, stored transposed (TiT, batch x Dt x Dt), and c_j (global neuron
+ * numbering, nbase has batch+1 entries); ws: 16*batch*Dt*Dt doubles.  Dt <= 33 (else -5). */
+int vt_kpcg_tbar(const double *Tm, const int *nbase, int Dt, int batch, double *ws, double *c,
+                 double *Ti, void *stream);
+/* M_b = S_b^T diag(c_b) S_b; descs = {const double *S, *c; double *M; long long lds, ldm;
+ * int N, Ds;} per block (48 bytes). */
+int vt_kpcg_mgram(const void *descs, int batch, int max_Ds, void *stream);
+/* Diagonal 64x64 blocks of L_b^-1 from the Cholesky block inverses (inv_ws +
+ * b*inv_stride + i*4096) into X + xoff[b] (dim[b] x dim[b], leading dimension dim[b]). */
+int vt_kpcg_place_diag(double *X, const long long *xoff, const int *dim, int batch, int max_dim,
+                       const double *inv_ws, long long inv_stride, void *stream);
+/* CG recurrences over concatenated per-block vectors (block b owns [seg[b], seg[b+1]));
+ * st: 4 doubles per block {rz, |r0|^2, active, iterations}; hist[b*hist_stride + k] =
+ * |r_k|^2.  init: x = 0, p = z, rz = <r,z>.  step_a: q += lam p, alpha = rz/<p,q>,
+ * x += alpha p, r -= alpha q, stop when |r|^2 <= stop_rel2 |r0|^2.  step_b: beta =
+ * <r,z>/rz, p = z + beta p.  Stopped blocks are left unchanged by later steps. */
+int vt_kpcg_init(const double *r, const double *z, double *p, double *x, const long long *seg,
+                 int batch, double *st, double *hist, int hist_stride, void *stream);
+int vt_kpcg_step_a(double *x, double *r, const double *p, double *q, const double *lam,
+                   const long long *seg, int batch, double *st, double *hist, int hist_stride,
+                   int it, double stop_rel2, void *stream);
+int vt_kpcg_step_b(const double *r, const double *z, double *p, const long long *seg, int batch,
+                   double *st, void *stream);
 
 #ifdef __cplusplus
 }

@@ -49,10 +49,16 @@ GRAM_LP_DTYPE = np.dtype([("S", np.uint64), ("Th", np.uint64), ("Tl", np.uint64)
 GRAM64_DTYPE = np.dtype([("S", np.uint64), ("Tm", np.uint64), ("A", np.uint64), ("lds", np.int64),
                          ("lda", np.int64), ("N", np.int32), ("Ds", np.int32), ("Dt", np.int32),
                          ("pad", np.int32)])
+KPM_DTYPE = np.dtype([("S", np.uint64), ("c", np.uint64), ("M", np.uint64), ("lds", np.int64),
+                      ("ldm", np.int64), ("N", np.int32), ("Ds", np.int32)])
+SK_DTYPE = np.dtype([("A", np.uint64), ("B", np.uint64), ("C", np.uint64), ("T", np.uint64),
+                     ("sam", np.int64), ("sak", np.int64), ("ldb", np.int64), ("ldc", np.int64),
+                     ("tstride", np.int64), ("M", np.int32), ("K", np.int32)])
 
 assert SPIKE_DTYPE.itemsize == 48 and TJ_DTYPE.itemsize == 40 and GEMM_DTYPE.itemsize == 96
 assert TN_DTYPE.itemsize == 64 and DIAG_DTYPE.itemsize == 24
-assert GRAM_LP_DTYPE.itemsize == 72 and GRAM64_DTYPE.itemsize == 56
+assert GRAM_LP_DTYPE.itemsize == 72 and GRAM64_DTYPE.itemsize == 56 and KPM_DTYPE.itemsize == 48
+assert SK_DTYPE.itemsize == 80
 
 
 def _to_dev(arr: np.ndarray, device) -> torch.Tensor:
@@ -78,19 +84,23 @@ class BatchHandle:
     precisions: list = dataclasses.field(default_factory=list)
 
 
-PRECISIONS = ("fp64", "mixed", "bf16x3", "fp16")
+PRECISIONS = ("fp64", "mixed", "bf16x3", "fp16", "kpcg")
 
 
 class BatchDecoder:
     """Decodes lists of BlockJobs, grouped by (temporal order Dt, precision).
 
-    precision per block: "fp64" (FP64 Gram + FP64 Cholesky), "mixed" (FP64 Gram, FP32
-    Cholesky, FP64 refinement), "bf16x3" / "fp16" (low-precision Gram, FP32 Cholesky,
-    FP64 refinement).  Low-precision blocks that fail the guard are redone together as
+    precision (solver form) per block: "fp64" (FP64 Gram + FP64 Cholesky), "mixed" (FP64
+    Gram, FP32 Cholesky, FP64 refinement), "bf16x3" / "fp16" (low-precision Gram, FP32
+    Cholesky, FP64 refinement), "kpcg" (no Gram: FP64 conjugate gradients through the
+    exact Kronecker operator, preconditioned by (S^T C S + lam I) kron Tbar).  Blocks of
+    the non-FP64 forms that fail the guard (relative residual of the FP64 system above
+    `tol`, or a factorisation that lost positive definiteness) are redone together as
     one batched FP64 group."""
 
     def __init__(self, device="cuda", precision: str = "bf16x3", lam_rel: float = 1e-6,
-                 refine_iters: int = 4, tol: float = 1e-10, verify: bool = True):
+                 refine_iters: int = 4, tol: float = 1e-10, verify: bool = True,
+                 cg_iters: int = 32, cg_stop: float = 1e-12):
         if precision not in PRECISIONS:
             raise ValueError(f"batched precision must be one of {PRECISIONS}")
         self.device = device
@@ -99,6 +109,8 @@ class BatchDecoder:
         self.refine_iters = refine_iters
         self.tol = tol
         self.verify = verify
+        self.cg_iters = cg_iters
+        self.cg_stop = cg_stop
         self.last_stats: Dict[str, Any] = {}
         self.timers: Optional[Dict[str, list]] = None     # stage -> [(ev0, ev1, work)]
 
@@ -238,6 +250,47 @@ class BatchDecoder:
         N_d = torch.from_numpy(N.astype(np.int32)).to(dev)
         lib.vt_kron_trace_batched(ptr(s2), ptr(Tm), ptr(N_d), ptr(s2off), ptr(s2off), Dt, ptr(n_b), 1,
                                   self.lam_rel, ptr(lam), B, s)
+        seg = torch.from_numpy(dbase.astype(np.int64)).to(dev)
+        elem_job = torch.from_numpy(np.repeat(np.arange(B, dtype=np.int32), D)).to(dev)
+        solve = self._solve_kpcg if prec == "kpcg" else self._solve_direct
+        self.cg_state = None
+        X, nrm, info, solve_keep = solve(prec, decs, N, Ds, D, Dt, nbase, dbase, Ntot, Dtot, Tm, r,
+                                         lam, tn_ws, ws_base, seg, elem_job)
+        cg_state = None if self.cg_state is None else self.cg_state[0]
+        # ---- synthesis ---------------------------------------------------------------------
+        T_out = np.array([d.theta_out.shape[0] for d in decs], np.int64)
+        P = np.array([d.geom.P for d in decs], np.int64)
+        Y2 = torch.empty(int((T_out * Ds).sum()), dtype=F64, device=dev)
+        y2base = np.concatenate([[0], np.cumsum(T_out * Ds)])
+        frames = [torch.empty((int(T_out[b]), int(P[b])), dtype=F64, device=dev) for b in range(B)]
+        gs1 = np.zeros(B, GEMM_DTYPE)
+        gs2 = np.zeros(B, GEMM_DTYPE)
+        for b, d in enumerate(decs):
+            y2p = Y2.data_ptr() + 8 * int(y2base[b])
+            gs1[b] = (d.theta_out.data_ptr(), X.data_ptr() + 8 * int(dbase[b]), y2p, Dt, 1, 1, Dt, Ds[b],
+                      T_out[b], Ds[b], Dt, 0, 1.0, 0.0)
+            gs2[b] = (y2p, d.Bs.data_ptr(), frames[b].data_ptr(), Ds[b], 1, 1, Ds[b], P[b],
+                      T_out[b], P[b], Ds[b], 0, 1.0, 0.0)
+        gs1_d, gs2_d = _to_dev(gs1, dev), _to_dev(gs2, dev)
+        lib.vt_gemm_grouped(ptr(gs1_d), B, int(T_out.max()), int(Ds.max()), s)
+        lib.vt_gemm_grouped(ptr(gs2_d), B, int(T_out.max()), int(P.max()), s)
+        # every buffer referenced by enqueued work stays alive with the handle
+        keep = (N_d, sd_d, ws, offsets, tk1, tk2, nid, base_d, q, Tm, tv, tn_ws, tn_r_d, s2off,
+                solve_keep, seg, elem_job, Y2, gs1_d, gs2_d, kappa, delta, bias, norm, s2, job_of, r)
+        results = []
+        for b in range(B):
+            Xb = X[int(dbase[b]):int(dbase[b + 1])]
+            st_b = None if cg_state is None else cg_state[4 * b:4 * b + 4]
+            results.append(DecodeResult(frames[b], Xb, lam[2 * b:2 * b + 2], n_b[b:b + 1],
+                                        info[b:b + 1], nrm[b], False, st_b))
+        return results, keep, (nrm, info)
+
+    def _solve_direct(self, prec, decs, N, Ds, D, Dt, nbase, dbase, Ntot, Dtot, Tm, r, lam, tn_ws,
+                      ws_base, seg, elem_job):
+        """Gram (FP64 or low precision) + Cholesky (FP64 or FP32) + FP64 refinement."""
+        dev = self.device
+        s = stream_ptr()
+        B = len(decs)
         # ---- Gram -------------------------------------------------------------------------
         maxD = int(D.max())
         nblk = (maxD + 63) // 64
@@ -309,8 +362,6 @@ class BatchDecoder:
         sync = torch.zeros(2 + 2 * B * nblk, dtype=I32, device=dev)
         # vectors in a padded (B x maxD) layout for the solves, concatenated elsewhere
         Xp = torch.zeros((B, maxD), dtype=F64, device=dev)
-        seg = torch.from_numpy(dbase.astype(np.int64)).to(dev)
-        elem_job = torch.from_numpy(np.repeat(np.arange(B, dtype=np.int32), D)).to(dev)
         pad_idx = torch.from_numpy(np.concatenate([b * maxD + np.arange(D[b]) for b in range(B)])).to(dev)
 
         def to_padded(vec, out):
@@ -354,31 +405,162 @@ class BatchDecoder:
             potrs(ptr(mats), B, maxD, ptr(inv), nblk * 4096, ptr(resp), maxD, ptr(sync), s)
             lib.vt_add_inplace(ptr(X), ptr(from_padded(resp)), Dtot, s)
         self._toc("solve_refine", e_ref, float(np.sum(D.astype(np.float64) ** 2)) * 2 * (iters + 1))
-        # ---- synthesis ---------------------------------------------------------------------
-        T_out = np.array([d.theta_out.shape[0] for d in decs], np.int64)
-        P = np.array([d.geom.P for d in decs], np.int64)
-        Y2 = torch.empty(int((T_out * Ds).sum()), dtype=F64, device=dev)
-        y2base = np.concatenate([[0], np.cumsum(T_out * Ds)])
-        frames = [torch.empty((int(T_out[b]), int(P[b])), dtype=F64, device=dev) for b in range(B)]
-        gs1 = np.zeros(B, GEMM_DTYPE)
-        gs2 = np.zeros(B, GEMM_DTYPE)
+        return X, nrm, info, (extra_keep, Apool, inv, mats, sync, Xp, pad_idx, Y, Z, AX, res, g1_d,
+                              tn_a_d, resp)
+
+    def _skinny(self, rows):
+        """Device descriptors of a skinny grouped product; rows of (A, B, C, T, sam, sak,
+        ldb, ldc, tstride, M, K) with pointers as ints (T = 0: no row factor)."""
+        d = np.zeros(len(rows), SK_DTYPE)
+        for i, row in enumerate(rows):
+            d[i] = row
+        return _to_dev(d, self.device)
+
+    def _operator(self, decs, N, Ds, Dt, nbase, dbase, Tm, src, dst, Z):
+        """Launcher of dst_b = A_b src_b for every block: Z_j = T_j (S src)_j in one skinny
+        product (T fused in its epilogue), dst = S^T Z in a second -- 4 N D flops, S read
+        twice, no Gram."""
+        B = len(decs)
+        E = Dt * Dt
+        fwd = self._skinny([(d.S.data_ptr(), src.data_ptr() + 8 * int(dbase[b]),
+                             Z.data_ptr() + 8 * int(nbase[b]) * Dt,
+                             Tm.data_ptr() + 8 * int(nbase[b]) * E,
+                             d.S.stride(0), 1, Dt, Dt, E, N[b], Ds[b]) for b, d in enumerate(decs)])
+        bwd = self._skinny([(d.S.data_ptr(), Z.data_ptr() + 8 * int(nbase[b]) * Dt,
+                             dst.data_ptr() + 8 * int(dbase[b]), 0,
+                             1, d.S.stride(0), Dt, Dt, 0, Ds[b], N[b]) for b, d in enumerate(decs)])
+        maxN, maxDs = int(N.max()), int(Ds.max())
+
+        def run(s):
+            lib.vt_gemm_skinny(ptr(fwd), B, maxN, Dt, s)
+            lib.vt_gemm_skinny(ptr(bwd), B, maxDs, Dt, s)
+
+        return run, (fwd, bwd)
+
+    def _spatial_inverse(self, decs, N, Ds, nbase, c, lam, s):
+        """Mi_b = (S_b^T diag(c_b) S_b + lam_b I)^-1 for every block, FP64: Cholesky, then
+        L^-1 by blocked forward substitution (64-row block steps, two grouped GEMMs each,
+        diagonal blocks from the factor's block inverses), then Mi = L^-T L^-1."""
+        dev = self.device
+        B = len(decs)
+        maxDs = int(Ds.max())
+        mbase = np.concatenate([[0], np.cumsum(Ds * Ds)])
+        Mpool = torch.empty(int(mbase[-1]), dtype=F64, device=dev)
+        kd = np.zeros(B, KPM_DTYPE)
+        dd = np.zeros(B, DIAG_DTYPE)
+        mat_items = []
         for b, d in enumerate(decs):
-            y2p = Y2.data_ptr() + 8 * int(y2base[b])
-            gs1[b] = (d.theta_out.data_ptr(), X.data_ptr() + 8 * int(dbase[b]), y2p, Dt, 1, 1, Dt, Ds[b],
-                      T_out[b], Ds[b], Dt, 0, 1.0, 0.0)
-            gs2[b] = (y2p, d.Bs.data_ptr(), frames[b].data_ptr(), Ds[b], 1, 1, Ds[b], P[b],
-                      T_out[b], P[b], Ds[b], 0, 1.0, 0.0)
-        gs1_d, gs2_d = _to_dev(gs1, dev), _to_dev(gs2, dev)
-        lib.vt_gemm_grouped(ptr(gs1_d), B, int(T_out.max()), int(Ds.max()), s)
-        lib.vt_gemm_grouped(ptr(gs2_d), B, int(T_out.max()), int(P.max()), s)
-        # every buffer referenced by enqueued work stays alive with the handle
-        keep = (N_d, sd_d, ws, offsets, tk1, tk2, nid, base_d, q, Tm, tv, tn_ws, tn_r_d, s2off,
-                extra_keep, Apool, inv, mats, sync, Xp, seg, elem_job, pad_idx,
-                      Y, Z, AX, res, g1_d, tn_a_d, resp, Y2, gs1_d, gs2_d, kappa, delta, bias, norm,
-                      s2, job_of, r)
-        results = []
+            mp = Mpool.data_ptr() + 8 * int(mbase[b])
+            kd[b] = (d.S.data_ptr(), c.data_ptr() + 8 * int(nbase[b]), mp, d.S.stride(0), Ds[b], N[b],
+                     Ds[b])
+            dd[b] = (mp, Ds[b], Ds[b], 0)
+            mat_items.append((mp, int(Ds[b]), int(Ds[b])))
+        kd_d, dd_d = _to_dev(kd, dev), _to_dev(dd, dev)
+        lib.vt_kpcg_mgram(ptr(kd_d), B, maxDs, s)
+        lib.vt_diag_add_batched(ptr(dd_d), B, 0, ptr(lam), maxDs, s)
+        nblk = (maxDs + 63) // 64
+        inv = torch.empty(B * nblk * 4096, dtype=F64, device=dev)
+        info = torch.zeros(B, dtype=I32, device=dev)
+        mats = mat_descs(mat_items, dev)
+        lib.vt_potrf_f64(ptr(mats), B, maxDs, ptr(inv), nblk * 4096, ptr(info), s)
+        # L^-1 (lower triangular, zero above the diagonal)
+        Lpool = torch.zeros(int(mbase[-1]), dtype=F64, device=dev)
+        xoff = torch.from_numpy(mbase[:-1].astype(np.int64)).to(dev)
+        dim = torch.from_numpy(Ds.astype(np.int32)).to(dev)
+        lib.vt_kpcg_place_diag(ptr(Lpool), ptr(xoff), ptr(dim), B, maxDs, ptr(inv), nblk * 4096, s)
+        Tb = torch.empty(B * 64 * maxDs, dtype=F64, device=dev)
+        keep = [Mpool, kd_d, dd_d, inv, mats, Lpool, xoff, dim, Tb]
+        for i in range(1, nblk):
+            g1 = np.zeros(B, GEMM_DTYPE)
+            g2 = np.zeros(B, GEMM_DTYPE)
+            for b in range(B):
+                D = int(Ds[b])
+                rows = max(0, min(64, D - 64 * i))
+                Mb = Mpool.data_ptr() + 8 * int(mbase[b])
+                Lb = Lpool.data_ptr() + 8 * int(mbase[b])
+                Tp = Tb.data_ptr() + 8 * b * 64 * maxDs
+                # T = L[i, :i] Linv[:i, :i];  Linv[i, :i] = -Dinv_i T
+                g1[b] = (Mb + 8 * 64 * i * D, Lb, Tp, D, 1, D, 1, maxDs, rows, 64 * i, 64 * i, 0, 1.0, 0.0)
+                # K = rows: T's rows beyond the block's own were never written
+                g2[b] = (inv.data_ptr() + 8 * (b * nblk + i) * 4096, Tp, Lb + 8 * 64 * i * D, 64, 1, maxDs,
+                         1, D, rows, 64 * i, rows, 0, -1.0, 0.0)
+            g1_d, g2_d = _to_dev(g1, dev), _to_dev(g2, dev)
+            lib.vt_gemm_grouped(ptr(g1_d), B, 64, 64 * i, s)
+            lib.vt_gemm_grouped(ptr(g2_d), B, 64, 64 * i, s)
+            keep += [g1_d, g2_d]
+        # Mi = Linv^T Linv (into Mpool: the factor is no longer needed)
+        g3 = np.zeros(B, GEMM_DTYPE)
         for b in range(B):
-            Xb = X[int(dbase[b]):int(dbase[b + 1])]
-            results.append(DecodeResult(frames[b], Xb, lam[2 * b:2 * b + 2], n_b[b:b + 1],
-                                        info[b:b + 1], nrm[b], False))
-        return results, keep, (nrm, info)
+            D = int(Ds[b])
+            Lb = Lpool.data_ptr() + 8 * int(mbase[b])
+            g3[b] = (Lb, Lb, Mpool.data_ptr() + 8 * int(mbase[b]), 1, D, D, 1, D, D, D, D, 0, 1.0, 0.0)
+        g3_d = _to_dev(g3, dev)
+        lib.vt_gemm_grouped(ptr(g3_d), B, maxDs, maxDs, s)
+        keep.append(g3_d)
+        return Mpool, mbase, info, keep
+
+    def _solve_kpcg(self, prec, decs, N, Ds, D, Dt, nbase, dbase, Ntot, Dtot, Tm, r, lam, tn_ws,
+                    ws_base, seg, elem_job):
+        """Kronecker-preconditioned CG on (A + lam I) X = r for every block of the group.
+
+        Setup: c_j = tr(T_j)/Dt, Tbar (trace Dt) and its inverse Ti; Mi = (S^T diag(c) S +
+        lam I)^-1 in FP64.  Each iteration applies A through its structure (two skinny
+        products) and the preconditioner as (Mi R) Ti (one skinny product with a shared
+        right factor).  A block stops once its recursive residual falls below cg_stop
+        relative; the guard then checks the true FP64 residual."""
+        dev = self.device
+        s = stream_ptr()
+        B = len(decs)
+        maxDs = int(Ds.max())
+        E = Dt * Dt
+        e_setup = self._tic()
+        nbase_d = torch.from_numpy(nbase.astype(np.int32)).to(dev)
+        c = torch.empty(Ntot, dtype=F64, device=dev)
+        TiT = torch.empty(B * E, dtype=F64, device=dev)
+        tws = torch.empty(16 * B * E, dtype=F64, device=dev)
+        lib.vt_kpcg_tbar(ptr(Tm), ptr(nbase_d), Dt, B, ptr(tws), ptr(c), ptr(TiT), s)
+        Mi, mbase, info, mkeep = self._spatial_inverse(decs, N, Ds, nbase, c, lam, s)
+        R = r.clone()
+        Zp = torch.empty(Dtot, dtype=F64, device=dev)
+        pre = self._skinny([(Mi.data_ptr() + 8 * int(mbase[b]), R.data_ptr() + 8 * int(dbase[b]),
+                             Zp.data_ptr() + 8 * int(dbase[b]), TiT.data_ptr() + 8 * b * E,
+                             Ds[b], 1, Dt, Dt, 0, Ds[b], Ds[b]) for b in range(B)])
+
+        def precond():
+            lib.vt_gemm_skinny(ptr(pre), B, maxDs, Dt, s)
+
+        self._toc("kpcg_setup", e_setup, float(np.sum(2.0 * N * Ds * Ds + 2.0 * Ds.astype(np.float64) ** 3)))
+        e_cg = self._tic()
+        X = torch.empty(Dtot, dtype=F64, device=dev)
+        Pv = torch.empty(Dtot, dtype=F64, device=dev)
+        Q = torch.empty(Dtot, dtype=F64, device=dev)
+        Z = torch.empty((Ntot, Dt), dtype=F64, device=dev)
+        op_p, kp1 = self._operator(decs, N, Ds, Dt, nbase, dbase, Tm, Pv, Q, Z)
+        op_x, kp2 = self._operator(decs, N, Ds, Dt, nbase, dbase, Tm, X, Q, Z)
+        iters = int(self.cg_iters)
+        hs = iters + 1
+        st = torch.zeros(4 * B, dtype=F64, device=dev)
+        hist = torch.zeros((B, hs), dtype=F64, device=dev)
+        precond()
+        lib.vt_kpcg_init(ptr(R), ptr(Zp), ptr(Pv), ptr(X), ptr(seg), B, ptr(st), ptr(hist), hs, s)
+        stop2 = float(self.cg_stop) ** 2
+        for it in range(iters):
+            op_p(s)
+            lib.vt_kpcg_step_a(ptr(X), ptr(R), ptr(Pv), ptr(Q), ptr(lam), ptr(seg), B, ptr(st), ptr(hist),
+                               hs, it, stop2, s)
+            precond()
+            lib.vt_kpcg_step_b(ptr(R), ptr(Zp), ptr(Pv), ptr(seg), B, ptr(st), s)
+        # true FP64 residual of the final iterate (the guard's input)
+        nrm = torch.zeros((B, 2), dtype=F64, device=dev)
+        lib.vt_seg_sumsq(ptr(r), ptr(seg), B, ptr(nrm), 2, s)
+        op_x(s)
+        res = torch.empty(Dtot, dtype=F64, device=dev)
+        lib.vt_residual_batched(ptr(res), ptr(r), ptr(Q), ptr(X), ptr(lam), ptr(elem_job), Dtot, s)
+        lib.vt_seg_sumsq(ptr(res), ptr(seg), B, ptr(nrm[:, 1:]), 2, s)
+        # algorithmic HBM bytes per iteration: S twice, T_j once, Z written and read, Mi
+        # once, about a dozen passes over the D-space vectors
+        nbytes = 8.0 * np.sum(2.0 * N * Ds + N * E + 2.0 * N * Dt + Ds * Ds + 12.0 * D)
+        self._toc("kpcg_iterate", e_cg, float(nbytes) * iters)
+        self.cg_state = (st, hist)
+        return X, nrm, info, (nbase_d, c, TiT, tws, Mi, mkeep, R, Zp, pre, X, Pv, Q, Z, kp1, kp2, st,
+                              hist, res)

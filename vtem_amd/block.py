@@ -197,6 +197,7 @@ class DecodeResult:
     info: Optional[torch.Tensor] = None
     resid: Optional[torch.Tensor] = None
     fallback: bool = False
+    state: Optional[torch.Tensor] = None   # iterative solver state (CG: rz, |r0|^2, active, iterations)
 
 
 class Decoder:

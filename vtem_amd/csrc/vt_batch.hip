@@ -372,3 +372,160 @@ VT_EXPORT int vt_seg_counts(const int *offsets, const int *base, int nseg, int *
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
+
+// ------------------------------------------------------------------------------------
+// Skinny grouped GEMM (fp64): problem b computes C_b (M x Nc) = A_b (M x K) B_b (K x Nc)
+// for Nc <= 32, with A_b(m, k) at A[m*sam + k*sak] (either orientation), B and C
+// row-major.  Optionally each output row m is then multiplied by the Nc x Nc matrix
+// T_b + m*tstride (C_m <- T_m C_m): with tstride = Nc*Nc this fuses Z_j = T_j (S P)_j
+// into the product; with tstride = 0 every row shares one matrix (right factor C T^T).
+//
+// One workgroup owns 64 rows; its 4 waves split K into 16-wide chunks (wave w takes
+// chunks w, w+4, ...).  Operands go from global memory straight into the MFMA f64
+// 16x16x4 operand layout: lane (r = l%16, g = l/16) loads the 4 consecutive k values
+// c*16 + 4g + s (s = 0..3) of its row, and MFMA step s pairs them with the B rows of
+// the same k, so each step sums k in {c*16 + 4g + s} and the 4 steps cover the chunk.
+// The wave partials are added in LDS in a fixed order ((w0 + w2) + (w1 + w3)).
+// ------------------------------------------------------------------------------------
+struct SkDesc {
+    const double *A, *B;
+    double *C;
+    const double *T;
+    long long sam, sak, ldb, ldc, tstride;
+    int M, K;
+};
+
+// operands of one 16-wide K chunk for lane (r, g)
+template <int NT>
+__device__ __forceinline__ void skinny_load(const SkDesc &d, int m0, int c, int r, int g, int Nc,
+                                            double (&a)[4][4], double (&bv)[NT][4])
+{
+    const int kb = c * 16 + 4 * g;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int m = m0 + 16 * t + r;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = kb + s;
+            a[t][s] = (m < d.M && k < d.K) ? d.A[(long long)m * d.sam + (long long)k * d.sak] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int n = 16 * j + r;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = kb + s;
+            bv[j][s] = (n < Nc && k < d.K) ? d.B[(long long)k * d.ldb + n] : 0.0;
+        }
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void skinny_mma(v4d (&acc)[4][NT], const double (&a)[4][4],
+                                           const double (&bv)[NT][4])
+{
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                acc[t][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][s], bv[j][s], acc[t][j], 0, 0, 0);
+}
+
+template <int NT>
+__global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ descs, int Nc)
+{
+    __shared__ double red[2][64][33];
+    const SkDesc d = descs[blockIdx.y];
+    const int m0 = blockIdx.x * 64;
+    if (m0 >= d.M) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    v4d acc[4][NT];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[t][j] = (v4d){0, 0, 0, 0};
+    const int nch = (d.K + 15) / 16;
+    // two chunks in flight per wave: chunks c and c + 4 (out-of-range chunks load zeros)
+    for (int c = wave; c < nch; c += 8) {
+        double a0[4][4], b0[NT][4], a1[4][4], b1[NT][4];
+        skinny_load<NT>(d, m0, c, r, g, Nc, a0, b0);
+        skinny_load<NT>(d, m0, c + 4, r, g, Nc, a1, b1);
+        skinny_mma<NT>(acc, a0, b0);
+        skinny_mma<NT>(acc, a1, b1);
+    }
+    // cross-wave reduction: (w0 + w2) + (w1 + w3)
+    if (wave >= 2) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    red[wave - 2][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)] = acc[t][j][i];
+    }
+    __syncthreads();
+    if (wave < 2) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    acc[t][j][i] += red[wave][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)];
+    }
+    __syncthreads();
+    if (wave == 1) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    red[0][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)] = acc[t][j][i];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = 16 * t + mfma_row_f64(lane, i), col = 16 * j + mfma_col(lane);
+                    red[1][row][col] = acc[t][j][i] + red[0][row][col];
+                }
+    }
+    __syncthreads();
+    // epilogue: C rows (optionally T_m C_m)
+    const int rows = min(64, d.M - m0);
+    for (int e = tid; e < rows * Nc; e += 256) {
+        const int row = e / Nc, col = e % Nc;
+        double v;
+        if (d.T) {
+            const double *Tr = d.T + (long long)(m0 + row) * d.tstride + (long long)col * Nc;
+            v = 0.0;
+            for (int k = 0; k < Nc; ++k) v += Tr[k] * red[1][row][k];
+        } else {
+            v = red[1][row][col];
+        }
+        d.C[(long long)(m0 + row) * d.ldc + col] = v;
+    }
+}
+
+VT_EXPORT int vt_gemm_skinny(const void *descs, int batch, int max_M, int Nc, void *stream)
+{
+    VT_CHECK_ARG(descs && batch >= 1 && batch <= 65535 && max_M >= 0 && Nc >= 1 && Nc <= 32);
+    if (max_M == 0) return VT_OK;
+    dim3 grid(vt_blocks(max_M, 64), batch);
+    hipStream_t s = (hipStream_t)stream;
+    if (Nc <= 16)
+        skinny_kernel<1><<<grid, 256, 0, s>>>((const SkDesc *)descs, Nc);
+    else
+        skinny_kernel<2><<<grid, 256, 0, s>>>((const SkDesc *)descs, Nc);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}

@@ -1,0 +1,342 @@
+// Kronecker-preconditioned conjugate gradients for the coefficient-space Tikhonov system
+//
+//     (A + lam I) X = r,     A = sum_j S_j S_j^T (x) T_j        (D = Ds * Dt unknowns)
+//
+// solved for a batch of independent blocks without forming A.  The operator is applied
+// through its structure (Y = S P, Z_j = T_j Y_j, Q = S^T Z: 4 N D flops, S read twice);
+// the preconditioner is the nearest separable operator
+//
+//     P = (S^T C S + lam I) (x) Tbar,   c_j = tr(T_j)/Dt,   Tbar = sum_j T_j / sum_j c_j,
+//
+// whose inverse is Mi (x) Ti with two small explicit inverses (Ds x Ds, Dt x Dt).  The
+// spatial part is inverted exactly, so the spatial ill-conditioning of A (modes weakly
+// covered by the receptive fields) does not slow the iteration; what remains is the
+// spread of each neuron's temporal moments about c_j Tbar, set by how much the interval
+// lengths vary within a neuron.
+//
+// Every reduction runs inside one workgroup per block in a fixed order, so a block's
+// iterates do not depend on the other blocks of the batch.
+//
+// The reference solves the spike-space system with a truncated SVD
+// (vtem/utils/linalg.py:591-631) and has no iterative Tikhonov solver.
+#include "vt_common.hpp"
+
+#define KP_CHUNKS 16
+#define KP_MAX_DT 33
+
+// ------------------------------------------------------------------------------------
+// Tbar, Ti and c_j.  Pass 1: grid (KP_CHUNKS, batch); each workgroup sums T_j over its
+// share of the block's neurons into ws[(b*KP_CHUNKS + g)*Dt^2 + e] and writes c_j.
+// ------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+kpcg_tbar_partial(const double *__restrict__ Tm, const int *__restrict__ nbase, int Dt,
+                  double *__restrict__ ws, double *__restrict__ c)
+{
+    const int b = blockIdx.y, g = blockIdx.x;
+    const int n0 = nbase[b], N = nbase[b + 1] - n0;
+    const int E = Dt * Dt;
+    const int chunk = (N + KP_CHUNKS - 1) / KP_CHUNKS;
+    const int j0 = g * chunk, j1 = min(N, j0 + chunk);
+    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+        double s = 0.0;
+        for (int j = j0; j < j1; ++j) s += Tm[(long long)(n0 + j) * E + e];
+        ws[((long long)b * KP_CHUNKS + g) * E + e] = s;
+    }
+    for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+        const double *T = Tm + (long long)(n0 + j) * E;
+        double t = 0.0;
+        for (int a = 0; a < Dt; ++a) t += T[a * Dt + a];
+        c[n0 + j] = t / Dt;
+    }
+}
+
+// Pass 2: one workgroup per block sums the partials (fixed order), scales to trace Dt and
+// inverts Tbar by Gauss-Jordan elimination in LDS (Tbar is SPD and well conditioned: it is
+// the interval-averaged temporal moment of the whole window).
+__global__ void __launch_bounds__(256)
+kpcg_tbar_final(const double *__restrict__ ws, int Dt, double *__restrict__ Ti)
+{
+    __shared__ double G[KP_MAX_DT][2 * KP_MAX_DT + 1];
+    __shared__ double piv[KP_MAX_DT];
+    __shared__ double tr;
+    const int b = blockIdx.x, E = Dt * Dt, W = 2 * Dt;
+    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+        double s = 0.0;
+        for (int g = 0; g < KP_CHUNKS; ++g) s += ws[((long long)b * KP_CHUNKS + g) * E + e];
+        G[e / Dt][e % Dt] = s;
+        G[e / Dt][Dt + e % Dt] = (e / Dt == e % Dt) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int a = 0; a < Dt; ++a) t += G[a][a];
+        tr = t;
+    }
+    __syncthreads();
+    // Tbar = Tsum * Dt / tr(Tsum)   (sum_j c_j = tr(Tsum)/Dt)
+    const double scale = Dt / tr;
+    for (int e = threadIdx.x; e < E; e += blockDim.x) G[e / Dt][e % Dt] *= scale;
+    __syncthreads();
+    for (int k = 0; k < Dt; ++k) {
+        const double inv = 1.0 / G[k][k];
+        __syncthreads();
+        for (int col = threadIdx.x; col < W; col += blockDim.x) G[k][col] *= inv;
+        __syncthreads();
+        if (threadIdx.x < Dt) piv[threadIdx.x] = G[threadIdx.x][k];
+        __syncthreads();
+        for (int e = threadIdx.x; e < Dt * W; e += blockDim.x) {
+            const int i = e / W, col = e % W;
+            if (i != k) G[i][col] -= piv[i] * G[k][col];
+        }
+        __syncthreads();
+    }
+    // transposed, so that a row vector y gives (y Ti)[c] = sum_k TiT[c][k] y[k]
+    for (int e = threadIdx.x; e < E; e += blockDim.x)
+        Ti[(long long)b * E + e] = G[e % Dt][Dt + e / Dt];
+}
+
+VT_EXPORT int vt_kpcg_tbar(const double *Tm, const int *nbase, int Dt, int batch, double *ws,
+                           double *c, double *Ti, void *stream)
+{
+    VT_CHECK_ARG(Tm && nbase && ws && c && Ti && batch >= 1 && batch <= 65535 && Dt >= 1);
+    if (Dt > KP_MAX_DT) return VT_ERANGE;
+    hipStream_t s = (hipStream_t)stream;
+    kpcg_tbar_partial<<<dim3(KP_CHUNKS, batch), 256, 0, s>>>(Tm, nbase, Dt, ws, c);
+    kpcg_tbar_final<<<batch, 256, 0, s>>>(ws, Dt, Ti);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Spatial preconditioner Gram  M_b = S_b^T diag(c_b) S_b  (Ds x Ds, K = N_b), FP64 MFMA
+// 16x16x4 on 64x64 tiles; grid (Ds tiles, Ds tiles, batch).
+// ------------------------------------------------------------------------------------
+struct KpMDesc {
+    const double *S;
+    const double *c;
+    double *M;
+    long long lds, ldm;
+    int N, Ds;
+};
+
+__global__ void __launch_bounds__(256) kpcg_mgram_kernel(const KpMDesc *__restrict__ descs)
+{
+    __shared__ double As[16][68];
+    __shared__ double Bs[16][68];
+    const KpMDesc d = descs[blockIdx.z];
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    if (m0 >= d.Ds || n0 >= d.Ds) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    v4d acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = (v4d){0, 0, 0, 0};
+    for (int k0 = 0; k0 < d.N; k0 += 16) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int e = tid + i * 256;
+            const int col = e & 63, k = e >> 6;
+            const int gk = k0 + k;
+            const bool kin = gk < d.N;
+            const double *row = d.S + (long long)gk * d.lds;
+            As[k][col] = (kin && m0 + col < d.Ds) ? row[m0 + col] : 0.0;
+            Bs[k][col] = (kin && n0 + col < d.Ds) ? d.c[gk] * row[n0 + col] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            double a0 = As[kr][wm + (lane & 15)], a1 = As[kr][wm + 16 + (lane & 15)];
+            double b0 = Bs[kr][wn + (lane & 15)], b1 = Bs[kr][wn + 16 + (lane & 15)];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm + i * 16 + mfma_row_f64(lane, r);
+                const int gn = n0 + wn + j * 16 + mfma_col(lane);
+                if (gm < d.Ds && gn < d.Ds) d.M[(long long)gm * d.ldm + gn] = acc[i][j][r];
+            }
+}
+
+VT_EXPORT int vt_kpcg_mgram(const void *descs, int batch, int max_Ds, void *stream)
+{
+    VT_CHECK_ARG(descs && batch >= 1 && batch <= 65535 && max_Ds >= 1);
+    const unsigned t = vt_blocks(max_Ds, 64);
+    kpcg_mgram_kernel<<<dim3(t, t, batch), 256, 0, (hipStream_t)stream>>>((const KpMDesc *)descs);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// Diagonal blocks of L^-1: X_b[64i.., 64i..] = Linv_ii (the 64x64 block inverses the
+// Cholesky left at inv_ws + b*inv_stride + i*4096).  grid (blocks of the largest
+// factor, batch).  X_b starts at X + xoff[b] with leading dimension dim[b].
+__global__ void __launch_bounds__(256)
+kpcg_place_diag_kernel(double *__restrict__ X, const long long *__restrict__ xoff,
+                       const int *__restrict__ dim, const double *__restrict__ inv_ws,
+                       long long inv_stride)
+{
+    const int b = blockIdx.y, i = blockIdx.x, D = dim[b];
+    if (i * 64 >= D) return;
+    const int nb = min(64, D - i * 64);
+    const double *W = inv_ws + (long long)b * inv_stride + (long long)i * 4096;
+    double *Xb = X + xoff[b] + (long long)i * 64 * D + i * 64;
+    for (int e = threadIdx.x; e < 4096; e += 256) {
+        const int rr = e >> 6, cc = e & 63;
+        if (rr < nb && cc < nb) Xb[(long long)rr * D + cc] = W[e];
+    }
+}
+
+VT_EXPORT int vt_kpcg_place_diag(double *X, const long long *xoff, const int *dim, int batch,
+                                 int max_dim, const double *inv_ws, long long inv_stride,
+                                 void *stream)
+{
+    VT_CHECK_ARG(X && xoff && dim && inv_ws && batch >= 1 && batch <= 65535 && max_dim >= 1);
+    kpcg_place_diag_kernel<<<dim3(vt_blocks(max_dim, 64), batch), 256, 0, (hipStream_t)stream>>>(
+        X, xoff, dim, inv_ws, inv_stride);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// CG recurrences over concatenated per-block vectors (block b owns [seg[b], seg[b+1])).
+// State per block (st + 4b): rz, rr0, active (0/1), iterations done.  hist[b*hs + k]
+// holds |r_k|^2 of the recursively updated residual.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double s = red[0];
+    __syncthreads();
+    return s;
+}
+
+// x = 0, p = z, rz = <r,z>, rr0 = <r,r>
+__global__ void __launch_bounds__(256)
+kpcg_init_kernel(const double *__restrict__ r, const double *__restrict__ z, double *__restrict__ p,
+                 double *__restrict__ x, const long long *__restrict__ seg, double *__restrict__ st,
+                 double *__restrict__ hist, int hs)
+{
+    __shared__ double red[256];
+    const long long a = seg[blockIdx.x], e = seg[blockIdx.x + 1];
+    double rz = 0.0, rr = 0.0;
+    for (long long i = a + threadIdx.x; i < e; i += 256) {
+        rz += r[i] * z[i];
+        rr += r[i] * r[i];
+        p[i] = z[i];
+        x[i] = 0.0;
+    }
+    rz = block_sum(rz, red);
+    rr = block_sum(rr, red);
+    if (threadIdx.x == 0) {
+        double *s = st + 4 * blockIdx.x;
+        s[0] = rz;
+        s[1] = rr;
+        s[2] = (rr > 0.0 && rz > 0.0) ? 1.0 : 0.0;
+        s[3] = 0.0;
+        hist[(long long)blockIdx.x * hs] = rr;
+    }
+}
+
+VT_EXPORT int vt_kpcg_init(const double *r, const double *z, double *p, double *x,
+                           const long long *seg, int batch, double *st, double *hist, int hist_stride,
+                           void *stream)
+{
+    VT_CHECK_ARG(r && z && p && x && seg && st && hist && batch >= 1 && hist_stride >= 1);
+    kpcg_init_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(r, z, p, x, seg, st, hist, hist_stride);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// q <- q + lam p; alpha = rz / <p, q>; x += alpha p; r -= alpha q; record |r|^2 and stop
+// the block once |r|^2 <= stop_rel2 * rr0.
+__global__ void __launch_bounds__(256)
+kpcg_step_a_kernel(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+                   double *__restrict__ q, const double *__restrict__ lam,
+                   const long long *__restrict__ seg, double *__restrict__ st,
+                   double *__restrict__ hist, int hs, int it, double stop_rel2)
+{
+    __shared__ double red[256];
+    double *s = st + 4 * blockIdx.x;
+    if (s[2] == 0.0) return;
+    const double l = lam[2 * blockIdx.x];
+    const long long a = seg[blockIdx.x], e = seg[blockIdx.x + 1];
+    double pq = 0.0;
+    for (long long i = a + threadIdx.x; i < e; i += 256) {
+        const double qi = q[i] + l * p[i];
+        q[i] = qi;
+        pq += p[i] * qi;
+    }
+    pq = block_sum(pq, red);
+    const double rz = s[0];
+    if (!(pq > 0.0)) {
+        if (threadIdx.x == 0) s[2] = 0.0;
+        return;
+    }
+    const double alpha = rz / pq;
+    double rr = 0.0;
+    for (long long i = a + threadIdx.x; i < e; i += 256) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        rr += ri * ri;
+    }
+    rr = block_sum(rr, red);
+    if (threadIdx.x == 0) {
+        hist[(long long)blockIdx.x * hs + it + 1] = rr;
+        s[3] = it + 1;
+        if (rr <= stop_rel2 * s[1]) s[2] = 0.0;
+    }
+}
+
+VT_EXPORT int vt_kpcg_step_a(double *x, double *r, const double *p, double *q, const double *lam,
+                             const long long *seg, int batch, double *st, double *hist,
+                             int hist_stride, int it, double stop_rel2, void *stream)
+{
+    VT_CHECK_ARG(x && r && p && q && lam && seg && st && hist && batch >= 1 && it >= 0 &&
+                 it + 1 < hist_stride);
+    kpcg_step_a_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(x, r, p, q, lam, seg, st, hist,
+                                                               hist_stride, it, stop_rel2);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// rz' = <r, z>; beta = rz'/rz; p = z + beta p
+__global__ void __launch_bounds__(256)
+kpcg_step_b_kernel(const double *__restrict__ r, const double *__restrict__ z, double *__restrict__ p,
+                   const long long *__restrict__ seg, double *__restrict__ st)
+{
+    __shared__ double red[256];
+    double *s = st + 4 * blockIdx.x;
+    if (s[2] == 0.0) return;
+    const long long a = seg[blockIdx.x], e = seg[blockIdx.x + 1];
+    double rz = 0.0;
+    for (long long i = a + threadIdx.x; i < e; i += 256) rz += r[i] * z[i];
+    rz = block_sum(rz, red);
+    const double beta = rz / s[0];
+    for (long long i = a + threadIdx.x; i < e; i += 256) p[i] = z[i] + beta * p[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s[0] = rz;
+        if (!(rz > 0.0)) s[2] = 0.0;
+    }
+}
+
+VT_EXPORT int vt_kpcg_step_b(const double *r, const double *z, double *p, const long long *seg,
+                             int batch, double *st, void *stream)
+{
+    VT_CHECK_ARG(r && z && p && seg && st && batch >= 1);
+    kpcg_step_b_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(r, z, p, seg, st);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
