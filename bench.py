@@ -44,6 +44,9 @@ if ROOT not in sys.path:
 PEAKS = {"gram_lp": (2500.0, "TFLOP/s", "mfma"), "gram_f64": (78.6, "TFLOP/s", "mfma"),
          "potrf_f32": (157.3, "TFLOP/s", "mfma"), "potrf_f64": (78.6, "TFLOP/s", "mfma"),
          "solve_refine": (157.3, "TFLOP/s", "mfma"), "kpcg_setup": (78.6, "TFLOP/s", "mfma"),
+         "kpcg_mgram": (78.6, "TFLOP/s", "mfma"), "kpcg_potrf": (78.6, "TFLOP/s", "mfma"),
+         "kpcg_inverse": (78.6, "TFLOP/s", "mfma"), "prologue": (8000.0, "GB/s", "hbm"),
+         "synthesis": (8000.0, "GB/s", "hbm"),
          "kpcg_iterate": (8000.0, "GB/s", "hbm")}
 WORK_NOTES = {"kpcg_iterate": "algorithmic HBM bytes per CG iteration: 8*(2*N*Ds + N*Dt^2 + 4*N*Dt "
                               "+ Ds^2 + 12*D) per block (S twice, T_j, Y/Z, Mi, vectors)",

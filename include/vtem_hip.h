@@ -259,13 +259,19 @@ int vt_seg_sumsq(const double *x, const long long *seg, int nseg, double *out, i
 /* A_b += lam[2b] I; descs = {void *A; long long lda; int D, pad;} (24 bytes). */
 int vt_diag_add_batched(const void *descs, int batch, int dtype, const double *lam, int max_D,
                         void *stream);
-/* Skinny grouped product, Nc <= 32: C_b (M x Nc) = A_b B_b with A_b(m,k) at
- * A[m*sam + k*sak], B and C row-major; if T != 0 each row is then multiplied by
- * T + m*tstride (C_m <- T_m C_m).  descs = {const double *A, *B; double *C; const double *T;
- * long long sam, sak, ldb, ldc, tstride; int M, K;} (80 bytes).  The structured operator
- * S^T (T_j (S X)) of the coefficient-space Gram (vtem/dirichlet.py:62-85 builds that Gram
- * in spike space) is two such products. */
-int vt_gemm_skinny(const void *descs, int batch, int max_M, int Nc, void *stream);
+/* Skinny grouped product: C_b (M x Nc) = A_b B_b, Nc <= max_Nc <= 32 per problem, with
+ * A_b(m,k) at A[m*sam + k*sak], B and C row-major; if T != 0 each row is then multiplied
+ * by T + m*tstride (C_m <- T_m C_m); a problem whose `active` flag (optional double) reads
+ * 0 is skipped.  descs = {const double *A, *B; double *C; const double *T; double *ws;
+ * const double *active; long long sam, sak, ldb, ldc, tstride; int M, K, Nc, pad;} (104
+ * bytes).  A problem with K > 1008 splits its reduction (in at most 8 parts, a function of
+ * its own K) and needs the ws that vt_gemm_skinny_workspace_size reports; max_K bounds
+ * every problem's K.  The structured operator S^T (T_j (S X)) of the coefficient-space
+ * Gram (vtem/dirichlet.py:62-85 builds that Gram in spike space) is two such products. */
+int vt_gemm_skinny(const void *descs, int batch, int max_M, int max_K, int max_Nc, void *stream);
+/* Doubles of ws one skinny problem (M x K, Nc columns) needs, written to the HOST pointer
+ * doubles_host (0 when its reduction is not split). */
+int vt_gemm_skinny_workspace_size(int M, int K, int Nc, long long *doubles_host);
 /* x += y. */
 int vt_add_inplace(double *x, const double *y, long long n, void *stream);
 /* out[b] = offsets[base[b+1]] - offsets[base[b]] (intervals per block). */

@@ -71,11 +71,12 @@ def test_gemm_skinny(torch_mod, tmode):
     optional row factor, against float64 torch."""
     torch = torch_mod
     from vtem_amd._lib import lib, ptr, stream_ptr
-    from vtem_amd.batch import SK_DTYPE, _to_dev
+    from vtem_amd.batch import SK_DTYPE, _to_dev, skinny_workspace
     _banner(f"skinny {tmode}")
     g = torch.Generator().manual_seed(7)
     for Nc in (5, 17, 32):
-        probs = [(300, 625, False), (64, 81, True), (1, 5, False), (2361, 625, True), (70, 1000, False)]
+        probs = [(300, 625, False), (64, 81, True), (1, 5, False), (2361, 625, True), (70, 1000, False),
+                 (625, 2361, True), (17, 5000, False), (100, 1009, True)]
         rows, refs, keep = [], [], []
         for M, K, trans in probs:
             A = torch.randn(K, M, generator=g, dtype=torch.float64) if trans else \
@@ -92,19 +93,48 @@ def test_gemm_skinny(torch_mod, tmode):
             Td = None if Tn is None else Tn.contiguous().cuda()
             sam, sak = (1, M) if trans else (K, 1)
             tstride = {"none": 0, "row": Nc * Nc, "shared": 0}[tmode]
+            wn = skinny_workspace(M, K, Nc)
+            Wd = torch.empty(max(1, wn), dtype=torch.float64, device="cuda")
             rows.append((Ad.data_ptr(), Bd.data_ptr(), Cd.data_ptr(), 0 if Td is None else Td.data_ptr(),
-                         sam, sak, Nc, Nc, tstride, M, K))
+                         Wd.data_ptr() if wn else 0, 0, sam, sak, Nc, Nc, tstride, M, K, Nc, 0))
+            keep.append(Wd)
             refs.append((ref, Cd))
             keep += [Ad, Bd, Cd, Td]
         d = np.zeros(len(rows), SK_DTYPE)
         for i, r in enumerate(rows):
             d[i] = r
         dd = _to_dev(d, "cuda")
-        lib.vt_gemm_skinny(ptr(dd), len(rows), max(p[0] for p in probs), Nc, stream_ptr())
+        lib.vt_gemm_skinny(ptr(dd), len(rows), max(p[0] for p in probs), max(p[1] for p in probs), Nc,
+                           stream_ptr())
         torch.cuda.synchronize()
         for (ref, Cd), (M, K, tr) in zip(refs, probs):
             err = (Cd.cpu() - ref).abs().max().item() / max(1.0, ref.abs().max().item())
             assert err < 1e-13, (Nc, M, K, tr, err)
+    # one launch with per-problem column counts; a problem whose active flag is 0 is skipped
+    flags = torch.tensor([1.0, 0.0, 1.0], dtype=torch.float64, device="cuda")
+    rows, refs, keep = [], [], []
+    for i, (M, K, Nc) in enumerate([(500, 1500, 5), (40, 300, 17), (130, 2000, 32)]):
+        A = torch.randn(M, K, generator=g, dtype=torch.float64)
+        Bm = torch.randn(K, Nc, generator=g, dtype=torch.float64)
+        Ad, Bd = A.cuda(), Bm.cuda()
+        Cd = torch.full((M, Nc), float("nan"), dtype=torch.float64, device="cuda")
+        wn = skinny_workspace(M, K, Nc)
+        Wd = torch.empty(max(1, wn), dtype=torch.float64, device="cuda")
+        rows.append((Ad.data_ptr(), Bd.data_ptr(), Cd.data_ptr(), 0, Wd.data_ptr() if wn else 0,
+                     flags.data_ptr() + 8 * i, K, 1, Nc, Nc, 0, M, K, Nc, 0))
+        refs.append((A @ Bm, Cd))
+        keep += [Ad, Bd, Cd, Wd]
+    d = np.zeros(len(rows), SK_DTYPE)
+    for i, r in enumerate(rows):
+        d[i] = r
+    dd = _to_dev(d, "cuda")
+    lib.vt_gemm_skinny(ptr(dd), 3, 500, 2000, 32, stream_ptr())
+    torch.cuda.synchronize()
+    for i, (ref, Cd) in enumerate(refs):
+        if i == 1:
+            assert torch.isnan(Cd).all()
+        else:
+            assert (Cd.cpu() - ref).abs().max().item() < 1e-12 * max(1.0, ref.abs().max().item())
 
 
 @pytest.mark.parametrize("M,Nc,K", [(289, 17, 2048), (5, 1, 3), (40, 32, 999)])

@@ -68,3 +68,13 @@ def test_dyadic_bank_counts():
     assert b2.num_neurons == 8192 and int((b2.kind == 0).sum()) == 7168
     b0 = syn.bank_config0()
     assert b0.num_neurons == 256 and set(np.round(b0.h_l / (np.pi / 8)).astype(int)) == set(range(8))
+
+
+def test_skinny_workspace_query():
+    """The split-K workspace query (host-only entry): a reduction splits only beyond 1008
+    terms, in at most 8 parts, and the size scales with M and Nc."""
+    from vtem_amd.batch import skinny_workspace
+    assert skinny_workspace(100, 1008, 17) == 0
+    assert skinny_workspace(100, 1009, 17) == 2 * 100 * 17
+    assert skinny_workspace(625, 2361, 17) == 4 * 625 * 17
+    assert skinny_workspace(3, 10 ** 6, 5) == 8 * 3 * 5

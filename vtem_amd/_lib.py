@@ -80,7 +80,8 @@ SIGNATURES = {
     "vt_kpcg_tbar": [P, P, I, I, P, P, P, P],
     "vt_kpcg_mgram": [P, I, I, P],
     "vt_kpcg_place_diag": [P, P, P, I, I, P, L, P],
-    "vt_gemm_skinny": [P, I, I, I, P],
+    "vt_gemm_skinny": [P, I, I, I, I, P],
+    "vt_gemm_skinny_workspace_size": [I, I, I, P],
     "vt_kpcg_init": [P, P, P, P, P, I, P, P, I, P],
     "vt_kpcg_step_a": [P, P, P, P, P, P, I, P, P, I, I, D, P],
     "vt_kpcg_step_b": [P, P, P, P, I, P, P],

@@ -19,6 +19,7 @@ depends on which other blocks are in the batch.
 
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import math
 from typing import Any, Dict, List, Optional
@@ -52,17 +53,34 @@ GRAM64_DTYPE = np.dtype([("S", np.uint64), ("Tm", np.uint64), ("A", np.uint64), 
 KPM_DTYPE = np.dtype([("S", np.uint64), ("c", np.uint64), ("M", np.uint64), ("lds", np.int64),
                       ("ldm", np.int64), ("N", np.int32), ("Ds", np.int32)])
 SK_DTYPE = np.dtype([("A", np.uint64), ("B", np.uint64), ("C", np.uint64), ("T", np.uint64),
-                     ("sam", np.int64), ("sak", np.int64), ("ldb", np.int64), ("ldc", np.int64),
-                     ("tstride", np.int64), ("M", np.int32), ("K", np.int32)])
+                     ("ws", np.uint64), ("active", np.uint64), ("sam", np.int64), ("sak", np.int64),
+                     ("ldb", np.int64), ("ldc", np.int64), ("tstride", np.int64), ("M", np.int32),
+                     ("K", np.int32), ("Nc", np.int32), ("pad", np.int32)])
 
 assert SPIKE_DTYPE.itemsize == 48 and TJ_DTYPE.itemsize == 40 and GEMM_DTYPE.itemsize == 96
 assert TN_DTYPE.itemsize == 64 and DIAG_DTYPE.itemsize == 24
 assert GRAM_LP_DTYPE.itemsize == 72 and GRAM64_DTYPE.itemsize == 56 and KPM_DTYPE.itemsize == 48
-assert SK_DTYPE.itemsize == 80
+assert SK_DTYPE.itemsize == 104
 
 
 def _to_dev(arr: np.ndarray, device) -> torch.Tensor:
-    return torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).copy()).to(device)
+    """Descriptor bytes to the device: staged in pinned host memory and copied without
+    blocking the host (the caching host allocator keeps the staging block until the copy
+    has run)."""
+    host = torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).copy()).pin_memory()
+    return host.to(device, non_blocking=True)
+
+
+def _h2d(arr: np.ndarray, device) -> torch.Tensor:
+    """A small index/offset array to the device (pinned staging, non-blocking)."""
+    return torch.from_numpy(np.ascontiguousarray(arr)).pin_memory().to(device, non_blocking=True)
+
+
+def skinny_workspace(M: int, K: int, Nc: int) -> int:
+    """Doubles of split-K workspace one skinny problem needs (queried from the library)."""
+    out = ctypes.c_longlong(0)
+    lib.vt_gemm_skinny_workspace_size(int(M), int(K), int(Nc), ctypes.addressof(out))
+    return int(out.value)
 
 
 @dataclasses.dataclass
@@ -73,6 +91,33 @@ class BlockJob:
     spikes: SpikeTrains
     t0: float = 0.0
     tag: Any = None
+
+
+@dataclasses.dataclass
+class _Prologue:
+    """Per-group products of the prologue (intervals -> q -> T_j, t_j -> r -> lam) in the
+    group's global neuron / D-space numbering."""
+    jobs: list
+    decs: list
+    Dt: int
+    B: int
+    N: np.ndarray
+    Ds: np.ndarray
+    D: np.ndarray
+    nbase: np.ndarray
+    dbase: np.ndarray
+    Ntot: int
+    Dtot: int
+    Tm: torch.Tensor
+    r: torch.Tensor
+    lam: torch.Tensor
+    n_b: torch.Tensor
+    tn_ws: torch.Tensor
+    ws_base: np.ndarray
+    seg: torch.Tensor
+    elem_job: torch.Tensor
+    nbase_d: torch.Tensor
+    keep: tuple
 
 
 @dataclasses.dataclass
@@ -137,7 +182,8 @@ class BatchDecoder:
         return self.finalize(self.launch(jobs, precisions))
 
     def launch(self, jobs: List[BlockJob], precisions=None) -> "BatchHandle":
-        """Enqueue every stage of every group on the current stream; no host sync."""
+        """Enqueue every stage of every group on the current stream; no host sync.  The
+        blocks of all "kpcg" groups share one joint CG solve."""
         if precisions is None:
             precisions = [self.precision] * len(jobs)
         h = BatchHandle(jobs, [None] * len(jobs), [], [])
@@ -147,12 +193,31 @@ class BatchDecoder:
             if pr not in PRECISIONS:
                 raise ValueError(f"unknown precision {pr}")
             groups.setdefault((jb.decoder.geom.Dt, pr), []).append(i)
+        kp = []
         for (Dt, pr), idx in sorted(groups.items()):
-            res, keep, checks = self._decode_group([jobs[i] for i in idx], Dt, pr)
+            g = self._prologue([jobs[i] for i in idx], Dt)
+            if pr == "kpcg":
+                kp.append((idx, g))
+                continue
+            X, nrm, info, skeep = self._solve_direct(g, pr)
+            res, keep = self._synthesis(g, X, nrm, info, None, 0)
             for i, r in zip(idx, res):
                 h.results[i] = r
-            h.keep.append(keep)
-            h.checks.append((idx, checks, pr))
+            h.keep.append((g.keep, skeep, keep))
+            h.checks.append((idx, (nrm, info), pr))
+        if kp:
+            gs = [g for _, g in kp]
+            X, nrm, info, st, boff, skeep = self._solve_kpcg(gs)
+            all_idx = []
+            for (idx, g), (b0, e0) in zip(kp, boff):
+                res, keep = self._synthesis(g, X[e0:e0 + g.Dtot], nrm[b0:b0 + g.B], info[b0:b0 + g.B],
+                                            st[4 * b0:4 * (b0 + g.B)], 0)
+                for i, r in zip(idx, res):
+                    h.results[i] = r
+                h.keep.append((g.keep, keep))
+                all_idx += idx
+            h.keep.append(skeep)
+            h.checks.append((all_idx, (nrm, info), "kpcg"))
         return h
 
     def finalize(self, h: "BatchHandle") -> List[DecodeResult]:
@@ -185,7 +250,9 @@ class BatchDecoder:
         return h.results
 
     # ----------------------------------------------------------------------------------
-    def _decode_group(self, jobs: List[BlockJob], Dt: int, prec: str):
+    def _prologue(self, jobs: List[BlockJob], Dt: int) -> _Prologue:
+        """Intervals, q, per-neuron temporal moments T_j and t_j, r = S^T t and lam for a
+        group of blocks of temporal order Dt (global neuron numbering)."""
         dev = self.device
         s = stream_ptr()
         B = len(jobs)
@@ -196,14 +263,14 @@ class BatchDecoder:
         nbase = np.concatenate([[0], np.cumsum(N)])
         dbase = np.concatenate([[0], np.cumsum(D)])
         Ntot, Dtot = int(nbase[-1]), int(dbase[-1])
-        # per-neuron parameters in global numbering
         kappa = torch.cat([d.kappa for d in decs])
         delta = torch.cat([d.delta for d in decs])
         bias = torch.cat([d.bias for d in decs])
         norm = torch.cat([d.norm for d in decs])
         s2 = torch.cat([d.s2 for d in decs])
-        job_of = torch.from_numpy(np.repeat(np.arange(B, dtype=np.int32), N)).to(dev)
-        # ---- intervals -------------------------------------------------------------------
+        job_of = _h2d(np.repeat(np.arange(B, dtype=np.int32), N), dev)
+        e_pro = self._tic()
+        # ---- intervals
         sd = np.zeros(B, SPIKE_DTYPE)
         cap = 0
         for b, jb in enumerate(jobs):
@@ -219,10 +286,10 @@ class BatchDecoder:
         nid = torch.empty(cap, dtype=I32, device=dev)
         lib.vt_intervals_batched(ptr(sd_d), ptr(job_of), Ntot, ptr(ws), ptr(offsets), ptr(tk1),
                                  ptr(tk2), ptr(nid), cap, s)
-        base_d = torch.from_numpy(nbase.astype(np.int32)).to(dev)
+        base_d = _h2d(nbase.astype(np.int32), dev)
         n_b = torch.empty(B, dtype=I32, device=dev)
         lib.vt_seg_counts(ptr(offsets), ptr(base_d), B, ptr(n_b), s)
-        # ---- q, T_j, t_j -----------------------------------------------------------------
+        # ---- q, T_j, t_j
         q = torch.empty(cap, dtype=F64, device=dev)
         lib.vt_q(ptr(q), ptr(tk1), ptr(tk2), ptr(nid), ptr(kappa), ptr(delta), ptr(bias), ptr(norm),
                  ptr(offsets[Ntot:]), cap, s)
@@ -234,7 +301,7 @@ class BatchDecoder:
         tj_d = _to_dev(tj, dev)
         lib.vt_neuron_T_batched(ptr(Tm), ptr(tv), ptr(tk1), ptr(tk2), ptr(q), ptr(norm),
                                 ptr(offsets), ptr(tj_d), B, int(N.max()), s)
-        # ---- rhs, lambda -----------------------------------------------------------------
+        # ---- rhs, lambda
         r = torch.empty(Dtot, dtype=F64, device=dev)
         tn_ws = torch.empty(int(16 * Dt * Ds.sum()), dtype=F64, device=dev)
         ws_base = np.concatenate([[0], np.cumsum(16 * Dt * Ds)])
@@ -246,18 +313,24 @@ class BatchDecoder:
         tn_r_d = _to_dev(tn_r, dev)
         lib.vt_gemm_tn_batched(ptr(tn_r_d), B, int(Ds.max()), Dt, s)
         lam = torch.empty(2 * B, dtype=F64, device=dev)
-        s2off = torch.from_numpy(nbase[:-1].astype(np.int64)).to(dev)
-        N_d = torch.from_numpy(N.astype(np.int32)).to(dev)
+        s2off = _h2d(nbase[:-1].astype(np.int64), dev)
+        N_d = _h2d(N.astype(np.int32), dev)
         lib.vt_kron_trace_batched(ptr(s2), ptr(Tm), ptr(N_d), ptr(s2off), ptr(s2off), Dt, ptr(n_b), 1,
                                   self.lam_rel, ptr(lam), B, s)
-        seg = torch.from_numpy(dbase.astype(np.int64)).to(dev)
-        elem_job = torch.from_numpy(np.repeat(np.arange(B, dtype=np.int32), D)).to(dev)
-        solve = self._solve_kpcg if prec == "kpcg" else self._solve_direct
-        self.cg_state = None
-        X, nrm, info, solve_keep = solve(prec, decs, N, Ds, D, Dt, nbase, dbase, Ntot, Dtot, Tm, r,
-                                         lam, tn_ws, ws_base, seg, elem_job)
-        cg_state = None if self.cg_state is None else self.cg_state[0]
-        # ---- synthesis ---------------------------------------------------------------------
+        self._toc("prologue", e_pro, 0.0)
+        seg = _h2d(dbase.astype(np.int64), dev)
+        elem_job = _h2d(np.repeat(np.arange(B, dtype=np.int32), D), dev)
+        keep = (N_d, sd_d, ws, offsets, tk1, tk2, nid, base_d, q, tv, tn_r_d, s2off, tj_d, kappa, delta,
+                bias, norm, s2, job_of)
+        return _Prologue(jobs, decs, Dt, B, N, Ds, D, nbase, dbase, Ntot, Dtot, Tm, r, lam, n_b, tn_ws,
+                         ws_base, seg, elem_job, base_d, keep)
+
+    def _synthesis(self, g: _Prologue, X, nrm, info, st, _unused):
+        """frames_b = Theta_b X_b^T B_s,b^T for every block; DecodeResults of the group."""
+        dev = self.device
+        s = stream_ptr()
+        B, decs, Ds, Dt, dbase = g.B, g.decs, g.Ds, g.Dt, g.dbase
+        e_syn = self._tic()
         T_out = np.array([d.theta_out.shape[0] for d in decs], np.int64)
         P = np.array([d.geom.P for d in decs], np.int64)
         Y2 = torch.empty(int((T_out * Ds).sum()), dtype=F64, device=dev)
@@ -274,23 +347,22 @@ class BatchDecoder:
         gs1_d, gs2_d = _to_dev(gs1, dev), _to_dev(gs2, dev)
         lib.vt_gemm_grouped(ptr(gs1_d), B, int(T_out.max()), int(Ds.max()), s)
         lib.vt_gemm_grouped(ptr(gs2_d), B, int(T_out.max()), int(P.max()), s)
-        # every buffer referenced by enqueued work stays alive with the handle
-        keep = (N_d, sd_d, ws, offsets, tk1, tk2, nid, base_d, q, Tm, tv, tn_ws, tn_r_d, s2off,
-                solve_keep, seg, elem_job, Y2, gs1_d, gs2_d, kappa, delta, bias, norm, s2, job_of, r)
+        self._toc("synthesis", e_syn, float(np.sum(8.0 * T_out * P)))
         results = []
         for b in range(B):
             Xb = X[int(dbase[b]):int(dbase[b + 1])]
-            st_b = None if cg_state is None else cg_state[4 * b:4 * b + 4]
-            results.append(DecodeResult(frames[b], Xb, lam[2 * b:2 * b + 2], n_b[b:b + 1],
+            st_b = None if st is None else st[4 * b:4 * b + 4]
+            results.append(DecodeResult(frames[b], Xb, g.lam[2 * b:2 * b + 2], g.n_b[b:b + 1],
                                         info[b:b + 1], nrm[b], False, st_b))
-        return results, keep, (nrm, info)
+        return results, (Y2, gs1_d, gs2_d, X)
 
-    def _solve_direct(self, prec, decs, N, Ds, D, Dt, nbase, dbase, Ntot, Dtot, Tm, r, lam, tn_ws,
-                      ws_base, seg, elem_job):
+    def _solve_direct(self, g: "_Prologue", prec: str):
         """Gram (FP64 or low precision) + Cholesky (FP64 or FP32) + FP64 refinement."""
         dev = self.device
         s = stream_ptr()
-        B = len(decs)
+        B, decs, N, Ds, D, Dt = g.B, g.decs, g.N, g.Ds, g.D, g.Dt
+        nbase, dbase, Ntot, Dtot, Tm, r, lam = g.nbase, g.dbase, g.Ntot, g.Dtot, g.Tm, g.r, g.lam
+        tn_ws, ws_base, seg, elem_job = g.tn_ws, g.ws_base, g.seg, g.elem_job
         # ---- Gram -------------------------------------------------------------------------
         maxD = int(D.max())
         nblk = (maxD + 63) // 64
@@ -362,7 +434,7 @@ class BatchDecoder:
         sync = torch.zeros(2 + 2 * B * nblk, dtype=I32, device=dev)
         # vectors in a padded (B x maxD) layout for the solves, concatenated elsewhere
         Xp = torch.zeros((B, maxD), dtype=F64, device=dev)
-        pad_idx = torch.from_numpy(np.concatenate([b * maxD + np.arange(D[b]) for b in range(B)])).to(dev)
+        pad_idx = _h2d(np.concatenate([b * maxD + np.arange(D[b]) for b in range(B)]), dev)
 
         def to_padded(vec, out):
             out.view(-1).index_copy_(0, pad_idx, vec)
@@ -409,64 +481,82 @@ class BatchDecoder:
                               tn_a_d, resp)
 
     def _skinny(self, rows):
-        """Device descriptors of a skinny grouped product; rows of (A, B, C, T, sam, sak,
-        ldb, ldc, tstride, M, K) with pointers as ints (T = 0: no row factor)."""
+        """Skinny grouped product: rows of (A, B, C, T, active, sam, sak, ldb, ldc, tstride,
+        M, K, Nc) with pointers as ints (T = 0: no row factor; active = 0: always run).
+        Returns a launcher and the buffers it needs alive (descriptors, split-K workspace)."""
+        wsz = [skinny_workspace(int(r[10]), int(r[11]), int(r[12])) for r in rows]
+        ws = torch.empty(max(1, sum(wsz)), dtype=F64, device=self.device)
         d = np.zeros(len(rows), SK_DTYPE)
-        for i, row in enumerate(rows):
-            d[i] = row
-        return _to_dev(d, self.device)
-
-    def _operator(self, decs, N, Ds, Dt, nbase, dbase, Tm, src, dst, Z):
-        """Launcher of dst_b = A_b src_b for every block: Z_j = T_j (S src)_j in one skinny
-        product (T fused in its epilogue), dst = S^T Z in a second -- 4 N D flops, S read
-        twice, no Gram."""
-        B = len(decs)
-        E = Dt * Dt
-        fwd = self._skinny([(d.S.data_ptr(), src.data_ptr() + 8 * int(dbase[b]),
-                             Z.data_ptr() + 8 * int(nbase[b]) * Dt,
-                             Tm.data_ptr() + 8 * int(nbase[b]) * E,
-                             d.S.stride(0), 1, Dt, Dt, E, N[b], Ds[b]) for b, d in enumerate(decs)])
-        bwd = self._skinny([(d.S.data_ptr(), Z.data_ptr() + 8 * int(nbase[b]) * Dt,
-                             dst.data_ptr() + 8 * int(dbase[b]), 0,
-                             1, d.S.stride(0), Dt, Dt, 0, Ds[b], N[b]) for b, d in enumerate(decs)])
-        maxN, maxDs = int(N.max()), int(Ds.max())
+        off = 0
+        for i, r in enumerate(rows):
+            A, Bp, C, T, act, sam, sak, ldb, ldc, tst, M, K, Nc = r
+            d[i] = (A, Bp, C, T, ws.data_ptr() + 8 * off if wsz[i] else 0, act, sam, sak, ldb, ldc, tst,
+                    M, K, Nc, 0)
+            off += wsz[i]
+        dd = _to_dev(d, self.device)
+        B = len(rows)
+        maxM = max(int(r[10]) for r in rows)
+        maxK = max(int(r[11]) for r in rows)
+        maxNc = max(int(r[12]) for r in rows)
 
         def run(s):
-            lib.vt_gemm_skinny(ptr(fwd), B, maxN, Dt, s)
-            lib.vt_gemm_skinny(ptr(bwd), B, maxDs, Dt, s)
+            lib.vt_gemm_skinny(ptr(dd), B, maxM, maxK, maxNc, s)
 
-        return run, (fwd, bwd)
+        return run, (dd, ws)
 
-    def _spatial_inverse(self, decs, N, Ds, nbase, c, lam, s):
+    def _operator(self, blocks, src, dst, Z, dbj, nzj, active=None):
+        """Launcher of dst_b = A_b src_b for every block (blocks: (S, Tm_ptr, N, Ds, Dt)):
+        Z_j = T_j (S src)_j in one skinny product (T fused in its epilogue), dst = S^T Z in
+        a second -- 4 N D flops, S read twice, no Gram.  With `active` (per-block flag
+        pointers) converged blocks skip both products."""
+        act = active or [0] * len(blocks)
+        fwd, k1 = self._skinny([(S.data_ptr(), src.data_ptr() + 8 * int(dbj[b]), Z.data_ptr() + 8 * int(nzj[b]),
+                                 Tp, act[b], S.stride(0), 1, Dt, Dt, Dt * Dt, N, Ds, Dt)
+                                for b, (S, Tp, N, Ds, Dt) in enumerate(blocks)])
+        bwd, k2 = self._skinny([(S.data_ptr(), Z.data_ptr() + 8 * int(nzj[b]), dst.data_ptr() + 8 * int(dbj[b]),
+                                 0, act[b], 1, S.stride(0), Dt, Dt, 0, Ds, N, Dt)
+                                for b, (S, Tp, N, Ds, Dt) in enumerate(blocks)])
+
+        def run(s):
+            fwd(s)
+            bwd(s)
+
+        return run, (k1, k2)
+
+    def _spatial_inverse(self, Ss, N, Ds, c_ptrs, lam, s):
         """Mi_b = (S_b^T diag(c_b) S_b + lam_b I)^-1 for every block, FP64: Cholesky, then
         L^-1 by blocked forward substitution (64-row block steps, two grouped GEMMs each,
         diagonal blocks from the factor's block inverses), then Mi = L^-T L^-1."""
         dev = self.device
-        B = len(decs)
+        B = len(Ss)
         maxDs = int(Ds.max())
         mbase = np.concatenate([[0], np.cumsum(Ds * Ds)])
         Mpool = torch.empty(int(mbase[-1]), dtype=F64, device=dev)
         kd = np.zeros(B, KPM_DTYPE)
         dd = np.zeros(B, DIAG_DTYPE)
         mat_items = []
-        for b, d in enumerate(decs):
+        for b, S in enumerate(Ss):
             mp = Mpool.data_ptr() + 8 * int(mbase[b])
-            kd[b] = (d.S.data_ptr(), c.data_ptr() + 8 * int(nbase[b]), mp, d.S.stride(0), Ds[b], N[b],
-                     Ds[b])
+            kd[b] = (S.data_ptr(), c_ptrs[b], mp, S.stride(0), Ds[b], N[b], Ds[b])
             dd[b] = (mp, Ds[b], Ds[b], 0)
             mat_items.append((mp, int(Ds[b]), int(Ds[b])))
         kd_d, dd_d = _to_dev(kd, dev), _to_dev(dd, dev)
+        e0 = self._tic()
         lib.vt_kpcg_mgram(ptr(kd_d), B, maxDs, s)
+        self._toc("kpcg_mgram", e0, float(np.sum(2.0 * N * Ds * Ds)))
         lib.vt_diag_add_batched(ptr(dd_d), B, 0, ptr(lam), maxDs, s)
         nblk = (maxDs + 63) // 64
         inv = torch.empty(B * nblk * 4096, dtype=F64, device=dev)
         info = torch.zeros(B, dtype=I32, device=dev)
         mats = mat_descs(mat_items, dev)
+        e0 = self._tic()
         lib.vt_potrf_f64(ptr(mats), B, maxDs, ptr(inv), nblk * 4096, ptr(info), s)
+        self._toc("kpcg_potrf", e0, float(np.sum(Ds.astype(np.float64) ** 3 / 3)))
+        e0 = self._tic()
         # L^-1 (lower triangular, zero above the diagonal)
         Lpool = torch.zeros(int(mbase[-1]), dtype=F64, device=dev)
-        xoff = torch.from_numpy(mbase[:-1].astype(np.int64)).to(dev)
-        dim = torch.from_numpy(Ds.astype(np.int32)).to(dev)
+        xoff = _h2d(mbase[:-1].astype(np.int64), dev)
+        dim = _h2d(Ds.astype(np.int32), dev)
         lib.vt_kpcg_place_diag(ptr(Lpool), ptr(xoff), ptr(dim), B, maxDs, ptr(inv), nblk * 4096, s)
         Tb = torch.empty(B * 64 * maxDs, dtype=F64, device=dev)
         keep = [Mpool, kd_d, dd_d, inv, mats, Lpool, xoff, dim, Tb]
@@ -496,71 +586,91 @@ class BatchDecoder:
             g3[b] = (Lb, Lb, Mpool.data_ptr() + 8 * int(mbase[b]), 1, D, D, 1, D, D, D, D, 0, 1.0, 0.0)
         g3_d = _to_dev(g3, dev)
         lib.vt_gemm_grouped(ptr(g3_d), B, maxDs, maxDs, s)
+        self._toc("kpcg_inverse", e0, float(np.sum(Ds.astype(np.float64) ** 3)))
         keep.append(g3_d)
         return Mpool, mbase, info, keep
 
-    def _solve_kpcg(self, prec, decs, N, Ds, D, Dt, nbase, dbase, Ntot, Dtot, Tm, r, lam, tn_ws,
-                    ws_base, seg, elem_job):
-        """Kronecker-preconditioned CG on (A + lam I) X = r for every block of the group.
+    def _solve_kpcg(self, gs: List[_Prologue]):
+        """Kronecker-preconditioned CG on (A + lam I) X = r for every block of the groups,
+        as one joint solve (one launch sequence whatever the blocks' temporal orders).
 
-        Setup: c_j = tr(T_j)/Dt, Tbar (trace Dt) and its inverse Ti; Mi = (S^T diag(c) S +
-        lam I)^-1 in FP64.  Each iteration applies A through its structure (two skinny
-        products) and the preconditioner as (Mi R) Ti (one skinny product with a shared
-        right factor).  A block stops once its recursive residual falls below cg_stop
-        relative; the guard then checks the true FP64 residual."""
+        Setup: c_j = tr(T_j)/Dt, Tbar (trace Dt) and its inverse Ti per group; Mi =
+        (S^T diag(c) S + lam I)^-1 in FP64 per block.  Each iteration applies A through its
+        structure (two skinny products) and the preconditioner as (Mi R) Ti (one skinny
+        product with a shared right factor).  A block stops once its recursive residual
+        falls below cg_stop relative (its products are skipped from then on); the guard
+        then checks the true FP64 residual.  Returns the joint X, residual norms, factor
+        info, CG state, per-group (block, element) offsets and the buffers to keep."""
         dev = self.device
         s = stream_ptr()
-        B = len(decs)
-        maxDs = int(Ds.max())
-        E = Dt * Dt
         e_setup = self._tic()
-        nbase_d = torch.from_numpy(nbase.astype(np.int32)).to(dev)
-        c = torch.empty(Ntot, dtype=F64, device=dev)
-        TiT = torch.empty(B * E, dtype=F64, device=dev)
-        tws = torch.empty(16 * B * E, dtype=F64, device=dev)
-        lib.vt_kpcg_tbar(ptr(Tm), ptr(nbase_d), Dt, B, ptr(tws), ptr(c), ptr(TiT), s)
-        Mi, mbase, info, mkeep = self._spatial_inverse(decs, N, Ds, nbase, c, lam, s)
+        blocks, c_ptrs, TiT_ptrs, keep = [], [], [], []
+        boff, b0, e0 = [], 0, 0
+        for g in gs:
+            E = g.Dt * g.Dt
+            c = torch.empty(g.Ntot, dtype=F64, device=dev)
+            TiT = torch.empty(g.B * E, dtype=F64, device=dev)
+            tws = torch.empty(16 * g.B * E, dtype=F64, device=dev)
+            lib.vt_kpcg_tbar(ptr(g.Tm), ptr(g.nbase_d), g.Dt, g.B, ptr(tws), ptr(c), ptr(TiT), s)
+            keep += [c, TiT, tws]
+            for b, d in enumerate(g.decs):
+                blocks.append((d.S, g.Tm.data_ptr() + 8 * int(g.nbase[b]) * E, int(g.N[b]), int(g.Ds[b]), g.Dt))
+                c_ptrs.append(c.data_ptr() + 8 * int(g.nbase[b]))
+                TiT_ptrs.append(TiT.data_ptr() + 8 * b * E)
+            boff.append((b0, e0))
+            b0 += g.B
+            e0 += g.Dtot
+        Bt = len(blocks)
+        N = np.array([b[2] for b in blocks], np.int64)
+        Ds = np.array([b[3] for b in blocks], np.int64)
+        Dt = np.array([b[4] for b in blocks], np.int64)
+        D = Ds * Dt
+        dbj = np.concatenate([[0], np.cumsum(D)])
+        nzj = np.concatenate([[0], np.cumsum(N * Dt)])
+        Dtot = int(dbj[-1])
+        r = torch.cat([g.r for g in gs]) if len(gs) > 1 else gs[0].r
+        lam = torch.cat([g.lam for g in gs]) if len(gs) > 1 else gs[0].lam
+        seg = _h2d(dbj.astype(np.int64), dev)
+        elem_job = _h2d(np.repeat(np.arange(Bt, dtype=np.int32), D), dev)
+        Mi, mbase, info, mkeep = self._spatial_inverse([b[0] for b in blocks], N, Ds, c_ptrs, lam, s)
         R = r.clone()
         Zp = torch.empty(Dtot, dtype=F64, device=dev)
-        pre = self._skinny([(Mi.data_ptr() + 8 * int(mbase[b]), R.data_ptr() + 8 * int(dbase[b]),
-                             Zp.data_ptr() + 8 * int(dbase[b]), TiT.data_ptr() + 8 * b * E,
-                             Ds[b], 1, Dt, Dt, 0, Ds[b], Ds[b]) for b in range(B)])
-
-        def precond():
-            lib.vt_gemm_skinny(ptr(pre), B, maxDs, Dt, s)
-
+        st = torch.zeros(4 * Bt, dtype=F64, device=dev)
+        st.view(Bt, 4)[:, 2] = 1.0          # active for the first preconditioner application
+        act = [st.data_ptr() + 8 * (4 * b + 2) for b in range(Bt)]
+        pre_run, pre = self._skinny([(Mi.data_ptr() + 8 * int(mbase[b]), R.data_ptr() + 8 * int(dbj[b]),
+                                      Zp.data_ptr() + 8 * int(dbj[b]), TiT_ptrs[b], act[b],
+                                      Ds[b], 1, Dt[b], Dt[b], 0, Ds[b], Ds[b], Dt[b]) for b in range(Bt)])
         self._toc("kpcg_setup", e_setup, float(np.sum(2.0 * N * Ds * Ds + 2.0 * Ds.astype(np.float64) ** 3)))
         e_cg = self._tic()
         X = torch.empty(Dtot, dtype=F64, device=dev)
         Pv = torch.empty(Dtot, dtype=F64, device=dev)
         Q = torch.empty(Dtot, dtype=F64, device=dev)
-        Z = torch.empty((Ntot, Dt), dtype=F64, device=dev)
-        op_p, kp1 = self._operator(decs, N, Ds, Dt, nbase, dbase, Tm, Pv, Q, Z)
-        op_x, kp2 = self._operator(decs, N, Ds, Dt, nbase, dbase, Tm, X, Q, Z)
+        Z = torch.empty(int(nzj[-1]), dtype=F64, device=dev)
+        op_p, kp1 = self._operator(blocks, Pv, Q, Z, dbj, nzj, act)
+        op_x, kp2 = self._operator(blocks, X, Q, Z, dbj, nzj)
         iters = int(self.cg_iters)
         hs = iters + 1
-        st = torch.zeros(4 * B, dtype=F64, device=dev)
-        hist = torch.zeros((B, hs), dtype=F64, device=dev)
-        precond()
-        lib.vt_kpcg_init(ptr(R), ptr(Zp), ptr(Pv), ptr(X), ptr(seg), B, ptr(st), ptr(hist), hs, s)
+        hist = torch.zeros((Bt, hs), dtype=F64, device=dev)
+        pre_run(s)
+        lib.vt_kpcg_init(ptr(R), ptr(Zp), ptr(Pv), ptr(X), ptr(seg), Bt, ptr(st), ptr(hist), hs, s)
         stop2 = float(self.cg_stop) ** 2
         for it in range(iters):
             op_p(s)
-            lib.vt_kpcg_step_a(ptr(X), ptr(R), ptr(Pv), ptr(Q), ptr(lam), ptr(seg), B, ptr(st), ptr(hist),
+            lib.vt_kpcg_step_a(ptr(X), ptr(R), ptr(Pv), ptr(Q), ptr(lam), ptr(seg), Bt, ptr(st), ptr(hist),
                                hs, it, stop2, s)
-            precond()
-            lib.vt_kpcg_step_b(ptr(R), ptr(Zp), ptr(Pv), ptr(seg), B, ptr(st), s)
+            pre_run(s)
+            lib.vt_kpcg_step_b(ptr(R), ptr(Zp), ptr(Pv), ptr(seg), Bt, ptr(st), s)
         # true FP64 residual of the final iterate (the guard's input)
-        nrm = torch.zeros((B, 2), dtype=F64, device=dev)
-        lib.vt_seg_sumsq(ptr(r), ptr(seg), B, ptr(nrm), 2, s)
+        nrm = torch.zeros((Bt, 2), dtype=F64, device=dev)
+        lib.vt_seg_sumsq(ptr(r), ptr(seg), Bt, ptr(nrm), 2, s)
         op_x(s)
         res = torch.empty(Dtot, dtype=F64, device=dev)
         lib.vt_residual_batched(ptr(res), ptr(r), ptr(Q), ptr(X), ptr(lam), ptr(elem_job), Dtot, s)
-        lib.vt_seg_sumsq(ptr(res), ptr(seg), B, ptr(nrm[:, 1:]), 2, s)
+        lib.vt_seg_sumsq(ptr(res), ptr(seg), Bt, ptr(nrm[:, 1:]), 2, s)
         # algorithmic HBM bytes per iteration: S twice, T_j once, Z written and read, Mi
         # once, about a dozen passes over the D-space vectors
-        nbytes = 8.0 * np.sum(2.0 * N * Ds + N * E + 2.0 * N * Dt + Ds * Ds + 12.0 * D)
+        nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + Ds * Ds + 12.0 * D)
         self._toc("kpcg_iterate", e_cg, float(nbytes) * iters)
-        self.cg_state = (st, hist)
-        return X, nrm, info, (nbase_d, c, TiT, tws, Mi, mkeep, R, Zp, pre, X, Pv, Q, Z, kp1, kp2, st,
-                              hist, res)
+        keep += [r, lam, seg, elem_job, Mi, mkeep, R, Zp, pre, X, Pv, Q, Z, kp1, kp2, st, hist, res]
+        return X, nrm, info, st, boff, keep

@@ -375,39 +375,54 @@ VT_EXPORT int vt_seg_counts(const int *offsets, const int *base, int nseg, int *
 
 // ------------------------------------------------------------------------------------
 // Skinny grouped GEMM (fp64): problem b computes C_b (M x Nc) = A_b (M x K) B_b (K x Nc)
-// for Nc <= 32, with A_b(m, k) at A[m*sam + k*sak] (either orientation), B and C
+// for its own Nc <= 32 (the launch's max_Nc bounds them all), with A_b(m, k) at A[m*sam + k*sak] (either orientation), B and C
 // row-major.  Optionally each output row m is then multiplied by the Nc x Nc matrix
 // T_b + m*tstride (C_m <- T_m C_m): with tstride = Nc*Nc this fuses Z_j = T_j (S P)_j
 // into the product; with tstride = 0 every row shares one matrix (right factor C T^T).
 //
-// One workgroup owns 64 rows; its 4 waves split K into 16-wide chunks (wave w takes
-// chunks w, w+4, ...).  Operands go from global memory straight into the MFMA f64
-// 16x16x4 operand layout: lane (r = l%16, g = l/16) loads the 4 consecutive k values
-// c*16 + 4g + s (s = 0..3) of its row, and MFMA step s pairs them with the B rows of
-// the same k, so each step sums k in {c*16 + 4g + s} and the 4 steps cover the chunk.
-// The wave partials are added in LDS in a fixed order ((w0 + w2) + (w1 + w3)).
+// One workgroup owns 64 rows of one K split; its 4 waves take 16-wide K chunks in turn
+// (wave w: chunks w, w+4, ...).  Operands go from global memory straight into the MFMA
+// f64 16x16x4 operand layout: lane (r = l%16, g = l/16) loads the 4 consecutive k values
+// c*16 + 4g + s (s = 0..3) of its row, and MFMA step s pairs them with the B rows of the
+// same k, so each step sums k in {c*16 + 4g + s} and the 4 steps cover the chunk.  The
+// wave partials are added in LDS in a fixed order ((w0 + w2) + (w1 + w3)).
+//
+// Long reductions are split: a problem with nch = ceil(K/16) chunks uses
+// ks = clamp(nch/32, 1, 8) splits (a function of its own K only, so a problem's rounding
+// does not depend on the batch), each writing a partial tile to its workspace; a second
+// pass adds the ks partials in order and applies the row factor.
 // ------------------------------------------------------------------------------------
 struct SkDesc {
     const double *A, *B;
     double *C;
     const double *T;
+    double *ws;                 // ks * M * Nc doubles when ks > 1
+    const double *active;       // optional: the problem is skipped while *active == 0
     long long sam, sak, ldb, ldc, tstride;
-    int M, K;
+    int M, K, Nc, pad;
 };
 
-// operands of one 16-wide K chunk for lane (r, g)
+__host__ __device__ __forceinline__ int skinny_splits(int K)
+{
+    const int nch = (K + 15) / 16;
+    const int ks = nch / 32;
+    return ks < 1 ? 1 : (ks > 8 ? 8 : ks);
+}
+
+// operands of one 16-wide K chunk for lane (r, g); chunks at or beyond c_end load zeros
 template <int NT>
-__device__ __forceinline__ void skinny_load(const SkDesc &d, int m0, int c, int r, int g, int Nc,
-                                            double (&a)[4][4], double (&bv)[NT][4])
+__device__ __forceinline__ void skinny_load(const SkDesc &d, int m0, int c, int c_end, int r, int g,
+                                            int Nc, double (&a)[4][4], double (&bv)[NT][4])
 {
     const int kb = c * 16 + 4 * g;
+    const bool cin = c < c_end;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int m = m0 + 16 * t + r;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int k = kb + s;
-            a[t][s] = (m < d.M && k < d.K) ? d.A[(long long)m * d.sam + (long long)k * d.sak] : 0.0;
+            a[t][s] = (cin && m < d.M && k < d.K) ? d.A[(long long)m * d.sam + (long long)k * d.sak] : 0.0;
         }
     }
 #pragma unroll
@@ -416,7 +431,7 @@ __device__ __forceinline__ void skinny_load(const SkDesc &d, int m0, int c, int 
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int k = kb + s;
-            bv[j][s] = (n < Nc && k < d.K) ? d.B[(long long)k * d.ldb + n] : 0.0;
+            bv[j][s] = (cin && n < Nc && k < d.K) ? d.B[(long long)k * d.ldb + n] : 0.0;
         }
     }
 }
@@ -434,13 +449,24 @@ __device__ __forceinline__ void skinny_mma(v4d (&acc)[4][NT], const double (&a)[
                 acc[t][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][s], bv[j][s], acc[t][j], 0, 0, 0);
 }
 
+__device__ __forceinline__ double skinny_row_factor(const SkDesc &d, int m, int col, int Nc,
+                                                    const double *row)
+{
+    const double *Tr = d.T + (long long)m * d.tstride + (long long)col * Nc;
+    double v = 0.0;
+    for (int k = 0; k < Nc; ++k) v += Tr[k] * row[k];
+    return v;
+}
+
 template <int NT>
-__global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ descs, int Nc)
+__global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ descs)
 {
     __shared__ double red[2][64][33];
     const SkDesc d = descs[blockIdx.y];
     const int m0 = blockIdx.x * 64;
-    if (m0 >= d.M) return;
+    const int ks = skinny_splits(d.K), z = blockIdx.z, Nc = d.Nc;
+    if (m0 >= d.M || z >= ks) return;
+    if (d.active && *d.active == 0.0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r = lane & 15, g = lane >> 4;
     v4d acc[4][NT];
@@ -449,11 +475,13 @@ __global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ 
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[t][j] = (v4d){0, 0, 0, 0};
     const int nch = (d.K + 15) / 16;
-    // two chunks in flight per wave: chunks c and c + 4 (out-of-range chunks load zeros)
-    for (int c = wave; c < nch; c += 8) {
+    const int per = (nch + ks - 1) / ks;
+    const int c_beg = z * per, c_end = min(nch, c_beg + per);
+    // two chunks in flight per wave: c and c + 4
+    for (int c = c_beg + wave; c < c_end; c += 8) {
         double a0[4][4], b0[NT][4], a1[4][4], b1[NT][4];
-        skinny_load<NT>(d, m0, c, r, g, Nc, a0, b0);
-        skinny_load<NT>(d, m0, c + 4, r, g, Nc, a1, b1);
+        skinny_load<NT>(d, m0, c, c_end, r, g, Nc, a0, b0);
+        skinny_load<NT>(d, m0, c + 4, c_end, r, g, Nc, a1, b1);
         skinny_mma<NT>(acc, a0, b0);
         skinny_mma<NT>(acc, a1, b1);
     }
@@ -500,32 +528,68 @@ __global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ 
                 }
     }
     __syncthreads();
-    // epilogue: C rows (optionally T_m C_m)
     const int rows = min(64, d.M - m0);
+    if (ks > 1) {
+        double *W = d.ws + (long long)z * d.M * Nc;
+        for (int e = tid; e < rows * Nc; e += 256) {
+            const int row = e / Nc, col = e % Nc;
+            W[(long long)(m0 + row) * Nc + col] = red[1][row][col];
+        }
+        return;
+    }
     for (int e = tid; e < rows * Nc; e += 256) {
         const int row = e / Nc, col = e % Nc;
-        double v;
-        if (d.T) {
-            const double *Tr = d.T + (long long)(m0 + row) * d.tstride + (long long)col * Nc;
-            v = 0.0;
-            for (int k = 0; k < Nc; ++k) v += Tr[k] * red[1][row][k];
-        } else {
-            v = red[1][row][col];
-        }
+        const double v = d.T ? skinny_row_factor(d, m0 + row, col, Nc, red[1][row]) : red[1][row][col];
         d.C[(long long)(m0 + row) * d.ldc + col] = v;
     }
 }
 
-VT_EXPORT int vt_gemm_skinny(const void *descs, int batch, int max_M, int Nc, void *stream)
+// second pass of split problems: C = (sum_z partial_z) (row factor applied)
+__global__ void __launch_bounds__(256) skinny_reduce_kernel(const SkDesc *__restrict__ descs)
 {
-    VT_CHECK_ARG(descs && batch >= 1 && batch <= 65535 && max_M >= 0 && Nc >= 1 && Nc <= 32);
+    __shared__ double rowbuf[8][33];
+    const SkDesc d = descs[blockIdx.y];
+    const int ks = skinny_splits(d.K), Nc = d.Nc;
+    const int m0 = blockIdx.x * 8;
+    if (ks == 1 || m0 >= d.M) return;
+    if (d.active && *d.active == 0.0) return;
+    const int rr = threadIdx.x >> 5, col = threadIdx.x & 31;
+    const int m = m0 + rr;
+    double v = 0.0;
+    if (m < d.M && col < Nc)
+        for (int zz = 0; zz < ks; ++zz) v += d.ws[((long long)zz * d.M + m) * Nc + col];
+    rowbuf[rr][col] = v;
+    __syncthreads();
+    if (m < d.M && col < Nc) {
+        const double o = d.T ? skinny_row_factor(d, m, col, Nc, rowbuf[rr]) : v;
+        d.C[(long long)m * d.ldc + col] = o;
+    }
+}
+
+// workspace (doubles) one problem of the skinny product needs: 0 unless its K is split
+VT_EXPORT int vt_gemm_skinny_workspace_size(int M, int K, int Nc, long long *doubles_host)
+{
+    VT_CHECK_ARG(M >= 0 && K >= 0 && Nc >= 1 && Nc <= 32 && doubles_host);
+    const int ks = skinny_splits(K);
+    *doubles_host = ks > 1 ? (long long)ks * M * Nc : 0;
+    return VT_OK;
+}
+
+VT_EXPORT int vt_gemm_skinny(const void *descs, int batch, int max_M, int max_K, int max_Nc,
+                             void *stream)
+{
+    VT_CHECK_ARG(descs && batch >= 1 && batch <= 65535 && max_M >= 0 && max_K >= 0 && max_Nc >= 1 &&
+                 max_Nc <= 32);
     if (max_M == 0) return VT_OK;
-    dim3 grid(vt_blocks(max_M, 64), batch);
+    const int ks = skinny_splits(max_K);
+    dim3 grid(vt_blocks(max_M, 64), batch, ks);
     hipStream_t s = (hipStream_t)stream;
-    if (Nc <= 16)
-        skinny_kernel<1><<<grid, 256, 0, s>>>((const SkDesc *)descs, Nc);
+    const SkDesc *d = (const SkDesc *)descs;
+    if (max_Nc <= 16)
+        skinny_kernel<1><<<grid, 256, 0, s>>>(d);
     else
-        skinny_kernel<2><<<grid, 256, 0, s>>>((const SkDesc *)descs, Nc);
+        skinny_kernel<2><<<grid, 256, 0, s>>>(d);
+    if (ks > 1) skinny_reduce_kernel<<<dim3(vt_blocks(max_M, 8), batch), 256, 0, s>>>(d);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
