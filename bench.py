@@ -41,12 +41,15 @@ if ROOT not in sys.path:
 
 # stage -> (peak, unit, bound); the stage's recorded work is flops for "mfma" stages and
 # HBM bytes for "hbm" stages
+RNN_SHARE = 0.2          # decoder mix: 80 % pseudoinverse (Tikhonov), 20 % RNN
+RNN_STEPS = 500
+
 PEAKS = {"gram_lp": (2500.0, "TFLOP/s", "mfma"), "gram_f64": (78.6, "TFLOP/s", "mfma"),
          "potrf_f32": (157.3, "TFLOP/s", "mfma"), "potrf_f64": (78.6, "TFLOP/s", "mfma"),
          "solve_refine": (157.3, "TFLOP/s", "mfma"), "kpcg_setup": (78.6, "TFLOP/s", "mfma"),
          "kpcg_mgram": (78.6, "TFLOP/s", "mfma"), "kpcg_potrf": (78.6, "TFLOP/s", "mfma"),
          "kpcg_inverse": (78.6, "TFLOP/s", "mfma"), "prologue": (8000.0, "GB/s", "hbm"),
-         "synthesis": (8000.0, "GB/s", "hbm"),
+         "synthesis": (8000.0, "GB/s", "hbm"), "rnn_iterate": (8000.0, "GB/s", "hbm"),
          "kpcg_iterate": (8000.0, "GB/s", "hbm")}
 WORK_NOTES = {"kpcg_iterate": "algorithmic HBM bytes per CG iteration: 8*(2*N*Ds + N*Dt^2 + 4*N*Dt "
                               "+ Ds^2 + 12*D) per block (S twice, T_j, Y/Z, Mi, vectors)",
@@ -78,31 +81,31 @@ def prepare(R, seed):
     from vtem_amd.basis import Geometry
     from vtem_amd.batch import BlockJob
     from vtem_amd.block import Decoder, Encoder, dev, spatial_coeffs
-    from vtem_amd.serve import footprint_bytes
     rng = np.random.default_rng(seed)
     items = []
     for i in range(R):
         side = int(rng.choice(syn.SERVE_SIDES))
         dur = float(rng.choice(syn.SERVE_DURATIONS))
+        method = "rnn" if rng.random() < RNN_SHARE else "tikhonov"
         blk = syn.serve_block(side, dur, seed=seed * 100003 + i)
         geom = Geometry(side, side, blk.Lx, blk.Lx, blk.Lt, dur)
         enc = Encoder(geom, blk.bank)
         st = enc.encode(dev(blk.video.frames.reshape(geom.frames, -1)), blk.iaf)
         dec = Decoder(geom, spatial_coeffs(enc.RF, geom), blk.iaf.kappa, blk.iaf.delta,
                       blk.iaf.bias)
-        job = BlockJob(dec, st, tag=i)
-        items.append((blk, geom, job, footprint_bytes(job)))
+        job = BlockJob(dec, st, tag=i, method=method, rnn_steps=RNN_STEPS)
+        items.append((blk, geom, job))
     torch.cuda.synchronize()
-    for blk, geom, job, _ in items:
+    for blk, geom, job in items:
         if int(job.spikes.counts.max().item()) >= job.spikes.spikes.shape[0]:
             raise RuntimeError("spike buffer overflow")
     return items
 
 
-def make_requests(items, arrivals):
-    from vtem_amd.serve import Request
-    return [Request(i, float(a), job, geom.frames, fp)
-            for i, ((blk, geom, job, fp), a) in enumerate(zip(items, arrivals))]
+def make_requests(items, arrivals, form="kpcg"):
+    from vtem_amd.serve import Request, footprint_bytes
+    return [Request(i, float(a), job, geom.frames, footprint_bytes(job, form))
+            for i, ((blk, geom, job), a) in enumerate(zip(items, arrivals))]
 
 
 def snr_db(u, urec):
@@ -114,7 +117,7 @@ def cpu_baseline(items, budget_s):
     """Float64 oracle decode of requests (same spikes) until ~budget_s seconds of work."""
     from oracle import vtem_oracle as O
     frames, t_total, used = 0, 0.0, 0
-    for blk, geom, job, _ in items:
+    for blk, geom, job in items:
         if t_total >= budget_s:
             break
         lists = job.spikes.to_lists()
@@ -128,7 +131,17 @@ def cpu_baseline(items, budget_s):
         q = O.compute_q(tk1, tk2, nid, blk.iaf.kappa, blk.iaf.delta, blk.iaf.bias, np.ones(N))
         Tm, tv = O.neuron_T(O.temporal_factors(tk1, tk2, geom.Lt, geom.W), q, nid, N)
         A = O.gram_kron(S, Tm)
-        X = O.solve_tikhonov(A, O.rhs_kron(S, tv), O.tikhonov_lambda(A, tk1.size))
+        lam = O.tikhonov_lambda(A, tk1.size)
+        r = O.rhs_kron(S, tv)
+        if job.method == "rnn":
+            v = np.ones(r.size)                   # 50 power steps for eta = 1/lambda_max
+            for _ in range(50):
+                w = A @ v + lam * v
+                rho = float(v @ w) / float(v @ v)
+                v = w / np.linalg.norm(w)
+            X = O.rnn_dspace(A, r, 1.0 / rho, job.rnn_steps, lam)
+        else:
+            X = O.solve_tikhonov(A, r, lam)
         O.synthesize(X, B, O.temporal_basis(geom.frame_times(), geom.Lt, geom.W))
         t_total += time.perf_counter() - t0
         frames += geom.frames
@@ -149,18 +162,18 @@ def main():
     require_gpu()
     items = prepare(args.requests, args.seed)
     R = len(items)
-    frames = sum(g.frames for _, g, _, _ in items)
-    D = np.array([g.D for _, g, _, _ in items])
+    frames = sum(g.frames for _, g, _ in items)
+    D = np.array([g.D for _, g, _ in items])
     zeros = np.zeros(R)                       # offered load above capacity
 
     def run_form(form):
         server = Server(max_batch=args.max_batch, streams=args.streams, precision=form,
                         adaptive=form != "kpcg")
         for _ in range(args.warmup):           # allocator, code objects, precision policy
-            server.serve(make_requests(items, zeros))
+            server.serve(make_requests(items, zeros, form))
         stats, reqs = [], None
         for _ in range(args.steps):
-            reqs = make_requests(items, zeros)
+            reqs = make_requests(items, zeros, form)
             torch.cuda.synchronize()
             stats.append(server.serve(reqs))
         wall = sum(st["wall_s"] for st in stats)
@@ -176,7 +189,7 @@ def main():
 
     def snr_diffs(reqs):
         d = []
-        for (blk, geom, job, _), rc, rp in zip(items, reqs, plain_reqs):
+        for (blk, geom, job), rc, rp in zip(items, reqs, plain_reqs):
             u = blk.video.frames.reshape(geom.frames, -1)
             d.append(abs(snr_db(u, rc.result.frames.cpu().numpy()) -
                          snr_db(u, rp.result.frames.cpu().numpy())))
@@ -184,7 +197,7 @@ def main():
 
     d_snr = snr_diffs(chosen_reqs)
     snrs = [snr_db(blk.video.frames.reshape(geom.frames, -1), rp.result.frames.cpu().numpy())
-            for (blk, geom, _, _), rp in zip(items, plain_reqs)]
+            for (blk, geom, _), rp in zip(items, plain_reqs)]
     compare = None
     if args.compare:
         _, cst, creqs, cwall = run_form(args.compare)

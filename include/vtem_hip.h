@@ -307,6 +307,35 @@ int vt_kpcg_step_a(double *x, double *r, const double *p, double *q, const doubl
 int vt_kpcg_step_b(const double *r, const double *z, double *p, const long long *seg, int batch,
                    double *st, void *stream);
 
+/* `iters` CG iterations after vt_kpcg_init, enqueued by one call: per iteration the
+ * operator (skinny products fwd then bwd, q = A p), step_a, the preconditioner (skinny
+ * product pre, z = Mi r Ti) and step_b.  Each product is given as its descriptors and
+ * its max_M / max_K; max_Nc bounds every problem's column count. */
+int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
+                    const void *pre, int pre_M, int pre_K, int batch, int max_Nc, double *x,
+                    double *r, double *p, double *q, double *z, const double *lam,
+                    const long long *seg, double *st, double *hist, int hist_stride, int iters,
+                    double stop_rel2, void *stream);
+
+/* ----------------------------------------- gradient-flow (RNN) decoder, batched -- */
+/* Coefficient-space image of neural_network.rnn3 (vtem/neural_network.py:15-99, update
+ * kernel :102-128): X <- X + eta_b (r - (A + lam_b I) X), steps+1 iterations from X = 0,
+ * A applied through the skinny operator launches fwd (reads X) / bwd (writes q).
+ * elem_job maps each element of the concatenated X to its block. */
+int vt_kron_rnn(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
+                int batch, int max_Nc, double *x, const double *r, double *q, const double *lam,
+                const double *eta, const int *elem_job, long long n, int steps, void *stream);
+/* Step sizes eta_b = 1/rho_b, rho_b the power-iteration estimate (iters steps from v,
+ * overwritten) of the largest eigenvalue of A_b + lam_b I: the step rule eta = 1/lam_max
+ * the configurations use in place of the reference's dt*alpha (vtem/vtdm.py:21). */
+int vt_kron_power(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
+                  int batch, int max_Nc, double *v, double *q, const double *lam,
+                  const long long *seg, int iters, double *eta, void *stream);
+
+/* One power step per block (q = A v given): w = q + lam v, eta_b = <v,v>/<v,w>, v = w/|w|. */
+int vt_power_step(double *v, const double *q, const double *lam, const long long *seg, int batch,
+                  double *eta, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

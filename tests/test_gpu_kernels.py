@@ -620,6 +620,52 @@ def test_kpcg_batch(torch_mod, oracle):
                                       err_msg=f"block {i} {specs[i]}")
 
 
+def test_rnn_batch(torch_mod, oracle):
+    """Batched gradient-flow decoder (blocks of different temporal orders in one joint
+    launch sequence) against the per-block path and the oracle's coefficient-space RNN
+    with the same step: iterates agree to 1e-9 relative, SNR within 0.1 dB."""
+    torch = torch_mod
+    from vtem_amd import synthetic as syn
+    from vtem_amd.batch import BatchDecoder, BlockJob
+    _banner("rnn batch")
+    specs = [(32, 0.5), (32, 2.0), (48, 1.0)]
+    steps = 60
+    blocks = [syn.serve_block(s, d, seed=700 + i) for i, (s, d) in enumerate(specs)]
+    jobs = []
+    for blk in blocks:
+        geom, st, dec = _encode_block(torch, blk)
+        jobs.append(BlockJob(dec, st, method="rnn", rnn_steps=steps))
+    res = BatchDecoder(precision="kpcg").decode(jobs)
+    for i, (blk, jb, r) in enumerate(zip(blocks, jobs, res)):
+        eta = float(r.state.cpu().numpy()[0])
+        alone = jb.decoder.decode(jb.spikes, method="rnn", rnn_steps=steps)
+        Xa, Xb = alone.X.cpu().numpy(), r.X.cpu().numpy()
+        assert np.abs(Xa - Xb).max() <= 1e-9 * np.abs(Xa).max()
+        # oracle: same spikes, same eta
+        d = jb.decoder
+        g = d.geom
+        lists = jb.spikes.to_lists()
+        S = d.S.cpu().numpy()
+        tk1, tk2, nid, _ = oracle.select_intervals(lists, (0.0, g.duration))
+        q = oracle.compute_q(tk1, tk2, nid, blk.iaf.kappa, blk.iaf.delta, blk.iaf.bias, np.ones(d.N))
+        Tm, tv = oracle.neuron_T(oracle.temporal_factors(tk1, tk2, g.Lt, g.W), q, nid, d.N)
+        A = oracle.gram_kron(S, Tm)
+        lam = oracle.tikhonov_lambda(A, tk1.size)
+        lam_max = np.linalg.eigvalsh(A + lam * np.eye(A.shape[0]))[-1]
+        Xo = oracle.rnn_dspace(A, oracle.rhs_kron(S, tv), eta, steps, lam)
+        err = np.abs(Xb - Xo).max() / np.abs(Xo).max()
+        u = blk.video.frames.reshape(g.frames, -1)
+        s_b = oracle.snr_db(u, r.frames.cpu().numpy())
+        s_a = oracle.snr_db(u, alone.frames.cpu().numpy())
+        print(f"block {specs[i]}: D={g.D} eta*lam_max={eta * lam_max:.4f} |X-X_oracle|/|X| {err:.1e} "
+              f"SNR batched {s_b:.3f} alone {s_a:.3f}")
+        # the Rayleigh quotient approaches lambda_max from below: eta*lam_max >= 1, and the
+        # iteration is stable for eta*lam_max < 2
+        assert 0.999 < eta * lam_max < 1.2
+        assert err < 1e-9
+        assert abs(s_b - s_a) < 0.1
+
+
 def test_server_alone_equals_mix(torch_mod):
     """Requests served with Poisson arrivals on 2 streams (continuous batching) give the
     same frames as each request decoded alone with the same precision decision."""

@@ -26,8 +26,9 @@ class _Dec:
 
 
 class _Job:
-    def __init__(self, Ds, Dt):
+    def __init__(self, Ds, Dt, method="tikhonov"):
         self.decoder = _Dec(Ds, Dt)
+        self.method = method
 
 
 def test_precision_policy_routes_failing_class_to_fp64():

@@ -86,11 +86,15 @@ def skinny_workspace(M: int, K: int, Nc: int) -> int:
 @dataclasses.dataclass
 class BlockJob:
     """One block to decode: a prepared Decoder (geometry, S, IAF parameters) and its
-    resident spike trains; window starts at t0 and lasts the decoder's duration."""
+    resident spike trains; window starts at t0 and lasts the decoder's duration.
+    method "tikhonov" (pseudoinverse, regularised) or "rnn" (rnn_steps + 1 gradient-flow
+    iterations with step 1/lambda_max)."""
     decoder: Decoder
     spikes: SpikeTrains
     t0: float = 0.0
     tag: Any = None
+    method: str = "tikhonov"
+    rnn_steps: int = 500
 
 
 @dataclasses.dataclass
@@ -121,6 +125,23 @@ class _Prologue:
 
 
 @dataclasses.dataclass
+class _Skinny:
+    """Launch record of one skinny grouped product."""
+    descs: torch.Tensor
+    batch: int
+    max_M: int
+    max_K: int
+    max_Nc: int
+    ws: torch.Tensor
+
+    def run(self, s):
+        lib.vt_gemm_skinny(ptr(self.descs), self.batch, self.max_M, self.max_K, self.max_Nc, s)
+
+    def args(self):
+        return ptr(self.descs), self.max_M, self.max_K
+
+
+@dataclasses.dataclass
 class BatchHandle:
     jobs: List[BlockJob]
     results: List[Optional[DecodeResult]]
@@ -130,6 +151,7 @@ class BatchHandle:
 
 
 PRECISIONS = ("fp64", "mixed", "bf16x3", "fp16", "kpcg")
+POWER_ITERS = 50          # power steps for the RNN step size 1/lambda_max
 
 
 class BatchDecoder:
@@ -192,12 +214,19 @@ class BatchDecoder:
         for i, (jb, pr) in enumerate(zip(jobs, precisions)):
             if pr not in PRECISIONS:
                 raise ValueError(f"unknown precision {pr}")
-            groups.setdefault((jb.decoder.geom.Dt, pr), []).append(i)
-        kp = []
-        for (Dt, pr), idx in sorted(groups.items()):
+            if jb.method not in ("tikhonov", "rnn"):
+                raise ValueError(f"unknown method {jb.method}")
+            key = (jb.decoder.geom.Dt, "rnn", jb.rnn_steps) if jb.method == "rnn" else \
+                (jb.decoder.geom.Dt, pr, 0)
+            groups.setdefault(key, []).append(i)
+        kp, rn = [], {}
+        for (Dt, pr, steps), idx in sorted(groups.items()):
             g = self._prologue([jobs[i] for i in idx], Dt)
             if pr == "kpcg":
                 kp.append((idx, g))
+                continue
+            if pr == "rnn":
+                rn.setdefault(steps, []).append((idx, g))
                 continue
             X, nrm, info, skeep = self._solve_direct(g, pr)
             res, keep = self._synthesis(g, X, nrm, info, None, 0)
@@ -218,6 +247,17 @@ class BatchDecoder:
                 all_idx += idx
             h.keep.append(skeep)
             h.checks.append((all_idx, (nrm, info), "kpcg"))
+        for steps, parts in sorted(rn.items()):
+            gs = [g for _, g in parts]
+            X, st, boff, skeep = self._solve_rnn(gs, steps)
+            for (idx, g), (b0, e0) in zip(parts, boff):
+                zero = torch.zeros((g.B, 2), dtype=F64, device=self.device)
+                info = torch.zeros(g.B, dtype=I32, device=self.device)
+                res, keep = self._synthesis(g, X[e0:e0 + g.Dtot], zero, info, st[4 * b0:4 * (b0 + g.B)], 0)
+                for i, r in zip(idx, res):
+                    h.results[i] = r
+                h.keep.append((g.keep, keep, zero, info))
+            h.keep.append(skeep)
         return h
 
     def finalize(self, h: "BatchHandle") -> List[DecodeResult]:
@@ -483,7 +523,7 @@ class BatchDecoder:
     def _skinny(self, rows):
         """Skinny grouped product: rows of (A, B, C, T, active, sam, sak, ldb, ldc, tstride,
         M, K, Nc) with pointers as ints (T = 0: no row factor; active = 0: always run).
-        Returns a launcher and the buffers it needs alive (descriptors, split-K workspace)."""
+        Returns the launch record (descriptors, max_M, max_K, max_Nc, split-K workspace)."""
         wsz = [skinny_workspace(int(r[10]), int(r[11]), int(r[12])) for r in rows]
         ws = torch.empty(max(1, sum(wsz)), dtype=F64, device=self.device)
         d = np.zeros(len(rows), SK_DTYPE)
@@ -493,35 +533,22 @@ class BatchDecoder:
             d[i] = (A, Bp, C, T, ws.data_ptr() + 8 * off if wsz[i] else 0, act, sam, sak, ldb, ldc, tst,
                     M, K, Nc, 0)
             off += wsz[i]
-        dd = _to_dev(d, self.device)
-        B = len(rows)
-        maxM = max(int(r[10]) for r in rows)
-        maxK = max(int(r[11]) for r in rows)
-        maxNc = max(int(r[12]) for r in rows)
-
-        def run(s):
-            lib.vt_gemm_skinny(ptr(dd), B, maxM, maxK, maxNc, s)
-
-        return run, (dd, ws)
+        return _Skinny(_to_dev(d, self.device), len(rows), max(int(r[10]) for r in rows),
+                       max(int(r[11]) for r in rows), max(int(r[12]) for r in rows), ws)
 
     def _operator(self, blocks, src, dst, Z, dbj, nzj, active=None):
-        """Launcher of dst_b = A_b src_b for every block (blocks: (S, Tm_ptr, N, Ds, Dt)):
-        Z_j = T_j (S src)_j in one skinny product (T fused in its epilogue), dst = S^T Z in
-        a second -- 4 N D flops, S read twice, no Gram.  With `active` (per-block flag
-        pointers) converged blocks skip both products."""
+        """dst_b = A_b src_b for every block (blocks: (S, Tm_ptr, N, Ds, Dt)) as two skinny
+        launches: Z_j = T_j (S src)_j (T fused in the epilogue), dst = S^T Z -- 4 N D
+        flops, S read twice, no Gram.  With `active` (per-block flag pointers) converged
+        blocks skip both products."""
         act = active or [0] * len(blocks)
-        fwd, k1 = self._skinny([(S.data_ptr(), src.data_ptr() + 8 * int(dbj[b]), Z.data_ptr() + 8 * int(nzj[b]),
-                                 Tp, act[b], S.stride(0), 1, Dt, Dt, Dt * Dt, N, Ds, Dt)
-                                for b, (S, Tp, N, Ds, Dt) in enumerate(blocks)])
-        bwd, k2 = self._skinny([(S.data_ptr(), Z.data_ptr() + 8 * int(nzj[b]), dst.data_ptr() + 8 * int(dbj[b]),
-                                 0, act[b], 1, S.stride(0), Dt, Dt, 0, Ds, N, Dt)
-                                for b, (S, Tp, N, Ds, Dt) in enumerate(blocks)])
-
-        def run(s):
-            fwd(s)
-            bwd(s)
-
-        return run, (k1, k2)
+        fwd = self._skinny([(S.data_ptr(), src.data_ptr() + 8 * int(dbj[b]), Z.data_ptr() + 8 * int(nzj[b]),
+                             Tp, act[b], S.stride(0), 1, Dt, Dt, Dt * Dt, N, Ds, Dt)
+                            for b, (S, Tp, N, Ds, Dt) in enumerate(blocks)])
+        bwd = self._skinny([(S.data_ptr(), Z.data_ptr() + 8 * int(nzj[b]), dst.data_ptr() + 8 * int(dbj[b]),
+                             0, act[b], 1, S.stride(0), Dt, Dt, 0, Ds, N, Dt)
+                            for b, (S, Tp, N, Ds, Dt) in enumerate(blocks)])
+        return fwd, bwd
 
     def _spatial_inverse(self, Ss, N, Ds, c_ptrs, lam, s):
         """Mi_b = (S_b^T diag(c_b) S_b + lam_b I)^-1 for every block, FP64: Cholesky, then
@@ -638,33 +665,31 @@ class BatchDecoder:
         st = torch.zeros(4 * Bt, dtype=F64, device=dev)
         st.view(Bt, 4)[:, 2] = 1.0          # active for the first preconditioner application
         act = [st.data_ptr() + 8 * (4 * b + 2) for b in range(Bt)]
-        pre_run, pre = self._skinny([(Mi.data_ptr() + 8 * int(mbase[b]), R.data_ptr() + 8 * int(dbj[b]),
-                                      Zp.data_ptr() + 8 * int(dbj[b]), TiT_ptrs[b], act[b],
-                                      Ds[b], 1, Dt[b], Dt[b], 0, Ds[b], Ds[b], Dt[b]) for b in range(Bt)])
+        pre = self._skinny([(Mi.data_ptr() + 8 * int(mbase[b]), R.data_ptr() + 8 * int(dbj[b]),
+                             Zp.data_ptr() + 8 * int(dbj[b]), TiT_ptrs[b], act[b],
+                             Ds[b], 1, Dt[b], Dt[b], 0, Ds[b], Ds[b], Dt[b]) for b in range(Bt)])
         self._toc("kpcg_setup", e_setup, float(np.sum(2.0 * N * Ds * Ds + 2.0 * Ds.astype(np.float64) ** 3)))
         e_cg = self._tic()
         X = torch.empty(Dtot, dtype=F64, device=dev)
         Pv = torch.empty(Dtot, dtype=F64, device=dev)
         Q = torch.empty(Dtot, dtype=F64, device=dev)
         Z = torch.empty(int(nzj[-1]), dtype=F64, device=dev)
-        op_p, kp1 = self._operator(blocks, Pv, Q, Z, dbj, nzj, act)
-        op_x, kp2 = self._operator(blocks, X, Q, Z, dbj, nzj)
+        fp, bp = self._operator(blocks, Pv, Q, Z, dbj, nzj, act)
+        fx, bx = self._operator(blocks, X, Q, Z, dbj, nzj)
         iters = int(self.cg_iters)
         hs = iters + 1
         hist = torch.zeros((Bt, hs), dtype=F64, device=dev)
-        pre_run(s)
+        maxNc = int(Dt.max())
+        pre.run(s)
         lib.vt_kpcg_init(ptr(R), ptr(Zp), ptr(Pv), ptr(X), ptr(seg), Bt, ptr(st), ptr(hist), hs, s)
-        stop2 = float(self.cg_stop) ** 2
-        for it in range(iters):
-            op_p(s)
-            lib.vt_kpcg_step_a(ptr(X), ptr(R), ptr(Pv), ptr(Q), ptr(lam), ptr(seg), Bt, ptr(st), ptr(hist),
-                               hs, it, stop2, s)
-            pre_run(s)
-            lib.vt_kpcg_step_b(ptr(R), ptr(Zp), ptr(Pv), ptr(seg), Bt, ptr(st), s)
+        lib.vt_kpcg_iterate(*fp.args(), *bp.args(), *pre.args(), Bt, maxNc, ptr(X), ptr(R), ptr(Pv),
+                            ptr(Q), ptr(Zp), ptr(lam), ptr(seg), ptr(st), ptr(hist), hs, iters,
+                            float(self.cg_stop) ** 2, s)
         # true FP64 residual of the final iterate (the guard's input)
         nrm = torch.zeros((Bt, 2), dtype=F64, device=dev)
         lib.vt_seg_sumsq(ptr(r), ptr(seg), Bt, ptr(nrm), 2, s)
-        op_x(s)
+        fx.run(s)
+        bx.run(s)
         res = torch.empty(Dtot, dtype=F64, device=dev)
         lib.vt_residual_batched(ptr(res), ptr(r), ptr(Q), ptr(X), ptr(lam), ptr(elem_job), Dtot, s)
         lib.vt_seg_sumsq(ptr(res), ptr(seg), Bt, ptr(nrm[:, 1:]), 2, s)
@@ -672,5 +697,59 @@ class BatchDecoder:
         # once, about a dozen passes over the D-space vectors
         nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + Ds * Ds + 12.0 * D)
         self._toc("kpcg_iterate", e_cg, float(nbytes) * iters)
-        keep += [r, lam, seg, elem_job, Mi, mkeep, R, Zp, pre, X, Pv, Q, Z, kp1, kp2, st, hist, res]
+        keep += [r, lam, seg, elem_job, Mi, mkeep, R, Zp, pre, X, Pv, Q, Z, fp, bp, fx, bx, st, hist, res]
         return X, nrm, info, st, boff, keep
+
+    def _joint_blocks(self, gs: List[_Prologue]):
+        """Blocks of several prologue groups in one joint numbering."""
+        blocks, boff, b0, e0 = [], [], 0, 0
+        for g in gs:
+            E = g.Dt * g.Dt
+            for b, d in enumerate(g.decs):
+                blocks.append((d.S, g.Tm.data_ptr() + 8 * int(g.nbase[b]) * E, int(g.N[b]), int(g.Ds[b]), g.Dt))
+            boff.append((b0, e0))
+            b0 += g.B
+            e0 += g.Dtot
+        N = np.array([b[2] for b in blocks], np.int64)
+        Ds = np.array([b[3] for b in blocks], np.int64)
+        Dt = np.array([b[4] for b in blocks], np.int64)
+        return blocks, boff, N, Ds, Dt
+
+    def _solve_rnn(self, gs: List[_Prologue], steps: int):
+        """Gradient-flow decoder for every block of the groups, as one joint launch
+        sequence: eta_b = 1/lambda_max(A_b + lam_b I) by 50 power steps, then steps + 1
+        iterations X <- X + eta_b (r - (A_b + lam_b I) X) from 0 through the structured
+        operator.  Returns the joint X, the per-block state [eta, 0, 0, iterations], the
+        per-group offsets and the buffers to keep."""
+        dev = self.device
+        s = stream_ptr()
+        e0 = self._tic()
+        blocks, boff, N, Ds, Dt = self._joint_blocks(gs)
+        Bt = len(blocks)
+        D = Ds * Dt
+        dbj = np.concatenate([[0], np.cumsum(D)])
+        nzj = np.concatenate([[0], np.cumsum(N * Dt)])
+        Dtot = int(dbj[-1])
+        r = torch.cat([g.r for g in gs]) if len(gs) > 1 else gs[0].r
+        lam = torch.cat([g.lam for g in gs]) if len(gs) > 1 else gs[0].lam
+        seg = _h2d(dbj.astype(np.int64), dev)
+        elem_job = _h2d(np.repeat(np.arange(Bt, dtype=np.int32), D), dev)
+        X = torch.empty(Dtot, dtype=F64, device=dev)
+        V = torch.ones(Dtot, dtype=F64, device=dev)
+        Q = torch.empty(Dtot, dtype=F64, device=dev)
+        Z = torch.empty(int(nzj[-1]), dtype=F64, device=dev)
+        fv, bv = self._operator(blocks, V, Q, Z, dbj, nzj)
+        fx, bx = self._operator(blocks, X, Q, Z, dbj, nzj)
+        st = torch.zeros(4 * Bt, dtype=F64, device=dev)
+        eta = torch.empty(Bt, dtype=F64, device=dev)
+        maxNc = int(Dt.max())
+        lib.vt_kron_power(*fv.args(), *bv.args(), Bt, maxNc, ptr(V), ptr(Q), ptr(lam), ptr(seg), POWER_ITERS,
+                          ptr(eta), s)
+        lib.vt_kron_rnn(*fx.args(), *bx.args(), Bt, maxNc, ptr(X), ptr(r), ptr(Q), ptr(lam), ptr(eta),
+                        ptr(elem_job), Dtot, steps, s)
+        sv = st.view(Bt, 4)
+        sv[:, 0] = eta
+        sv[:, 3] = float(steps + 1)
+        nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + 4.0 * D)
+        self._toc("rnn_iterate", e0, float(nbytes) * (steps + 1 + POWER_ITERS))
+        return X, st, boff, [r, lam, seg, elem_job, X, V, Q, Z, fv, bv, fx, bx, st, eta]

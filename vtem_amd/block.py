@@ -387,6 +387,19 @@ class Decoder:
         self._keep_solve = (A32, inv, mats, res, AX)
         return X, lam, info, nrm
 
+    def power_eta(self, Tm, lam, iters: int = 50) -> float:
+        """1/rho, rho the power-iteration estimate (from the all-ones vector) of the largest
+        eigenvalue of A + lam I: the RNN step 1/lambda_max."""
+        g = self.geom
+        v = torch.ones(g.D, dtype=F64, device=self.device)
+        q = torch.empty_like(v)
+        seg = torch.tensor([0, g.D], dtype=torch.int64, device=self.device)
+        eta = torch.empty(1, dtype=F64, device=self.device)
+        for _ in range(iters):
+            self.apply_A(Tm, v, out=q)
+            lib.vt_power_step(ptr(v), ptr(q), ptr(lam), ptr(seg), 1, ptr(eta), stream_ptr())
+        return float(eta.item())
+
     def solve_rnn(self, Tm, r, lam, steps: int, eta: float):
         """X <- X + eta (r - (A + lam I) X), steps+1 iterations from 0 (the D-space image of
         vtem/neural_network.py:78-82), with A applied through its Kronecker structure."""
@@ -425,9 +438,8 @@ class Decoder:
                     fallback = True
         elif method == "rnn":
             lam = self.lam_exact(Tm, n_dev, lam_rel)
-            if rnn_eta is None:
-                raise ValueError("rnn_eta is required for the RNN decoder")
-            X = self.solve_rnn(Tm, r, lam, rnn_steps, rnn_eta)
+            eta = self.power_eta(Tm, lam) if rnn_eta is None else rnn_eta
+            X = self.solve_rnn(Tm, r, lam, rnn_steps, eta)
             info, nrm = None, None
         else:
             raise ValueError(f"unknown method {method}")

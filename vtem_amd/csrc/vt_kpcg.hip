@@ -340,3 +340,143 @@ VT_EXPORT int vt_kpcg_step_b(const double *r, const double *z, double *p, const 
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
+
+// ------------------------------------------------------------------------------------
+// Whole iteration loops on the host side of the library (one C call enqueues every
+// launch of the solve, with no per-launch round trip through the caller).
+// The operator and preconditioner are skinny grouped products (vt_gemm_skinny).
+// ------------------------------------------------------------------------------------
+extern "C" int vt_gemm_skinny(const void *descs, int batch, int max_M, int max_K, int max_Nc,
+                              void *stream);
+
+struct VtSkinnyLaunch {
+    const void *descs;
+    int max_M, max_K;
+};
+
+static int skinny_run(const VtSkinnyLaunch &l, int batch, int max_Nc, void *stream)
+{
+    return vt_gemm_skinny(l.descs, batch, l.max_M, l.max_K, max_Nc, stream);
+}
+
+#define VT_TRY(call)                 \
+    do {                             \
+        int st__ = (call);           \
+        if (st__ != VT_OK) return st__; \
+    } while (0)
+
+// `iters` CG iterations (after vt_kpcg_init): launches = {fwd, bwd, pre} as
+// {descs, max_M, max_K} triples for the operator halves and the preconditioner.
+VT_EXPORT int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M,
+                              int bwd_K, const void *pre, int pre_M, int pre_K, int batch, int max_Nc,
+                              double *x, double *r, double *p, double *q, double *z, const double *lam,
+                              const long long *seg, double *st, double *hist, int hist_stride,
+                              int iters, double stop_rel2, void *stream)
+{
+    VT_CHECK_ARG(fwd && bwd && pre && iters >= 0 && iters < hist_stride);
+    const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K}, Pr{pre, pre_M, pre_K};
+    for (int it = 0; it < iters; ++it) {
+        VT_TRY(skinny_run(F, batch, max_Nc, stream));
+        VT_TRY(skinny_run(Bk, batch, max_Nc, stream));
+        VT_TRY(vt_kpcg_step_a(x, r, p, q, lam, seg, batch, st, hist, hist_stride, it, stop_rel2, stream));
+        VT_TRY(skinny_run(Pr, batch, max_Nc, stream));
+        VT_TRY(vt_kpcg_step_b(r, z, p, seg, batch, st, stream));
+    }
+    return VT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Gradient-flow (RNN) decoder in coefficient space: X <- X + eta_b (r - (A + lam I) X),
+// the image under u = Phi^T c of the reference's c <- c + eta (q - G c).  eta_b = 1/rho_b
+// with rho_b the power-iteration estimate of the largest eigenvalue of A + lam I.
+// ------------------------------------------------------------------------------------
+// power step per block: w = q + lam v; rho = <v,w>/<v,v>; v = w/|w|; eta = 1/rho
+__global__ void __launch_bounds__(256)
+power_step_kernel(double *__restrict__ v, const double *__restrict__ q, const double *__restrict__ lam,
+                  const long long *__restrict__ seg, double *__restrict__ eta)
+{
+    __shared__ double red[256];
+    const double l = lam[2 * blockIdx.x];
+    const long long a = seg[blockIdx.x], e = seg[blockIdx.x + 1];
+    double vw = 0.0, vv = 0.0, ww = 0.0;
+    for (long long i = a + threadIdx.x; i < e; i += 256) {
+        const double w = q[i] + l * v[i];
+        vw += v[i] * w;
+        vv += v[i] * v[i];
+        ww += w * w;
+    }
+    vw = block_sum(vw, red);
+    vv = block_sum(vv, red);
+    ww = block_sum(ww, red);
+    const double inv = ww > 0.0 ? 1.0 / sqrt(ww) : 0.0;
+    for (long long i = a + threadIdx.x; i < e; i += 256) v[i] = (q[i] + l * v[i]) * inv;
+    if (threadIdx.x == 0) eta[blockIdx.x] = (vw > 0.0 && vv > 0.0) ? vv / vw : 0.0;
+}
+
+__global__ void zero_kernel(double *__restrict__ x, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] = 0.0;
+}
+
+// x += eta_b (r - q - lam_b x)
+__global__ void rnn_update_kernel(double *__restrict__ x, const double *__restrict__ r,
+                                  const double *__restrict__ q, const double *__restrict__ lam,
+                                  const double *__restrict__ eta, const int *__restrict__ elem_job,
+                                  long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int b = elem_job[i];
+        x[i] += eta[b] * (r[i] - q[i] - lam[2 * b] * x[i]);
+    }
+}
+
+VT_EXPORT int vt_power_step(double *v, const double *q, const double *lam, const long long *seg,
+                            int batch, double *eta, void *stream)
+{
+    VT_CHECK_ARG(v && q && lam && seg && eta && batch >= 1 && batch <= 65535);
+    power_step_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(v, q, lam, seg, eta);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// eta (batch doubles) from `iters` power steps started at v (overwritten); the operator
+// launches read v and write q.
+VT_EXPORT int vt_kron_power(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
+                            int batch, int max_Nc, double *v, double *q, const double *lam,
+                            const long long *seg, int iters, double *eta, void *stream)
+{
+    VT_CHECK_ARG(fwd && bwd && v && q && lam && seg && eta && batch >= 1 && iters >= 1);
+    const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
+    hipStream_t s = (hipStream_t)stream;
+    for (int it = 0; it < iters; ++it) {
+        VT_TRY(skinny_run(F, batch, max_Nc, stream));
+        VT_TRY(skinny_run(Bk, batch, max_Nc, stream));
+        power_step_kernel<<<batch, 256, 0, s>>>(v, q, lam, seg, eta);
+        VT_LAUNCH_CHECK();
+    }
+    return VT_OK;
+}
+
+// steps+1 gradient-flow iterations from x = 0 (the operator launches read x, write q)
+VT_EXPORT int vt_kron_rnn(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
+                          int batch, int max_Nc, double *x, const double *r, double *q,
+                          const double *lam, const double *eta, const int *elem_job, long long n,
+                          int steps, void *stream)
+{
+    VT_CHECK_ARG(fwd && bwd && x && r && q && lam && eta && elem_job && n >= 0 && steps >= 0);
+    const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
+    hipStream_t s = (hipStream_t)stream;
+    unsigned g = vt_blocks(n, 256);
+    if (g > 4096) g = 4096;
+    zero_kernel<<<g, 256, 0, s>>>(x, n);   // a kernel, not a memset node, if captured
+    for (int it = 0; it <= steps; ++it) {
+        VT_TRY(skinny_run(F, batch, max_Nc, stream));
+        VT_TRY(skinny_run(Bk, batch, max_Nc, stream));
+        rnn_update_kernel<<<g, 256, 0, s>>>(x, r, q, lam, eta, elem_job, n);
+        VT_LAUNCH_CHECK();
+    }
+    return VT_OK;
+}

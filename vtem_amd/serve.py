@@ -49,17 +49,23 @@ class Request:
     batch: Optional[int] = None
 
 
-def footprint_bytes(job: BlockJob) -> int:
-    """In-flight HBM of one decode: FP32 factor + block inverses + moments + intervals +
-    split moments + frames + vectors (the decoder's S and the spike trains are resident
-    before admission and not counted)."""
+def footprint_bytes(job: BlockJob, form: str = "kpcg") -> int:
+    """In-flight HBM of one decode (the decoder's S and the spike trains are resident
+    before admission and not counted): intervals, moments T_j/t_j, output frames and the
+    solver's own state -- for the Gram forms the FP32 D x D factor, its block inverses and
+    the split moments; for "kpcg" the spatial preconditioner (two Ds x Ds) and about a
+    dozen D-vectors; for the RNN a handful of D-vectors."""
     d = job.decoder
     g = d.geom
-    D = g.D
-    nblk = (D + 63) // 64
+    D, Ds = g.D, g.Ds
     cap = job.spikes.spikes.shape[0] * d.N
-    return int(4 * D * D + 4 * nblk * 4096 + 8 * d.N * g.Dt * (g.Dt + 1) * 2 + 4 * d.N * g.Dt * g.Dt
-               + 20 * cap + 8 * g.frames * g.P + 8 * g.frames * g.Ds + 64 * D)
+    common = 20 * cap + 8 * d.N * g.Dt * (g.Dt + 3) + 8 * g.frames * g.P + 8 * g.frames * g.Ds
+    if job.method == "rnn":
+        return int(common + 8 * 6 * D)
+    if form == "kpcg":
+        return int(common + 16 * Ds * Ds + 8 * 64 * Ds + 8 * 12 * D)
+    nblk = (D + 63) // 64
+    return int(common + 4 * D * D + 4 * nblk * 4096 + 4 * d.N * g.Dt * g.Dt + 64 * D)
 
 
 class PrecisionPolicy:
@@ -86,7 +92,7 @@ class PrecisionPolicy:
         return self.low
 
     def record(self, job: BlockJob, precision: str, failed: bool):
-        if precision == "fp64":
+        if precision == "fp64" or job.method != "tikhonov":
             return
         st = self.stats.setdefault(self.key(job), [0, 0])
         st[0] += 1
@@ -194,7 +200,8 @@ class Server:
                 time.sleep(r.arrival - now)
             r.start = clock() - t0
             jb = r.job
-            r.result = jb.decoder.decode(jb.spikes, jb.t0, precision="fp64")
+            r.result = jb.decoder.decode(jb.spikes, jb.t0, precision="fp64", method=jb.method,
+                                         rnn_steps=jb.rnn_steps)
             torch.cuda.current_stream().synchronize()
             r.done = clock() - t0
         wall = clock() - t0
