@@ -336,6 +336,38 @@ int vt_kron_power(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bw
 int vt_power_step(double *v, const double *q, const double *lam, const long long *seg, int batch,
                   double *eta, void *stream);
 
+/* ------------------------------------------ spatial Dirichlet side, pseudoinverse -- */
+/* Mode table (P x 2M, row stride ld >= 2M, M = (2Mx+1)(2My+1)): row q*Px + p holds
+ * [cos theta_m | -sin theta_m], theta_m = mx Ox x_p + my Oy y_q, x_p = x_start + p dx,
+ * y_q = -(y_start + q dy), column m = (my+My)(2Mx+1) + (mx+Mx).  RF times this table is
+ * the direct DFT that replaces the cuFFT path of compute_Ds (vtem/vrf.py:333-377,
+ * fft2Ds kernel :1293-1350); [Re Ds | Im Ds] times its transpose is compute_dirich_space
+ * (vtem/vrf.py:380-499, kernels :1353-1433). */
+int vt_dirichlet_table(double *out, long long ld, int Px, int Py, double x_start, double dx,
+                       double y_start, double dy, int Mx, int My, double Ox, double Oy,
+                       void *stream);
+/* d_Ds (N x (2My+1) x (2Mx+1) complex128, interleaved) = scale (C[:, m] + i C[:, M+m]) from
+ * C = RF table (row stride ldc); mask 1 = the reference's ellipse (vtem/vrf.py:1331), 0 = all
+ * modes. */
+int vt_ds_assemble(double *Ds, const double *C, long long ldc, int N, int Mx, int My, int mask,
+                   double scale, void *stream);
+/* W[i,j] *= norm[i] norm[j] (put_norm of compute_Dsw, vtem/vrf.py:529-533, :540-559). */
+int vt_scale_outer(double *W, long long ldw, int n, const double *norm, void *stream);
+/* Symmetric eigendecomposition by rocSOLVER (the counterpart of the reference's CULA call,
+ * vtem/utils/linalg.py:366-523): w ascending; A's rows become the eigenvectors.  E: n
+ * doubles of workspace; info (device int) > 0 if the iteration did not converge.  -2 if
+ * no rocBLAS handle could be created. */
+int vt_eig_sym(double *A, int lda, int n, double *w, double *E, int *info, void *stream);
+/* y_k <- y_k / w_k where |w_k| > rcond max|w|, else 0: the truncation of solve_eq_sym
+ * (vtem/utils/linalg.py:591-631, sq_Kernel :790-831). */
+int vt_eig_filter(double *y, const double *w, int n, double rcond, void *stream);
+/* Stitching: for each block descriptor {const double *blk, *wt, *wy, *wx; int T, Py, Px,
+ * t0, y0, x0;} (56 bytes), out[t0+t, y0+y, x0+x] += wt[t] wy[y] wx[x] blk[t, y, x].  Blocks
+ * of one call must not overlap.  Replaces the tile / window accumulation of decode_video
+ * and VTDM (vtem/vtdm.py:236-302, :604-683). */
+int vt_stitch_accumulate(double *out, long long ld_t, long long ld_y, const void *descs,
+                         int nblocks, long long max_elems, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -426,13 +426,13 @@ def add_diag(G: np.ndarray, lamb: float) -> np.ndarray:
 def solve_eq_sym(G: np.ndarray, q: np.ndarray, rcond: float = 1e-8) -> np.ndarray:
     """Truncated pseudoinverse c = U diag(1/s_i if s_i > rcond s_0 else 0) U^T q
     (vtem/utils/linalg.py:591-631, sq_Kernel :790-831).  For symmetric G the singular
-    values are |eigenvalues| and U's columns are eigenvectors up to sign, so eigh gives
-    the same c."""
+    values are |eigenvalues| and the right singular vectors are the eigenvectors times
+    sign(eigenvalue), so the pseudoinverse is U diag(1/w_i) U^T over the kept |w_i|."""
     w, U = np.linalg.eigh(G)
     s = np.abs(w)
     qq = U.T @ q
     keep = s > rcond * s.max()
-    qq = np.where(keep, qq / np.where(keep, s, 1.0), 0.0)
+    qq = np.where(keep, qq / np.where(keep, w, 1.0), 0.0)
     return U @ qq
 
 
@@ -565,3 +565,100 @@ def snr_db(u: np.ndarray, urec: np.ndarray) -> float:
     u = np.asarray(u, np.float64)
     e = u - np.asarray(urec, np.float64)
     return 10.0 * math.log10(float(np.sum(u * u)) / float(np.sum(e * e)))
+
+
+def vtdm_windows(decode_window, start_time: float, end_time: float, dt: float,
+                 stitching: bool = False, stitch_interval=None) -> np.ndarray:
+    """Temporal window loop of VTDM (vtem/vtdm.py:570-692): windows of stitch_interval
+    overlapping by 20 % (4 frames when that is under 3 frames and the window has at least
+    8), each decoded by decode_window(start, end) -> (frames, Py, Px) sampled at dt from
+    the window start, then written with a Hann crossfade over the overlaps."""
+    end_time = float(end_time)
+    if stitching:
+        stitch_interval = 20 * dt if stitch_interval is None else float(stitch_interval)
+        stitch_interval = min(stitch_interval, end_time)
+        end = round(float(min(start_time + stitch_interval, end_time)) / dt) * dt
+        overlap = 0.2 * stitch_interval
+    else:
+        stitch_interval = end_time
+        end = end_time
+        overlap = 0.0
+    total_l = int(round((end_time - start_time) / dt))
+    overlap_l = int(round(overlap / dt))
+    if overlap_l < 3:
+        if round(stitch_interval / dt) >= 8:
+            overlap_l, overlap = 4, 4 * dt
+        else:
+            overlap_l, overlap = 0, 0.0
+    interval_l = int(round(stitch_interval / dt))
+    lw = hann_ramp(overlap_l)[:, None, None]
+    rw = 1.0 - lw
+    start = float(start_time)
+    u = None
+    i = 0
+    prev = None
+    while True:
+        u_rec = decode_window(start, end)
+        if u is None:
+            u = np.empty((total_l,) + u_rec.shape[1:])
+        if overlap_l:
+            if i == 0:
+                if end_time - end < dt:
+                    return u_rec
+                u[0:interval_l - overlap_l] = u_rec[0:-overlap_l]
+                prev = u_rec[-overlap_l:]
+            else:
+                l_ind = i * (interval_l - overlap_l)
+                r_ind = (i + 1) * (interval_l - overlap_l)
+                u[l_ind:l_ind + overlap_l] = u_rec[0:overlap_l] * lw + prev * rw
+                if end_time - end < dt:
+                    u[l_ind + overlap_l:] = u_rec[-u.shape[0] + l_ind + overlap_l:]
+                    return u
+                u[l_ind + overlap_l:r_ind] = u_rec[overlap_l:interval_l - overlap_l]
+                prev = u_rec[-overlap_l:]
+        else:
+            u[i * interval_l:i * interval_l + u_rec.shape[0]] = u_rec
+            if end_time - end < dt:
+                return u
+        start = start + round(stitch_interval / dt) * dt - round(overlap / dt) * dt
+        end = min(start + round(stitch_interval / dt) * dt, end_time)
+        if end_time - end < overlap:
+            end = end_time
+        i += 1
+
+
+def decode_tiles(decode_tile, domain, dx: float, dy: float, spatial_interval, total_l: int) -> np.ndarray:
+    """Spatial tile loop of decode_video (vtem/vtdm.py:179-309): tiles of spatial_interval
+    pixels overlapping by 20 %, placed from the right / top edges (the last tile clamped to
+    the domain), each decoded by decode_tile(domain) -> (total_l, py, px) and accumulated
+    with [Hann ramp, ones, 1 - Hann ramp] windows on the sides shared with other tiles."""
+    xs, xe, ys, ye = domain
+    decPx, decPy = int(round((xe - xs) / dx)), int(round((ye - ys) / dy))
+    sx, sy = min(spatial_interval[0], decPx), min(spatial_interval[1], decPy)
+    ox, oy = int(round(0.2 * sx)), int(round(0.2 * sy))
+    u = np.zeros((total_l, decPy, decPx))
+    start_y, end_y = max(ye - sy * dy, ys), ye
+    y0, y1 = 0, sy
+    while True:
+        start_x, end_x = max(xe - sx * dx, xs), xe
+        x0, x1 = decPx - sx, decPx
+        while True:
+            left = 0 if x0 == 0 else ox
+            right = 0 if x1 == decPx else ox
+            top = 0 if y0 == 0 else oy
+            bottom = 0 if y1 == decPy else oy
+            w = np.outer(tile_window(y1 - y0, top, bottom), tile_window(x1 - x0, left, right))
+            u[:, y0:y1, x0:x1] += decode_tile([start_x, end_x, start_y, end_y]) * w[None]
+            if start_x == xs:
+                break
+            start_x = max(start_x - sx * dx + ox * dx, xs)
+            end_x = end_x - sx * dx + ox * dx
+            x0 = max(x0 - sx + ox, 0)
+            x1 = x1 - sx + ox
+        if start_y == ys:
+            break
+        start_y = max(start_y - sy * dy + oy * dy, ys)
+        end_y = end_y - sy * dy + oy * dy
+        y0 = y1 - oy
+        y1 = min(y0 + sy, decPy)
+    return u

@@ -21,8 +21,10 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvtem_hip.so")
 ARCH = os.environ.get("VTEM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["vt_encode.hip", "vt_gemm.hip", "vt_decode.hip", "vt_gram.hip", "vt_chol.hip", "vt_kpcg.hip",
-           "vt_batch.hip"]
+SOURCES = ["vt_encode.hip", "vt_gemm.hip", "vt_decode.hip", "vt_gram.hip", "vt_chol.hip",
+           "vt_kpcg.hip", "vt_batch.hip", "vt_dirichlet.hip"]
+# rocSOLVER (symmetric eigendecomposition of the truncated pseudoinverse) and its rocBLAS
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
          "-Xclang", "-packed-fp32-ops", "-fvisibility=hidden", "-Wno-unused-result",
@@ -51,7 +53,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, *LIBS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
