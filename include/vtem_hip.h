@@ -264,9 +264,9 @@ int vt_diag_add_batched(const void *descs, int batch, int dtype, const double *l
  * by T + m*tstride (C_m <- T_m C_m); a problem whose `active` flag (optional double) reads
  * 0 is skipped.  descs = {const double *A, *B; double *C; const double *T; double *ws;
  * const double *active; long long sam, sak, ldb, ldc, tstride; int M, K, Nc, pad;} (104
- * bytes).  A problem with K > 1008 splits its reduction (in at most 8 parts, a function of
- * its own K) and needs the ws that vt_gemm_skinny_workspace_size reports; max_K bounds
- * every problem's K.  The structured operator S^T (T_j (S X)) of the coefficient-space
+ * bytes).  A problem with K > 240 splits its reduction (one part per 128 terms, at most
+ * 8, a function of its own K) and needs the ws that vt_gemm_skinny_workspace_size
+ * reports; max_K bounds every problem's K.  The structured operator S^T (T_j (S X)) of the coefficient-space
  * Gram (vtem/dirichlet.py:62-85 builds that Gram in spike space) is two such products. */
 int vt_gemm_skinny(const void *descs, int batch, int max_M, int max_K, int max_Nc, void *stream);
 /* Doubles of ws one skinny problem (M x K, Nc columns) needs, written to the HOST pointer

@@ -380,7 +380,7 @@ VT_EXPORT int vt_seg_counts(const int *offsets, const int *base, int nseg, int *
 // T_b + m*tstride (C_m <- T_m C_m): with tstride = Nc*Nc this fuses Z_j = T_j (S P)_j
 // into the product; with tstride = 0 every row shares one matrix (right factor C T^T).
 //
-// One workgroup owns 64 rows of one K split; its 4 waves take 16-wide K chunks in turn
+// One workgroup owns 16 or 64 rows of one K split; its 4 waves take 16-wide K chunks in turn
 // (wave w: chunks w, w+4, ...).  Operands go from global memory straight into the MFMA
 // f64 16x16x4 operand layout: lane (r = l%16, g = l/16) loads the 4 consecutive k values
 // c*16 + 4g + s (s = 0..3) of its row, and MFMA step s pairs them with the B rows of the
@@ -388,7 +388,7 @@ VT_EXPORT int vt_seg_counts(const int *offsets, const int *base, int nseg, int *
 // wave partials are added in LDS in a fixed order ((w0 + w2) + (w1 + w3)).
 //
 // Long reductions are split: a problem with nch = ceil(K/16) chunks uses
-// ks = clamp(nch/32, 1, 8) splits (a function of its own K only, so a problem's rounding
+// ks = clamp(nch/8, 1, 8) splits (a function of its own K only, so a problem's rounding
 // does not depend on the batch), each writing a partial tile to its workspace; a second
 // pass adds the ks partials in order and applies the row factor.
 // ------------------------------------------------------------------------------------
@@ -405,19 +405,19 @@ struct SkDesc {
 __host__ __device__ __forceinline__ int skinny_splits(int K)
 {
     const int nch = (K + 15) / 16;
-    const int ks = nch / 32;
+    const int ks = nch / 8;
     return ks < 1 ? 1 : (ks > 8 ? 8 : ks);
 }
 
 // operands of one 16-wide K chunk for lane (r, g); chunks at or beyond c_end load zeros
-template <int NT>
+template <int NT, int RT>
 __device__ __forceinline__ void skinny_load(const SkDesc &d, int m0, int c, int c_end, int r, int g,
-                                            int Nc, double (&a)[4][4], double (&bv)[NT][4])
+                                            int Nc, double (&a)[RT][4], double (&bv)[NT][4])
 {
     const int kb = c * 16 + 4 * g;
     const bool cin = c < c_end;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < RT; ++t) {
         const int m = m0 + 16 * t + r;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -436,14 +436,14 @@ __device__ __forceinline__ void skinny_load(const SkDesc &d, int m0, int c, int 
     }
 }
 
-template <int NT>
-__device__ __forceinline__ void skinny_mma(v4d (&acc)[4][NT], const double (&a)[4][4],
+template <int NT, int RT>
+__device__ __forceinline__ void skinny_mma(v4d (&acc)[RT][NT], const double (&a)[RT][4],
                                            const double (&bv)[NT][4])
 {
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int j = 0; j < NT; ++j)
                 acc[t][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][s], bv[j][s], acc[t][j], 0, 0, 0);
@@ -458,20 +458,23 @@ __device__ __forceinline__ double skinny_row_factor(const SkDesc &d, int m, int 
     return v;
 }
 
-template <int NT>
+// RT row tiles of 16 per workgroup (the row tiling does not change any sum's order, so the
+// launch may pick it from the batch size)
+template <int NT, int RT>
 __global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ descs)
 {
-    __shared__ double red[2][64][33];
+    constexpr int R = 16 * RT;
+    __shared__ double red[2][R][33];
     const SkDesc d = descs[blockIdx.y];
-    const int m0 = blockIdx.x * 64;
+    const int m0 = blockIdx.x * R;
     const int ks = skinny_splits(d.K), z = blockIdx.z, Nc = d.Nc;
     if (m0 >= d.M || z >= ks) return;
     if (d.active && *d.active == 0.0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r = lane & 15, g = lane >> 4;
-    v4d acc[4][NT];
+    v4d acc[RT][NT];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < RT; ++t)
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[t][j] = (v4d){0, 0, 0, 0};
     const int nch = (d.K + 15) / 16;
@@ -479,16 +482,16 @@ __global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ 
     const int c_beg = z * per, c_end = min(nch, c_beg + per);
     // two chunks in flight per wave: c and c + 4
     for (int c = c_beg + wave; c < c_end; c += 8) {
-        double a0[4][4], b0[NT][4], a1[4][4], b1[NT][4];
-        skinny_load<NT>(d, m0, c, c_end, r, g, Nc, a0, b0);
-        skinny_load<NT>(d, m0, c + 4, c_end, r, g, Nc, a1, b1);
-        skinny_mma<NT>(acc, a0, b0);
-        skinny_mma<NT>(acc, a1, b1);
+        double a0[RT][4], b0[NT][4], a1[RT][4], b1[NT][4];
+        skinny_load<NT, RT>(d, m0, c, c_end, r, g, Nc, a0, b0);
+        skinny_load<NT, RT>(d, m0, c + 4, c_end, r, g, Nc, a1, b1);
+        skinny_mma<NT, RT>(acc, a0, b0);
+        skinny_mma<NT, RT>(acc, a1, b1);
     }
     // cross-wave reduction: (w0 + w2) + (w1 + w3)
     if (wave >= 2) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int j = 0; j < NT; ++j)
 #pragma unroll
@@ -498,7 +501,7 @@ __global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ 
     __syncthreads();
     if (wave < 2) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int j = 0; j < NT; ++j)
 #pragma unroll
@@ -508,7 +511,7 @@ __global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ 
     __syncthreads();
     if (wave == 1) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int j = 0; j < NT; ++j)
 #pragma unroll
@@ -518,7 +521,7 @@ __global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ 
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int j = 0; j < NT; ++j)
 #pragma unroll
@@ -528,7 +531,7 @@ __global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ 
                 }
     }
     __syncthreads();
-    const int rows = min(64, d.M - m0);
+    const int rows = min(R, d.M - m0);
     if (ks > 1) {
         double *W = d.ws + (long long)z * d.M * Nc;
         for (int e = tid; e < rows * Nc; e += 256) {
@@ -582,14 +585,25 @@ VT_EXPORT int vt_gemm_skinny(const void *descs, int batch, int max_M, int max_K,
                  max_Nc <= 32);
     if (max_M == 0) return VT_OK;
     const int ks = skinny_splits(max_K);
-    dim3 grid(vt_blocks(max_M, 64), batch, ks);
     hipStream_t s = (hipStream_t)stream;
     const SkDesc *d = (const SkDesc *)descs;
-    if (max_Nc <= 16)
-        skinny_kernel<1><<<grid, 256, 0, s>>>(d);
-    else
-        skinny_kernel<2><<<grid, 256, 0, s>>>(d);
+    // small launches take 16-row tiles (4x the workgroups), large ones 64-row tiles
+    const long long wg64 = (long long)batch * vt_blocks(max_M, 64) * ks;
+    if (wg64 < 1024) {
+        dim3 grid(vt_blocks(max_M, 16), batch, ks);
+        if (max_Nc <= 16)
+            skinny_kernel<1, 1><<<grid, 256, 0, s>>>(d);
+        else
+            skinny_kernel<2, 1><<<grid, 256, 0, s>>>(d);
+    } else {
+        dim3 grid(vt_blocks(max_M, 64), batch, ks);
+        if (max_Nc <= 16)
+            skinny_kernel<1, 4><<<grid, 256, 0, s>>>(d);
+        else
+            skinny_kernel<2, 4><<<grid, 256, 0, s>>>(d);
+    }
     if (ks > 1) skinny_reduce_kernel<<<dim3(vt_blocks(max_M, 8), batch), 256, 0, s>>>(d);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
+
