@@ -71,6 +71,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-sweep", action="store_true", help="skip the latency sweep")
     p.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
+    p.add_argument("--no-precision", action="store_true", help="skip the 128x128 precision report")
+    p.add_argument("--precision-only", action="store_true", help="only the 128x128 precision report")
     p.add_argument("--roofline-only", action="store_true",
                    help="only time the dominant kernel at the workload's shapes (for rocprofv3 runs)")
     return p.parse_args()
@@ -154,6 +156,81 @@ def dominant_kernel(items, max_batch, reps=200):
             "blocks": len(blocks), "avg_us": us, "bytes": nbytes, "achieved_GBs": nbytes / us / 1e3}
 
 
+def precision_report(seed=0):
+    """Precision form (128x128 px, 2 s, Lx=Ly=12, Lt=8, D = 10625, 8192 neurons of the
+    config-1 laws): Gram relative Frobenius error of each low-precision Gram against the
+    FP64 Gram, SNR of each solve form against the video and against the plain FP64 path,
+    and the Gram's MFMA throughput (algorithmic flops N D (D+1), HIP events)."""
+    import torch
+    from vtem_amd import synthetic as syn
+    from vtem_amd.basis import Geometry
+    from vtem_amd.batch import BatchDecoder, BlockJob
+    from vtem_amd.block import Decoder, Encoder, dev, spatial_coeffs
+    geom = Geometry(128, 128, 12, 12, 8, 2.0)
+    vid = syn.make_video(128, 128, 2.0, 100.0, 12, 12, 8, seed=seed)
+    bank = syn.bank_dyadic(128.0, 128.0)
+    iaf = syn.iaf_params(bank.num_neurons, 2.5, 1.0, 0.04, 0.004, np.random.default_rng(seed + 1))
+    enc = Encoder(geom, bank)
+    st = enc.encode(dev(vid.frames.reshape(geom.frames, -1)), iaf)
+    dec = Decoder(geom, spatial_coeffs(enc.RF, geom), iaf.kappa, iaf.delta, iaf.bias)
+    u = vid.frames.reshape(geom.frames, -1)
+    tk1, tk2, nid, off = dec.intervals(st)
+    q, Tm, tv = dec.moments(tk1, tk2, nid, off)
+    n = int(off[dec.N].item())
+    D, N = geom.D, dec.N
+    flops = float(N) * D * (D + 1)
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()                                   # warm
+        torch.cuda.synchronize()
+        e0.record()
+        out = fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return out, e0.elapsed_time(e1)
+
+    A64, t64 = timed(lambda: dec.gram(Tm))
+    L64 = torch.tril(A64)
+    ref = float(torch.linalg.norm(L64))
+    forms = {"fp64": {"gram_ms": round(t64, 3), "gram_tflops": round(flops / t64 / 1e9, 2),
+                      "gram_frac_of_fp64_peak": round(flops / t64 / 1e9 / 78.6, 4)}}
+    del L64
+    for mode, name, peak in ((0, "bf16x3", 2500.0 / 3), (1, "fp16", 2500.0)):
+        A, t = timed(lambda: dec.gram_lp(Tm, mode))
+        err = float(torch.linalg.norm(torch.tril(A.double() - A64))) / ref
+        forms[name] = {"gram_rel_frobenius_err": float(f"{err:.3g}"), "gram_ms": round(t, 3),
+                       "gram_tflops": round(flops / t / 1e9, 2),
+                       "gram_frac_of_mfma_peak": round(flops / t / 1e9 / peak, 4),
+                       "peak_tflops": round(peak, 1)}
+        del A
+    del A64
+    torch.cuda.empty_cache()
+    plain = dec.decode(st, precision="fp64")
+    fr64 = plain.frames.cpu().numpy()
+    s64 = snr_db(u, fr64)
+    forms["fp64"]["snr_db"] = round(s64, 4)
+    for name in ("mixed", "bf16x3", "fp16"):
+        r = dec.decode(st, precision=name, refine_iters=6)
+        h = np.sqrt(r.resid.cpu().numpy()) if r.resid is not None else None
+        f = forms.setdefault(name, {})
+        f["snr_db"] = round(snr_db(u, r.frames.cpu().numpy()), 4)
+        f["d_snr_vs_fp64_db"] = float(f"{abs(f['snr_db'] - s64):.3g}")
+        f["final_rel_residual"] = None if h is None else float(f"{h[-1] / h[0]:.3g}")
+        f["fp64_fallback"] = bool(r.fallback)
+    torch.cuda.synchronize()
+    res, tk = timed(lambda: BatchDecoder(precision="kpcg").decode([BlockJob(dec, st)]))
+    r = res[0]
+    h = r.resid.cpu().numpy()
+    forms["kpcg"] = {"snr_db": round(snr_db(u, r.frames.cpu().numpy()), 4),
+                     "d_snr_vs_fp64_db": float(f"{abs(snr_db(u, r.frames.cpu().numpy()) - s64):.3g}"),
+                     "iterations": int(r.state.cpu().numpy()[3]),
+                     "final_rel_residual": float(f"{math.sqrt(h[1] / h[0]):.3g}"),
+                     "decode_ms": round(tk, 3)}
+    return {"workload": "128x128 px, 2 s, Lx=Ly=12, Lt=8, D=10625, 8192 neurons (dyadic Gabor + DoG)",
+            "intervals": n, "forms": forms}
+
+
 def copy_calibration(nbytes=512 << 20, reps=5):
     """A plain device copy of known size (the scale reference for HBM byte counters)."""
     import torch
@@ -212,6 +289,9 @@ def main():
     from vtem_amd._lib import require_gpu
     from vtem_amd.serve import Server, poisson_arrivals
     require_gpu()
+    if args.precision_only:
+        print(json.dumps({"precision_c2": precision_report(args.seed)}), flush=True)
+        return
     items = prepare(args.requests, args.seed)
     if args.roofline_only:
         dk = dominant_kernel(items, args.max_batch)
@@ -330,6 +410,8 @@ def main():
             sweep.append({"offered_rps": round(rate, 2), "frames_per_s": round(st["frames_per_s"], 2),
                           "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3)})
         line["latency_sweep"] = sweep
+    if not args.no_precision:
+        line["precision_c2"] = precision_report(args.seed)
     if not args.no_cpu:
         fps, used, t = cpu_baseline(items, args.cpu_budget)
         line["cpu_baseline"] = {"value": round(fps, 3), "unit": "frames/s", "cores": os.cpu_count(),

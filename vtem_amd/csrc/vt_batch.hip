@@ -380,18 +380,25 @@ VT_EXPORT int vt_seg_counts(const int *offsets, const int *base, int nseg, int *
 // T_b + m*tstride (C_m <- T_m C_m): with tstride = Nc*Nc this fuses Z_j = T_j (S P)_j
 // into the product; with tstride = 0 every row shares one matrix (right factor C T^T).
 //
-// One workgroup owns 16 or 64 rows of one K split; its 4 waves take 16-wide K chunks in turn
-// (wave w: chunks w, w+4, ...).  Operands go from global memory straight into the MFMA
+// One workgroup owns 16 or 64 rows of one K split; its 8 waves take 16-wide K chunks in turn
+// (wave w: chunks w, w+8, ...).  Operands go from global memory straight into the MFMA
 // f64 16x16x4 operand layout: lane (r = l%16, g = l/16) loads the 4 consecutive k values
 // c*16 + 4g + s (s = 0..3) of its row, and MFMA step s pairs them with the B rows of the
 // same k, so each step sums k in {c*16 + 4g + s} and the 4 steps cover the chunk.  The
-// wave partials are added in LDS in a fixed order ((w0 + w2) + (w1 + w3)).
+// wave partials are added in LDS in a fixed tree order.
 //
 // Long reductions are split: a problem with nch = ceil(K/16) chunks uses
 // ks = clamp(nch/8, 1, 8) splits (a function of its own K only, so a problem's rounding
 // does not depend on the batch), each writing a partial tile to its workspace; a second
 // pass adds the ks partials in order and applies the row factor.
 // ------------------------------------------------------------------------------------
+#ifndef VT_SKINNY_CHUNKS_PER_SPLIT
+#define VT_SKINNY_CHUNKS_PER_SPLIT 32
+#endif
+#ifndef VT_SKINNY_SMALL_WG
+#define VT_SKINNY_SMALL_WG 4096
+#endif
+
 struct SkDesc {
     const double *A, *B;
     double *C;
@@ -405,7 +412,7 @@ struct SkDesc {
 __host__ __device__ __forceinline__ int skinny_splits(int K)
 {
     const int nch = (K + 15) / 16;
-    const int ks = nch / 8;
+    const int ks = nch / VT_SKINNY_CHUNKS_PER_SPLIT;
     return ks < 1 ? 1 : (ks > 8 ? 8 : ks);
 }
 
@@ -459,12 +466,14 @@ __device__ __forceinline__ double skinny_row_factor(const SkDesc &d, int m, int 
 }
 
 // RT row tiles of 16 per workgroup (the row tiling does not change any sum's order, so the
-// launch may pick it from the batch size)
+// launch may pick it from the batch size); SK_WAVES waves split K and are reduced in LDS
+// in a fixed tree order.
+#define SK_WAVES 8
 template <int NT, int RT>
-__global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ descs)
+__global__ void __launch_bounds__(64 * SK_WAVES) skinny_kernel(const SkDesc *__restrict__ descs)
 {
     constexpr int R = 16 * RT;
-    __shared__ double red[2][R][33];
+    __shared__ double red[SK_WAVES / 2][R][33];
     const SkDesc d = descs[blockIdx.y];
     const int m0 = blockIdx.x * R;
     const int ks = skinny_splits(d.K), z = blockIdx.z, Nc = d.Nc;
@@ -480,36 +489,39 @@ __global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ 
     const int nch = (d.K + 15) / 16;
     const int per = (nch + ks - 1) / ks;
     const int c_beg = z * per, c_end = min(nch, c_beg + per);
-    // two chunks in flight per wave: c and c + 4
-    for (int c = c_beg + wave; c < c_end; c += 8) {
+    // two chunks in flight per wave: c and c + SK_WAVES
+    for (int c = c_beg + wave; c < c_end; c += 2 * SK_WAVES) {
         double a0[RT][4], b0[NT][4], a1[RT][4], b1[NT][4];
         skinny_load<NT, RT>(d, m0, c, c_end, r, g, Nc, a0, b0);
-        skinny_load<NT, RT>(d, m0, c + 4, c_end, r, g, Nc, a1, b1);
+        skinny_load<NT, RT>(d, m0, c + SK_WAVES, c_end, r, g, Nc, a1, b1);
         skinny_mma<NT, RT>(acc, a0, b0);
         skinny_mma<NT, RT>(acc, a1, b1);
     }
-    // cross-wave reduction: (w0 + w2) + (w1 + w3)
-    if (wave >= 2) {
+    // cross-wave tree reduction: at each level the upper half adds into the lower half
 #pragma unroll
-        for (int t = 0; t < RT; ++t)
+    for (int half = SK_WAVES / 2; half >= 1; half >>= 1) {
+        if (wave >= half && wave < 2 * half) {
 #pragma unroll
-            for (int j = 0; j < NT; ++j)
+            for (int t = 0; t < RT; ++t)
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    red[wave - 2][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)] = acc[t][j][i];
+                for (int j = 0; j < NT; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        red[wave - half][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)] = acc[t][j][i];
+        }
+        __syncthreads();
+        if (wave < half) {
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        acc[t][j][i] += red[wave][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)];
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (wave < 2) {
-#pragma unroll
-        for (int t = 0; t < RT; ++t)
-#pragma unroll
-            for (int j = 0; j < NT; ++j)
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    acc[t][j][i] += red[wave][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)];
-    }
-    __syncthreads();
-    if (wave == 1) {
+    if (wave == 0) {
 #pragma unroll
         for (int t = 0; t < RT; ++t)
 #pragma unroll
@@ -519,30 +531,18 @@ __global__ void __launch_bounds__(256) skinny_kernel(const SkDesc *__restrict__ 
                     red[0][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)] = acc[t][j][i];
     }
     __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-        for (int t = 0; t < RT; ++t)
-#pragma unroll
-            for (int j = 0; j < NT; ++j)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int row = 16 * t + mfma_row_f64(lane, i), col = 16 * j + mfma_col(lane);
-                    red[1][row][col] = acc[t][j][i] + red[0][row][col];
-                }
-    }
-    __syncthreads();
     const int rows = min(R, d.M - m0);
     if (ks > 1) {
         double *W = d.ws + (long long)z * d.M * Nc;
-        for (int e = tid; e < rows * Nc; e += 256) {
+        for (int e = tid; e < rows * Nc; e += 64 * SK_WAVES) {
             const int row = e / Nc, col = e % Nc;
-            W[(long long)(m0 + row) * Nc + col] = red[1][row][col];
+            W[(long long)(m0 + row) * Nc + col] = red[0][row][col];
         }
         return;
     }
-    for (int e = tid; e < rows * Nc; e += 256) {
+    for (int e = tid; e < rows * Nc; e += 64 * SK_WAVES) {
         const int row = e / Nc, col = e % Nc;
-        const double v = d.T ? skinny_row_factor(d, m0 + row, col, Nc, red[1][row]) : red[1][row][col];
+        const double v = d.T ? skinny_row_factor(d, m0 + row, col, Nc, red[0][row]) : red[0][row][col];
         d.C[(long long)(m0 + row) * d.ldc + col] = v;
     }
 }
@@ -589,18 +589,18 @@ VT_EXPORT int vt_gemm_skinny(const void *descs, int batch, int max_M, int max_K,
     const SkDesc *d = (const SkDesc *)descs;
     // small launches take 16-row tiles (4x the workgroups), large ones 64-row tiles
     const long long wg64 = (long long)batch * vt_blocks(max_M, 64) * ks;
-    if (wg64 < 1024) {
+    if (wg64 < VT_SKINNY_SMALL_WG) {
         dim3 grid(vt_blocks(max_M, 16), batch, ks);
         if (max_Nc <= 16)
-            skinny_kernel<1, 1><<<grid, 256, 0, s>>>(d);
+            skinny_kernel<1, 1><<<grid, 64 * SK_WAVES, 0, s>>>(d);
         else
-            skinny_kernel<2, 1><<<grid, 256, 0, s>>>(d);
+            skinny_kernel<2, 1><<<grid, 64 * SK_WAVES, 0, s>>>(d);
     } else {
         dim3 grid(vt_blocks(max_M, 64), batch, ks);
         if (max_Nc <= 16)
-            skinny_kernel<1, 4><<<grid, 256, 0, s>>>(d);
+            skinny_kernel<1, 4><<<grid, 64 * SK_WAVES, 0, s>>>(d);
         else
-            skinny_kernel<2, 4><<<grid, 256, 0, s>>>(d);
+            skinny_kernel<2, 4><<<grid, 64 * SK_WAVES, 0, s>>>(d);
     }
     if (ks > 1) skinny_reduce_kernel<<<dim3(vt_blocks(max_M, 8), batch), 256, 0, s>>>(d);
     VT_LAUNCH_CHECK();

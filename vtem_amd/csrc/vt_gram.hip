@@ -174,6 +174,9 @@ VT_EXPORT int vt_symmetrize(void *A, int dtype, long long lda, int D, void *stre
 // T_j is split once per decode (vt_gram_split_T) into two N x Dt^2 16-bit arrays; W =
 // S[j,ps] S[j,ps'] is formed in FP32 registers and split per k-step.  T is staged in LDS
 // transposed ([column][neuron]) so each lane's 8-neuron B operand is one 16-byte read.
+// A workgroup owns a 16 x 16 block of spatial pairs (256 pair rows, so T_j is streamed
+// once per 256 rows) and all Dt^2 columns; its 8 waves form a 4 (64-row groups) x 2
+// (column halves) grid, so every B operand read from LDS feeds 4 row tiles.
 // ------------------------------------------------------------------------------------
 struct GramDesc {
     const double *S;      // N x Ds (row stride lds)
@@ -185,20 +188,23 @@ struct GramDesc {
 };
 
 #define KL 32
+#define LP_BP 16          // spatial indices per block: a workgroup owns 16 x 16 = 256 pair rows
+#define LP_WAVES 8        // 4 row groups (64 pair rows each) x 2 column groups
 
 template <int NCT, int MODE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * LP_WAVES)
 gram_lp_kernel(const GramDesc *__restrict__ descs)
 {
     constexpr int NC = NCT * 16;
-    constexpr int KP = KL + 8;      // padded k stride (16-bit elements)
-    __shared__ float S1[KL][BP];
-    __shared__ float S2[KL][BP];
+    constexpr int NCT2 = (NCT + 1) / 2;      // column tiles per column group
+    constexpr int KP = KL + 8;               // padded k stride (16-bit elements)
+    __shared__ float S1[KL][LP_BP];
+    __shared__ float S2[KL][LP_BP];
     __shared__ __attribute__((aligned(16))) unsigned short Th[NC][KP];
     __shared__ __attribute__((aligned(16))) unsigned short Tl[MODE == 0 ? NC : 1][KP];
 
     const GramDesc d = descs[blockIdx.y];
-    const int nbp = (d.Ds + BP - 1) / BP;
+    const int nbp = (d.Ds + LP_BP - 1) / LP_BP;
     const int pid = blockIdx.x;
     if (pid >= nbp * (nbp + 1) / 2) return;
     int b1 = (int)((sqrt(8.0 * pid + 1.0) - 1.0) * 0.5);
@@ -206,78 +212,98 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
     while (b1 * (b1 + 1) / 2 > pid) --b1;
     const int b2 = pid - b1 * (b1 + 1) / 2;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rg = wave & 3, cg = wave >> 2;            // row group (64 rows), column group
+    const int c0 = cg * NCT2;                            // first column tile of the group
     const int Dt = d.Dt, DT2 = Dt * Dt, N = d.N;
     const float sW = MODE == 1 ? d.scales[1] : 1.0f;
-
-    v4f acc[NCT];
-#pragma unroll
-    for (int c = 0; c < NCT; ++c) acc[c] = (v4f){0, 0, 0, 0};
-    const int r = wave * 16 + (lane & 15);
-    const int i1 = r >> 3, i2 = r & 7;
     const int g = lane >> 4;
 
+    v4f acc[4][NCT2];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int c = 0; c < NCT2; ++c) acc[t][c] = (v4f){0, 0, 0, 0};
+
     for (int j0 = 0; j0 < N; j0 += KL) {
-        for (int e = tid; e < KL * BP; e += 256) {
-            int kk = e / BP, i = e % BP;
-            int j = j0 + kk, p1 = b1 * BP + i, p2 = b2 * BP + i;
+        for (int e = tid; e < KL * LP_BP; e += 64 * LP_WAVES) {
+            const int kk = e / LP_BP, i = e % LP_BP;
+            const int j = j0 + kk, p1 = b1 * LP_BP + i, p2 = b2 * LP_BP + i;
             S1[kk][i] = (j < N && p1 < d.Ds) ? (float)d.S[(long long)j * d.lds + p1] : 0.f;
             S2[kk][i] = (j < N && p2 < d.Ds) ? (float)d.S[(long long)j * d.lds + p2] : 0.f;
         }
-        for (int e = tid; e < KL * NC; e += 256) {
-            int kk = e / NC, c = e % NC;
-            bool ok = (c < DT2 && j0 + kk < N);
-            long long gi = (long long)(j0 + kk) * DT2 + c;
+        for (int e = tid; e < KL * NC; e += 64 * LP_WAVES) {
+            const int kk = e / NC, c = e % NC;
+            const bool ok = (c < DT2 && j0 + kk < N);
+            const long long gi = (long long)(j0 + kk) * DT2 + c;
             Th[c][kk] = ok ? d.Th[gi] : (unsigned short)0;
             if (MODE == 0) Tl[c][kk] = ok ? d.Tl[gi] : (unsigned short)0;
         }
         __syncthreads();
-        // A operand: W for row r, neurons 8g..8g+7 of this chunk
-        float w[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) w[i] = S1[8 * g + i][i1] * S2[8 * g + i][i2];
+        // A operands: W = S1 S2 for this wave's 4 row tiles, neurons 8g..8g+7 of the chunk
         if (MODE == 0) {
-            v8bf wh, wl;
+            v8bf wh[4], wl[4];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                __bf16 h = (__bf16)w[i];
-                wh[i] = h;
-                wl[i] = (__bf16)(w[i] - (float)h);
+            for (int t = 0; t < 4; ++t) {
+                const int r = rg * 64 + t * 16 + (lane & 15);
+                const int i1 = r >> 4, i2 = r & 15;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float w = S1[8 * g + i][i1] * S2[8 * g + i][i2];
+                    const __bf16 h = (__bf16)w;
+                    wh[t][i] = h;
+                    wl[t][i] = (__bf16)(w - (float)h);
+                }
             }
 #pragma unroll
-            for (int c = 0; c < NCT; ++c) {
-                const int cc = c * 16 + (lane & 15);
-                v8bf bh = *(const v8bf *)&Th[cc][8 * g];
-                v8bf bl = *(const v8bf *)&Tl[cc][8 * g];
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, bh, acc[c], 0, 0, 0);
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh, bl, acc[c], 0, 0, 0);
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl, bh, acc[c], 0, 0, 0);
+            for (int c = 0; c < NCT2; ++c) {
+                if (c0 + c >= NCT) break;
+                const int cc = (c0 + c) * 16 + (lane & 15);
+                const v8bf bh = *(const v8bf *)&Th[cc][8 * g];
+                const v8bf bl = *(const v8bf *)&Tl[cc][8 * g];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[t], bh, acc[t][c], 0, 0, 0);
+                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[t], bl, acc[t][c], 0, 0, 0);
+                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[t], bh, acc[t][c], 0, 0, 0);
+                }
             }
         } else {
-            v8h wh;
+            v8h wh[4];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) wh[i] = (_Float16)(w[i] * sW);
+            for (int t = 0; t < 4; ++t) {
+                const int r = rg * 64 + t * 16 + (lane & 15);
+                const int i1 = r >> 4, i2 = r & 15;
 #pragma unroll
-            for (int c = 0; c < NCT; ++c) {
-                const int cc = c * 16 + (lane & 15);
-                v8h bh = *(const v8h *)&Th[cc][8 * g];
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wh, bh, acc[c], 0, 0, 0);
+                for (int i = 0; i < 8; ++i) wh[t][i] = (_Float16)(S1[8 * g + i][i1] * S2[8 * g + i][i2] * sW);
+            }
+#pragma unroll
+            for (int c = 0; c < NCT2; ++c) {
+                if (c0 + c >= NCT) break;
+                const int cc = (c0 + c) * 16 + (lane & 15);
+                const v8h bh = *(const v8h *)&Th[cc][8 * g];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wh[t], bh, acc[t][c], 0, 0, 0);
             }
         }
         __syncthreads();
     }
     const float unscale = MODE == 1 ? d.scales[2] : 1.0f;
 #pragma unroll
-    for (int c = 0; c < NCT; ++c) {
-        const int col = c * 16 + mfma_col(lane);
+    for (int c = 0; c < NCT2; ++c) {
+        if (c0 + c >= NCT) break;
+        const int col = (c0 + c) * 16 + mfma_col(lane);
         if (col >= DT2) continue;
         const int pt = col / Dt, ptp = col % Dt;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int rr = wave * 16 + mfma_row(lane, q);
-            const int ps = b1 * BP + (rr >> 3), psp = b2 * BP + (rr & 7);
-            if (ps < d.Ds && psp < d.Ds)
-                d.A[(long long)(ps * Dt + pt) * d.lda + psp * Dt + ptp] = acc[c][q] * unscale;
-        }
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = rg * 64 + t * 16 + mfma_row(lane, q);
+                const int ps = b1 * LP_BP + (rr >> 4), psp = b2 * LP_BP + (rr & 15);
+                if (ps < d.Ds && psp < d.Ds)
+                    d.A[(long long)(ps * Dt + pt) * d.lda + psp * Dt + ptp] = acc[t][c][q] * unscale;
+            }
     }
 }
 
@@ -353,15 +379,15 @@ VT_EXPORT int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int m
 {
     VT_CHECK_ARG(descs && batch >= 1 && batch <= 65535 && max_Ds >= 1 && Dt >= 1 &&
                  (mode == 0 || mode == 1));
-    const int nbp = (max_Ds + BP - 1) / BP;
+    const int nbp = (max_Ds + LP_BP - 1) / LP_BP;
     dim3 grid(nbp * (nbp + 1) / 2, batch);
     const int nct = (Dt * Dt + 15) / 16;
     hipStream_t s = (hipStream_t)stream;
     const GramDesc *dd = (const GramDesc *)descs;
 #define LAUNCH_LP(NCTV)                                                                       \
     do {                                                                                      \
-        if (mode == 0) gram_lp_kernel<NCTV, 0><<<grid, 256, 0, s>>>(dd);                      \
-        else gram_lp_kernel<NCTV, 1><<<grid, 256, 0, s>>>(dd);                                \
+        if (mode == 0) gram_lp_kernel<NCTV, 0><<<grid, 64 * LP_WAVES, 0, s>>>(dd);            \
+        else gram_lp_kernel<NCTV, 1><<<grid, 64 * LP_WAVES, 0, s>>>(dd);                      \
     } while (0)
     if (nct <= 2) LAUNCH_LP(2);
     else if (nct <= 6) LAUNCH_LP(6);
