@@ -143,12 +143,17 @@ int vt_neuron_T_batched(double *Tm, double *tv, const double *tk1, const double 
  * vtem/dirichlet.py:62-85) through the push-through identity. */
 int vt_gram_f64(const double *S, long long lds, const double *Tm, int N, int Ds, int Dt,
                 double *A, long long lda, void *stream);
-/* Split T (n = N*Dt^2 doubles) for the low-precision Gram: mode 0 -> hi/lo bf16 arrays;
- * mode 1 -> fp16(T*sT) in hi, with power-of-two scales [sT, sW, 1/(sT sW)] computed on
- * device from max|T| and max|S| (S: nS doubles) into `scales` (3 floats).  No reference
+/* Split one block's T (N x Dt^2 doubles) for the low-precision Gram into the tiled 16-bit
+ * layout the Gram kernel stages with 16-byte copies: per 32-neuron chunk a contiguous
+ * [NC][32] tile, NC = 16 * column tiles of Dt^2 (zero padded); the element count per array
+ * comes from vt_gram_split_T_size.  mode 0 -> hi/lo bf16 arrays; mode 1 -> fp16(T*sT) in
+ * hi, with power-of-two scales [sT, sW, 1/(sT sW)] computed on device from max|T| and
+ * max|S| (S: nS doubles) into `scales` (3 floats).  Dt <= 17 (else -5).  No reference
  * counterpart (the reference has only fp32/fp64 paths). */
-int vt_gram_split_T(const double *Tm, long long n, void *hi, void *lo, int mode, const double *S,
+int vt_gram_split_T(const double *Tm, int N, int Dt, void *hi, void *lo, int mode, const double *S,
                     long long nS, float *scales, void *stream);
+/* 16-bit elements of one block's tiled T (per array), written to the HOST pointer. */
+int vt_gram_split_T_size(int N, int Dt, long long *elems_host);
 /* Batched low-precision Gram (mode 0 split-BF16 3-product, mode 1 FP16) on the 16x16x32
  * MFMAs with fp32 accumulation into fp32 A (lower block triangle).  descs: device array of
  * {const double *S; const u16 *Th, *Tl; float *A; long long lds, lda; int N, Ds, Dt, pad;
@@ -264,7 +269,7 @@ int vt_diag_add_batched(const void *descs, int batch, int dtype, const double *l
  * by T + m*tstride (C_m <- T_m C_m); a problem whose `active` flag (optional double) reads
  * 0 is skipped.  descs = {const double *A, *B; double *C; const double *T; double *ws;
  * const double *active; long long sam, sak, ldb, ldc, tstride; int M, K, Nc, pad;} (104
- * bytes).  A problem with K > 240 splits its reduction (one part per 128 terms, at most
+ * bytes).  A problem with K > 1008 splits its reduction (one part per 512 terms, at most
  * 8, a function of its own K) and needs the ws that vt_gemm_skinny_workspace_size
  * reports; max_K bounds every problem's K.  The structured operator S^T (T_j (S X)) of the coefficient-space
  * Gram (vtem/dirichlet.py:62-85 builds that Gram in spike space) is two such products. */

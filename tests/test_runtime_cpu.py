@@ -72,12 +72,12 @@ def test_dyadic_bank_counts():
 
 
 def test_skinny_workspace_query():
-    """The split-K workspace query (host-only entry): a reduction splits only beyond 240
-    terms (one split per 128 terms), in at most 8 parts, and the size scales with M and
+    """The split-K workspace query (host-only entry): a reduction splits only beyond 1008
+    terms (one split per 512 terms), in at most 8 parts, and the size scales with M and
     Nc."""
     from vtem_amd.batch import skinny_workspace
-    assert skinny_workspace(100, 240, 17) == 0
-    assert skinny_workspace(100, 241, 17) == 2 * 100 * 17
-    assert skinny_workspace(625, 1000, 17) == 7 * 625 * 17
-    assert skinny_workspace(625, 2361, 17) == 8 * 625 * 17
+    assert skinny_workspace(100, 1008, 17) == 0
+    assert skinny_workspace(100, 1009, 17) == 2 * 100 * 17
+    assert skinny_workspace(625, 625, 17) == 0
+    assert skinny_workspace(625, 2361, 17) == 4 * 625 * 17
     assert skinny_workspace(3, 10 ** 6, 5) == 8 * 3 * 5

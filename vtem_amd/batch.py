@@ -28,7 +28,8 @@ import numpy as np
 import torch
 
 from ._lib import lib, ptr, stream_ptr
-from .block import (F64, I32, Decoder, DecodeResult, SpikeTrains, gram_descs, mat_descs)
+from .block import (F64, I32, Decoder, DecodeResult, SpikeTrains, gram_descs, mat_descs,
+                    split_T_size)
 
 SPIKE_DTYPE = np.dtype([("spikes", np.uint64), ("counts", np.uint64), ("ld", np.int64),
                         ("max_rows", np.int32), ("neuron_base", np.int32), ("t0", np.float64),
@@ -416,22 +417,24 @@ class BatchDecoder:
         extra_keep = []
         if low:
             mode = 0 if prec == "bf16x3" else 1
-            nT = Ntot * Dt * Dt
-            Th = torch.empty(nT, dtype=torch.int16, device=dev)
-            Tl = torch.empty(nT if mode == 0 else 1, dtype=torch.int16, device=dev)
-            scales = torch.ones(3, dtype=torch.float32, device=dev)
-            if mode == 0:
-                lib.vt_gram_split_T(ptr(Tm), nT, ptr(Th), ptr(Tl), 0, 0, 0, 0, s)
-            else:
-                S_all = torch.cat([d.S.reshape(-1) for d in decs])
-                extra_keep.append(S_all)
-                lib.vt_gram_split_T(ptr(Tm), nT, ptr(Th), ptr(Tl), 1, ptr(S_all), S_all.numel(),
-                                    ptr(scales), s)
+            # per block: tiled 16-bit T (and, for FP16, the block's own power-of-two scales,
+            # so a block's result does not depend on the batch)
+            tsz = np.array([split_T_size(int(N[b]), Dt) for b in range(B)], np.int64)
+            tbase = np.concatenate([[0], np.cumsum(tsz)])
+            Th = torch.empty(int(tbase[-1]), dtype=torch.int16, device=dev)
+            Tl = torch.empty(int(tbase[-1]) if mode == 0 else 1, dtype=torch.int16, device=dev)
+            scales = torch.ones(3 * B, dtype=torch.float32, device=dev)
+            E2 = Dt * Dt
+            for b, d in enumerate(decs):
+                lib.vt_gram_split_T(ptr(Tm) + 8 * int(nbase[b]) * E2, int(N[b]), Dt, ptr(Th) + 2 * int(tbase[b]),
+                                    ptr(Tl) + (2 * int(tbase[b]) if mode == 0 else 0), mode, ptr(d.S),
+                                    d.S.numel(), ptr(scales) + 12 * b, s)
             gd = np.zeros(B, GRAM_LP_DTYPE)
             for b, d in enumerate(decs):
-                off = int(nbase[b]) * Dt * Dt * 2
+                off = 2 * int(tbase[b])
                 gd[b] = (d.S.data_ptr(), Th.data_ptr() + off, Tl.data_ptr() + (off if mode == 0 else 0),
-                         As[b].data_ptr(), d.S.stride(0), D[b], N[b], Ds[b], Dt, 0, scales.data_ptr())
+                         As[b].data_ptr(), d.S.stride(0), D[b], N[b], Ds[b], Dt, 0,
+                         scales.data_ptr() + 12 * b)
             gd_d = _to_dev(gd, dev)
             e0 = self._tic()
             lib.vt_gram_lp(ptr(gd_d), B, int(Ds.max()), Dt, mode, s)

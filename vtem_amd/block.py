@@ -18,6 +18,7 @@ Decode  (vtem/vtdm.py:630-652 -> dirichlet.compute_q/compute_G, linalg.solve_eq_
 
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import math
 from typing import Optional
@@ -55,6 +56,13 @@ def gram_descs(items, device) -> torch.Tensor:
         arr[i] = (S.data_ptr(), Th.data_ptr(), Tl.data_ptr(), A.data_ptr(), S.stride(0),
                   A.stride(0), N, Ds, Dt, 0, scales.data_ptr())
     return torch.from_numpy(arr.view(np.uint8).copy()).to(device)
+
+
+def split_T_size(N: int, Dt: int) -> int:
+    """16-bit elements of one block's tiled low-precision T (queried from the library)."""
+    out = ctypes.c_longlong(0)
+    lib.vt_gram_split_T_size(int(N), int(Dt), ctypes.addressof(out))
+    return int(out.value)
 
 
 def dev(a, dtype=F64, device="cuda"):
@@ -313,11 +321,11 @@ class Decoder:
         """Low-precision Gram into FP32 (mode 0 split-BF16 x3, mode 1 FP16)."""
         g = self.geom
         D, N = g.D, self.N
-        n = N * g.Dt * g.Dt
+        n = split_T_size(N, g.Dt)
         hi = torch.empty(n, dtype=torch.int16, device=self.device)
         lo = torch.empty(n if mode == 0 else 1, dtype=torch.int16, device=self.device)
         scales = torch.ones(3, dtype=torch.float32, device=self.device)
-        lib.vt_gram_split_T(ptr(Tm), n, ptr(hi), ptr(lo), mode, ptr(self.S), self.S.numel(),
+        lib.vt_gram_split_T(ptr(Tm), N, g.Dt, ptr(hi), ptr(lo), mode, ptr(self.S), self.S.numel(),
                             ptr(scales), stream_ptr())
         A = torch.empty((D, D), dtype=torch.float32, device=self.device) if out is None else out
         desc = gram_descs([(self.S, hi, lo, A, N, g.Ds, g.Dt, scales)], self.device)

@@ -388,7 +388,7 @@ VT_EXPORT int vt_seg_counts(const int *offsets, const int *base, int nseg, int *
 // wave partials are added in LDS in a fixed tree order.
 //
 // Long reductions are split: a problem with nch = ceil(K/16) chunks uses
-// ks = clamp(nch/8, 1, 8) splits (a function of its own K only, so a problem's rounding
+// ks = clamp(nch/32, 1, 8) splits (a function of its own K only, so a problem's rounding
 // does not depend on the batch), each writing a partial tile to its workspace; a second
 // pass adds the ks partials in order and applies the row factor.
 // ------------------------------------------------------------------------------------

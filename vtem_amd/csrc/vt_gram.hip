@@ -171,9 +171,11 @@ VT_EXPORT int vt_symmetrize(void *A, int dtype, long long lda, int D, void *stre
 //   mode 1 (FP16, 1 product): W and T pre-scaled by powers of two (scales on device) so
 //          they sit in FP16's normal range; the product is unscaled in the epilogue.
 //
-// T_j is split once per decode (vt_gram_split_T) into two N x Dt^2 16-bit arrays; W =
-// S[j,ps] S[j,ps'] is formed in FP32 registers and split per k-step.  T is staged in LDS
-// transposed ([column][neuron]) so each lane's 8-neuron B operand is one 16-byte read.
+// T_j is split once per decode (vt_gram_split_T) into 16-bit arrays tiled per 32-neuron
+// chunk as [column][neuron], so a chunk is staged into LDS with 16-byte copies and each
+// lane's 8-neuron B operand is one 16-byte LDS read; W = S[j,ps] S[j,ps'] is formed in
+// FP32 registers and split per k-step.  The LDS stage is double-buffered: chunk k+1 is
+// copied while chunk k feeds the MFMAs, one barrier per chunk.
 // A workgroup owns a 16 x 16 block of spatial pairs (256 pair rows, so T_j is streamed
 // once per 256 rows) and all Dt^2 columns; its 8 waves form a 4 (64-row groups) x 2
 // (column halves) grid, so every B operand read from LDS feeds 4 row tiles.
@@ -191,6 +193,13 @@ struct GramDesc {
 #define LP_BP 16          // spatial indices per block: a workgroup owns 16 x 16 = 256 pair rows
 #define LP_WAVES 8        // 4 row groups (64 pair rows each) x 2 column groups
 
+// columns of the tiled T layout for a temporal order Dt (the Gram kernels' column tiles)
+__host__ __device__ __forceinline__ int gram_lp_nct(int Dt)
+{
+    const int nct = (Dt * Dt + 15) / 16;
+    return nct <= 2 ? 2 : (nct <= 6 ? 6 : (nct <= 19 ? 19 : 0));
+}
+
 template <int NCT, int MODE>
 __global__ void __launch_bounds__(64 * LP_WAVES)
 gram_lp_kernel(const GramDesc *__restrict__ descs)
@@ -198,10 +207,11 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
     constexpr int NC = NCT * 16;
     constexpr int NCT2 = (NCT + 1) / 2;      // column tiles per column group
     constexpr int KP = KL + 8;               // padded k stride (16-bit elements)
-    __shared__ float S1[KL][LP_BP];
-    __shared__ float S2[KL][LP_BP];
-    __shared__ __attribute__((aligned(16))) unsigned short Th[NC][KP];
-    __shared__ __attribute__((aligned(16))) unsigned short Tl[MODE == 0 ? NC : 1][KP];
+    constexpr int NV = NC * KL / 8;          // 16-byte vectors per T tile
+    __shared__ float S1[2][KL][LP_BP];
+    __shared__ float S2[2][KL][LP_BP];
+    __shared__ __attribute__((aligned(16))) unsigned short Th[2][NC][KP];
+    __shared__ __attribute__((aligned(16))) unsigned short Tl[MODE == 0 ? 2 : 1][MODE == 0 ? NC : 1][KP];
 
     const GramDesc d = descs[blockIdx.y];
     const int nbp = (d.Ds + LP_BP - 1) / LP_BP;
@@ -217,6 +227,25 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
     const int Dt = d.Dt, DT2 = Dt * Dt, N = d.N;
     const float sW = MODE == 1 ? d.scales[1] : 1.0f;
     const int g = lane >> 4;
+    const int nsteps = (N + KL - 1) / KL;
+
+    // stage chunk k (neurons 32k..32k+31) into buffer bf: S slices and the pre-tiled T
+    auto stage = [&](int k, int bf) {
+        const int j0 = k * KL;
+        for (int e = tid; e < KL * LP_BP; e += 64 * LP_WAVES) {
+            const int kk = e / LP_BP, i = e % LP_BP;
+            const int j = j0 + kk, p1 = b1 * LP_BP + i, p2 = b2 * LP_BP + i;
+            S1[bf][kk][i] = (j < N && p1 < d.Ds) ? (float)d.S[(long long)j * d.lds + p1] : 0.f;
+            S2[bf][kk][i] = (j < N && p2 < d.Ds) ? (float)d.S[(long long)j * d.lds + p2] : 0.f;
+        }
+        const uint4 *th = (const uint4 *)(d.Th + (long long)k * NC * KL);
+        const uint4 *tl = (const uint4 *)(d.Tl + (long long)k * NC * KL);
+        for (int e = tid; e < NV; e += 64 * LP_WAVES) {
+            const int c = e >> 2, kk = (e & 3) * 8;
+            *(uint4 *)&Th[bf][c][kk] = th[e];
+            if (MODE == 0) *(uint4 *)&Tl[bf][c][kk] = tl[e];
+        }
+    };
 
     v4f acc[4][NCT2];
 #pragma unroll
@@ -224,21 +253,11 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
 #pragma unroll
         for (int c = 0; c < NCT2; ++c) acc[t][c] = (v4f){0, 0, 0, 0};
 
-    for (int j0 = 0; j0 < N; j0 += KL) {
-        for (int e = tid; e < KL * LP_BP; e += 64 * LP_WAVES) {
-            const int kk = e / LP_BP, i = e % LP_BP;
-            const int j = j0 + kk, p1 = b1 * LP_BP + i, p2 = b2 * LP_BP + i;
-            S1[kk][i] = (j < N && p1 < d.Ds) ? (float)d.S[(long long)j * d.lds + p1] : 0.f;
-            S2[kk][i] = (j < N && p2 < d.Ds) ? (float)d.S[(long long)j * d.lds + p2] : 0.f;
-        }
-        for (int e = tid; e < KL * NC; e += 64 * LP_WAVES) {
-            const int kk = e / NC, c = e % NC;
-            const bool ok = (c < DT2 && j0 + kk < N);
-            const long long gi = (long long)(j0 + kk) * DT2 + c;
-            Th[c][kk] = ok ? d.Th[gi] : (unsigned short)0;
-            if (MODE == 0) Tl[c][kk] = ok ? d.Tl[gi] : (unsigned short)0;
-        }
-        __syncthreads();
+    stage(0, 0);
+    __syncthreads();
+    for (int k = 0; k < nsteps; ++k) {
+        const int bf = k & 1;
+        if (k + 1 < nsteps) stage(k + 1, bf ^ 1);       // the other buffer: free since the last barrier
         // A operands: W = S1 S2 for this wave's 4 row tiles, neurons 8g..8g+7 of the chunk
         if (MODE == 0) {
             v8bf wh[4], wl[4];
@@ -248,7 +267,7 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
                 const int i1 = r >> 4, i2 = r & 15;
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    const float w = S1[8 * g + i][i1] * S2[8 * g + i][i2];
+                    const float w = S1[bf][8 * g + i][i1] * S2[bf][8 * g + i][i2];
                     const __bf16 h = (__bf16)w;
                     wh[t][i] = h;
                     wl[t][i] = (__bf16)(w - (float)h);
@@ -258,8 +277,8 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
             for (int c = 0; c < NCT2; ++c) {
                 if (c0 + c >= NCT) break;
                 const int cc = (c0 + c) * 16 + (lane & 15);
-                const v8bf bh = *(const v8bf *)&Th[cc][8 * g];
-                const v8bf bl = *(const v8bf *)&Tl[cc][8 * g];
+                const v8bf bh = *(const v8bf *)&Th[bf][cc][8 * g];
+                const v8bf bl = *(const v8bf *)&Tl[bf][cc][8 * g];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[t], bh, acc[t][c], 0, 0, 0);
@@ -274,13 +293,14 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
                 const int r = rg * 64 + t * 16 + (lane & 15);
                 const int i1 = r >> 4, i2 = r & 15;
 #pragma unroll
-                for (int i = 0; i < 8; ++i) wh[t][i] = (_Float16)(S1[8 * g + i][i1] * S2[8 * g + i][i2] * sW);
+                for (int i = 0; i < 8; ++i)
+                    wh[t][i] = (_Float16)(S1[bf][8 * g + i][i1] * S2[bf][8 * g + i][i2] * sW);
             }
 #pragma unroll
             for (int c = 0; c < NCT2; ++c) {
                 if (c0 + c >= NCT) break;
                 const int cc = (c0 + c) * 16 + (lane & 15);
-                const v8h bh = *(const v8h *)&Th[cc][8 * g];
+                const v8h bh = *(const v8h *)&Th[bf][cc][8 * g];
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wh[t], bh, acc[t][c], 0, 0, 0);
@@ -307,22 +327,28 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
     }
 }
 
-// mode 0: hi/lo bf16 split of T (n doubles).  mode 1: fp16 of T * scales[0].
-__global__ void split_T_kernel(const double *__restrict__ T, long long n, unsigned short *hi,
-                               unsigned short *lo, int mode, const float *scales)
+// Split T_j (N x Dt^2 doubles) into the tiled 16-bit layout of the Gram kernels: chunk k of
+// 32 neurons is a contiguous [NC][32] tile (column-major within the chunk, zero padded), so
+// the kernels stage it with 16-byte copies.  mode 0: hi / lo bf16; mode 1: fp16 of T*scales[0].
+__global__ void split_T_kernel(const double *__restrict__ T, int N, int Dt, int NC, long long total,
+                               unsigned short *hi, unsigned short *lo, int mode, const float *scales)
 {
     const float s = mode == 1 ? scales[0] : 1.0f;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+    const int DT2 = Dt * Dt;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
-        float t = (float)T[i];
+        const int kk = (int)(i % KL);
+        const long long r = i / KL;
+        const int c = (int)(r % NC);
+        const int j = (int)(r / NC) * KL + kk;
+        const float t = (j < N && c < DT2) ? (float)T[(long long)j * DT2 + c] : 0.f;
         if (mode == 0) {
-            __bf16 h = (__bf16)t;
-            __bf16 l = (__bf16)(t - (float)h);
+            const __bf16 h = (__bf16)t;
+            const __bf16 l = (__bf16)(t - (float)h);
             hi[i] = __builtin_bit_cast(unsigned short, h);
             lo[i] = __builtin_bit_cast(unsigned short, l);
         } else {
-            _Float16 h = (_Float16)(t * s);
-            hi[i] = __builtin_bit_cast(unsigned short, h);
+            hi[i] = __builtin_bit_cast(unsigned short, (_Float16)(t * s));
         }
     }
 }
@@ -359,15 +385,28 @@ fp16_scales_kernel(const double *__restrict__ T, long long nT, const double *__r
 
 // Split T (n = N*Dt^2 doubles) for the low-precision Gram.  mode 0: hi, lo (bf16);
 // mode 1: hi = fp16(T*sT) with scales computed here from T and S (nS doubles).
-VT_EXPORT int vt_gram_split_T(const double *Tm, long long n, void *hi, void *lo, int mode,
+// 16-bit elements of one block's tiled T (per array: hi, and lo in mode 0)
+VT_EXPORT int vt_gram_split_T_size(int N, int Dt, long long *elems_host)
+{
+    VT_CHECK_ARG(N >= 0 && Dt >= 1 && elems_host);
+    const int nct = gram_lp_nct(Dt);
+    if (!nct) return VT_ERANGE;
+    *elems_host = (long long)((N + KL - 1) / KL) * nct * 16 * KL;
+    return VT_OK;
+}
+
+VT_EXPORT int vt_gram_split_T(const double *Tm, int N, int Dt, void *hi, void *lo, int mode,
                               const double *S, long long nS, float *scales, void *stream)
 {
-    VT_CHECK_ARG(Tm && hi && n >= 0 && (mode == 0 || mode == 1));
+    VT_CHECK_ARG(Tm && hi && N >= 1 && Dt >= 1 && (mode == 0 || mode == 1));
     VT_CHECK_ARG(mode == 1 ? (S && scales && nS > 0) : (lo != nullptr));
+    const int nct = gram_lp_nct(Dt);
+    if (!nct) return VT_ERANGE;
     hipStream_t s = (hipStream_t)stream;
-    if (mode == 1) fp16_scales_kernel<<<1, 1024, 0, s>>>(Tm, n, S, nS, scales);
-    unsigned g = vt_blocks(n, 256);
-    split_T_kernel<<<g < 4096 ? g : 4096, 256, 0, s>>>(Tm, n, (unsigned short *)hi,
+    if (mode == 1) fp16_scales_kernel<<<1, 1024, 0, s>>>(Tm, (long long)N * Dt * Dt, S, nS, scales);
+    const long long total = (long long)((N + KL - 1) / KL) * nct * 16 * KL;
+    unsigned g = vt_blocks(total, 256);
+    split_T_kernel<<<g < 4096 ? g : 4096, 256, 0, s>>>(Tm, N, Dt, nct * 16, total, (unsigned short *)hi,
                                                       (unsigned short *)lo, mode, scales);
     VT_LAUNCH_CHECK();
     return VT_OK;
@@ -381,7 +420,7 @@ VT_EXPORT int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int m
                  (mode == 0 || mode == 1));
     const int nbp = (max_Ds + LP_BP - 1) / LP_BP;
     dim3 grid(nbp * (nbp + 1) / 2, batch);
-    const int nct = (Dt * Dt + 15) / 16;
+    const int nct = gram_lp_nct(Dt);
     hipStream_t s = (hipStream_t)stream;
     const GramDesc *dd = (const GramDesc *)descs;
 #define LAUNCH_LP(NCTV)                                                                       \
@@ -389,9 +428,9 @@ VT_EXPORT int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int m
         if (mode == 0) gram_lp_kernel<NCTV, 0><<<grid, 64 * LP_WAVES, 0, s>>>(dd);            \
         else gram_lp_kernel<NCTV, 1><<<grid, 64 * LP_WAVES, 0, s>>>(dd);                      \
     } while (0)
-    if (nct <= 2) LAUNCH_LP(2);
-    else if (nct <= 6) LAUNCH_LP(6);
-    else if (nct <= 19) LAUNCH_LP(19);
+    if (nct == 2) LAUNCH_LP(2);
+    else if (nct == 6) LAUNCH_LP(6);
+    else if (nct == 19) LAUNCH_LP(19);
     else return VT_ERANGE;
 #undef LAUNCH_LP
     VT_LAUNCH_CHECK();
