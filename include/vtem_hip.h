@@ -148,7 +148,8 @@ int vt_gram_f64(const double *S, long long lds, const double *Tm, int N, int Ds,
  * [NC][32] tile, NC = 16 * column tiles of Dt^2 (zero padded); the element count per array
  * comes from vt_gram_split_T_size.  mode 0 -> hi/lo bf16 arrays; mode 1 -> fp16(T*sT) in
  * hi, with power-of-two scales [sT, sW, 1/(sT sW)] computed on device from max|T| and
- * max|S| (S: nS doubles) into `scales` (3 floats).  Dt <= 17 (else -5).  No reference
+ * max|S| (S: nS doubles) into `scales` (5 floats: the last two are reduction scratch).
+ * Dt <= 17 (else -5).  No reference
  * counterpart (the reference has only fp32/fp64 paths). */
 int vt_gram_split_T(const double *Tm, int N, int Dt, void *hi, void *lo, int mode, const double *S,
                     long long nS, float *scales, void *stream);

@@ -423,18 +423,18 @@ class BatchDecoder:
             tbase = np.concatenate([[0], np.cumsum(tsz)])
             Th = torch.empty(int(tbase[-1]), dtype=torch.int16, device=dev)
             Tl = torch.empty(int(tbase[-1]) if mode == 0 else 1, dtype=torch.int16, device=dev)
-            scales = torch.ones(3 * B, dtype=torch.float32, device=dev)
+            scales = torch.ones(5 * B, dtype=torch.float32, device=dev)
             E2 = Dt * Dt
             for b, d in enumerate(decs):
                 lib.vt_gram_split_T(ptr(Tm) + 8 * int(nbase[b]) * E2, int(N[b]), Dt, ptr(Th) + 2 * int(tbase[b]),
                                     ptr(Tl) + (2 * int(tbase[b]) if mode == 0 else 0), mode, ptr(d.S),
-                                    d.S.numel(), ptr(scales) + 12 * b, s)
+                                    d.S.numel(), ptr(scales) + 20 * b, s)
             gd = np.zeros(B, GRAM_LP_DTYPE)
             for b, d in enumerate(decs):
                 off = 2 * int(tbase[b])
                 gd[b] = (d.S.data_ptr(), Th.data_ptr() + off, Tl.data_ptr() + (off if mode == 0 else 0),
                          As[b].data_ptr(), d.S.stride(0), D[b], N[b], Ds[b], Dt, 0,
-                         scales.data_ptr() + 12 * b)
+                         scales.data_ptr() + 20 * b)
             gd_d = _to_dev(gd, dev)
             e0 = self._tic()
             lib.vt_gram_lp(ptr(gd_d), B, int(Ds.max()), Dt, mode, s)

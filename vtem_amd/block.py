@@ -324,7 +324,7 @@ class Decoder:
         n = split_T_size(N, g.Dt)
         hi = torch.empty(n, dtype=torch.int16, device=self.device)
         lo = torch.empty(n if mode == 0 else 1, dtype=torch.int16, device=self.device)
-        scales = torch.ones(3, dtype=torch.float32, device=self.device)
+        scales = torch.ones(5, dtype=torch.float32, device=self.device)
         lib.vt_gram_split_T(ptr(Tm), N, g.Dt, ptr(hi), ptr(lo), mode, ptr(self.S), self.S.numel(),
                             ptr(scales), stream_ptr())
         A = torch.empty((D, D), dtype=torch.float32, device=self.device) if out is None else out

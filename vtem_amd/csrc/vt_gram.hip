@@ -354,37 +354,48 @@ __global__ void split_T_kernel(const double *__restrict__ T, int N, int Dt, int 
 }
 
 // FP16 scales from max|T| and max|S|: powers of two putting max|T| * sT and
-// max|S|^2 * sW near 2^14 (FP16 max 65504), stored as [sT, sW, 1/(sT sW)].
-__global__ void __launch_bounds__(1024)
-fp16_scales_kernel(const double *__restrict__ T, long long nT, const double *__restrict__ S,
-                   long long nS, float *scales)
+// max|S|^2 * sW near 2^14 (FP16 max 65504), stored as [sT, sW, 1/(sT sW)].  The maxima
+// are reduced over a grid with atomicMax on the bit patterns of non-negative floats (which
+// order like the values), so the result does not depend on the reduction order.
+__global__ void __launch_bounds__(256)
+absmax_kernel(const double *__restrict__ T, long long nT, const double *__restrict__ S, long long nS,
+              unsigned int *bits)
 {
-    __shared__ double red[2][1024];
-    double mt = 0.0, ms = 0.0;
-    for (long long i = threadIdx.x; i < nT; i += 1024) mt = fmax(mt, fabs(T[i]));
-    for (long long i = threadIdx.x; i < nS; i += 1024) ms = fmax(ms, fabs(S[i]));
+    __shared__ float red[2][256];
+    float mt = 0.f, ms = 0.f;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nT; i += (long long)gridDim.x * 256)
+        mt = fmaxf(mt, (float)fabs(T[i]));
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nS; i += (long long)gridDim.x * 256)
+        ms = fmaxf(ms, (float)fabs(S[i]));
     red[0][threadIdx.x] = mt;
     red[1][threadIdx.x] = ms;
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
+    for (int o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) {
-            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
-            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+            red[0][threadIdx.x] = fmaxf(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+            red[1][threadIdx.x] = fmaxf(red[1][threadIdx.x], red[1][threadIdx.x + o]);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double tmax = red[0][0] > 0 ? red[0][0] : 1.0, smax = red[1][0] > 0 ? red[1][0] : 1.0;
-        double sT = exp2(floor(log2(16384.0 / tmax)));
-        double sW = exp2(floor(log2(16384.0 / (smax * smax))));
-        scales[0] = (float)sT;
-        scales[1] = (float)sW;
-        scales[2] = (float)(1.0 / (sT * sW));
+        atomicMax(bits, __float_as_uint(red[0][0]));
+        atomicMax(bits + 1, __float_as_uint(red[1][0]));
     }
 }
 
-// Split T (n = N*Dt^2 doubles) for the low-precision Gram.  mode 0: hi, lo (bf16);
-// mode 1: hi = fp16(T*sT) with scales computed here from T and S (nS doubles).
+__global__ void fp16_scales_kernel(const unsigned int *bits, float *scales)
+{
+    const double tm = __uint_as_float(bits[0]), sm = __uint_as_float(bits[1]);
+    const double tmax = tm > 0 ? tm : 1.0, smax = sm > 0 ? sm : 1.0;
+    const double sT = exp2(floor(log2(16384.0 / tmax)));
+    const double sW = exp2(floor(log2(16384.0 / (smax * smax))));
+    scales[0] = (float)sT;
+    scales[1] = (float)sW;
+    scales[2] = (float)(1.0 / (sT * sW));
+}
+
+__global__ void zero_bits_kernel(unsigned int *bits) { bits[0] = 0u; bits[1] = 0u; }
+
 // 16-bit elements of one block's tiled T (per array: hi, and lo in mode 0)
 VT_EXPORT int vt_gram_split_T_size(int N, int Dt, long long *elems_host)
 {
@@ -403,7 +414,13 @@ VT_EXPORT int vt_gram_split_T(const double *Tm, int N, int Dt, void *hi, void *l
     const int nct = gram_lp_nct(Dt);
     if (!nct) return VT_ERANGE;
     hipStream_t s = (hipStream_t)stream;
-    if (mode == 1) fp16_scales_kernel<<<1, 1024, 0, s>>>(Tm, (long long)N * Dt * Dt, S, nS, scales);
+    if (mode == 1) {
+        // scales[3..4] hold the maxima's bit patterns while they are reduced
+        unsigned int *bits = (unsigned int *)(scales + 3);
+        zero_bits_kernel<<<1, 1, 0, s>>>(bits);
+        absmax_kernel<<<256, 256, 0, s>>>(Tm, (long long)N * Dt * Dt, S, nS, bits);
+        fp16_scales_kernel<<<1, 1, 0, s>>>(bits, scales);
+    }
     const long long total = (long long)((N + KL - 1) / KL) * nct * 16 * KL;
     unsigned g = vt_blocks(total, 256);
     split_T_kernel<<<g < 4096 ? g : 4096, 256, 0, s>>>(Tm, N, Dt, nct * 16, total, (unsigned short *)hi,
