@@ -98,6 +98,41 @@ int vt_iaf_encode_trig(const double *coef, long long coef_ld, const double *basi
                        double *v0, const double *kappa, const double *bias,
                        const double *delta, double *time_count, int *spike_count, double dt,
                        int root_mode, void *stream);
+/* Time-parallel forms of vt_iaf_encode and vt_iaf_encode_trig: same arguments, state
+ * semantics and outputs (counts equal, spike times equal to rounding; the reference's
+ * IAF_encode.encode, vtem/ensemble_encode.py:106-168, kernel :438-552).  Spike k is the
+ * first passage of the integrated input to the level k*delta - v0, so the samples are cut
+ * into segments integrated in parallel, joined by a per-neuron scan of (area, maximum),
+ * and only segments holding spikes are walked again for the crossing times.  ws holds
+ * vt_iaf_par_workspace_size(N, size) bytes of device memory. */
+int vt_iaf_encode_trig_par(const double *coef, long long coef_ld, const double *basis, int Dt,
+                           int N, int size, double *spikes, long long spike_ld, int max_spike,
+                           double *v0, const double *kappa, const double *bias,
+                           const double *delta, double *time_count, int *spike_count,
+                           double dt, int root_mode, void *ws, long long ws_bytes,
+                           void *stream);
+int vt_iaf_encode_par(const void *input, int in_dtype, long long input_ld, int N, int size,
+                      double *spikes, long long spike_ld, int max_spike, double *v0,
+                      const double *kappa, const double *bias, const double *delta,
+                      double *time_count, int *spike_count, double dt, int root_mode, void *ws,
+                      long long ws_bytes, void *stream);
+/* Host query: workspace bytes of one time-parallel encoder call. */
+int vt_iaf_par_workspace_size(int N, int size, long long *bytes_host);
+/* Batched time-parallel trig encoder: nd descriptors (device array) of the layout
+ *   struct { const double *coef; long long coef_ld, coef_ns; const double *basis;
+ *            const void *input; long long input_ld;          (unused here)
+ *            double *spikes; long long spike_ld; double *v0, *time_count;
+ *            const double *kappa, *bias, *delta; int *spike_count; void *ws;
+ *            int *overflow; double dt; int N, size, Dt, max_spike, root_mode, in_dtype; }
+ * (160 bytes), each one an independent vt_iaf_encode_trig_par call with coefficient pt of
+ * neuron n at coef[pt*coef_ld + n*coef_ns]; *overflow (if not null) is set to 1 when a
+ * neuron fires more than max_spike times.  maxN, maxSize and maxDt bound the descriptors'
+ * N, size and Dt.  Encodes many blocks (served requests, stitching tiles)
+ * in one launch sequence; the reference encodes one video at a time (vtem/vtem.py:165-200). */
+int vt_iaf_encode_trig_batched(const void *descs, int nd, int maxN, int maxSize, int maxDt,
+                               void *stream);
+/* Host query: the descriptor size above, for layout checks. */
+int vt_iaf_par_desc_size(int *bytes_host);
 /* Random-threshold IAF: spikes holds N(0,1) draws on entry; delta_value carried in/out.
  * Replaces IAF_encode_rt.encode (vtem/ensemble_encode.py:265-329, kernel :555-674). */
 int vt_iaf_encode_rt(const double *input, long long input_ld, int N, int size, double *spikes,

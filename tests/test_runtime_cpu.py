@@ -81,3 +81,21 @@ def test_skinny_workspace_query():
     assert skinny_workspace(625, 625, 17) == 0
     assert skinny_workspace(625, 2361, 17) == 4 * 625 * 17
     assert skinny_workspace(3, 10 ** 6, 5) == 8 * 3 * 5
+
+
+def test_iaf_par_host_queries():
+    """Host-only entries of the time-parallel encoder: the descriptor layout the batched
+    launch reads matches the host dtype, and the workspace holds (segments + 2) x N pairs
+    of doubles with one segment per 64 sample intervals."""
+    import ctypes
+    from vtem_amd._lib import lib
+    from vtem_amd.batch_encode import IAFP_DTYPE
+    from vtem_amd.block import iaf_par_workspace_bytes
+    b = ctypes.c_int(0)
+    lib.vt_iaf_par_desc_size(ctypes.addressof(b))
+    assert b.value == IAFP_DTYPE.itemsize == 160
+    assert iaf_par_workspace_bytes(100, 0) == 2 * 100 * 16
+    assert iaf_par_workspace_bytes(100, 1) == 2 * 100 * 16
+    assert iaf_par_workspace_bytes(100, 65) == 3 * 100 * 16
+    assert iaf_par_workspace_bytes(100, 66) == 4 * 100 * 16
+    assert iaf_par_workspace_bytes(2361, 20001) == (313 + 2) * 2361 * 16

@@ -1,0 +1,191 @@
+"""Batched encoder: many video blocks filtered and IAF-encoded together on the GPU.
+
+Each block runs the reference's encode chain (vtem/vtem.py:165-200): filtering by its
+receptive-field bank (vtem/vrf.py:264-292), then integrate-and-fire
+(vtem/ensemble_encode.py:106-168).  The filter and the temporal-coefficient projection
+are one GEMM each per block; the IAF of all blocks is one time-parallel launch sequence
+(vt_iaf_encode_trig_batched), so blocks of different sizes and durations share the GPU
+instead of each occupying it with one serial lane per neuron.
+
+The reference encodes one video at a time and has no batched form.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List
+
+import numpy as np
+import torch
+
+from ._lib import lib, ptr, stream_ptr
+from .batch import SK_DTYPE, _to_dev, skinny_workspace
+from .block import Encoder, SpikeTrains, F64, I32, dev, iaf_par_workspace_bytes
+from .synthetic import IAFParams
+
+# Descriptor of one time-parallel IAF call (layout of the C struct in vt_iaf_par.hip).
+IAFP_DTYPE = np.dtype([
+    ("coef", "<u8"), ("coef_ld", "<i8"), ("coef_ns", "<i8"), ("basis", "<u8"), ("input", "<u8"),
+    ("input_ld", "<i8"), ("spikes", "<u8"), ("spike_ld", "<i8"), ("v0", "<u8"),
+    ("time_count", "<u8"), ("kappa", "<u8"), ("bias", "<u8"), ("delta", "<u8"),
+    ("spike_count", "<u8"), ("ws", "<u8"), ("overflow", "<u8"), ("dt", "<f8"), ("N", "<i4"),
+    ("size", "<i4"), ("Dt", "<i4"), ("max_spike", "<i4"), ("root_mode", "<i4"),
+    ("in_dtype", "<i4")])
+assert IAFP_DTYPE.itemsize == 160
+
+
+@dataclasses.dataclass
+class IAFDevice:
+    """An encoder population's IAF parameters and carried state, resident on the GPU."""
+    kappa: torch.Tensor
+    bias: torch.Tensor
+    delta: torch.Tensor
+    v0: torch.Tensor
+    time_count: torch.Tensor
+
+    @classmethod
+    def from_params(cls, p: IAFParams, device="cuda"):
+        N = p.kappa.size
+        return cls(dev(p.kappa, F64, device), dev(p.bias, F64, device), dev(p.delta, F64, device),
+                   dev(p.v0, F64, device), torch.zeros(N, dtype=F64, device=device))
+
+
+@dataclasses.dataclass
+class EncodeJob:
+    """One block to encode: its Encoder (geometry, RF bank), frames (F x Py*Px, device),
+    IAF state (updated in place) and the spike buffer's rows (max_spikes_bound gives a
+    rate bound that needs no device data)."""
+    encoder: Encoder
+    frames: torch.Tensor
+    iaf: IAFDevice
+    max_spike: int
+
+
+def rf_l1(enc: Encoder) -> torch.Tensor:
+    """dx dy ||RF_j||_1 per neuron (device), cached on the encoder."""
+    r = getattr(enc, "_rf_l1", None)
+    if r is None:
+        g = enc.geom
+        r = enc.RF.abs().sum(1).to(F64) * (g.dx * g.dy)
+        enc._rf_l1 = r
+        enc._rf_l1_max = float(r.max().item()) if r.numel() else 0.0
+    return r
+
+
+def max_spikes_bound(enc: Encoder, iaf_host: IAFParams, umax: float) -> int:
+    """Spike buffer rows from a rate bound that needs no device data: neuron j's input is
+    at most dx dy ||RF_j||_1 umax, so it fires at most (that + b_j) / (kappa_j delta_j) times
+    per second.  Safe but loose (the L1 bound is far above typical inputs)."""
+    rf_l1(enc)
+    rate = (enc._rf_l1_max * umax + np.max(np.abs(iaf_host.bias))) / \
+        np.min(iaf_host.kappa * iaf_host.delta)
+    return int(math.ceil(enc.geom.duration * rate * 1.05)) + 8
+
+
+def max_spikes_rate(geom, iaf_host: IAFParams, margin: float = 2.0) -> int:
+    """Spike buffer rows from the bias-driven rate b / (kappa delta) of the fastest neuron
+    times `margin` (the input's own swing adds at most |v| / (kappa delta); margin 2 covers
+    |v| up to b).  Not a bound: a block that exceeds it is reported through the encoder's
+    overflow flag and encoded again with an exact buffer."""
+    rate = np.max(np.abs(iaf_host.bias) / (iaf_host.kappa * iaf_host.delta))
+    return int(math.ceil(geom.duration * rate * margin)) + 16
+
+
+def _theta_scaled(enc: Encoder) -> torch.Tensor:
+    """Theta_f dx dy / frames (frames x Dt), cached on the encoder."""
+    t = getattr(enc, "_theta_fs", None)
+    if t is None:
+        g = enc.geom
+        t = (enc.theta_f.to(F64) * (g.dx * g.dy / g.frames)).contiguous()
+        enc._theta_fs = t
+    return t
+
+
+def _skinny(descs: np.ndarray, maxM: int, maxK: int, maxNc: int, device):
+    dd = _to_dev(descs, device)
+    lib.vt_gemm_skinny(ptr(dd), len(descs), maxM, maxK, maxNc, stream_ptr())
+
+
+class BatchEncoder:
+    """Filter + IAF of many blocks.  The temporal coefficients of each neuron's filtered
+    input, C_j = (1/F) Theta_f^T (dx dy V RF_j^T), are formed as RF (V^T Theta_f dx dy / F):
+    the same product reassociated, so the F x P frames are contracted to Dt x P before the
+    receptive fields are applied (Dt / F of the filtering flops).  Both products are grouped
+    skinny GEMMs over all blocks; the IAF is the batched time-parallel encoder."""
+
+    def __init__(self, device="cuda", root_mode: int = 0):
+        self.device = device
+        self.root_mode = int(root_mode)
+
+    def coefficients(self, jobs: List[EncodeJob]) -> List[torch.Tensor]:
+        """C^T (N x Dt) per job: coefficient pt of neuron j at [j, pt]."""
+        dvc = self.device
+        Wt, Ct, ws1, ws2 = [], [], [], []
+        for job in jobs:
+            g = job.encoder.geom
+            N = job.encoder.RF.shape[0]
+            if job.encoder.RF.dtype != F64:
+                raise ValueError("the batched encoder filters with an FP64 RF bank")
+            Wt.append(torch.empty((g.P, g.Dt), dtype=F64, device=dvc))
+            Ct.append(torch.empty((N, g.Dt), dtype=F64, device=dvc))
+            ws1.append(skinny_workspace(g.P, g.frames, g.Dt))
+            ws2.append(skinny_workspace(N, g.P, g.Dt))
+        w1 = torch.empty(max(1, sum(ws1)), dtype=F64, device=dvc)
+        w2 = torch.empty(max(1, sum(ws2)), dtype=F64, device=dvc)
+        d1 = np.zeros(len(jobs), SK_DTYPE)
+        d2 = np.zeros(len(jobs), SK_DTYPE)
+        o1 = o2 = 0
+        for i, job in enumerate(jobs):
+            enc, V = job.encoder, job.frames
+            g = enc.geom
+            th = _theta_scaled(enc)
+            RF = enc.RF
+            N = RF.shape[0]
+            # W^T (P x Dt) = V^T Theta_f dx dy / F: A(m=pixel, k=frame) = V[k, m]
+            d1[i] = (ptr(V), ptr(th), ptr(Wt[i]), 0, ptr(w1) + 8 * o1, 0, 1, V.stride(0), g.Dt,
+                     g.Dt, 0, g.P, g.frames, g.Dt, 0)
+            # C^T (N x Dt) = RF W^T
+            d2[i] = (ptr(RF), ptr(Wt[i]), ptr(Ct[i]), 0, ptr(w2) + 8 * o2, 0, RF.stride(0), 1,
+                     g.Dt, g.Dt, 0, N, g.P, g.Dt, 0)
+            o1 += ws1[i]
+            o2 += ws2[i]
+        if jobs:
+            geo = [j.encoder.geom for j in jobs]
+            maxDt = max(g.Dt for g in geo)
+            _skinny(d1, max(g.P for g in geo), max(g.frames for g in geo), maxDt, dvc)
+            _skinny(d2, max(j.encoder.RF.shape[0] for j in jobs), max(g.P for g in geo), maxDt, dvc)
+        return Ct
+
+    def encode(self, jobs: List[EncodeJob], overflow: torch.Tensor = None) -> List[SpikeTrains]:
+        """Encode every job; returns resident spike trains (no host synchronisation).  Job
+        j's IAF state advances as in a serial encode.  overflow (int32, len(jobs), zeroed
+        by the caller) gets 1 where a job's buffer was too small."""
+        dvc = self.device
+        coefs = self.coefficients(jobs)
+        out, ws_sizes = [], []
+        for job, C in zip(jobs, coefs):
+            g = job.encoder.geom
+            N = C.shape[0]
+            spikes = torch.empty((job.max_spike, N), dtype=F64, device=dvc)
+            counts = torch.empty(N, dtype=I32, device=dvc)
+            out.append(SpikeTrains(spikes, counts))
+            ws_sizes.append((iaf_par_workspace_bytes(N, g.iaf_samples) + 255) // 256 * 256)
+        ws = torch.empty(max(1, sum(ws_sizes)), dtype=torch.uint8, device=dvc)
+        d = np.zeros(len(jobs), IAFP_DTYPE)
+        off = 0
+        maxN = maxS = maxDt = 0
+        for i, (job, C, st) in enumerate(zip(jobs, coefs, out)):
+            g = job.encoder.geom
+            N = C.shape[0]
+            d[i] = (ptr(C), 1, C.stride(0), ptr(job.encoder.theta_iaf), 0, 0, ptr(st.spikes), N,
+                    ptr(job.iaf.v0), ptr(job.iaf.time_count), ptr(job.iaf.kappa),
+                    ptr(job.iaf.bias), ptr(job.iaf.delta), ptr(st.counts), ptr(ws) + off,
+                    0 if overflow is None else ptr(overflow) + 4 * i, g.dt, N, g.iaf_samples,
+                    g.Dt, job.max_spike, self.root_mode, 0)
+            off += ws_sizes[i]
+            maxN, maxS, maxDt = max(maxN, N), max(maxS, g.iaf_samples), max(maxDt, g.Dt)
+        if jobs:
+            dd = _to_dev(d, dvc)
+            lib.vt_iaf_encode_trig_batched(ptr(dd), len(jobs), maxN, maxS, maxDt, stream_ptr())
+        return out

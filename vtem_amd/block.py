@@ -168,14 +168,17 @@ class Encoder:
         return int(math.ceil(self.geom.duration * float(rate.max()) * 1.05)) + 8
 
     def encode(self, V: torch.Tensor, iaf: IAFParams, root_mode: int = 0,
-               max_spike: Optional[int] = None):
-        g = self.geom
+               max_spike: Optional[int] = None, parallel: bool = True):
         v = self.filter(V)
         C = self.trig_coeffs(v)
-        return self.encode_coeffs(C, iaf, root_mode, max_spike)
+        return self.encode_coeffs(C, iaf, root_mode, max_spike, parallel)
 
     def encode_coeffs(self, C: torch.Tensor, iaf: IAFParams, root_mode: int = 0,
-                      max_spike: Optional[int] = None):
+                      max_spike: Optional[int] = None, parallel: bool = True):
+        """IAF over the dt grid with the input evaluated from its temporal coefficients.
+        parallel=True runs the time-parallel encoder (segments joined by a scan of the
+        integrated input); parallel=False the one-lane-per-neuron serial loop of the
+        reference (vtem/ensemble_encode.py:438-552).  Counts agree, times to rounding."""
         g = self.geom
         N = C.shape[1]
         if max_spike is None:
@@ -185,11 +188,27 @@ class Encoder:
         v0 = dev(iaf.v0, F64, self.device)
         tc = torch.zeros(N, dtype=F64, device=self.device)
         kap, bia, dlt = (dev(a, F64, self.device) for a in (iaf.kappa, iaf.bias, iaf.delta))
-        lib.vt_iaf_encode_trig(ptr(C), C.stride(0), ptr(self.theta_iaf), g.Dt, N, g.iaf_samples,
-                               ptr(spikes), N, max_spike, ptr(v0), ptr(kap), ptr(bia), ptr(dlt),
-                               ptr(tc), ptr(counts), g.dt, root_mode, stream_ptr())
+        args = (ptr(C), C.stride(0), ptr(self.theta_iaf), g.Dt, N, g.iaf_samples, ptr(spikes), N,
+                max_spike, ptr(v0), ptr(kap), ptr(bia), ptr(dlt), ptr(tc), ptr(counts), g.dt,
+                root_mode)
+        if parallel:
+            ws = iaf_par_workspace(N, g.iaf_samples, self.device)
+            lib.vt_iaf_encode_trig_par(*args, ptr(ws), ws.numel(), stream_ptr())
+        else:
+            lib.vt_iaf_encode_trig(*args, stream_ptr())
         self.last_state = (v0, tc)
         return SpikeTrains(spikes, counts)
+
+
+def iaf_par_workspace_bytes(N: int, size: int) -> int:
+    """Device bytes of one time-parallel IAF call (segment partials and prefixes)."""
+    b = ctypes.c_longlong()
+    lib.vt_iaf_par_workspace_size(int(N), int(size), ctypes.addressof(b))
+    return int(b.value)
+
+
+def iaf_par_workspace(N: int, size: int, device="cuda") -> torch.Tensor:
+    return torch.empty(iaf_par_workspace_bytes(N, size), dtype=torch.uint8, device=device)
 
 
 # --------------------------------------------------------------------------------------

@@ -8,6 +8,7 @@
 // every lane is bounds-guarded (the reference writes state past num_neurons,
 // ensemble_encode.py:543-545).
 #include "vt_common.hpp"
+#include "vt_iaf.hpp"
 
 #define VT_PI 3.141592653589793238462643383279
 
@@ -208,21 +209,12 @@ iaf_trig_kernel(const double *__restrict__ coef, long long coef_ld, const double
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    double c[DT];
-#pragma unroll
-    for (int p = 0; p < DT; ++p) c[p] = p < Dt ? coef[(long long)p * coef_ld + n] : 0.0;
+    TrigSource<DT> eval;
+    eval.load(coef, coef_ld, basis, Dt, n);
     const double kappa = st.kappa[n], bias = st.bias[n], delta = st.delta[n];
     double v0 = st.v0[n], time_count = st.time_count[n], ddt = st.dt;
     int spike_count = 0;
     double *col = st.spikes + n;
-    auto eval = [&](int i) {
-        const double *b = basis + (long long)i * Dt;
-        double s = 0.0;
-#pragma unroll
-        for (int p = 0; p < DT; ++p)
-            if (p < Dt) s += c[p] * b[p];
-        return s;
-    };
     double y_next = size > 0 ? (eval(0) + bias) / kappa : 0.0;
     for (int i = 0; i + 1 < size; ++i) {
         double y1 = y_next;

@@ -1,0 +1,403 @@
+// Time-parallel IAF encoder.
+//
+// The reference integrates each neuron serially over all samples, one thread per neuron
+// (vtem/ensemble_encode.py:438-552).  With reset to zero, spike k of a neuron is the first
+// time the integral F(t) = int_0^t y reaches the level L_k = k delta - v0 (v0 the carried
+// voltage): after spike k-1 the voltage is F(t) - L_{k-1}, so the next threshold crossing is
+// F = L_{k-1} + delta, and no earlier point can have reached a higher level.  The spike
+// count up to t is therefore the number of levels at or below the running maximum of F,
+// and the time axis splits into independent segments:
+//
+//   1. iafp_partial: per (neuron, segment of IAFP_SEG sample intervals) the segment's area
+//      A and the maximum M of F over the segment relative to its start (step ends, and the
+//      interior peak of a falling step y1 > 0 > y2);
+//   2. iafp_scan: per neuron, the exclusive prefix (F_s, M_s) of every segment under
+//      (A, M) o (A', M') = (A + A', max(M, A + M')), the totals, and the final state;
+//   3. iafp_walk: per segment holding spikes (level count at its end above the count at its
+//      start), the segment is integrated again from F_s and every level crossed gets its
+//      time from the same quadratic root as the reference (:501, :521).
+//
+// The samples, the trapezoid areas, the branch conditions and the root formulas are the
+// reference's; only the bookkeeping differs (F relative to the call start instead of a
+// voltage reset at each spike), so counts agree with the serial encoder and times agree to
+// rounding.  Work per neuron drops from `size` dependent steps to IAFP_SEG plus a scan of
+// size / IAFP_SEG segments, and the launch holds N x size / IAFP_SEG lanes.
+#include "vt_common.hpp"
+#include "vt_iaf.hpp"
+
+#define IAFP_SEG 64        // sample intervals per segment
+#define IAFP_WAVES 4       // segments per workgroup (one wave each, 64 neurons per wave)
+
+// One encoder call on one block of neurons.  ws holds (S + 2) x N pairs of doubles,
+// S = ceil((size - 1) / IAFP_SEG): segment partials, then prefixes in place; row S the
+// totals (F, M); row S + 1 the initial (v0, time_count).
+struct IafParDesc {
+    const double *coef;     // trig source: coefficient pt of neuron n at
+    long long coef_ld;      //   coef[pt * coef_ld + n * coef_ns], and the basis
+    long long coef_ns;
+    const double *basis;    // size x Dt
+    const void *input;      // sampled source: size x N (ld input_ld), FP64 or FP32
+    long long input_ld;
+    double *spikes;         // max_spike x N (ld spike_ld)
+    long long spike_ld;
+    double *v0, *time_count;
+    const double *kappa, *bias, *delta;
+    int *spike_count;
+    double2 *ws;
+    int *overflow;          // optional: set to 1 when a neuron fires more than max_spike times
+    double dt;
+    int N, size, Dt, max_spike, root_mode, in_dtype;
+};
+
+static __host__ __device__ inline int iafp_segments(int size)
+{
+    return size > 1 ? (size - 1 + IAFP_SEG - 1) / IAFP_SEG : 0;
+}
+
+// Number of levels k >= 1 with k delta - v0 <= M, at most cap, evaluated with the same
+// level expression the walk compares against.
+__device__ __forceinline__ int iafp_levels(double M, double v0, double delta, int cap)
+{
+#pragma clang fp contract(off)
+    if (!(M + v0 >= delta)) return 0;    // also M = -inf (no step yet) and NaN
+    double q = floor((M + v0) / delta);
+    if (!(q < (double)cap)) return cap;
+    int k = (int)q;
+    if (k < cap && (double)(k + 1) * delta - v0 <= M)
+        ++k;
+    else if (k >= 1 && (double)k * delta - v0 > M)
+        --k;
+    return k;
+}
+
+template <class Src>
+__device__ __forceinline__ void iafp_partial_dev(const IafParDesc &d, const Src &src, int n, int s,
+                                                 int S)
+{
+#pragma clang fp contract(off)
+    const int steps = d.size - 1;
+    const int i0 = s * IAFP_SEG, i1 = min(i0 + IAFP_SEG, steps);
+    const double kappa = d.kappa[n], bias = d.bias[n], dt = d.dt;
+    double y1 = (src(i0) + bias) / kappa;
+    double P = 0.0, M = -INFINITY;
+    for (int i = i0; i < i1; ++i) {
+        const double y2 = (src(i + 1) + bias) / kappa;
+        const double area = (y1 + y2) / 2 * dt;
+        double top;
+        if (y2 >= 0 || y1 <= 0) {
+            top = P + area;
+        } else {
+            const double a = (y2 - y1) / dt;
+            top = P + y1 * y1 / (-2 * a);
+        }
+        M = fmax(M, top);
+        P += area;
+        y1 = y2;
+    }
+    d.ws[(long long)s * d.N + n] = make_double2(P, M);
+}
+
+template <class Src>
+__device__ __forceinline__ void iafp_walk_dev(const IafParDesc &d, const Src &src, int n, int s,
+                                              int S)
+{
+#pragma clang fp contract(off)
+    const long long N = d.N;
+    const int cap = d.max_spike + 1;
+    const double delta = d.delta[n];
+    const double2 init = d.ws[(long long)(S + 1) * N + n];
+    const double v0 = init.x, tc0 = init.y;
+    const double2 pre = d.ws[(long long)s * N + n];
+    const double2 nxt = d.ws[(long long)(s + 1) * N + n];
+    int k = iafp_levels(pre.y, v0, delta, cap);
+    const int k1 = iafp_levels(nxt.y, v0, delta, cap);
+    if (k >= k1) return;
+    const int kfinal = iafp_levels(d.ws[(long long)S * N + n].y, v0, delta, cap);
+    double *col = d.spikes + n;
+    auto emit = [&](int j, double t) {
+        // Rows beyond the buffer: the reference keeps overwriting the last row
+        // (ensemble_encode.py:503-507), so it ends holding spike max_spike; skip the
+        // spike that the later one overwrites.
+        if (j == d.max_spike - 1 && kfinal == cap) return;
+        const int row = j < d.max_spike - 1 ? j : d.max_spike - 1;
+        col[(long long)row * d.spike_ld] = t;
+    };
+    const int steps = d.size - 1;
+    const int i0 = s * IAFP_SEG, i1 = min(i0 + IAFP_SEG, steps);
+    const double kappa = d.kappa[n], bias = d.bias[n], dt = d.dt;
+    const int mode = d.root_mode;
+    double y1 = (src(i0) + bias) / kappa;
+    double P = 0.0;
+    double best = -INFINITY, tbest = tc0 + (double)i0 * dt;
+    for (int i = i0; i < i1 && k < k1; ++i) {
+        const double y2 = (src(i + 1) + bias) / kappa;
+        const double area = (y1 + y2) / 2 * dt;
+        const double F = pre.x + P;
+        const double t0 = tc0 + (double)i * dt;
+        if (y2 >= 0 || y1 <= 0) {
+            // rising / normal step (ensemble_encode.py:493-514)
+            const double a = (y2 - y1) / dt;
+            while (k < k1) {
+                const double L = (double)(k + 1) * delta - v0;
+                if (!(F + area >= L)) break;
+                const double dv = L - F;
+                double remain;
+                if (mode == 0)
+                    remain = (fabs(a) <= 1e-12) ? (dv / y1) : (sqrt(y1 * y1 + 2 * dv * a) - y1) / (a);
+                else
+                    remain = 2 * dv / (sqrt(y1 * y1 + 2 * dv * a) + y1);
+                emit(k, t0 + remain);
+                ++k;
+            }
+            if (F + area > best) { best = F + area; tbest = t0 + dt; }
+        } else {
+            // falling step y1 > 0 > y2 (ensemble_encode.py:515-535)
+            const double a = (y2 - y1) / dt;
+            while (k < k1) {
+                const double L = (double)(k + 1) * delta - v0;
+                const double dv = L - F;
+                const double disc = y1 * y1 + 2 * dv * a;
+                if (!(disc >= 0)) break;
+                double remain;
+                if (mode == 0)
+                    remain = (sqrt(disc) - y1) / a;
+                else
+                    remain = 2 * dv / (sqrt(disc) + y1);
+                emit(k, t0 + remain);
+                ++k;
+            }
+            const double top = F + y1 * y1 / (-2 * a);
+            if (top > best) { best = top; tbest = t0 + y1 / (-a); }
+        }
+        P += area;
+        y1 = y2;
+    }
+    // Levels the prefix maximum counts in this segment but the step tests missed by a
+    // rounding tie are reached at the segment's highest point.
+    for (; k < k1; ++k) emit(k, tbest);
+}
+
+// Workgroup x: neuron group g = x % G (64 lanes), segment quad x / G (IAFP_WAVES waves).
+__device__ __forceinline__ bool iafp_task(const IafParDesc &d, int &n, int &s, int &S)
+{
+    S = iafp_segments(d.size);
+    const int G = (d.N + 63) / 64;
+    const int x = blockIdx.x;
+    const int quad = x / G;
+    n = (x - quad * G) * 64 + (threadIdx.x & 63);
+    // wave-uniform segment: the basis row addresses become scalar loads shared by the wave
+    s = quad * IAFP_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    return n < d.N && s < S;
+}
+
+// BATCHED selects the descriptor source at compile time (a runtime select between the
+// array and the by-value argument would turn every field into a generic-pointer load,
+// defeating scalar loads of the uniform fields).
+template <int DT, int PHASE, bool BATCHED>
+__global__ void __launch_bounds__(64 * IAFP_WAVES)
+iafp_trig_kernel(const IafParDesc *__restrict__ descs, IafParDesc d0)
+{
+    const IafParDesc d = BATCHED ? descs[blockIdx.y] : d0;
+    int n, s, S;
+    if (!iafp_task(d, n, s, S)) return;
+    TrigSource<DT> src;
+    src.load(d.coef, d.coef_ld, d.basis, d.Dt, n, d.coef_ns);
+    if (PHASE == 0)
+        iafp_partial_dev(d, src, n, s, S);
+    else
+        iafp_walk_dev(d, src, n, s, S);
+}
+
+template <typename TIn, int PHASE>
+__global__ void __launch_bounds__(64 * IAFP_WAVES)
+iafp_sampled_kernel(IafParDesc d)
+{
+    int n, s, S;
+    if (!iafp_task(d, n, s, S)) return;
+    SampledSource<TIn> src{(const TIn *)d.input + n, d.input_ld};
+    if (PHASE == 0)
+        iafp_partial_dev(d, src, n, s, S);
+    else
+        iafp_walk_dev(d, src, n, s, S);
+}
+
+// Per neuron: exclusive prefixes of the segment partials, totals, and the carried state
+// the serial encoder would leave (ensemble_encode.py:541-545): v0 <- F_T - L_K with K the
+// spike count (at most max_spike + 1, the reference's overflow stop), time_count advanced
+// by the call's size - 1 intervals.
+template <bool BATCHED>
+__global__ void __launch_bounds__(256)
+iafp_scan_kernel(const IafParDesc *__restrict__ descs, IafParDesc d0)
+{
+#pragma clang fp contract(off)
+    const IafParDesc d = BATCHED ? descs[blockIdx.y] : d0;
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= d.N) return;
+    const long long N = d.N;
+    const int S = iafp_segments(d.size);
+    const double v0 = d.v0[n], tc0 = d.time_count[n], delta = d.delta[n];
+    double F = 0.0, M = -INFINITY;
+    double2 *w = d.ws + n;
+    int s = 0;
+    for (; s + 8 <= S; s += 8) {
+        double2 p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] = w[(long long)(s + u) * N];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            w[(long long)(s + u) * N] = make_double2(F, M);
+            M = fmax(M, F + p[u].y);
+            F += p[u].x;
+        }
+    }
+    for (; s < S; ++s) {
+        const double2 p = w[(long long)s * N];
+        w[(long long)s * N] = make_double2(F, M);
+        M = fmax(M, F + p.y);
+        F += p.x;
+    }
+    w[(long long)S * N] = make_double2(F, M);
+    w[(long long)(S + 1) * N] = make_double2(v0, tc0);
+    if (S == 0) {
+        d.spike_count[n] = 0;
+        return;
+    }
+    const int K = iafp_levels(M, v0, delta, d.max_spike + 1);
+    d.spike_count[n] = K;
+    if (K > d.max_spike && d.overflow) *d.overflow = 1;
+    d.v0[n] = K > 0 ? F - ((double)K * delta - v0) : v0 + F;
+    d.time_count[n] = tc0 + (double)(d.size - 1) * d.dt;
+}
+
+static int iafp_check(const IafParDesc &d)
+{
+    VT_CHECK_ARG(d.N >= 0 && d.size >= 0);
+    if (d.N == 0) return VT_OK;
+    VT_CHECK_ARG(d.spikes && d.v0 && d.kappa && d.bias && d.delta && d.time_count &&
+                 d.spike_count && d.ws);
+    VT_CHECK_ARG(d.max_spike >= 1 && d.spike_ld >= d.N && d.dt > 0);
+    VT_CHECK_ARG(d.root_mode == 0 || d.root_mode == 1);
+    return VT_OK;
+}
+
+static long long iafp_ws_bytes(int N, int size)
+{
+    return (long long)(iafp_segments(size) + 2) * N * (long long)sizeof(double2);
+}
+
+VT_EXPORT int vt_iaf_par_workspace_size(int N, int size, long long *bytes)
+{
+    VT_CHECK_ARG(N >= 0 && size >= 0 && bytes);
+    *bytes = iafp_ws_bytes(N, size);
+    return VT_OK;
+}
+
+template <int DT>
+static void iafp_trig_launch(const IafParDesc *descs, const IafParDesc &d0, dim3 g, dim3 gs,
+                             hipStream_t s)
+{
+    if (descs) {
+        iafp_trig_kernel<DT, 0, true><<<g, 64 * IAFP_WAVES, 0, s>>>(descs, d0);
+        iafp_scan_kernel<true><<<gs, 256, 0, s>>>(descs, d0);
+        iafp_trig_kernel<DT, 1, true><<<g, 64 * IAFP_WAVES, 0, s>>>(descs, d0);
+    } else {
+        iafp_trig_kernel<DT, 0, false><<<g, 64 * IAFP_WAVES, 0, s>>>(descs, d0);
+        iafp_scan_kernel<false><<<gs, 256, 0, s>>>(descs, d0);
+        iafp_trig_kernel<DT, 1, false><<<g, 64 * IAFP_WAVES, 0, s>>>(descs, d0);
+    }
+}
+
+template <typename TIn>
+static void iafp_sampled_launch(const IafParDesc &d, dim3 g, dim3 gs, hipStream_t s)
+{
+    iafp_sampled_kernel<TIn, 0><<<g, 64 * IAFP_WAVES, 0, s>>>(d);
+    iafp_scan_kernel<false><<<gs, 256, 0, s>>>(nullptr, d);
+    iafp_sampled_kernel<TIn, 1><<<g, 64 * IAFP_WAVES, 0, s>>>(d);
+}
+
+static int iafp_trig_dispatch(const IafParDesc *descs, const IafParDesc &d0, int maxDt, dim3 g,
+                              dim3 gs, hipStream_t s)
+{
+    if (maxDt <= 9)
+        iafp_trig_launch<9>(descs, d0, g, gs, s);
+    else if (maxDt <= 17)
+        iafp_trig_launch<17>(descs, d0, g, gs, s);
+    else if (maxDt <= 33)
+        iafp_trig_launch<33>(descs, d0, g, gs, s);
+    else if (maxDt <= 65)
+        iafp_trig_launch<65>(descs, d0, g, gs, s);
+    else
+        return VT_ERANGE;
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+static dim3 iafp_grid(int maxN, int maxSize, int nd)
+{
+    const int G = (maxN + 63) / 64;
+    const int quads = (iafp_segments(maxSize) + IAFP_WAVES - 1) / IAFP_WAVES;
+    return dim3((unsigned)(G * (quads > 0 ? quads : 1)), (unsigned)nd);
+}
+
+VT_EXPORT int vt_iaf_encode_trig_par(const double *coef, long long coef_ld, const double *basis,
+                                     int Dt, int N, int size, double *spikes, long long spike_ld,
+                                     int max_spike, double *v0, const double *kappa,
+                                     const double *bias, const double *delta, double *time_count,
+                                     int *spike_count, double dt, int root_mode, void *ws,
+                                     long long ws_bytes, void *stream)
+{
+    IafParDesc d{coef, coef_ld, 1, basis, nullptr, 0, spikes, spike_ld, v0, time_count, kappa,
+                 bias, delta, spike_count, (double2 *)ws, nullptr, dt, N, size, Dt, max_spike,
+                 root_mode, 0};
+    int r = iafp_check(d);
+    if (r || N == 0) return r;
+    VT_CHECK_ARG(coef && basis && Dt >= 1 && coef_ld >= N);
+    VT_CHECK_ARG(ws_bytes >= iafp_ws_bytes(N, size));
+    return iafp_trig_dispatch(nullptr, d, Dt, iafp_grid(N, size, 1), dim3(vt_blocks(N, 256), 1),
+                              (hipStream_t)stream);
+}
+
+VT_EXPORT int vt_iaf_encode_par(const void *input, int in_dtype, long long input_ld, int N,
+                                int size, double *spikes, long long spike_ld, int max_spike,
+                                double *v0, const double *kappa, const double *bias,
+                                const double *delta, double *time_count, int *spike_count,
+                                double dt, int root_mode, void *ws, long long ws_bytes,
+                                void *stream)
+{
+    IafParDesc d{nullptr, 0, 1, nullptr, input, input_ld, spikes, spike_ld, v0, time_count,
+                 kappa, bias, delta, spike_count, (double2 *)ws, nullptr, dt, N, size, 0,
+                 max_spike, root_mode, in_dtype};
+    int r = iafp_check(d);
+    if (r || N == 0) return r;
+    VT_CHECK_ARG((input || size == 0) && input_ld >= N && (in_dtype == 0 || in_dtype == 1));
+    VT_CHECK_ARG(ws_bytes >= iafp_ws_bytes(N, size));
+    hipStream_t s = (hipStream_t)stream;
+    dim3 g = iafp_grid(N, size, 1), gs(vt_blocks(N, 256), 1);
+    if (in_dtype == 0)
+        iafp_sampled_launch<double>(d, g, gs, s);
+    else
+        iafp_sampled_launch<float>(d, g, gs, s);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// Batched trig encode: one descriptor per block (device array), launched together.  The
+// host passes the maxima of N, size and Dt over the descriptors for the grid; workgroups
+// beyond a block's own extent exit at once.
+VT_EXPORT int vt_iaf_encode_trig_batched(const void *descs, int nd, int maxN, int maxSize,
+                                         int maxDt, void *stream)
+{
+    VT_CHECK_ARG(nd >= 0 && maxN >= 0 && maxSize >= 0 && maxDt >= 1);
+    VT_CHECK_ARG(descs || nd == 0);
+    if (nd == 0 || maxN == 0) return VT_OK;
+    IafParDesc d0{};
+    return iafp_trig_dispatch((const IafParDesc *)descs, d0, maxDt, iafp_grid(maxN, maxSize, nd),
+                              dim3(vt_blocks(maxN, 256), (unsigned)nd), (hipStream_t)stream);
+}
+
+VT_EXPORT int vt_iaf_par_desc_size(int *bytes)
+{
+    VT_CHECK_ARG(bytes);
+    *bytes = (int)sizeof(IafParDesc);
+    return VT_OK;
+}

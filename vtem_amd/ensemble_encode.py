@@ -2,9 +2,10 @@
 
 IAF_encode (deterministic thresholds) and IAF_encode_rt (per-spike random thresholds)
 keep the reference's constructor, load_parameters, encode(neural_inputs, startbias,
-avg_rate) -> (list of N spike arrays, int32 counts) and reset_timer.  The integration
-runs in vt_iaf_encode / vt_iaf_encode_rt, one lane per neuron; v0 and the time counter
-stay on the GPU between calls, as the reference's state does.  encode_device returns the
+avg_rate) -> (list of N spike arrays, int32 counts) and reset_timer.  IAF_encode runs
+the time-parallel vt_iaf_encode_par by default (parallel=False: vt_iaf_encode, one lane per
+neuron, bit-identical to the reference's loop); IAF_encode_rt runs vt_iaf_encode_rt.  v0
+and the time counter stay on the GPU between calls, as the reference's state does.  encode_device returns the
 resident spike buffer instead of host lists (the decode path's input).
 
 Defaults draw v0 (and the rt thresholds) from numpy's global generator, as the reference
@@ -20,7 +21,7 @@ import numpy as np
 import torch
 
 from ._lib import lib, ptr, stream_ptr
-from .block import SpikeTrains
+from .block import SpikeTrains, iaf_par_workspace
 
 F64 = torch.float64
 
@@ -32,7 +33,8 @@ def _dev(a, dtype=F64, device="cuda"):
 
 
 class pop_encode:
-    def __init__(self, num_neurons, dt, dtype=np.float64, device="cuda", rng=None, root_mode=0):
+    def __init__(self, num_neurons, dt, dtype=np.float64, device="cuda", rng=None, root_mode=0,
+                 parallel=True):
         if np.dtype(dtype) != np.float64:
             raise ValueError("the encoder stores spike times in float64")
         self.num_neurons = int(num_neurons)
@@ -41,6 +43,8 @@ class pop_encode:
         self.device = device
         self.rng = rng
         self.root_mode = int(root_mode)
+        # time-parallel encoder (same counts, times to rounding) or the serial loop
+        self.parallel = bool(parallel)
 
     def _uniform(self, n):
         return (self.rng.uniform(0, 1, n) if self.rng is not None else np.random.uniform(0, 1, n))
@@ -113,10 +117,14 @@ class IAF_encode(pop_encode):
         spikes = torch.empty((max_spike, N), dtype=F64, device=self.device)
         counts = torch.empty(N, dtype=torch.int32, device=self.device)
         o = int(startbias)
-        lib.vt_iaf_encode(ptr(x), 0, x.stride(0), N, T, ptr(spikes), N, max_spike, ptr(self.d_v0) + 8 * o,
-                          ptr(self.d_kappa) + 8 * o, ptr(self.d_bias) + 8 * o, ptr(self.d_delta) + 8 * o,
-                          ptr(self.d_time_count) + 8 * o, ptr(counts), self.dt, self.root_mode,
-                          stream_ptr())
+        args = (ptr(x), 0, x.stride(0), N, T, ptr(spikes), N, max_spike, ptr(self.d_v0) + 8 * o,
+                ptr(self.d_kappa) + 8 * o, ptr(self.d_bias) + 8 * o, ptr(self.d_delta) + 8 * o,
+                ptr(self.d_time_count) + 8 * o, ptr(counts), self.dt, self.root_mode)
+        if self.parallel:
+            ws = iaf_par_workspace(N, T, self.device)
+            lib.vt_iaf_encode_par(*args, ptr(ws), ws.numel(), stream_ptr())
+        else:
+            lib.vt_iaf_encode(*args, stream_ptr())
         return SpikeTrains(spikes, counts)
 
     def encode(self, neural_inputs, startbias=0, avg_rate=0.1) -> Tuple[List[np.ndarray], np.ndarray]:
