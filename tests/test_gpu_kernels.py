@@ -692,3 +692,54 @@ def test_server_alone_equals_mix(torch_mod):
         alone = bd.decode([r.job])[0]
         assert alone.fallback == r.result.fallback
         np.testing.assert_array_equal(r.result.frames.cpu().numpy(), alone.frames.cpu().numpy())
+
+
+def test_server_encode_decode_alone_equals_mix(torch_mod):
+    """Encode+decode requests (video resident, encoded by the server's batched encoder)
+    served with Poisson arrivals on 2 streams: every request's spikes and frames are
+    bit-identical to the request encoded and decoded alone (KPCG form, Tikhonov and RNN
+    mixed).  One request gets a 4-row spike buffer: the server detects the overflow and
+    redoes it with an exact buffer, giving the alone result with a large buffer."""
+    torch = torch_mod
+    import dataclasses
+    import numpy as np
+    from vtem_amd import synthetic as syn
+    from vtem_amd.basis import Geometry
+    from vtem_amd.batch import BatchDecoder, BlockJob
+    from vtem_amd.batch_encode import BatchEncoder, EncodeJob, IAFDevice, max_spikes_rate
+    from vtem_amd.block import Decoder, Encoder, dev, spatial_coeffs
+    from vtem_amd.serve import Request, Server, footprint_bytes, poisson_arrivals
+    _banner("server: encode+decode alone == in the mix, overflow redo")
+    specs = [(32, 2.0), (48, 1.0), (32, 1.0), (48, 2.0), (64, 1.0), (32, 0.5), (48, 0.5)]
+    reqs, parts = [], []
+    arr = poisson_arrivals(len(specs), 200.0, np.random.default_rng(5))
+    for i, ((sd, du), a) in enumerate(zip(specs, arr)):
+        blk = syn.serve_block(sd, du, seed=700 + i)
+        geom = Geometry(sd, sd, blk.Lx, blk.Lx, blk.Lt, du)
+        enc = Encoder(geom, blk.bank)
+        ms = 4 if i == 3 else max_spikes_rate(geom, blk.iaf)
+        ej = EncodeJob(enc, dev(blk.video.frames.reshape(geom.frames, -1)),
+                       IAFDevice.from_params(blk.iaf), ms)
+        dec = Decoder(geom, spatial_coeffs(enc.RF, geom), blk.iaf.kappa, blk.iaf.delta,
+                      blk.iaf.bias)
+        job = BlockJob(dec, None, tag=i, method="rnn" if i % 3 == 2 else "tikhonov",
+                       rnn_steps=100)
+        parts.append((ej, job, max_spikes_rate(geom, blk.iaf)))
+        reqs.append(Request(i, float(a), job, geom.frames, footprint_bytes(job, "kpcg", encode=ej),
+                            encode=ej, iaf=blk.iaf))
+    server = Server(max_batch=3, streams=2, precision="kpcg", adaptive=False)
+    st = server.serve(reqs)
+    print(f"served: batches={st['batches']} overflows={st['overflows']} p50={st['p50_ms']:.2f} ms",
+          flush=True)
+    assert st["overflows"] == 1
+    be, bd = BatchEncoder(), BatchDecoder(precision="kpcg")
+    for r, (ej, job, ms_ok) in zip(reqs, parts):
+        ej1 = dataclasses.replace(ej, max_spike=max(ej.max_spike, ms_ok)) if ej.max_spike == 4 else ej
+        s1 = be.encode([ej1], carry=False)[0]
+        alone = bd.decode([dataclasses.replace(job, spikes=s1)])[0]
+        c = s1.counts.cpu().numpy()
+        np.testing.assert_array_equal(r.spikes.counts.cpu().numpy(), c)
+        g, o = r.spikes.spikes.cpu().numpy(), s1.spikes.cpu().numpy()
+        for j in range(c.size):                 # rows beyond a neuron's count are unwritten
+            np.testing.assert_array_equal(g[:c[j], j], o[:c[j], j])
+        np.testing.assert_array_equal(r.result.frames.cpu().numpy(), alone.frames.cpu().numpy())

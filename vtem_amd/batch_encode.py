@@ -92,6 +92,17 @@ def max_spikes_rate(geom, iaf_host: IAFParams, margin: float = 2.0) -> int:
     return int(math.ceil(geom.duration * rate * margin)) + 16
 
 
+def encode_footprint(job: EncodeJob) -> int:
+    """In-flight HBM of one encode (the RF bank and the frames are resident before
+    admission): spike buffer and counts, W^T and C^T, the split-K workspaces, the IAF
+    workspace and the carried state."""
+    g = job.encoder.geom
+    N = job.encoder.RF.shape[0]
+    return int(8 * job.max_spike * N + 4 * N + 8 * (g.P + N) * g.Dt
+               + 8 * (skinny_workspace(g.P, g.frames, g.Dt) + skinny_workspace(N, g.P, g.Dt))
+               + iaf_par_workspace_bytes(N, g.iaf_samples) + 16 * N)
+
+
 def _theta_scaled(enc: Encoder) -> torch.Tensor:
     """Theta_f dx dy / frames (frames x Dt), cached on the encoder."""
     t = getattr(enc, "_theta_fs", None)
@@ -157,12 +168,26 @@ class BatchEncoder:
             _skinny(d2, max(j.encoder.RF.shape[0] for j in jobs), max(g.P for g in geo), maxDt, dvc)
         return Ct
 
-    def encode(self, jobs: List[EncodeJob], overflow: torch.Tensor = None) -> List[SpikeTrains]:
-        """Encode every job; returns resident spike trains (no host synchronisation).  Job
-        j's IAF state advances as in a serial encode.  overflow (int32, len(jobs), zeroed
-        by the caller) gets 1 where a job's buffer was too small."""
+    def encode(self, jobs: List[EncodeJob], overflow: torch.Tensor = None,
+               carry: bool = True) -> List[SpikeTrains]:
+        """Encode every job; returns resident spike trains (no host synchronisation).
+        carry=True advances each job's IAF state (v0, time_count) in place, as the
+        reference's encoder state (vtem/ensemble_encode.py:541-545); carry=False leaves it
+        and puts the advanced state in self.last_states.  overflow (int32, len(jobs),
+        zeroed by the caller) gets 1 where a job's buffer was too small."""
         dvc = self.device
         coefs = self.coefficients(jobs)
+        if carry:
+            states = [(j.iaf.v0, j.iaf.time_count) for j in jobs]
+        else:
+            v0 = torch.cat([j.iaf.v0 for j in jobs]) if jobs else None
+            tc = torch.cat([j.iaf.time_count for j in jobs]) if jobs else None
+            states, o = [], 0
+            for j in jobs:
+                n = j.iaf.v0.numel()
+                states.append((v0[o:o + n], tc[o:o + n]))
+                o += n
+        self.last_states = states
         out, ws_sizes = [], []
         for job, C in zip(jobs, coefs):
             g = job.encoder.geom
@@ -175,11 +200,11 @@ class BatchEncoder:
         d = np.zeros(len(jobs), IAFP_DTYPE)
         off = 0
         maxN = maxS = maxDt = 0
-        for i, (job, C, st) in enumerate(zip(jobs, coefs, out)):
+        for i, (job, C, st, (v0, tc)) in enumerate(zip(jobs, coefs, out, states)):
             g = job.encoder.geom
             N = C.shape[0]
             d[i] = (ptr(C), 1, C.stride(0), ptr(job.encoder.theta_iaf), 0, 0, ptr(st.spikes), N,
-                    ptr(job.iaf.v0), ptr(job.iaf.time_count), ptr(job.iaf.kappa),
+                    ptr(v0), ptr(tc), ptr(job.iaf.kappa),
                     ptr(job.iaf.bias), ptr(job.iaf.delta), ptr(st.counts), ptr(ws) + off,
                     0 if overflow is None else ptr(overflow) + 4 * i, g.dt, N, g.iaf_samples,
                     g.Dt, job.max_spike, self.root_mode, 0)

@@ -1,21 +1,26 @@
-"""Serving runtime: continuous batching of decode requests on one GPU.
+"""Serving runtime: continuous batching of encode+decode requests on one GPU.
 
-Requests (video blocks of different sizes and durations, with resident spike trains and
-prepared decoders) arrive over time.  The server:
+Requests (video blocks of different sizes and durations: frames resident in HBM with the
+block's receptive-field bank, IAF parameters and prepared decoder) arrive over time.  A
+request either carries its video to encode (EncodeJob) or already-encoded spike trains.
+The server:
 
 * admits arrived requests into a ready queue;
 * whenever one of its streams is free, forms a batch from the head of the queue, bounded
-  by a request count and by an HBM budget (each request's in-flight footprint: its
-  D x D FP32 factor, block inverses, moments, intervals and output frames);
-* launches the whole batched decode on that stream without a host synchronisation, so
-  batches formed from requests that arrive while others are on the GPU run concurrently
-  on the device;
-* retires a batch when its completion event fires: runs the precision guard (FP64 redo
-  of any block whose low-precision solve missed tolerance), releases its memory and
-  stamps each request's completion time.
+  by a request count and by an HBM budget (each request's in-flight footprint: spike
+  buffer and encoder workspaces, intervals, moments, the solver's state and the output
+  frames);
+* launches the whole batch on that stream without a host synchronisation -- the batched
+  encode (filter products and time-parallel IAF of every block) and then the batched
+  decode -- so batches formed from requests that arrive while others are on the GPU run
+  concurrently on the device;
+* retires a batch when its completion event fires: re-runs any block whose spike buffer
+  overflowed (with an exact buffer) or whose low-precision solve missed tolerance,
+  releases its memory and stamps each request's completion time.
 
 Latency is arrival -> result resident in HBM.  The plain mode is the per-block FP64
-path: one request at a time, in arrival order, no batching.
+path: one request at a time, in arrival order, no batching: the reference's encoder
+(frame filtering, serial IAF loop) and the FP64 Gram + Cholesky decode.
 
 The reference has no serving layer; its decode drivers run one spike file at a time
 (vtem/vtdm.py:18-309).
@@ -33,23 +38,27 @@ import numpy as np
 import torch
 
 from .batch import BatchDecoder, BlockJob
-from .block import DecodeResult
+from .batch_encode import BatchEncoder, EncodeJob, encode_footprint
+from .block import DecodeResult, SpikeTrains
 
 
 @dataclasses.dataclass
 class Request:
     rid: int
     arrival: float                 # seconds after the start of the run
-    job: BlockJob
+    job: BlockJob                  # decode part (spikes None when `encode` is given)
     frames: int                    # frames produced by the decode
     footprint: int                 # HBM bytes while in flight
+    encode: Optional[EncodeJob] = None
+    iaf: Any = None                # host IAF parameters (plain mode's encoder)
     start: Optional[float] = None
     done: Optional[float] = None
     result: Optional[DecodeResult] = None
     batch: Optional[int] = None
+    spikes: Optional[SpikeTrains] = None
 
 
-def footprint_bytes(job: BlockJob, form: str = "kpcg") -> int:
+def footprint_bytes(job: BlockJob, form: str = "kpcg", encode: EncodeJob = None) -> int:
     """In-flight HBM of one decode (the decoder's S and the spike trains are resident
     before admission and not counted): intervals, moments T_j/t_j, output frames and the
     solver's own state -- for the Gram forms the FP32 D x D factor, its block inverses and
@@ -58,8 +67,11 @@ def footprint_bytes(job: BlockJob, form: str = "kpcg") -> int:
     d = job.decoder
     g = d.geom
     D, Ds = g.D, g.Ds
-    cap = job.spikes.spikes.shape[0] * d.N
+    rows = encode.max_spike if encode is not None else job.spikes.spikes.shape[0]
+    cap = rows * d.N
     common = 20 * cap + 8 * d.N * g.Dt * (g.Dt + 3) + 8 * g.frames * g.P + 8 * g.frames * g.Ds
+    if encode is not None:
+        common += encode_footprint(encode)
     if job.method == "rnn":
         return int(common + 8 * 6 * D)
     if form == "kpcg":
@@ -121,6 +133,8 @@ class Server:
                 raise ValueError(f"request {r.rid} needs {r.footprint} B, above the HBM budget")
         bd = BatchDecoder(precision=self.precision, verify=self.verify)
         bd.instrument(instrument)
+        be = BatchEncoder()
+        overflows = 0
         streams = [torch.cuda.Stream() for _ in range(self.n_streams)]
         busy = [None] * self.n_streams          # (event, handle, reqs, bytes)
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
@@ -137,9 +151,16 @@ class Server:
             # retire finished batches
             for k, b in enumerate(busy):
                 if b is not None and b[0].query():
-                    ev, h, reqs, nbytes = b
+                    ev, h, reqs, nbytes, flags = b
                     with torch.cuda.stream(streams[k]):
                         res = bd.finalize(h)
+                        if flags is not None:
+                            over = [r for r, f in zip([r for r in reqs if r.encode is not None],
+                                                      flags.cpu().numpy()) if f]
+                            for r in over:
+                                i = reqs.index(r)
+                                res[i] = self._redo_overflow(r, be, bd)
+                            overflows += len(over)
                     fallbacks += bd.last_stats.get("fallbacks", 0)
                     if self.policy is not None:
                         failed = set(bd.last_stats.get("fallback_index", []))
@@ -167,6 +188,7 @@ class Server:
                 precs = [self.policy.choose(r.job) if self.policy else self.precision
                          for r in batch]
                 with torch.cuda.stream(streams[k]):
+                    flags = self._encode(batch, be, bd)
                     h = bd.launch([r.job for r in batch], precs)
                     ev = torch.cuda.Event()
                     ev.record()
@@ -174,7 +196,7 @@ class Server:
                     r.start, r.batch = tstart, nbatch
                 nbatch += 1
                 in_use += nbytes
-                busy[k] = (ev, h, batch, nbytes)
+                busy[k] = (ev, h, batch, nbytes, flags)
                 launched = True
             if not launched:
                 # nothing to launch: wait for the next arrival or a completion
@@ -185,10 +207,49 @@ class Server:
                     time.sleep(min(nxt, 5e-3))
         torch.cuda.synchronize()
         wall = clock() - t0
-        extra = {"batches": nbatch, "fallbacks": fallbacks}
+        extra = {"batches": nbatch, "fallbacks": fallbacks, "overflows": overflows}
         if instrument:
             extra["timers"] = bd.timers
         return self._stats(requests, wall, extra)
+
+    @staticmethod
+    def _encode(batch: List[Request], be: BatchEncoder, bd: BatchDecoder):
+        """Batched encode of the requests that carry video; their decode jobs get the
+        resident spike trains.  Returns the per-request overflow flags (device) or None.
+        Instrumented as stage "encode" with its algorithmic HBM bytes (frames, RF bank,
+        coefficients and spike buffers)."""
+        enc = [r for r in batch if r.encode is not None]
+        if not enc:
+            return None
+        e0 = bd._tic()
+        flags = torch.zeros(len(enc), dtype=torch.int32, device=be.device)
+        sts = be.encode([r.encode for r in enc], flags, carry=False)
+        if e0 is not None:
+            work = 0.0
+            for r in enc:
+                g, N = r.encode.encoder.geom, r.encode.encoder.RF.shape[0]
+                work += 8.0 * (g.frames * g.P + N * g.P + 2 * (g.P + N) * g.Dt + 4 * N)
+            bd._toc("encode", e0, work)
+        for r, st in zip(enc, sts):
+            r.spikes = st
+            r.job = dataclasses.replace(r.job, spikes=st)
+        return flags
+
+    @staticmethod
+    def _redo_overflow(r: Request, be: BatchEncoder, bd: BatchDecoder) -> DecodeResult:
+        """A block fired more often than its buffer held: encode it again with the buffer
+        the counts call for and decode it alone (FP64-guarded solve of the batch's form)."""
+        need = int(r.spikes.counts.max().item())
+        while True:
+            job = dataclasses.replace(r.encode, max_spike=2 * max(need, r.encode.max_spike))
+            flags = torch.zeros(1, dtype=torch.int32, device=be.device)
+            st = be.encode([job], flags, carry=False)[0]
+            if not int(flags.item()):
+                break
+            need = int(st.counts.max().item())
+        r.spikes = st
+        r.job = dataclasses.replace(r.job, spikes=st)
+        return bd.decode([r.job])[0]
 
     def _serve_plain(self, requests, clock):
         order = sorted(requests, key=lambda r: r.arrival)
@@ -200,7 +261,14 @@ class Server:
                 time.sleep(r.arrival - now)
             r.start = clock() - t0
             jb = r.job
-            r.result = jb.decoder.decode(jb.spikes, jb.t0, precision="fp64", method=jb.method,
+            spikes = jb.spikes
+            if r.encode is not None:
+                # the reference's encoder per block: every frame filtered, then the
+                # temporal coefficients, then the serial one-lane-per-neuron IAF loop
+                e = r.encode
+                spikes = e.encoder.encode(e.frames, r.iaf, parallel=False)
+                r.spikes = spikes
+            r.result = jb.decoder.decode(spikes, jb.t0, precision="fp64", method=jb.method,
                                          rnn_steps=jb.rnn_steps)
             torch.cuda.current_stream().synchronize()
             r.done = clock() - t0
