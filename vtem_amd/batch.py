@@ -220,14 +220,29 @@ class BatchDecoder:
             key = (jb.decoder.geom.Dt, "rnn", jb.rnn_steps) if jb.method == "rnn" else \
                 (jb.decoder.geom.Dt, pr, 0)
             groups.setdefault(key, []).append(i)
-        kp, rn = [], {}
-        for (Dt, pr, steps), idx in sorted(groups.items()):
+        main = torch.cuda.current_stream()
+        rnn_keys = [k for k in sorted(groups) if k[1] == "rnn"]
+        other_keys = [k for k in sorted(groups) if k[1] != "rnn"]
+        side = None
+        if rnn_keys and other_keys:
+            # The RNN groups (hundreds of small dependent launches) run on a side stream,
+            # concurrently with the other groups' solves; the main stream waits for them
+            # at the end, so the batch's completion covers both.
+            side = self._side_stream(main)
+            fork = torch.cuda.Event()
+            fork.record(main)
+            side.wait_event(fork)
+            with torch.cuda.stream(side):
+                self._launch_rnn(h, jobs, groups, rnn_keys)
+        elif rnn_keys:
+            self._launch_rnn(h, jobs, groups, rnn_keys)
+        kp = []
+        for key in other_keys:
+            Dt, pr, _ = key
+            idx = groups[key]
             g = self._prologue([jobs[i] for i in idx], Dt)
             if pr == "kpcg":
                 kp.append((idx, g))
-                continue
-            if pr == "rnn":
-                rn.setdefault(steps, []).append((idx, g))
                 continue
             X, nrm, info, skeep = self._solve_direct(g, pr)
             res, keep = self._synthesis(g, X, nrm, info, None, 0)
@@ -248,6 +263,31 @@ class BatchDecoder:
                 all_idx += idx
             h.keep.append(skeep)
             h.checks.append((all_idx, (nrm, info), "kpcg"))
+        if side is not None:
+            join = torch.cuda.Event()
+            join.record(side)
+            main.wait_event(join)
+            for r in h.results:
+                if r is not None:       # frames made on the side stream, used on main
+                    r.frames.record_stream(main)
+        return h
+
+    def _side_stream(self, main) -> torch.cuda.Stream:
+        ss = getattr(self, "_sides", None)
+        if ss is None:
+            ss = self._sides = {}
+        key = main.cuda_stream
+        if key not in ss:
+            ss[key] = torch.cuda.Stream(device=main.device)     # default priority
+        return ss[key]
+
+    def _launch_rnn(self, h: "BatchHandle", jobs, groups, keys):
+        """Prologue, joint RNN iterations and synthesis of the RNN groups (current stream)."""
+        rn: Dict[int, list] = {}
+        for key in keys:
+            Dt, _, steps = key
+            idx = groups[key]
+            rn.setdefault(steps, []).append((idx, self._prologue([jobs[i] for i in idx], Dt)))
         for steps, parts in sorted(rn.items()):
             gs = [g for _, g in parts]
             X, st, boff, skeep = self._solve_rnn(gs, steps)
@@ -259,7 +299,6 @@ class BatchDecoder:
                     h.results[i] = r
                 h.keep.append((g.keep, keep, zero, info))
             h.keep.append(skeep)
-        return h
 
     def finalize(self, h: "BatchHandle") -> List[DecodeResult]:
         """Guard: blocks whose low-precision factor failed or whose refinement did not
