@@ -449,8 +449,9 @@ def main():
         fps, used, t = cpu_baseline(items, args.cpu_budget)
         line["cpu_baseline"] = {"value": round(fps, 3), "unit": "frames/s", "cores": os.cpu_count(),
                                 "kind": "port",
-                                "sample": f"float64 oracle decode of the first {used} requests "
-                                          f"({t:.1f} s, same spikes)"}
+                                "sample": f"float64 oracle encode (RF bank, frame filtering, "
+                                          f"upsampling, serial IAF) + decode of the first {used} "
+                                          f"requests ({t:.1f} s)"}
     print(json.dumps(line), flush=True)
 
 

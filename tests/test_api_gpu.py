@@ -110,6 +110,12 @@ def test_iaf_encode_api(torch_mod, oracle):
         np.testing.assert_allclose(joined, lists[j], rtol=0, atol=1e-12)
     with pytest.raises(ValueError):
         iaf.encode(v, avg_rate=50.0)
+    # time-parallel form: same counts, times within 1e-9 s
+    iaf3 = IAF_encode(N, 0.01, parallel=True)
+    iaf3.load_parameters(kappa, bias, delta, h_v0=v0)
+    l3, c3 = iaf3.encode(v)
+    np.testing.assert_array_equal(c3, co)
+    assert max(np.abs(a - b).max(initial=0) for a, b in zip(l3, lo)) < 1e-9
 
 
 def test_vtem_iaf_chunks(torch_mod, oracle):

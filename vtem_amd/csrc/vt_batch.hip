@@ -468,7 +468,12 @@ __device__ __forceinline__ double skinny_row_factor(const SkDesc &d, int m, int 
 // RT row tiles of 16 per workgroup (the row tiling does not change any sum's order, so the
 // launch may pick it from the batch size); SK_WAVES waves split K and are reduced in LDS
 // in a fixed tree order.
+#ifndef SK_WAVES
 #define SK_WAVES 8
+#endif
+#ifndef SK_INFLIGHT
+#define SK_INFLIGHT 2      // chunks each wave has in flight
+#endif
 template <int NT, int RT>
 __global__ void __launch_bounds__(64 * SK_WAVES) skinny_kernel(const SkDesc *__restrict__ descs)
 {
@@ -489,13 +494,14 @@ __global__ void __launch_bounds__(64 * SK_WAVES) skinny_kernel(const SkDesc *__r
     const int nch = (d.K + 15) / 16;
     const int per = (nch + ks - 1) / ks;
     const int c_beg = z * per, c_end = min(nch, c_beg + per);
-    // two chunks in flight per wave: c and c + SK_WAVES
-    for (int c = c_beg + wave; c < c_end; c += 2 * SK_WAVES) {
-        double a0[RT][4], b0[NT][4], a1[RT][4], b1[NT][4];
-        skinny_load<NT, RT>(d, m0, c, c_end, r, g, Nc, a0, b0);
-        skinny_load<NT, RT>(d, m0, c + SK_WAVES, c_end, r, g, Nc, a1, b1);
-        skinny_mma<NT, RT>(acc, a0, b0);
-        skinny_mma<NT, RT>(acc, a1, b1);
+    // SK_INFLIGHT chunks in flight per wave: c, c + SK_WAVES, ... (added in that order)
+    for (int c = c_beg + wave; c < c_end; c += SK_INFLIGHT * SK_WAVES) {
+        double a[SK_INFLIGHT][RT][4], b[SK_INFLIGHT][NT][4];
+#pragma unroll
+        for (int u = 0; u < SK_INFLIGHT; ++u)
+            skinny_load<NT, RT>(d, m0, c + u * SK_WAVES, c_end, r, g, Nc, a[u], b[u]);
+#pragma unroll
+        for (int u = 0; u < SK_INFLIGHT; ++u) skinny_mma<NT, RT>(acc, a[u], b[u]);
     }
     // cross-wave tree reduction: at each level the upper half adds into the lower half
 #pragma unroll

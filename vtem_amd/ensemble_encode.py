@@ -3,8 +3,9 @@
 IAF_encode (deterministic thresholds) and IAF_encode_rt (per-spike random thresholds)
 keep the reference's constructor, load_parameters, encode(neural_inputs, startbias,
 avg_rate) -> (list of N spike arrays, int32 counts) and reset_timer.  IAF_encode runs
-the time-parallel vt_iaf_encode_par by default (parallel=False: vt_iaf_encode, one lane per
-neuron, bit-identical to the reference's loop); IAF_encode_rt runs vt_iaf_encode_rt.  v0
+vt_iaf_encode, one lane per neuron, bit-identical to the reference's loop
+(parallel=True: the time-parallel vt_iaf_encode_par, same counts, times to rounding, for
+long inputs); IAF_encode_rt runs vt_iaf_encode_rt.  v0
 and the time counter stay on the GPU between calls, as the reference's state does.  encode_device returns the
 resident spike buffer instead of host lists (the decode path's input).
 
@@ -34,7 +35,7 @@ def _dev(a, dtype=F64, device="cuda"):
 
 class pop_encode:
     def __init__(self, num_neurons, dt, dtype=np.float64, device="cuda", rng=None, root_mode=0,
-                 parallel=True):
+                 parallel=False):
         if np.dtype(dtype) != np.float64:
             raise ValueError("the encoder stores spike times in float64")
         self.num_neurons = int(num_neurons)
@@ -43,7 +44,8 @@ class pop_encode:
         self.device = device
         self.rng = rng
         self.root_mode = int(root_mode)
-        # time-parallel encoder (same counts, times to rounding) or the serial loop
+        # parallel=True: the time-parallel encoder (same counts, times to rounding);
+        # default: the serial loop, bit-identical to the reference's
         self.parallel = bool(parallel)
 
     def _uniform(self, n):
