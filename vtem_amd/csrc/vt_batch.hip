@@ -8,6 +8,7 @@
 // blocks live in shared arrays and the single-problem kernels (vt_q, vt_neuron_T_batched,
 // vt_gram_split_T, vt_tj_apply) run over the whole batch unchanged.
 #include "vt_common.hpp"
+#include "vt_skinny.hpp"
 
 // ------------------------------------------------------------------------------------
 // Intervals over a batch.  job_of[j] maps a global neuron to its block; descriptor per
@@ -392,29 +393,9 @@ VT_EXPORT int vt_seg_counts(const int *offsets, const int *base, int nseg, int *
 // does not depend on the batch), each writing a partial tile to its workspace; a second
 // pass adds the ks partials in order and applies the row factor.
 // ------------------------------------------------------------------------------------
-#ifndef VT_SKINNY_CHUNKS_PER_SPLIT
-#define VT_SKINNY_CHUNKS_PER_SPLIT 32
-#endif
 #ifndef VT_SKINNY_SMALL_WG
 #define VT_SKINNY_SMALL_WG 4096
 #endif
-
-struct SkDesc {
-    const double *A, *B;
-    double *C;
-    const double *T;
-    double *ws;                 // ks * M * Nc doubles when ks > 1
-    const double *active;       // optional: the problem is skipped while *active == 0
-    long long sam, sak, ldb, ldc, tstride;
-    int M, K, Nc, pad;
-};
-
-__host__ __device__ __forceinline__ int skinny_splits(int K)
-{
-    const int nch = (K + 15) / 16;
-    const int ks = nch / VT_SKINNY_CHUNKS_PER_SPLIT;
-    return ks < 1 ? 1 : (ks > 8 ? 8 : ks);
-}
 
 // operands of one 16-wide K chunk for lane (r, g); chunks at or beyond c_end load zeros
 template <int NT, int RT>
@@ -584,14 +565,10 @@ VT_EXPORT int vt_gemm_skinny_workspace_size(int M, int K, int Nc, long long *dou
     return VT_OK;
 }
 
-VT_EXPORT int vt_gemm_skinny(const void *descs, int batch, int max_M, int max_K, int max_Nc,
-                             void *stream)
+int vt_skinny_first_pass(const void *descs, int batch, int max_M, int max_K, int max_Nc,
+                         hipStream_t s)
 {
-    VT_CHECK_ARG(descs && batch >= 1 && batch <= 65535 && max_M >= 0 && max_K >= 0 && max_Nc >= 1 &&
-                 max_Nc <= 32);
-    if (max_M == 0) return VT_OK;
     const int ks = skinny_splits(max_K);
-    hipStream_t s = (hipStream_t)stream;
     const SkDesc *d = (const SkDesc *)descs;
     // small launches take 16-row tiles (4x the workgroups), large ones 64-row tiles
     const long long wg64 = (long long)batch * vt_blocks(max_M, 64) * ks;
@@ -608,7 +585,21 @@ VT_EXPORT int vt_gemm_skinny(const void *descs, int batch, int max_M, int max_K,
         else
             skinny_kernel<2, 4><<<grid, 64 * SK_WAVES, 0, s>>>(d);
     }
-    if (ks > 1) skinny_reduce_kernel<<<dim3(vt_blocks(max_M, 8), batch), 256, 0, s>>>(d);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+VT_EXPORT int vt_gemm_skinny(const void *descs, int batch, int max_M, int max_K, int max_Nc,
+                             void *stream)
+{
+    VT_CHECK_ARG(descs && batch >= 1 && batch <= 65535 && max_M >= 0 && max_K >= 0 && max_Nc >= 1 &&
+                 max_Nc <= 32);
+    if (max_M == 0) return VT_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int r = vt_skinny_first_pass(descs, batch, max_M, max_K, max_Nc, s);
+    if (r) return r;
+    if (skinny_splits(max_K) > 1)
+        skinny_reduce_kernel<<<dim3(vt_blocks(max_M, 8), batch), 256, 0, s>>>((const SkDesc *)descs);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }

@@ -20,6 +20,7 @@
 // The reference solves the spike-space system with a truncated SVD
 // (vtem/utils/linalg.py:591-631) and has no iterative Tikhonov solver.
 #include "vt_common.hpp"
+#include "vt_skinny.hpp"
 
 #define KP_CHUNKS 16
 #define KP_MAX_DT 33
@@ -420,16 +421,29 @@ __global__ void zero_kernel(double *__restrict__ x, long long n)
         x[i] = 0.0;
 }
 
-// x += eta_b (r - q - lam_b x)
-__global__ void rnn_update_kernel(double *__restrict__ x, const double *__restrict__ r,
-                                  const double *__restrict__ q, const double *__restrict__ lam,
-                                  const double *__restrict__ eta, const int *__restrict__ elem_job,
-                                  long long n)
+// x += eta_b (r - q - lam_b x), with q = S^T Z finished here: blocks whose backward product
+// split its reduction sum their partials (in split order, as the product's own second pass
+// does); the others read the product's output.  One launch instead of two per iteration.
+__global__ void rnn_reduce_update_kernel(double *__restrict__ x, const double *__restrict__ r,
+                                         const double *__restrict__ q, const SkDesc *__restrict__ bwd,
+                                         const double *__restrict__ lam, const double *__restrict__ eta,
+                                         const int *__restrict__ elem_job, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         const int b = elem_job[i];
-        x[i] += eta[b] * (r[i] - q[i] - lam[2 * b] * x[i]);
+        const SkDesc &d = bwd[b];
+        const int ks = skinny_splits(d.K);
+        double qi;
+        if (ks > 1) {
+            const long long loc = i - (long long)(d.C - q);     // index within the block
+            const long long mn = (long long)d.M * d.Nc;
+            qi = 0.0;
+            for (int z = 0; z < ks; ++z) qi += d.ws[z * mn + loc];
+        } else {
+            qi = q[i];
+        }
+        x[i] += eta[b] * (r[i] - qi - lam[2 * b] * x[i]);
     }
 }
 
@@ -474,8 +488,9 @@ VT_EXPORT int vt_kron_rnn(const void *fwd, int fwd_M, int fwd_K, const void *bwd
     zero_kernel<<<g, 256, 0, s>>>(x, n);   // a kernel, not a memset node, if captured
     for (int it = 0; it <= steps; ++it) {
         VT_TRY(skinny_run(F, batch, max_Nc, stream));
-        VT_TRY(skinny_run(Bk, batch, max_Nc, stream));
-        rnn_update_kernel<<<g, 256, 0, s>>>(x, r, q, lam, eta, elem_job, n);
+        if (Bk.max_M > 0) VT_TRY(vt_skinny_first_pass(Bk.descs, batch, Bk.max_M, Bk.max_K, max_Nc, s));
+        rnn_reduce_update_kernel<<<g, 256, 0, s>>>(x, r, q, (const SkDesc *)Bk.descs, lam, eta,
+                                                   elem_job, n);
         VT_LAUNCH_CHECK();
     }
     return VT_OK;
