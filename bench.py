@@ -48,6 +48,7 @@ if ROOT not in sys.path:
 # HBM bytes for "hbm" stages
 RNN_SHARE = 0.2          # decoder mix: 80 % pseudoinverse (Tikhonov), 20 % RNN
 RNN_STEPS = 500
+LAT_REQUESTS = 1024      # requests per latency-sweep rate point
 
 PEAKS = {"gram_lp": (2500.0, "TFLOP/s", "mfma"), "gram_f64": (78.6, "TFLOP/s", "mfma"),
          "potrf_f32": (157.3, "TFLOP/s", "mfma"), "potrf_f64": (78.6, "TFLOP/s", "mfma"),
@@ -55,8 +56,13 @@ PEAKS = {"gram_lp": (2500.0, "TFLOP/s", "mfma"), "gram_f64": (78.6, "TFLOP/s", "
          "kpcg_mgram": (78.6, "TFLOP/s", "mfma"), "kpcg_potrf": (78.6, "TFLOP/s", "mfma"),
          "kpcg_inverse": (78.6, "TFLOP/s", "mfma"), "prologue": (8000.0, "GB/s", "hbm"),
          "synthesis": (8000.0, "GB/s", "hbm"), "rnn_iterate": (8000.0, "GB/s", "hbm"),
-         "kpcg_iterate": (8000.0, "GB/s", "hbm"), "encode": (8000.0, "GB/s", "hbm")}
-WORK_NOTES = {"kpcg_iterate": "algorithmic HBM bytes per CG iteration: 8*(2*N*Ds + N*Dt^2 + 4*N*Dt "
+         "kpcg_iterate": (8000.0, "GB/s", "hbm"), "encode_coeffs": (8000.0, "GB/s", "hbm"),
+         "iaf": (78.6, "TFLOP/s", "fp64")}
+WORK_NOTES = {"encode_coeffs": "algorithmic HBM bytes 8*(F*P + N*P + 2*(P+N)*Dt) per block (frames, "
+                               "RF bank, the two reassociated coefficient products)",
+              "iaf": "algorithmic FP64 flops 2*Dt*N*(T_f-1) per block (the trigonometric input's "
+                     "evaluation; the walk's second pass and the step arithmetic are extra)",
+              "kpcg_iterate": "algorithmic HBM bytes per CG iteration: 8*(2*N*Ds + N*Dt^2 + 4*N*Dt "
                               "+ Ds^2 + 12*D) per block (S twice, T_j, Y/Z, Mi, vectors)",
               "gram_lp": "algorithmic flops N*D*(D+1) per block",
               "potrf_f32": "algorithmic flops D^3/3 per block"}
@@ -425,6 +431,11 @@ def main():
         "snr": {"mean_db": round(float(np.mean(snrs)), 3), "min_db": round(float(np.min(snrs)), 3),
                 "max_abs_diff_vs_plain_db": float(f"{max(d_snr):.3g}")},
         "stage_ms": {k: round(v[0], 3) for k, v in stage.items()},
+        "stage_roofline": {k: {"achieved": round(v[1] / (v[0] * 1e-3) / (1e9 if PEAKS[k][1] == "GB/s" else 1e12), 3),
+                               "unit": PEAKS[k][1], "peak": PEAKS[k][0],
+                               "frac": round(v[1] / (v[0] * 1e-3) / (1e9 if PEAKS[k][1] == "GB/s" else 1e12)
+                                             / PEAKS[k][0], 5)}
+                           for k, v in stage.items() if k in PEAKS and v[1] > 0},
         "dominant_stage": {"stage": dom, "bound": bound, "achieved": round(achieved, 3), "peak": peak,
                            "unit": unit, "frac": round(achieved / peak, 5),
                            "work_note": WORK_NOTES.get(dom, "")},
@@ -434,13 +445,16 @@ def main():
                      "algorithmic_bytes": dk["bytes"], "blocks": dk["blocks"]},
     }
     if not args.no_sweep:
+        # p50/p99 over LAT_REQUESTS requests per rate point: the prepared blocks, cycled
         cap_rps = R / (wall / args.steps)
         sweep = []
+        cyc = [items[i % R] for i in range(LAT_REQUESTS)]
         for frac in (0.5, 0.8):
             rate = frac * cap_rps
-            arr = poisson_arrivals(R, rate, np.random.default_rng(args.seed + 17))
-            st = server.serve(make_requests(items, arr))
-            sweep.append({"offered_rps": round(rate, 2), "frames_per_s": round(st["frames_per_s"], 2),
+            arr = poisson_arrivals(LAT_REQUESTS, rate, np.random.default_rng(args.seed + 17))
+            st = server.serve(make_requests(cyc, arr))
+            sweep.append({"offered_rps": round(rate, 2), "requests": LAT_REQUESTS,
+                          "frames_per_s": round(st["frames_per_s"], 2),
                           "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3)})
         line["latency_sweep"] = sweep
     if not args.no_precision:

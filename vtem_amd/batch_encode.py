@@ -169,14 +169,22 @@ class BatchEncoder:
         return Ct
 
     def encode(self, jobs: List[EncodeJob], overflow: torch.Tensor = None,
-               carry: bool = True) -> List[SpikeTrains]:
+               carry: bool = True, timers=None) -> List[SpikeTrains]:
         """Encode every job; returns resident spike trains (no host synchronisation).
         carry=True advances each job's IAF state (v0, time_count) in place, as the
         reference's encoder state (vtem/ensemble_encode.py:541-545); carry=False leaves it
         and puts the advanced state in self.last_states.  overflow (int32, len(jobs),
         zeroed by the caller) gets 1 where a job's buffer was too small."""
         dvc = self.device
+        e0 = timers._tic() if timers is not None else None
         coefs = self.coefficients(jobs)
+        if e0 is not None:
+            work = 0.0
+            for j in jobs:
+                g, N = j.encoder.geom, j.encoder.RF.shape[0]
+                work += 8.0 * (g.frames * g.P + N * g.P + 2 * (g.P + N) * g.Dt)
+            timers._toc("encode_coeffs", e0, work)
+            e0 = timers._tic()
         if carry:
             states = [(j.iaf.v0, j.iaf.time_count) for j in jobs]
         else:
@@ -213,4 +221,8 @@ class BatchEncoder:
         if jobs:
             dd = _to_dev(d, dvc)
             lib.vt_iaf_encode_trig_batched(ptr(dd), len(jobs), maxN, maxS, maxDt, stream_ptr())
+        if e0 is not None:
+            # algorithmic work: the input's evaluation, 2 Dt flops per sample interval and neuron
+            timers._toc("iaf", e0, float(sum(2.0 * j.encoder.geom.Dt * j.encoder.RF.shape[0] *
+                                              (j.encoder.geom.iaf_samples - 1) for j in jobs)))
         return out

@@ -216,20 +216,15 @@ class Server:
     def _encode(batch: List[Request], be: BatchEncoder, bd: BatchDecoder):
         """Batched encode of the requests that carry video; their decode jobs get the
         resident spike trains.  Returns the per-request overflow flags (device) or None.
-        Instrumented as stage "encode" with its algorithmic HBM bytes (frames, RF bank,
-        coefficients and spike buffers)."""
+        Instrumented as stages "encode_coeffs" (algorithmic HBM bytes: frames, RF banks and
+        the two coefficient products) and "iaf" (algorithmic FP64 flops of the input's
+        evaluation)."""
         enc = [r for r in batch if r.encode is not None]
         if not enc:
             return None
-        e0 = bd._tic()
         flags = torch.zeros(len(enc), dtype=torch.int32, device=be.device)
-        sts = be.encode([r.encode for r in enc], flags, carry=False)
-        if e0 is not None:
-            work = 0.0
-            for r in enc:
-                g, N = r.encode.encoder.geom, r.encode.encoder.RF.shape[0]
-                work += 8.0 * (g.frames * g.P + N * g.P + 2 * (g.P + N) * g.Dt + 4 * N)
-            bd._toc("encode", e0, work)
+        sts = be.encode([r.encode for r in enc], flags, carry=False,
+                        timers=bd if bd.timers is not None else None)
         for r, st in zip(enc, sts):
             r.spikes = st
             r.job = dataclasses.replace(r.job, spikes=st)
