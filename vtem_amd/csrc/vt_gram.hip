@@ -208,8 +208,10 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
     constexpr int NCT2 = (NCT + 1) / 2;      // column tiles per column group
     constexpr int KP = KL + 8;               // padded k stride (16-bit elements)
     constexpr int NV = NC * KL / 8;          // 16-byte vectors per T tile
-    __shared__ float S1[2][KL][LP_BP];
-    __shared__ float S2[2][KL][LP_BP];
+    // rows padded to LP_BP + 2 floats: the four 16-lane groups of a wave read rows 8 apart,
+    // which a 16-float stride maps onto the same banks
+    __shared__ float S1[2][KL][LP_BP + 2];
+    __shared__ float S2[2][KL][LP_BP + 2];
     __shared__ __attribute__((aligned(16))) unsigned short Th[2][NC][KP];
     __shared__ __attribute__((aligned(16))) unsigned short Tl[MODE == 0 ? 2 : 1][MODE == 0 ? NC : 1][KP];
 
@@ -236,7 +238,8 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
             const int kk = e / LP_BP, i = e % LP_BP;
             const int j = j0 + kk, p1 = b1 * LP_BP + i, p2 = b2 * LP_BP + i;
             S1[bf][kk][i] = (j < N && p1 < d.Ds) ? (float)d.S[(long long)j * d.lds + p1] : 0.f;
-            S2[bf][kk][i] = (j < N && p2 < d.Ds) ? (float)d.S[(long long)j * d.lds + p2] : 0.f;
+            // FP16: S2 carries the power-of-two scale sW (exact), so W = S1 S2 sW is one product
+            S2[bf][kk][i] = (j < N && p2 < d.Ds) ? (float)d.S[(long long)j * d.lds + p2] * sW : 0.f;
         }
         const uint4 *th = (const uint4 *)(d.Th + (long long)k * NC * KL);
         const uint4 *tl = (const uint4 *)(d.Tl + (long long)k * NC * KL);
@@ -294,7 +297,7 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
                 const int i1 = r >> 4, i2 = r & 15;
 #pragma unroll
                 for (int i = 0; i < 8; ++i)
-                    wh[t][i] = (_Float16)(S1[bf][8 * g + i][i1] * S2[bf][8 * g + i][i2] * sW);
+                    wh[t][i] = (_Float16)(S1[bf][8 * g + i][i1] * S2[bf][8 * g + i][i2]);
             }
 #pragma unroll
             for (int c = 0; c < NCT2; ++c) {
