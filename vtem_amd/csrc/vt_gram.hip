@@ -190,6 +190,7 @@ struct GramDesc {
 };
 
 #define KL 32
+#define KP (KL + 8)           // padded k stride (16-bit elements) of a T tile row, in HBM and LDS
 #define LP_BP 16          // spatial indices per block: a workgroup owns 16 x 16 = 256 pair rows
 #define LP_WAVES 8        // 4 row groups (64 pair rows each) x 2 column groups
 
@@ -206,7 +207,6 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
 {
     constexpr int NC = NCT * 16;
     constexpr int NCT2 = (NCT + 1) / 2;      // column tiles per column group
-    constexpr int KP = KL + 8;               // padded k stride (16-bit elements)
     constexpr int NV = NC * KL / 8;          // 16-byte vectors per T tile
     // rows padded to LP_BP + 2 floats: the four 16-lane groups of a wave read rows 8 apart,
     // which a 16-float stride maps onto the same banks
@@ -231,22 +231,46 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
     const int g = lane >> 4;
     const int nsteps = (N + KL - 1) / KL;
 
-    // stage chunk k (neurons 32k..32k+31) into buffer bf: S slices and the pre-tiled T
-    auto stage = [&](int k, int bf) {
-        const int j0 = k * KL;
-        for (int e = tid; e < KL * LP_BP; e += 64 * LP_WAVES) {
-            const int kk = e / LP_BP, i = e % LP_BP;
-            const int j = j0 + kk, p1 = b1 * LP_BP + i, p2 = b2 * LP_BP + i;
-            S1[bf][kk][i] = (j < N && p1 < d.Ds) ? (float)d.S[(long long)j * d.lds + p1] : 0.f;
+    // Chunk k (neurons 32k..32k+31).  The pre-tiled T rows go from HBM straight into LDS
+    // (global_load_lds, 16 bytes per lane at the wave's LDS base + 16 * lane; the tile's
+    // HBM layout already carries the LDS row padding), and the S slices through one
+    // register per thread: the next chunk's loads are issued before this chunk's MFMAs and
+    // waited for at the step's barrier, so their latency overlaps the math.
+    typedef __attribute__((address_space(1))) const void *gptr_t;
+    typedef __attribute__((address_space(3))) void *lptr_t;
+    constexpr int NTH = 64 * LP_WAVES;
+    constexpr int TV = NC * KP / 8;                      // 16-byte vectors per T tile
+    float rs1 = 0.f, rs2 = 0.f;
+    const int se_kk = tid / LP_BP, se_i = tid % LP_BP;  // S element of this thread (tid < KL * LP_BP)
+    auto load_S = [&](int k) {
+        if (tid < KL * LP_BP) {
+            const int j = k * KL + se_kk, p1 = b1 * LP_BP + se_i, p2 = b2 * LP_BP + se_i;
+            rs1 = (j < N && p1 < d.Ds) ? (float)d.S[(long long)j * d.lds + p1] : 0.f;
             // FP16: S2 carries the power-of-two scale sW (exact), so W = S1 S2 sW is one product
-            S2[bf][kk][i] = (j < N && p2 < d.Ds) ? (float)d.S[(long long)j * d.lds + p2] * sW : 0.f;
+            rs2 = (j < N && p2 < d.Ds) ? (float)d.S[(long long)j * d.lds + p2] * sW : 0.f;
         }
-        const uint4 *th = (const uint4 *)(d.Th + (long long)k * NC * KL);
-        const uint4 *tl = (const uint4 *)(d.Tl + (long long)k * NC * KL);
-        for (int e = tid; e < NV; e += 64 * LP_WAVES) {
-            const int c = e >> 2, kk = (e & 3) * 8;
-            *(uint4 *)&Th[bf][c][kk] = th[e];
-            if (MODE == 0) *(uint4 *)&Tl[bf][c][kk] = tl[e];
+    };
+    auto store_S = [&](int bf) {
+        if (tid < KL * LP_BP) {
+            S1[bf][se_kk][se_i] = rs1;
+            S2[bf][se_kk][se_i] = rs2;
+        }
+    };
+    auto dma_T = [&](int k, int bf) {
+        const unsigned short *th = d.Th + (long long)k * NC * KP;
+        const unsigned short *tl = d.Tl + (long long)k * NC * KP;
+#pragma unroll
+        for (int q = 0; q < (TV + NTH - 1) / NTH; ++q) {
+            const int v0 = q * NTH + wave * 64;          // this wave's first vector (uniform)
+            if (v0 >= TV) break;
+            const int v = v0 + lane;
+            if (v < TV) {
+                __builtin_amdgcn_global_load_lds((gptr_t)(th + 8 * v), (lptr_t)(&Th[bf][0][0] + 8 * v0),
+                                                 16, 0, 0);
+                if (MODE == 0)
+                    __builtin_amdgcn_global_load_lds((gptr_t)(tl + 8 * v), (lptr_t)(&Tl[bf][0][0] + 8 * v0),
+                                                     16, 0, 0);
+            }
         }
     };
 
@@ -256,11 +280,17 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
 #pragma unroll
         for (int c = 0; c < NCT2; ++c) acc[t][c] = (v4f){0, 0, 0, 0};
 
-    stage(0, 0);
+    load_S(0);
+    store_S(0);
+    dma_T(0, 0);
+    __builtin_amdgcn_s_waitcnt(0);                      // this wave's tile loads have landed
     __syncthreads();
     for (int k = 0; k < nsteps; ++k) {
         const int bf = k & 1;
-        if (k + 1 < nsteps) stage(k + 1, bf ^ 1);       // the other buffer: free since the last barrier
+        if (k + 1 < nsteps) {                           // the other buffer: free since the last barrier
+            load_S(k + 1);
+            dma_T(k + 1, bf ^ 1);
+        }
         // A operands: W = S1 S2 for this wave's 4 row tiles, neurons 8g..8g+7 of the chunk
         if (MODE == 0) {
             v8bf wh[4], wl[4];
@@ -309,6 +339,8 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wh[t], bh, acc[t][c], 0, 0, 0);
             }
         }
+        if (k + 1 < nsteps) store_S(bf ^ 1);
+        __builtin_amdgcn_s_waitcnt(0);                  // the next chunk's tile loads have landed
         __syncthreads();
     }
     const float unscale = MODE == 1 ? d.scales[2] : 1.0f;
@@ -331,8 +363,9 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
 }
 
 // Split T_j (N x Dt^2 doubles) into the tiled 16-bit layout of the Gram kernels: chunk k of
-// 32 neurons is a contiguous [NC][32] tile (column-major within the chunk, zero padded), so
-// the kernels stage it with 16-byte copies.  mode 0: hi / lo bf16; mode 1: fp16 of T*scales[0].
+// 32 neurons is a contiguous [NC][KP] tile (column-major within the chunk, rows padded from
+// 32 to KP elements with zeros, as the kernels keep it in LDS), so the kernels copy it to
+// LDS with 16-byte direct loads.  mode 0: hi / lo bf16; mode 1: fp16 of T*scales[0].
 __global__ void split_T_kernel(const double *__restrict__ T, int N, int Dt, int NC, long long total,
                                unsigned short *hi, unsigned short *lo, int mode, const float *scales)
 {
@@ -340,11 +373,11 @@ __global__ void split_T_kernel(const double *__restrict__ T, int N, int Dt, int 
     const int DT2 = Dt * Dt;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
-        const int kk = (int)(i % KL);
-        const long long r = i / KL;
+        const int kk = (int)(i % KP);
+        const long long r = i / KP;
         const int c = (int)(r % NC);
         const int j = (int)(r / NC) * KL + kk;
-        const float t = (j < N && c < DT2) ? (float)T[(long long)j * DT2 + c] : 0.f;
+        const float t = (kk < KL && j < N && c < DT2) ? (float)T[(long long)j * DT2 + c] : 0.f;
         if (mode == 0) {
             const __bf16 h = (__bf16)t;
             const __bf16 l = (__bf16)(t - (float)h);
@@ -405,7 +438,7 @@ VT_EXPORT int vt_gram_split_T_size(int N, int Dt, long long *elems_host)
     VT_CHECK_ARG(N >= 0 && Dt >= 1 && elems_host);
     const int nct = gram_lp_nct(Dt);
     if (!nct) return VT_ERANGE;
-    *elems_host = (long long)((N + KL - 1) / KL) * nct * 16 * KL;
+    *elems_host = (long long)((N + KL - 1) / KL) * nct * 16 * KP;
     return VT_OK;
 }
 
@@ -424,7 +457,7 @@ VT_EXPORT int vt_gram_split_T(const double *Tm, int N, int Dt, void *hi, void *l
         absmax_kernel<<<256, 256, 0, s>>>(Tm, (long long)N * Dt * Dt, S, nS, bits);
         fp16_scales_kernel<<<1, 1, 0, s>>>(bits, scales);
     }
-    const long long total = (long long)((N + KL - 1) / KL) * nct * 16 * KL;
+    const long long total = (long long)((N + KL - 1) / KL) * nct * 16 * KP;
     unsigned g = vt_blocks(total, 256);
     split_T_kernel<<<g < 4096 ? g : 4096, 256, 0, s>>>(Tm, N, Dt, nct * 16, total, (unsigned short *)hi,
                                                       (unsigned short *)lo, mode, scales);
