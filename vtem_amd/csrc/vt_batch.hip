@@ -407,6 +407,17 @@ __device__ __forceinline__ void skinny_load(const SkDesc &d, int m0, int c, int 
 #pragma unroll
     for (int t = 0; t < RT; ++t) {
         const int m = m0 + 16 * t + r;
+        if (d.sak == 1 && cin && m < d.M && kb + 3 < d.K) {
+            // row-major A: the lane's 4 consecutive k values as two 16-byte loads
+            typedef double d2a8 __attribute__((ext_vector_type(2), aligned(8)));
+            const double *p = d.A + (long long)m * d.sam + kb;
+            const d2a8 x = *(const d2a8 *)p, y = *(const d2a8 *)(p + 2);
+            a[t][0] = x[0];
+            a[t][1] = x[1];
+            a[t][2] = y[0];
+            a[t][3] = y[1];
+            continue;
+        }
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int k = kb + s;
