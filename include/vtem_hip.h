@@ -197,6 +197,11 @@ int vt_gram_split_T_size(int N, int Dt, long long *elems_host);
 int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int mode, void *stream);
 /* s2[j] = |S_j|^2 (once per neuron set). */
 int vt_row_sumsq(const double *S, long long lds, int N, int Ds, double *s2, void *stream);
+/* dst (cols x rows, row stride ldd) = src^T (rows x cols, row stride lds).  Makes the
+ * decoder's resident S^T (Ds x N), so the operator's second product Q = S^T Z streams rows
+ * as the first does.  No reference counterpart (the reference forms no such operator). */
+int vt_transpose_f64(const double *src, long long lds, int rows, int cols, double *dst,
+                     long long ldd, void *stream);
 /* Exact tr(A) = sum_j s2_j tr(T_j) in fp64; lam_out = {lam_rel tr(A)/n, tr(A)}.  Replaces
  * the trace needed for lambda = 1e-6 tr(G)/n without forming G or A. */
 int vt_kron_trace(const double *s2, const double *Tm, int N, int Dt, const int *n_dev,

@@ -67,6 +67,7 @@ SIGNATURES = {
     "vt_kron_trace": [P, P, I, I, P, D, P, P],
     "vt_kron_trace_batched": [P, P, P, P, P, I, P, I, D, P, I, P],
     "vt_row_sumsq": [P, L, I, I, P, P],
+    "vt_transpose_f64": [P, L, I, I, P, L, P],
     "vt_symmetrize": [P, I, L, I, P],
     "vt_potrf_f64": [P, I, I, P, L, P, P],
     "vt_potrf_f32": [P, I, I, P, L, P, P],

@@ -580,16 +580,26 @@ class BatchDecoder:
 
     def _operator(self, blocks, src, dst, Z, dbj, nzj, active=None):
         """dst_b = A_b src_b for every block (blocks: (S, Tm_ptr, N, Ds, Dt)) as two skinny
-        launches: Z_j = T_j (S src)_j (T fused in the epilogue), dst = S^T Z -- 4 N D
-        flops, S read twice, no Gram.  With `active` (per-block flag pointers) converged
+        launches: Z_j = T_j (S src)_j (T fused in the epilogue), dst = S^T Z (from the
+        decoder's resident S^T) -- 4 N D flops, S read twice, no Gram.  With `active` (per-block flag pointers) converged
         blocks skip both products."""
         act = active or [0] * len(blocks)
+        # forward descriptors largest-first (their order changes no result; the backward
+        # descriptors stay in block order, which the fused RNN update indexes)
+        order = sorted(range(len(blocks)), key=lambda b: -blocks[b][2] * blocks[b][3])
         fwd = self._skinny([(S.data_ptr(), src.data_ptr() + 8 * int(dbj[b]), Z.data_ptr() + 8 * int(nzj[b]),
                              Tp, act[b], S.stride(0), 1, Dt, Dt, Dt * Dt, N, Ds, Dt)
-                            for b, (S, Tp, N, Ds, Dt) in enumerate(blocks)])
-        bwd = self._skinny([(S.data_ptr(), Z.data_ptr() + 8 * int(nzj[b]), dst.data_ptr() + 8 * int(dbj[b]),
-                             0, act[b], 1, S.stride(0), Dt, Dt, 0, Ds, N, Dt)
-                            for b, (S, Tp, N, Ds, Dt) in enumerate(blocks)])
+                            for b in order for (S, Tp, N, Ds, Dt) in [blocks[b][:5]]])
+
+        rows = []
+        for b, blk in enumerate(blocks):
+            S, Tp, N, Ds, Dt = blk[:5]
+            St = blk[5] if len(blk) > 5 else None
+            # the resident S^T streams rows like the forward product; else S read by columns
+            A, sam, sak = (St.data_ptr(), St.stride(0), 1) if St is not None else (S.data_ptr(), 1, S.stride(0))
+            rows.append((A, Z.data_ptr() + 8 * int(nzj[b]), dst.data_ptr() + 8 * int(dbj[b]), 0, act[b],
+                         sam, sak, Dt, Dt, 0, Ds, N, Dt))
+        bwd = self._skinny(rows)
         return fwd, bwd
 
     def _spatial_inverse(self, Ss, N, Ds, c_ptrs, lam, s):
@@ -683,7 +693,8 @@ class BatchDecoder:
             lib.vt_kpcg_tbar(ptr(g.Tm), ptr(g.nbase_d), g.Dt, g.B, ptr(tws), ptr(c), ptr(TiT), s)
             keep += [c, TiT, tws]
             for b, d in enumerate(g.decs):
-                blocks.append((d.S, g.Tm.data_ptr() + 8 * int(g.nbase[b]) * E, int(g.N[b]), int(g.Ds[b]), g.Dt))
+                blocks.append((d.S, g.Tm.data_ptr() + 8 * int(g.nbase[b]) * E, int(g.N[b]), int(g.Ds[b]), g.Dt,
+                               d.St))
                 c_ptrs.append(c.data_ptr() + 8 * int(g.nbase[b]))
                 TiT_ptrs.append(TiT.data_ptr() + 8 * b * E)
             boff.append((b0, e0))
@@ -743,12 +754,14 @@ class BatchDecoder:
         return X, nrm, info, st, boff, keep
 
     def _joint_blocks(self, gs: List[_Prologue]):
-        """Blocks of several prologue groups in one joint numbering."""
+        """Blocks of several prologue groups in one joint numbering: (S, T_j pointer, N, Ds,
+        Dt, S^T) per block."""
         blocks, boff, b0, e0 = [], [], 0, 0
         for g in gs:
             E = g.Dt * g.Dt
             for b, d in enumerate(g.decs):
-                blocks.append((d.S, g.Tm.data_ptr() + 8 * int(g.nbase[b]) * E, int(g.N[b]), int(g.Ds[b]), g.Dt))
+                blocks.append((d.S, g.Tm.data_ptr() + 8 * int(g.nbase[b]) * E, int(g.N[b]), int(g.Ds[b]), g.Dt,
+                               d.St))
             boff.append((b0, e0))
             b0 += g.B
             e0 += g.Dtot

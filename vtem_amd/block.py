@@ -250,6 +250,10 @@ class Decoder:
         self._tn_ws = torch.empty(16 * geom.Ds * geom.Dt, dtype=F64, device=device)
         self.s2 = torch.empty(N, dtype=F64, device=device)
         lib.vt_row_sumsq(ptr(self.S), self.S.stride(0), N, geom.Ds, ptr(self.s2), stream_ptr())
+        # S^T (Ds x N), resident beside S: the operator's product Q = S^T Z reads it by rows
+        self.St = torch.empty((geom.Ds, N), dtype=F64, device=device)
+        lib.vt_transpose_f64(ptr(self.S), self.S.stride(0), N, geom.Ds, ptr(self.St), N,
+                             stream_ptr())
 
     # -- spikes -> intervals -> moments ------------------------------------------------
     def intervals(self, st: SpikeTrains, t0: float = 0.0, Tw: float = None):

@@ -494,6 +494,38 @@ VT_EXPORT int vt_row_sumsq(const double *S, long long lds, int N, int Ds, double
     return VT_OK;
 }
 
+// dst (cols x rows, row stride ldd) = src^T (src rows x cols, row stride lds): 32 x 32
+// tiles through LDS (padded rows), coalesced reads and writes.
+__global__ void __launch_bounds__(256)
+transpose_kernel(const double *__restrict__ src, long long lds, int rows, int cols,
+                 double *__restrict__ dst, long long ldd)
+{
+    __shared__ double tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) tile[k][tx] = src[(long long)r * lds + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) dst[(long long)c * ldd + r] = tile[tx][k];
+    }
+}
+
+VT_EXPORT int vt_transpose_f64(const double *src, long long lds, int rows, int cols, double *dst,
+                               long long ldd, void *stream)
+{
+    VT_CHECK_ARG(src && dst && rows >= 0 && cols >= 0 && lds >= cols && ldd >= rows);
+    if (rows == 0 || cols == 0) return VT_OK;
+    VT_CHECK_ARG(vt_blocks(rows, 32) <= 65535);
+    dim3 g(vt_blocks(cols, 32), vt_blocks(rows, 32));
+    transpose_kernel<<<g, 256, 0, (hipStream_t)stream>>>(src, lds, rows, cols, dst, ldd);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
 // Exact trace of the Kronecker Gram without forming it: tr(A) = sum_j s2_j tr(T_j) with
 // s2_j = |S_j|^2.  lam_out[0] = lam_rel * tr(A) / n (n from n_dev), lam_out[1] = tr(A).
 // Batched: problem b uses s2 + s2_off[b], Tm + T_off[b] (in neurons), N[b], n_dev[b].
