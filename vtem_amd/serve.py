@@ -5,7 +5,8 @@ block's receptive-field bank, IAF parameters and prepared decoder) arrive over t
 request either carries its video to encode (EncodeJob) or already-encoded spike trains.
 The server:
 
-* admits arrived requests into a ready queue;
+* admits arrived requests into a ready queue, RNN-decoded requests into a queue of their
+  own;
 * whenever one of its streams is free, forms a batch from the head of the queue, bounded
   by a request count and by an HBM budget (each request's in-flight footprint: spike
   buffer and encoder workspaces, intervals, moments, the solver's state and the output
@@ -13,7 +14,9 @@ The server:
 * launches the whole batch on that stream without a host synchronisation -- the batched
   encode (filter products and time-parallel IAF of every block) and then the batched
   decode -- so batches formed from requests that arrive while others are on the GPU run
-  concurrently on the device;
+  concurrently on the device; the RNN queue feeds one lane of its own, whose batch takes
+  every RNN request waiting (up to rnn_max_batch), so RNN requests share one sequence of
+  iteration launches;
 * retires a batch when its completion event fires: re-runs any block whose spike buffer
   overflowed (with an exact buffer) or whose low-precision solve missed tolerance,
   releases its memory and stamps each request's completion time.
@@ -113,10 +116,16 @@ class PrecisionPolicy:
 
 class Server:
     def __init__(self, hbm_budget: int = 64 << 30, max_batch: int = 32, streams: int = 2,
-                 precision: str = "bf16x3", verify: bool = True, adaptive: bool = True):
+                 precision: str = "bf16x3", verify: bool = True, adaptive: bool = True,
+                 rnn_lane: bool = True, rnn_max_batch: int = 64):
         self.hbm_budget = int(hbm_budget)
         self.max_batch = int(max_batch)
         self.n_streams = int(streams)
+        # RNN requests (hundreds of dependent operator applications each) get a lane of
+        # their own: one stream whose batch takes every RNN request waiting when it frees
+        # up, so their iterations share launches instead of each batch running its own loop
+        self.rnn_lane = bool(rnn_lane)
+        self.rnn_max_batch = int(rnn_max_batch)
         self.precision = precision
         self.verify = verify
         self.policy = PrecisionPolicy(precision) if adaptive else None
@@ -135,19 +144,25 @@ class Server:
         bd.instrument(instrument)
         be = BatchEncoder()
         overflows = 0
-        streams = [torch.cuda.Stream() for _ in range(self.n_streams)]
-        busy = [None] * self.n_streams          # (event, handle, reqs, bytes)
-        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        # lanes: (stream, queue, max batch); lane queues hold arrived, not yet launched requests
         ready: collections.deque = collections.deque()
+        ready_rnn: collections.deque = collections.deque()
+        lanes = [(torch.cuda.Stream(), ready, self.max_batch) for _ in range(self.n_streams)]
+        if self.rnn_lane:
+            lanes.append((torch.cuda.Stream(), ready_rnn, self.rnn_max_batch))
+        streams = [ln[0] for ln in lanes]
+        busy = [None] * len(lanes)              # (event, handle, reqs, bytes, overflow flags)
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         in_use = 0
         nbatch = 0
         fallbacks = 0
         torch.cuda.synchronize()
         t0 = clock()
-        while pending or ready or any(b is not None for b in busy):
+        while pending or ready or ready_rnn or any(b is not None for b in busy):
             now = clock() - t0
             while pending and pending[0].arrival <= now:
-                ready.append(pending.popleft())
+                r = pending.popleft()
+                (ready_rnn if self.rnn_lane and r.job.method == "rnn" else ready).append(r)
             # retire finished batches
             for k, b in enumerate(busy):
                 if b is not None and b[0].query():
@@ -171,19 +186,19 @@ class Server:
                         r.result, r.done = rr, tdone
                     in_use -= nbytes
                     busy[k] = None
-            # launch on a free stream
+            # launch on a free lane
             launched = False
-            for k in range(self.n_streams):
-                if busy[k] is not None or not ready:
+            for k, (_, queue, cap) in enumerate(lanes):
+                if busy[k] is not None or not queue:
                     continue
                 batch, nbytes = [], 0
-                while ready and len(batch) < self.max_batch and \
-                        in_use + nbytes + ready[0].footprint <= self.hbm_budget:
-                    r = ready.popleft()
+                while queue and len(batch) < cap and \
+                        in_use + nbytes + queue[0].footprint <= self.hbm_budget:
+                    r = queue.popleft()
                     batch.append(r)
                     nbytes += r.footprint
                 if not batch:
-                    break
+                    continue
                 tstart = clock() - t0
                 precs = [self.policy.choose(r.job) if self.policy else self.precision
                          for r in batch]
