@@ -130,9 +130,10 @@ class BatchEncoder:
         self.root_mode = int(root_mode)
 
     def coefficients(self, jobs: List[EncodeJob]) -> List[torch.Tensor]:
-        """C^T (N x Dt) per job: coefficient pt of neuron j at [j, pt]."""
+        """C^T (N x Dt) per job: coefficient pt of neuron j at [j, pt].  Temporal orders with
+        Dt > 32 are computed in column chunks of at most 32 (the skinny product's width)."""
         dvc = self.device
-        Wt, Ct, ws1, ws2 = [], [], [], []
+        Wt, Ct, rows1, rows2 = [], [], [], []
         for job in jobs:
             g = job.encoder.geom
             N = job.encoder.RF.shape[0]
@@ -140,32 +141,33 @@ class BatchEncoder:
                 raise ValueError("the batched encoder filters with an FP64 RF bank")
             Wt.append(torch.empty((g.P, g.Dt), dtype=F64, device=dvc))
             Ct.append(torch.empty((N, g.Dt), dtype=F64, device=dvc))
-            ws1.append(skinny_workspace(g.P, g.frames, g.Dt))
-            ws2.append(skinny_workspace(N, g.P, g.Dt))
-        w1 = torch.empty(max(1, sum(ws1)), dtype=F64, device=dvc)
-        w2 = torch.empty(max(1, sum(ws2)), dtype=F64, device=dvc)
-        d1 = np.zeros(len(jobs), SK_DTYPE)
-        d2 = np.zeros(len(jobs), SK_DTYPE)
-        o1 = o2 = 0
-        for i, job in enumerate(jobs):
+        for job, W, C in zip(jobs, Wt, Ct):
             enc, V = job.encoder, job.frames
             g = enc.geom
             th = _theta_scaled(enc)
             RF = enc.RF
             N = RF.shape[0]
-            # W^T (P x Dt) = V^T Theta_f dx dy / F: A(m=pixel, k=frame) = V[k, m]
-            d1[i] = (ptr(V), ptr(th), ptr(Wt[i]), 0, ptr(w1) + 8 * o1, 0, 1, V.stride(0), g.Dt,
-                     g.Dt, 0, g.P, g.frames, g.Dt, 0)
-            # C^T (N x Dt) = RF W^T
-            d2[i] = (ptr(RF), ptr(Wt[i]), ptr(Ct[i]), 0, ptr(w2) + 8 * o2, 0, RF.stride(0), 1,
-                     g.Dt, g.Dt, 0, N, g.P, g.Dt, 0)
-            o1 += ws1[i]
-            o2 += ws2[i]
-        if jobs:
-            geo = [j.encoder.geom for j in jobs]
-            maxDt = max(g.Dt for g in geo)
-            _skinny(d1, max(g.P for g in geo), max(g.frames for g in geo), maxDt, dvc)
-            _skinny(d2, max(j.encoder.RF.shape[0] for j in jobs), max(g.P for g in geo), maxDt, dvc)
+            for c0 in range(0, g.Dt, 32):
+                nc = min(32, g.Dt - c0)
+                # W^T (P x Dt) = V^T Theta_f dx dy / F: A(m=pixel, k=frame) = V[k, m]
+                rows1.append((ptr(V), ptr(th) + 8 * c0, ptr(W) + 8 * c0, 1, V.stride(0), g.Dt, g.Dt,
+                              g.P, g.frames, nc))
+                # C^T (N x Dt) = RF W^T
+                rows2.append((ptr(RF), ptr(W) + 8 * c0, ptr(C) + 8 * c0, RF.stride(0), 1, g.Dt, g.Dt,
+                              N, g.P, nc))
+        for rows in (rows1, rows2):
+            if not rows:
+                continue
+            wsz = [skinny_workspace(r[7], r[8], r[9]) for r in rows]
+            ws = torch.empty(max(1, sum(wsz)), dtype=F64, device=dvc)
+            d = np.zeros(len(rows), SK_DTYPE)
+            off = 0
+            for i, (A, B, Cp, sam, sak, ldb, ldc, M, K, nc) in enumerate(rows):
+                d[i] = (A, B, Cp, 0, ptr(ws) + 8 * off if wsz[i] else 0, 0, sam, sak, ldb, ldc, 0,
+                        M, K, nc, 0)
+                off += wsz[i]
+            _skinny(d, max(r[7] for r in rows), max(r[8] for r in rows), max(r[9] for r in rows), dvc)
+            self._ws = ws                     # alive until the next call (stream-ordered reuse)
         return Ct
 
     def encode(self, jobs: List[EncodeJob], overflow: torch.Tensor = None,

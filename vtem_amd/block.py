@@ -18,6 +18,7 @@ Decode  (vtem/vtdm.py:630-652 -> dirichlet.compute_q/compute_G, linalg.solve_eq_
 
 from __future__ import annotations
 
+import copy
 import ctypes
 import dataclasses
 import math
@@ -254,6 +255,17 @@ class Decoder:
         self.St = torch.empty((geom.Ds, N), dtype=F64, device=device)
         lib.vt_transpose_f64(ptr(self.S), self.S.stride(0), N, geom.Ds, ptr(self.St), N,
                              stream_ptr())
+
+    def with_iaf(self, kappa, delta, bias, gain: float = 1.0) -> "Decoder":
+        """The same decoder (shared S, S^T and tables) for a population with other IAF
+        parameters.  gain c: the neurons integrated c u + b, which decodes as u with
+        kappa / c and b / c (q_k = kappa delta - b (t_k+1 - t_k) scales by 1 / c)."""
+        d = copy.copy(self)
+        d.kappa = dev(np.asarray(kappa, np.float64) / gain, F64, self.device)
+        d.delta = dev(delta, F64, self.device)
+        d.bias = dev(np.asarray(bias, np.float64) / gain, F64, self.device)
+        d._tn_ws = torch.empty_like(self._tn_ws)
+        return d
 
     # -- spikes -> intervals -> moments ------------------------------------------------
     def intervals(self, st: SpikeTrains, t0: float = 0.0, Tw: float = None):
