@@ -84,6 +84,8 @@ def parse():
     p.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
     p.add_argument("--no-precision", action="store_true", help="skip the 128x128 precision report")
     p.add_argument("--precision-only", action="store_true", help="only the 128x128 precision report")
+    p.add_argument("--no-c3", action="store_true", help="skip the 512x512 stitched batching form")
+    p.add_argument("--c3-only", action="store_true", help="only the 512x512 stitched batching form")
     p.add_argument("--roofline-only", action="store_true",
                    help="only time the dominant kernel at the workload's shapes (for rocprofv3 runs)")
     return p.parse_args()
@@ -264,6 +266,64 @@ def precision_report(seed=0):
             "intervals": n, "forms": forms}
 
 
+def stitched_c3(seed=0, precision="kpcg", max_batch=64):
+    """Batching form: a 512 x 512 px, 4 s video (trigonometric polynomial with the frame's
+    and the duration's periods, orders 64 / 32, i.e. the blocks' bandwidth) in 64 px tiles
+    with 16 px overlap and 1 s windows with 0.25 s overlap: 11 x 11 tiles x 5 windows =
+    605 blocks of Lx = Ly = 8, Lt = 8, each tile with 2048 neurons of the config-1 laws, IAF
+    b = 2.5, delta ~ N(0.04, 0.004^2), and a contrast gain log-uniform in [0.25, 4].  Timed
+    (video, RF bank and decoders resident): the batched encode of all tiles, the batched
+    decode of all blocks and the raised-cosine blending.  SNR over the full video and its
+    interior (16 px border and the first / last 0.25 s excluded); the same codec on a video
+    with the blocks' own periods (64 px, 1 s), untimed, checks the machinery."""
+    import torch
+    from vtem_amd import synthetic as syn
+    from vtem_amd.stitch import TiledCodec
+    vid = syn.make_video(512, 512, 4.0, 100.0, 64, 64, 32, seed=seed)
+    rng = np.random.default_rng(seed + 3)
+    codec = TiledCodec(512, 512, 4.0, 32, syn.bank_dyadic(64.0, 64.0))
+    n = len(codec.plan.tiles)
+    N = codec.encoder.RF.shape[0]
+    iaf = [syn.iaf_params(N, 2.5, 1.0, 0.04, 0.004, rng) for _ in range(n)]
+    contrast = np.exp(rng.uniform(math.log(0.25), math.log(4.0), n))
+    decs = codec.decoders(iaf, contrast)
+
+    def run(frames):
+        V = torch.tensor(frames, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        spikes = codec.encode(V, iaf, contrast)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        out = codec.decode(spikes, decs, precision=precision, max_batch=max_batch)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        u = out.cpu().numpy()
+        nsp = int(sum(int(s_.counts.sum().item()) for s_ in spikes))
+        del spikes, out, V
+        torch.cuda.empty_cache()
+        return u, t1 - t0, t2 - t1, nsp
+
+    run(vid.frames[:, :, :])                              # warm: code objects, allocator
+    u, te, td, nsp = run(vid.frames)
+    x = vid.frames
+    inner = (slice(25, -25), slice(16, -16), slice(16, -16))
+    res = {"workload": "512x512 px, 4 s; 64 px tiles / 16 px overlap x 1 s windows / 0.25 s overlap; "
+                       "Lx=Ly=8, Lt=8, 2048 neurons per tile",
+           "blocks": len(codec.plan.blocks), "tiles": n, "windows": len(codec.plan.f0),
+           "precision": precision, "spikes": nsp,
+           "encode_ms": round(te * 1e3, 2), "decode_blend_ms": round(td * 1e3, 2),
+           "frames_per_s_512x512": round(400 / (te + td), 2),
+           "block_frames_per_s": round(len(codec.plan.blocks) * codec.plan.window / (te + td), 1),
+           "snr_db": round(snr_db(x, u), 3), "snr_interior_db": round(snr_db(x[inner], u[inner]), 3)}
+    c = syn.video_coefficients(8, 8, 8, np.random.default_rng(seed + 5))
+    per = syn.evaluate_video(c, 512, 512, np.arange(400) / 100.0, Sx=64.0, Sy=64.0, T=1.0)
+    per /= np.abs(per).max()
+    up, *_ = run(per)
+    res["snr_db_block_periodic_video"] = round(snr_db(per, up), 3)
+    return res
+
+
 def copy_calibration(nbytes=512 << 20, reps=5):
     """A plain device copy of known size (the scale reference for HBM byte counters)."""
     import torch
@@ -330,6 +390,9 @@ def main():
     require_gpu()
     if args.precision_only:
         print(json.dumps({"precision_c2": precision_report(args.seed)}), flush=True)
+        return
+    if args.c3_only:
+        print(json.dumps({"stitched_c3": stitched_c3(args.seed)}), flush=True)
         return
     items = prepare(args.requests, args.seed)
     if args.roofline_only:
@@ -459,6 +522,8 @@ def main():
         line["latency_sweep"] = sweep
     if not args.no_precision:
         line["precision_c2"] = precision_report(args.seed)
+    if not args.no_c3:
+        line["stitched_c3"] = stitched_c3(args.seed)
     if not args.no_cpu:
         fps, used, t = cpu_baseline(items, args.cpu_budget)
         line["cpu_baseline"] = {"value": round(fps, 3), "unit": "frames/s", "cores": os.cpu_count(),

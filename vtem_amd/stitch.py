@@ -10,9 +10,9 @@ reference's tiling of decode_video and windowing of VTDM (vtem/vtdm.py:179-309,
 * All tiles share one receptive-field bank, laid out on the tile's own grid, so one
   Encoder (and one spatial-coefficient matrix S) serves every tile; the IAF parameters
   (thresholds, initial voltages) are drawn per tile.
-* A per-tile contrast multiplier scales what the tile's neurons see (their input gain),
-  so spike counts are uneven across blocks; each decoded block is divided by it before
-  blending.
+* A per-tile contrast multiplier c is the tile's input gain: its neurons integrate
+  (c v + b) / kappa, i.e. v with kappa / c and b / c.  The encoder and the tile's decoders
+  use those scaled parameters, so each block decodes the video itself.
 * The encoder integrates the whole duration per tile (its input is evaluated from the
   tile's temporal coefficients over the video's period); each window's block selects its
   intervals from the tile's spike trains (BlockJob.t0).
@@ -154,10 +154,16 @@ class TiledCodec:
             for k in range(g0, min(len(tiles), g0 + group)):
                 iy, ix = tiles[k]
                 y0, x0 = p.y0[iy], p.x0[ix]
-                tv = (V[:, y0:y0 + T, x0:x0 + T] * float(contrast[k])).reshape(V.shape[0], T * T)
-                m = 2.0 * max(1.0, float(contrast[k]))
-                jobs.append(EncodeJob(self.encoder, tv.contiguous(), IAFDevice.from_params(iaf[k], self.device),
-                                      max_spikes_rate(self.enc_geom, iaf[k], margin=m)))
+                # the tile's frames (a copy: the product's operand has one pixel stride)
+                tv = V[:, y0:y0 + T, x0:x0 + T].reshape(V.shape[0], T * T).contiguous()
+                # input gain c: the neuron integrates (c v + b) / kappa = (v + b/c) / (kappa/c)
+                c = float(contrast[k])
+                pk = dataclasses.replace(iaf[k], kappa=iaf[k].kappa / c, bias=iaf[k].bias / c)
+                # rows for the bias-driven rate, widened for the input's swing c |v| (|v| of
+                # order 1 for unit-peak videos); an overflow is redone below
+                m = 1.5 * (1.0 + 0.6 * c)
+                jobs.append(EncodeJob(self.encoder, tv, IAFDevice.from_params(pk, self.device),
+                                      max_spikes_rate(self.enc_geom, pk, margin=m)))
                 margins.append(m)
             flags = torch.zeros(len(jobs), dtype=torch.int32, device=self.device)
             sts = be.encode(jobs, flags, carry=False)
@@ -166,7 +172,8 @@ class TiledCodec:
                 m = margins[i]
                 while True:
                     m *= 2.0
-                    j2 = dataclasses.replace(jobs[i], max_spike=max_spikes_rate(self.enc_geom, iaf[g0 + i], margin=m))
+                    j2 = dataclasses.replace(jobs[i], max_spike=max_spikes_rate(self.enc_geom, iaf[g0 + i],
+                                                                                margin=m))
                     fl = torch.zeros(1, dtype=torch.int32, device=self.device)
                     st = be.encode([j2], fl, carry=False)[0]
                     if not int(fl.item()):
@@ -176,8 +183,8 @@ class TiledCodec:
         return out
 
     def decoders(self, iaf: Sequence[IAFParams], contrast: Sequence[float]) -> List[Decoder]:
-        """One decoder per tile: the shared S (and S^T), the tile's IAF parameters; the
-        input gain c enters as kappa / c (the neuron integrates c u + b with kappa)."""
+        """One decoder per tile: the shared S (and S^T), the tile's IAF parameters with the
+        input gain c folded in (kappa / c, b / c), as the encoder used them."""
         base = Decoder(self.dec_geom, self.S, iaf[0].kappa, iaf[0].delta, iaf[0].bias,
                        device=self.device)
         out = []
