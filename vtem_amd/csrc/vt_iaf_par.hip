@@ -135,11 +135,12 @@ __device__ __forceinline__ void iafp_walk_dev(const IafParDesc &d, const Src &sr
         const double F = pre.x + P;
         const double t0 = tc0 + (double)i * dt;
         if (y2 >= 0 || y1 <= 0) {
-            // rising / normal step (ensemble_encode.py:493-514)
-            const double a = (y2 - y1) / dt;
+            // rising / normal step (ensemble_encode.py:493-514); the slope (a division)
+            // only where a level is crossed
             while (k < k1) {
                 const double L = (double)(k + 1) * delta - v0;
                 if (!(F + area >= L)) break;
+                const double a = (y2 - y1) / dt;
                 const double dv = L - F;
                 double remain;
                 if (mode == 0)
