@@ -99,3 +99,37 @@ def test_iaf_par_host_queries():
     assert iaf_par_workspace_bytes(100, 65) == 3 * 100 * 16
     assert iaf_par_workspace_bytes(100, 66) == 4 * 100 * 16
     assert iaf_par_workspace_bytes(2361, 20001) == (313 + 2) * 2361 * 16
+
+
+def test_batch_formation_and_rnn_lane_routing():
+    """Batches take requests in arrival order up to the request cap and the free HBM;
+    RNN-decoded requests are routed to their own lane."""
+    import collections
+    from vtem_amd.serve import Request, form_batch, is_rnn
+
+    class _J:
+        def __init__(self, method):
+            self.method = method
+
+    reqs = [Request(i, 0.001 * i, _J("rnn" if i % 4 == 3 else "tikhonov"), 100, 10 * (i + 1))
+            for i in range(12)]
+    assert [is_rnn(r) for r in reqs[:4]] == [False, False, False, True]
+    q = collections.deque(reqs)
+    b, nb = form_batch(q, 3, 10 ** 9)
+    assert [r.rid for r in b] == [0, 1, 2] and nb == 60 and q[0].rid == 3
+    b, nb = form_batch(q, 10, 95)                  # footprints 40 + 50 fit, 60 does not
+    assert [r.rid for r in b] == [3, 4] and nb == 90
+    b, nb = form_batch(q, 10, 10)                  # nothing fits
+    assert b == [] and nb == 0 and len(q) == 7
+
+
+def test_spike_buffer_rate_rows():
+    """Spike rows from the fastest neuron's bias-driven rate b / (kappa delta) over the
+    window, times the margin, plus 16."""
+    from vtem_amd.basis import Geometry
+    from vtem_amd.batch_encode import max_spikes_rate
+    from vtem_amd.synthetic import IAFParams
+    g = Geometry(32, 32, 4, 4, 4, 2.0)
+    p = IAFParams(np.ones(3), np.full(3, 2.5), np.array([0.05, 0.025, 0.04]), np.zeros(3), np.zeros(3))
+    assert max_spikes_rate(g, p, margin=2.0) == int(np.ceil(2.0 * (2.5 / 0.025) * 2.0)) + 16
+    assert max_spikes_rate(g, p, margin=1.0) == 200 + 16

@@ -162,7 +162,7 @@ class Server:
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 r = pending.popleft()
-                (ready_rnn if self.rnn_lane and r.job.method == "rnn" else ready).append(r)
+                (ready_rnn if self.rnn_lane and is_rnn(r) else ready).append(r)
             # retire finished batches
             for k, b in enumerate(busy):
                 if b is not None and b[0].query():
@@ -191,12 +191,7 @@ class Server:
             for k, (_, queue, cap) in enumerate(lanes):
                 if busy[k] is not None or not queue:
                     continue
-                batch, nbytes = [], 0
-                while queue and len(batch) < cap and \
-                        in_use + nbytes + queue[0].footprint <= self.hbm_budget:
-                    r = queue.popleft()
-                    batch.append(r)
-                    nbytes += r.footprint
+                batch, nbytes = form_batch(queue, cap, self.hbm_budget - in_use)
                 if not batch:
                     continue
                 tstart = clock() - t0
@@ -297,6 +292,22 @@ class Server:
                "mean_ms": float(lat.mean())}
         out.update(extra)
         return out
+
+
+def is_rnn(r: Request) -> bool:
+    """Requests decoded by the gradient-flow RNN go to the server's RNN lane."""
+    return r.job.method == "rnn"
+
+
+def form_batch(queue: collections.deque, cap: int, free_bytes: int):
+    """Take requests from the head of `queue` (arrival order) while the batch holds fewer
+    than `cap` and their footprints fit in `free_bytes`.  Returns (batch, bytes)."""
+    batch, nbytes = [], 0
+    while queue and len(batch) < cap and nbytes + queue[0].footprint <= free_bytes:
+        r = queue.popleft()
+        batch.append(r)
+        nbytes += r.footprint
+    return batch, nbytes
 
 
 def poisson_arrivals(n: int, rate: float, rng: np.random.Generator) -> np.ndarray:
