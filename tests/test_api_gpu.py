@@ -62,6 +62,9 @@ def test_vrf_gabor_api(torch_mod, oracle):
     psi = g.compute_dirich_space(Ds, 5, 5, 32, 32, 32.0, 32.0, W, W).cpu().numpy()
     psi_o = oracle.compute_psi(Ds_o, 32, 32, 32.0, 32.0, W / 5, W / 5, mask="rect")
     assert _rel(psi, psi_o) < 1e-11
+    # the reference's FFT form of psi (vrf.py:437-499) gives the same functions
+    psi_f = g.compute_dirich_space_fft(Ds, 5, 5, 32, 32, 32.0, 32.0, W, W).cpu().numpy()
+    assert _rel(psi_f, psi_o) < 1e-11
 
 
 def test_vrf_cs_api(torch_mod, oracle):
@@ -138,6 +141,39 @@ def test_vtem_iaf_chunks(torch_mod, oracle):
     np.testing.assert_array_equal(enc.spike_count, co)
     for a, b in zip(enc.spikes, lo):
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["gabor", "cs"])
+def test_vtem_default_bank_drivers(torch_mod, kind):
+    """VTEM_Gabor_IAF / VTEM_CS_IAF (vtem/vtem.py:209-427) build the reference's default
+    bank on the video's grid and encode with it: the same spikes as VTEM_IAF called with
+    that bank and the same IAF state."""
+    _banner(f"VTEM default-bank driver {kind}")
+    from vtem_amd import vrf as vrf_mod
+    from vtem_amd.ensemble_encode import IAF_encode
+    from vtem_amd.vtem import VTEM_CS_IAF, VTEM_Gabor_IAF, VTEM_IAF
+    vid, _ = _c0()
+    frames = vid.frames[:40]
+    drv = VTEM_Gabor_IAF if kind == "gabor" else VTEM_CS_IAF
+    cls = vrf_mod.vrf_gabor if kind == "gabor" else vrf_mod.vrf_cs
+    rf = cls((32, 32), dx=1.0 / 16, dy=1.0 / 16)
+    rf.load_parameters()
+    N = rf.num_neurons
+    rng = np.random.default_rng(8)
+    v0 = rng.uniform(0, 1, N) * 0.03
+
+    def iaf():
+        e = IAF_encode(N, 0.01)
+        e.load_parameters(np.ones(N), np.full(N, 0.8), np.full(N, 0.03), h_v0=v0)
+        return e
+
+    a = drv(frames, None, 2 * math.pi * 4, npinput=True, h5input=False, iaf=iaf())
+    b = VTEM_IAF(frames, None, 2 * math.pi * 4, rf, dx=1.0 / 16, dy=1.0 / 16, npinput=True,
+                 h5input=False, iaf=iaf())
+    assert N > 0 and a.spike_count.sum() > 0
+    np.testing.assert_array_equal(a.spike_count, b.spike_count)
+    for x, y in zip(a.spikes, b.spikes):
+        np.testing.assert_array_equal(x, y)
 
 
 def _spike_space_inputs(oracle, sub):
