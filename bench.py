@@ -302,16 +302,17 @@ def stitched_c3(seed=0, precision="kpcg", max_batch=64):
         nsp = int(sum(int(s_.counts.sum().item()) for s_ in spikes))
         del spikes, out, V
         torch.cuda.empty_cache()
-        return u, t1 - t0, t2 - t1, nsp
+        return u, t1 - t0, t2 - t1, nsp, dict(codec.last_stats)
 
     run(vid.frames[:, :, :])                              # warm: code objects, allocator
-    u, te, td, nsp = run(vid.frames)
+    u, te, td, nsp, dst = run(vid.frames)
     x = vid.frames
     inner = (slice(25, -25), slice(16, -16), slice(16, -16))
     res = {"workload": "512x512 px, 4 s; 64 px tiles / 16 px overlap x 1 s windows / 0.25 s overlap; "
                        "Lx=Ly=8, Lt=8, 2048 neurons per tile",
            "blocks": len(codec.plan.blocks), "tiles": n, "windows": len(codec.plan.f0),
-           "precision": precision, "spikes": nsp,
+           "precision": precision, "spikes": nsp, "kpcg_retries": dst["kpcg_retries"],
+           "fp64_fallbacks": dst["fallbacks"],
            "encode_ms": round(te * 1e3, 2), "decode_blend_ms": round(td * 1e3, 2),
            "frames_per_s_512x512": round(400 / (te + td), 2),
            "block_frames_per_s": round(len(codec.plan.blocks) * codec.plan.window / (te + td), 1),

@@ -123,14 +123,19 @@ int vt_iaf_par_workspace_size(int N, int size, long long *bytes_host);
  *            const void *input; long long input_ld;          (unused here)
  *            double *spikes; long long spike_ld; double *v0, *time_count;
  *            const double *kappa, *bias, *delta; int *spike_count; void *ws;
- *            int *overflow; double dt; int N, size, Dt, max_spike, root_mode, in_dtype; }
- * (160 bytes), each one an independent vt_iaf_encode_trig_par call with coefficient pt of
+ *            int *overflow; double dt; int N, size, Dt, max_spike, root_mode, in_dtype;
+ *            double *ybuf; long long y_ld; }
+ * (176 bytes), each one an independent vt_iaf_encode_trig_par call with coefficient pt of
  * neuron n at coef[pt*coef_ld + n*coef_ns]; *overflow (if not null) is set to 1 when a
  * neuron fires more than max_spike times.  maxN, maxSize and maxDt bound the descriptors'
- * N, size and Dt.  Encodes many blocks (served requests, stitching tiles)
- * in one launch sequence; the reference encodes one video at a time (vtem/vtem.py:165-200). */
+ * N, size and Dt.  flags bit 0: every descriptor's ybuf (size x N, ld y_ld >= N) receives
+ * the scaled input y = (v + b)/kappa at every sample in the first pass, and the crossing
+ * walk reads it back instead of evaluating the input again (same results; 8 bytes of
+ * device memory per neuron and sample); with flags 0 ybuf is unused.  Encodes many blocks
+ * (served requests, stitching tiles) in one launch sequence; the reference encodes one
+ * video at a time (vtem/vtem.py:165-200). */
 int vt_iaf_encode_trig_batched(const void *descs, int nd, int maxN, int maxSize, int maxDt,
-                               void *stream);
+                               int flags, void *stream);
 /* Host query: the descriptor size above, for layout checks. */
 int vt_iaf_par_desc_size(int *bytes_host);
 /* Random-threshold IAF: spikes holds N(0,1) draws on entry; delta_value carried in/out.

@@ -241,3 +241,34 @@ def test_batched_encode(torch_mod):
     be.encode([j for j, _ in small], flags)
     torch.cuda.synchronize()
     assert flags.cpu().numpy().tolist() == [0, 1, 0]
+
+
+def test_batched_encode_stored_input(torch_mod):
+    """The crossing walk reading the input stored by the first pass gives bit-identical
+    counts, spike times and carried state to the walk that evaluates the trigonometric input
+    again (Dt 5, 9, 17 and 65 in one launch; a non-power-of-two kappa, so the stored value
+    is a rounded quotient)."""
+    torch = torch_mod
+    from vtem_amd import synthetic as syn
+    from vtem_amd.batch_encode import BatchEncoder, EncodeJob, IAFDevice
+    _banner("batched encode, stored input")
+    specs = [(32, 5, 4, 1.0, syn.bank_config0(), 3), (64, 8, 8, 2.0, syn.bank_dyadic(64., 64.), 4),
+             (32, 4, 2, 0.5, syn.bank_config0(), 5), (32, 5, 32, 1.0, syn.bank_config0(), 6)]
+    jobs = [j for j, _ in _serve_jobs(torch, specs)]
+    jobs[1].iaf.kappa.mul_(0.7)
+    jobs[1].iaf.bias.mul_(0.7)
+    be = BatchEncoder()
+    a = be.encode(jobs, carry=False, store_input=True)
+    sa = [(v.clone(), t.clone()) for v, t in be.last_states]
+    b = be.encode(jobs, carry=False, store_input=False)
+    sb = be.last_states
+    torch.cuda.synchronize()
+    for x, y, (v1, t1), (v2, t2) in zip(a, b, sa, sb):
+        c = x.counts.cpu().numpy()
+        assert c.sum() > 0
+        np.testing.assert_array_equal(c, y.counts.cpu().numpy())
+        g, o = x.spikes.cpu().numpy(), y.spikes.cpu().numpy()
+        for j in range(c.size):
+            np.testing.assert_array_equal(g[:c[j], j], o[:c[j], j])
+        torch.testing.assert_close(v1, v2, rtol=0, atol=0)
+        torch.testing.assert_close(t1, t2, rtol=0, atol=0)

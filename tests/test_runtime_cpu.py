@@ -93,7 +93,7 @@ def test_iaf_par_host_queries():
     from vtem_amd.block import iaf_par_workspace_bytes
     b = ctypes.c_int(0)
     lib.vt_iaf_par_desc_size(ctypes.addressof(b))
-    assert b.value == IAFP_DTYPE.itemsize == 160
+    assert b.value == IAFP_DTYPE.itemsize == 176
     assert iaf_par_workspace_bytes(100, 0) == 2 * 100 * 16
     assert iaf_par_workspace_bytes(100, 1) == 2 * 100 * 16
     assert iaf_par_workspace_bytes(100, 65) == 3 * 100 * 16

@@ -32,7 +32,7 @@ SIGNATURES = {
     "vt_iaf_encode_trig": [P, L, P, I, I, I, P, L, I, P, P, P, P, P, P, D, I, P],
     "vt_iaf_encode_trig_par": [P, L, P, I, I, I, P, L, I, P, P, P, P, P, P, D, I, P, L, P],
     "vt_iaf_encode_par": [P, I, L, I, I, P, L, I, P, P, P, P, P, P, D, I, P, L, P],
-    "vt_iaf_encode_trig_batched": [P, I, I, I, I, P],
+    "vt_iaf_encode_trig_batched": [P, I, I, I, I, I, P],
     "vt_iaf_par_workspace_size": [I, I, P],
     "vt_iaf_par_desc_size": [P],
     "vt_iaf_encode_rt": [P, L, I, I, P, L, I, P, P, P, P, P, P, D, P, P, I, P],

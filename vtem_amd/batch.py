@@ -164,11 +164,12 @@ class BatchDecoder:
     exact Kronecker operator, preconditioned by (S^T C S + lam I) kron Tbar).  Blocks of
     the non-FP64 forms that fail the guard (relative residual of the FP64 system above
     `tol`, or a factorisation that lost positive definiteness) are redone together as
-    one batched FP64 group."""
+    one batched FP64 group; kpcg blocks first get one more CG solve with retry_iters
+    iterations."""
 
     def __init__(self, device="cuda", precision: str = "bf16x3", lam_rel: float = 1e-6,
                  refine_iters: int = 4, tol: float = 1e-10, verify: bool = True,
-                 cg_iters: int = 32, cg_stop: float = 1e-12):
+                 cg_iters: int = 32, cg_stop: float = 1e-12, retry_iters: Optional[int] = None):
         if precision not in PRECISIONS:
             raise ValueError(f"batched precision must be one of {PRECISIONS}")
         self.device = device
@@ -179,6 +180,9 @@ class BatchDecoder:
         self.verify = verify
         self.cg_iters = cg_iters
         self.cg_stop = cg_stop
+        # kpcg blocks that fail the guard are solved again with this many iterations
+        # before the FP64 redo (CG that ran out of iterations on a harder system)
+        self.retry_iters = 4 * cg_iters if retry_iters is None else retry_iters
         self.last_stats: Dict[str, Any] = {}
         self.timers: Optional[Dict[str, list]] = None     # stage -> [(ev0, ev1, work)]
 
@@ -300,23 +304,49 @@ class BatchDecoder:
                 h.keep.append((g.keep, keep, zero, info))
             h.keep.append(skeep)
 
+    def _guard(self, checks):
+        """(job index, precision) of every non-FP64 result that failed the guard (factor
+        lost positive definiteness, or FP64 relative residual above tol), and the number of
+        FP64 factorisations that failed."""
+        redo, failed_fp64 = [], 0
+        for idx, (nrm, info), pr in checks:
+            hn = nrm.cpu().numpy()
+            inf = info.cpu().numpy()
+            for k, i in enumerate(idx):
+                if pr == "fp64":
+                    failed_fp64 += int(inf[k] != 0)
+                    continue
+                ok = inf[k] == 0 and np.isfinite(hn[k]).all() and \
+                    math.sqrt(hn[k, -1]) <= self.tol * math.sqrt(hn[k, 0])
+                if not ok:
+                    redo.append((i, pr))
+        return redo, failed_fp64
+
     def finalize(self, h: "BatchHandle") -> List[DecodeResult]:
-        """Guard: blocks whose low-precision factor failed or whose refinement did not
-        reach `tol` relative residual are redone as one batched FP64 group (flagged)."""
-        redo = []
-        failed_fp64 = 0
+        """Guard: kpcg blocks that did not reach `tol` are solved again with retry_iters CG
+        iterations; blocks still failing, and low-precision blocks whose factor failed or
+        whose refinement did not reach `tol`, are redone as one batched FP64 group
+        (flagged)."""
+        redo, failed_fp64, retried = [], 0, []
         if self.verify:
-            for idx, (nrm, info), pr in h.checks:
-                hn = nrm.cpu().numpy()
-                inf = info.cpu().numpy()
-                for k, i in enumerate(idx):
-                    if pr == "fp64":
-                        failed_fp64 += int(inf[k] != 0)
-                        continue
-                    ok = inf[k] == 0 and np.isfinite(hn[k]).all() and \
-                        math.sqrt(hn[k, -1]) <= self.tol * math.sqrt(hn[k, 0])
-                    if not ok:
-                        redo.append(i)
+            redo, failed_fp64 = self._guard(h.checks)
+            kp = [i for i, pr in redo if pr == "kpcg"]
+            if kp and self.retry_iters > self.cg_iters:
+                base = self.cg_iters
+                self.cg_iters = self.retry_iters
+                try:
+                    h1 = self.launch([h.jobs[i] for i in kp], ["kpcg"] * len(kp))
+                finally:
+                    self.cg_iters = base
+                bad, _ = self._guard(h1.checks)
+                bad = {j for j, _ in bad}
+                for j, (i, rr) in enumerate(zip(kp, h1.results)):
+                    if j not in bad:
+                        h.results[i] = rr
+                h.keep.append(h1.keep)
+                retried = kp
+                redo = [(i, pr) for i, pr in redo if pr != "kpcg"] + [(kp[j], "kpcg") for j in sorted(bad)]
+        redo = [i for i, _ in redo]
         if redo:
             h2 = self.launch([h.jobs[i] for i in redo], ["fp64"] * len(redo))
             for i, rr in zip(redo, h2.results):
@@ -324,7 +354,8 @@ class BatchDecoder:
                 h.results[i] = rr
             h.keep.append(h2.keep)
         self.last_stats = {"groups": len(h.checks), "fallbacks": len(redo),
-                           "fp64_not_pd": failed_fp64, "fallback_index": redo}
+                           "fp64_not_pd": failed_fp64, "fallback_index": redo,
+                           "kpcg_retries": len(retried)}
         torch.cuda.current_stream().synchronize()
         h.keep = []
         return h.results

@@ -31,8 +31,8 @@ IAFP_DTYPE = np.dtype([
     ("time_count", "<u8"), ("kappa", "<u8"), ("bias", "<u8"), ("delta", "<u8"),
     ("spike_count", "<u8"), ("ws", "<u8"), ("overflow", "<u8"), ("dt", "<f8"), ("N", "<i4"),
     ("size", "<i4"), ("Dt", "<i4"), ("max_spike", "<i4"), ("root_mode", "<i4"),
-    ("in_dtype", "<i4")])
-assert IAFP_DTYPE.itemsize == 160
+    ("in_dtype", "<i4"), ("ybuf", "<u8"), ("y_ld", "<i8")])
+assert IAFP_DTYPE.itemsize == 176
 
 
 @dataclasses.dataclass
@@ -95,12 +95,12 @@ def max_spikes_rate(geom, iaf_host: IAFParams, margin: float = 2.0) -> int:
 def encode_footprint(job: EncodeJob) -> int:
     """In-flight HBM of one encode (the RF bank and the frames are resident before
     admission): spike buffer and counts, W^T and C^T, the split-K workspaces, the IAF
-    workspace and the carried state."""
+    workspace, the stored input y (N x samples) and the carried state."""
     g = job.encoder.geom
     N = job.encoder.RF.shape[0]
     return int(8 * job.max_spike * N + 4 * N + 8 * (g.P + N) * g.Dt
                + 8 * (skinny_workspace(g.P, g.frames, g.Dt) + skinny_workspace(N, g.P, g.Dt))
-               + iaf_par_workspace_bytes(N, g.iaf_samples) + 16 * N)
+               + iaf_par_workspace_bytes(N, g.iaf_samples) + 8 * N * g.iaf_samples + 16 * N)
 
 
 def _theta_scaled(enc: Encoder) -> torch.Tensor:
@@ -171,12 +171,14 @@ class BatchEncoder:
         return Ct
 
     def encode(self, jobs: List[EncodeJob], overflow: torch.Tensor = None,
-               carry: bool = True, timers=None) -> List[SpikeTrains]:
+               carry: bool = True, timers=None, store_input: bool = True) -> List[SpikeTrains]:
         """Encode every job; returns resident spike trains (no host synchronisation).
         carry=True advances each job's IAF state (v0, time_count) in place, as the
         reference's encoder state (vtem/ensemble_encode.py:541-545); carry=False leaves it
         and puts the advanced state in self.last_states.  overflow (int32, len(jobs),
-        zeroed by the caller) gets 1 where a job's buffer was too small."""
+        zeroed by the caller) gets 1 where a job's buffer was too small.  store_input keeps
+        the scaled input y (8 bytes per neuron and sample) between the two passes, so the
+        crossing walk reads it instead of evaluating the trigonometric input again."""
         dvc = self.device
         e0 = timers._tic() if timers is not None else None
         coefs = self.coefficients(jobs)
@@ -207,8 +209,12 @@ class BatchEncoder:
             out.append(SpikeTrains(spikes, counts))
             ws_sizes.append((iaf_par_workspace_bytes(N, g.iaf_samples) + 255) // 256 * 256)
         ws = torch.empty(max(1, sum(ws_sizes)), dtype=torch.uint8, device=dvc)
+        ybuf = None
+        if store_input and jobs:
+            ybuf = torch.empty(sum(C.shape[0] * j.encoder.geom.iaf_samples for j, C in zip(jobs, coefs)),
+                               dtype=F64, device=dvc)
         d = np.zeros(len(jobs), IAFP_DTYPE)
-        off = 0
+        off = yoff = 0
         maxN = maxS = maxDt = 0
         for i, (job, C, st, (v0, tc)) in enumerate(zip(jobs, coefs, out, states)):
             g = job.encoder.geom
@@ -217,12 +223,15 @@ class BatchEncoder:
                     ptr(v0), ptr(tc), ptr(job.iaf.kappa),
                     ptr(job.iaf.bias), ptr(job.iaf.delta), ptr(st.counts), ptr(ws) + off,
                     0 if overflow is None else ptr(overflow) + 4 * i, g.dt, N, g.iaf_samples,
-                    g.Dt, job.max_spike, self.root_mode, 0)
+                    g.Dt, job.max_spike, self.root_mode, 0,
+                    0 if ybuf is None else ptr(ybuf) + 8 * yoff, N)
             off += ws_sizes[i]
+            yoff += N * g.iaf_samples
             maxN, maxS, maxDt = max(maxN, N), max(maxS, g.iaf_samples), max(maxDt, g.Dt)
         if jobs:
             dd = _to_dev(d, dvc)
-            lib.vt_iaf_encode_trig_batched(ptr(dd), len(jobs), maxN, maxS, maxDt, stream_ptr())
+            lib.vt_iaf_encode_trig_batched(ptr(dd), len(jobs), maxN, maxS, maxDt,
+                                           0 if ybuf is None else 1, stream_ptr())
         if e0 is not None:
             # algorithmic work: the input's evaluation, 2 Dt flops per sample interval and neuron
             timers._toc("iaf", e0, float(sum(2.0 * j.encoder.geom.Dt * j.encoder.RF.shape[0] *

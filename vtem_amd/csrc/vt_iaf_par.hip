@@ -27,6 +27,12 @@
 
 #define IAFP_SEG 64        // sample intervals per segment
 #define IAFP_WAVES 4       // segments per workgroup (one wave each, 64 neurons per wave)
+#ifndef IAFP_ILP
+#define IAFP_ILP 4         // samples whose inputs are evaluated together
+#endif
+#ifndef IAFP_TRIG_WAVES
+#define IAFP_TRIG_WAVES 3  // minimum waves per SIMD of the trigonometric-input kernels
+#endif
 
 // One encoder call on one block of neurons.  ws holds (S + 2) x N pairs of doubles,
 // S = ceil((size - 1) / IAFP_SEG): segment partials, then prefixes in place; row S the
@@ -47,6 +53,8 @@ struct IafParDesc {
     int *overflow;          // optional: set to 1 when a neuron fires more than max_spike times
     double dt;
     int N, size, Dt, max_spike, root_mode, in_dtype;
+    double *ybuf;           // optional: size x N (ld y_ld), receives y = (v + b) / kappa at
+    long long y_ld;         //   every sample in pass 1; the walk then reads it back
 };
 
 static __host__ __device__ inline int iafp_segments(int size)
@@ -78,10 +86,15 @@ __device__ __forceinline__ void iafp_partial_dev(const IafParDesc &d, const Src 
     const int steps = d.size - 1;
     const int i0 = s * IAFP_SEG, i1 = min(i0 + IAFP_SEG, steps);
     const double kappa = d.kappa[n], bias = d.bias[n], dt = d.dt;
+    double *yout = d.ybuf ? d.ybuf + n : nullptr;
+    const long long yld = d.y_ld;
     double y1 = (src(i0) + bias) / kappa;
+    if (yout && s == 0) yout[0] = y1;
     double P = 0.0, M = -INFINITY;
-    for (int i = i0; i < i1; ++i) {
-        const double y2 = (src(i + 1) + bias) / kappa;
+    auto step = [&](int i, double v2) {
+#pragma clang fp contract(off)
+        const double y2 = (v2 + bias) / kappa;
+        if (yout) yout[(long long)(i + 1) * yld] = y2;
         const double area = (y1 + y2) / 2 * dt;
         double top;
         if (y2 >= 0 || y1 <= 0) {
@@ -93,12 +106,52 @@ __device__ __forceinline__ void iafp_partial_dev(const IafParDesc &d, const Src 
         M = fmax(M, top);
         P += area;
         y1 = y2;
+    };
+    int i = i0;
+    for (; i + IAFP_ILP <= i1; i += IAFP_ILP) {
+        double v[IAFP_ILP];
+        src.template eval<IAFP_ILP>(i + 1, v);
+#pragma unroll
+        for (int r = 0; r < IAFP_ILP; ++r) step(i + r, v[r]);
     }
+    for (; i < i1; ++i) step(i, src(i + 1));
     d.ws[(long long)s * d.N + n] = make_double2(P, M);
 }
 
+// The walk's input: y = (v + b) / kappa from a source of v, or y as pass 1 stored it.
 template <class Src>
-__device__ __forceinline__ void iafp_walk_dev(const IafParDesc &d, const Src &src, int n, int s,
+struct ScaledInput {
+    const Src &src;
+    double bias, kappa;
+    __device__ __forceinline__ double operator()(int i) const
+    {
+#pragma clang fp contract(off)
+        return (src(i) + bias) / kappa;
+    }
+    template <int R>
+    __device__ __forceinline__ void eval(int i, double *out) const
+    {
+#pragma clang fp contract(off)
+        src.template eval<R>(i, out);
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[r] = (out[r] + bias) / kappa;
+    }
+};
+
+struct StoredInput {
+    const double *y;
+    long long ld;
+    __device__ __forceinline__ double operator()(int i) const { return y[(long long)i * ld]; }
+    template <int R>
+    __device__ __forceinline__ void eval(int i, double *out) const
+    {
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[r] = y[(long long)(i + r) * ld];
+    }
+};
+
+template <class Ysrc>
+__device__ __forceinline__ void iafp_walk_dev(const IafParDesc &d, const Ysrc &ys, int n, int s,
                                               int S)
 {
 #pragma clang fp contract(off)
@@ -124,13 +177,13 @@ __device__ __forceinline__ void iafp_walk_dev(const IafParDesc &d, const Src &sr
     };
     const int steps = d.size - 1;
     const int i0 = s * IAFP_SEG, i1 = min(i0 + IAFP_SEG, steps);
-    const double kappa = d.kappa[n], bias = d.bias[n], dt = d.dt;
+    const double dt = d.dt;
     const int mode = d.root_mode;
-    double y1 = (src(i0) + bias) / kappa;
+    double y1 = ys(i0);
     double P = 0.0;
     double best = -INFINITY, tbest = tc0 + (double)i0 * dt;
-    for (int i = i0; i < i1 && k < k1; ++i) {
-        const double y2 = (src(i + 1) + bias) / kappa;
+    auto step = [&](int i, double y2) {
+#pragma clang fp contract(off)
         const double area = (y1 + y2) / 2 * dt;
         const double F = pre.x + P;
         const double t0 = tc0 + (double)i * dt;
@@ -172,54 +225,86 @@ __device__ __forceinline__ void iafp_walk_dev(const IafParDesc &d, const Src &sr
         }
         P += area;
         y1 = y2;
+    };
+    // samples in groups of IAFP_ILP (their inputs evaluated together), each step taken
+    // only while levels remain, as in a one-sample loop
+    int i = i0;
+    for (; i + IAFP_ILP <= i1 && k < k1; i += IAFP_ILP) {
+        double v[IAFP_ILP];
+        ys.template eval<IAFP_ILP>(i + 1, v);
+#pragma unroll
+        for (int r = 0; r < IAFP_ILP; ++r) {
+            if (k >= k1) break;
+            step(i + r, v[r]);
+        }
     }
+    for (; i < i1 && k < k1; ++i) step(i, ys(i + 1));
     // Levels the prefix maximum counts in this segment but the step tests missed by a
     // rounding tie are reached at the segment's highest point.
     for (; k < k1; ++k) emit(k, tbest);
 }
 
-// Workgroup x: neuron group g = x % G (64 lanes), segment quad x / G (IAFP_WAVES waves).
-__device__ __forceinline__ bool iafp_task(const IafParDesc &d, int &n, int &s, int &S)
+// A workgroup's IAFP_WAVES waves cover 2^wl neuron groups (64 lanes each) on each of
+// IAFP_WAVES / 2^wl consecutive segments.  With wl = 2 the four waves evaluate the same
+// basis rows at about the same time, so three of them find the rows in the scalar cache.
+// Workgroup x: neuron-group tile x % GW, segment group x / GW.
+__device__ __forceinline__ bool iafp_task(const IafParDesc &d, int wl, int &n, int &s, int &S)
 {
     S = iafp_segments(d.size);
     const int G = (d.N + 63) / 64;
+    const int GW = (G + (1 << wl) - 1) >> wl;
     const int x = blockIdx.x;
-    const int quad = x / G;
-    n = (x - quad * G) * 64 + (threadIdx.x & 63);
+    const int sg = x / GW;
     // wave-uniform segment: the basis row addresses become scalar loads shared by the wave
-    s = quad * IAFP_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    n = (((x - sg * GW) << wl) + (w & ((1 << wl) - 1))) * 64 + (threadIdx.x & 63);
+    s = sg * (IAFP_WAVES >> wl) + (w >> wl);
     return n < d.N && s < S;
 }
 
 // BATCHED selects the descriptor source at compile time (a runtime select between the
 // array and the by-value argument would turn every field into a generic-pointer load,
 // defeating scalar loads of the uniform fields).
+// At least 3 waves per SIMD: the 65-term order needs 130 VGPRs of coefficients, and a few
+// more would leave 2 waves to hide the latency of the basis loads.
 template <int DT, int PHASE, bool BATCHED>
-__global__ void __launch_bounds__(64 * IAFP_WAVES)
-iafp_trig_kernel(const IafParDesc *__restrict__ descs, IafParDesc d0)
+__global__ void __launch_bounds__(64 * IAFP_WAVES) __attribute__((amdgpu_waves_per_eu(IAFP_TRIG_WAVES)))
+iafp_trig_kernel(const IafParDesc *__restrict__ descs, IafParDesc d0, int wl)
 {
     const IafParDesc d = BATCHED ? descs[blockIdx.y] : d0;
     int n, s, S;
-    if (!iafp_task(d, n, s, S)) return;
+    if (!iafp_task(d, wl, n, s, S)) return;
     TrigSource<DT> src;
     src.load(d.coef, d.coef_ld, d.basis, d.Dt, n, d.coef_ns);
     if (PHASE == 0)
         iafp_partial_dev(d, src, n, s, S);
     else
-        iafp_walk_dev(d, src, n, s, S);
+        iafp_walk_dev(d, ScaledInput<TrigSource<DT>>{src, d.bias[n], d.kappa[n]}, n, s, S);
+}
+
+// Pass 2 over the input that pass 1 stored (every descriptor has ybuf): a stream of
+// y = (v + b) / kappa instead of the trigonometric evaluation.
+template <bool BATCHED>
+__global__ void __launch_bounds__(64 * IAFP_WAVES)
+iafp_walk_stored_kernel(const IafParDesc *__restrict__ descs, IafParDesc d0, int wl)
+{
+    const IafParDesc d = BATCHED ? descs[blockIdx.y] : d0;
+    int n, s, S;
+    if (!iafp_task(d, wl, n, s, S)) return;
+    iafp_walk_dev(d, StoredInput{d.ybuf + n, d.y_ld}, n, s, S);
 }
 
 template <typename TIn, int PHASE>
 __global__ void __launch_bounds__(64 * IAFP_WAVES)
-iafp_sampled_kernel(IafParDesc d)
+iafp_sampled_kernel(IafParDesc d, int wl)
 {
     int n, s, S;
-    if (!iafp_task(d, n, s, S)) return;
+    if (!iafp_task(d, wl, n, s, S)) return;
     SampledSource<TIn> src{(const TIn *)d.input + n, d.input_ld};
     if (PHASE == 0)
         iafp_partial_dev(d, src, n, s, S);
     else
-        iafp_walk_dev(d, src, n, s, S);
+        iafp_walk_dev(d, ScaledInput<SampledSource<TIn>>{src, d.bias[n], d.kappa[n]}, n, s, S);
 }
 
 // Per neuron: exclusive prefixes of the segment partials, totals, and the carried state
@@ -293,51 +378,69 @@ VT_EXPORT int vt_iaf_par_workspace_size(int N, int size, long long *bytes)
     return VT_OK;
 }
 
+// Grid of the segment kernels for blocks of at most maxN neurons and maxSize samples, and
+// the number 2^wl of neuron groups a workgroup holds per segment: 4 once there are enough
+// groups to fill the workgroups (the waves then share basis rows), fewer for small blocks.
+struct IafpGrid {
+    dim3 g;
+    int wl;
+};
+
+static IafpGrid iafp_grid(int maxN, int maxSize, int nd)
+{
+    const int G = (maxN + 63) / 64;
+#ifdef IAFP_WL
+    const int wl = IAFP_WL;
+#else
+    const int wl = G >= 8 ? 2 : (G >= 2 ? 1 : 0);
+#endif
+    const int GW = (G + (1 << wl) - 1) >> wl;
+    const int per = IAFP_WAVES >> wl;
+    const int sg = (iafp_segments(maxSize) + per - 1) / per;
+    return {dim3((unsigned)(GW * (sg > 0 ? sg : 1)), (unsigned)nd), wl};
+}
+
 template <int DT>
-static void iafp_trig_launch(const IafParDesc *descs, const IafParDesc &d0, dim3 g, dim3 gs,
-                             hipStream_t s)
+static void iafp_trig_launch(const IafParDesc *descs, const IafParDesc &d0, IafpGrid g, dim3 gs,
+                             bool stored, hipStream_t s)
 {
     if (descs) {
-        iafp_trig_kernel<DT, 0, true><<<g, 64 * IAFP_WAVES, 0, s>>>(descs, d0);
+        iafp_trig_kernel<DT, 0, true><<<g.g, 64 * IAFP_WAVES, 0, s>>>(descs, d0, g.wl);
         iafp_scan_kernel<true><<<gs, 256, 0, s>>>(descs, d0);
-        iafp_trig_kernel<DT, 1, true><<<g, 64 * IAFP_WAVES, 0, s>>>(descs, d0);
+        if (stored)
+            iafp_walk_stored_kernel<true><<<g.g, 64 * IAFP_WAVES, 0, s>>>(descs, d0, g.wl);
+        else
+            iafp_trig_kernel<DT, 1, true><<<g.g, 64 * IAFP_WAVES, 0, s>>>(descs, d0, g.wl);
     } else {
-        iafp_trig_kernel<DT, 0, false><<<g, 64 * IAFP_WAVES, 0, s>>>(descs, d0);
+        iafp_trig_kernel<DT, 0, false><<<g.g, 64 * IAFP_WAVES, 0, s>>>(descs, d0, g.wl);
         iafp_scan_kernel<false><<<gs, 256, 0, s>>>(descs, d0);
-        iafp_trig_kernel<DT, 1, false><<<g, 64 * IAFP_WAVES, 0, s>>>(descs, d0);
+        iafp_trig_kernel<DT, 1, false><<<g.g, 64 * IAFP_WAVES, 0, s>>>(descs, d0, g.wl);
     }
 }
 
 template <typename TIn>
-static void iafp_sampled_launch(const IafParDesc &d, dim3 g, dim3 gs, hipStream_t s)
+static void iafp_sampled_launch(const IafParDesc &d, IafpGrid g, dim3 gs, hipStream_t s)
 {
-    iafp_sampled_kernel<TIn, 0><<<g, 64 * IAFP_WAVES, 0, s>>>(d);
+    iafp_sampled_kernel<TIn, 0><<<g.g, 64 * IAFP_WAVES, 0, s>>>(d, g.wl);
     iafp_scan_kernel<false><<<gs, 256, 0, s>>>(nullptr, d);
-    iafp_sampled_kernel<TIn, 1><<<g, 64 * IAFP_WAVES, 0, s>>>(d);
+    iafp_sampled_kernel<TIn, 1><<<g.g, 64 * IAFP_WAVES, 0, s>>>(d, g.wl);
 }
 
-static int iafp_trig_dispatch(const IafParDesc *descs, const IafParDesc &d0, int maxDt, dim3 g,
-                              dim3 gs, hipStream_t s)
+static int iafp_trig_dispatch(const IafParDesc *descs, const IafParDesc &d0, int maxDt,
+                              IafpGrid g, dim3 gs, bool stored, hipStream_t s)
 {
     if (maxDt <= 9)
-        iafp_trig_launch<9>(descs, d0, g, gs, s);
+        iafp_trig_launch<9>(descs, d0, g, gs, stored, s);
     else if (maxDt <= 17)
-        iafp_trig_launch<17>(descs, d0, g, gs, s);
+        iafp_trig_launch<17>(descs, d0, g, gs, stored, s);
     else if (maxDt <= 33)
-        iafp_trig_launch<33>(descs, d0, g, gs, s);
+        iafp_trig_launch<33>(descs, d0, g, gs, stored, s);
     else if (maxDt <= 65)
-        iafp_trig_launch<65>(descs, d0, g, gs, s);
+        iafp_trig_launch<65>(descs, d0, g, gs, stored, s);
     else
         return VT_ERANGE;
     VT_LAUNCH_CHECK();
     return VT_OK;
-}
-
-static dim3 iafp_grid(int maxN, int maxSize, int nd)
-{
-    const int G = (maxN + 63) / 64;
-    const int quads = (iafp_segments(maxSize) + IAFP_WAVES - 1) / IAFP_WAVES;
-    return dim3((unsigned)(G * (quads > 0 ? quads : 1)), (unsigned)nd);
 }
 
 VT_EXPORT int vt_iaf_encode_trig_par(const double *coef, long long coef_ld, const double *basis,
@@ -355,7 +458,7 @@ VT_EXPORT int vt_iaf_encode_trig_par(const double *coef, long long coef_ld, cons
     VT_CHECK_ARG(coef && basis && Dt >= 1 && coef_ld >= N);
     VT_CHECK_ARG(ws_bytes >= iafp_ws_bytes(N, size));
     return iafp_trig_dispatch(nullptr, d, Dt, iafp_grid(N, size, 1), dim3(vt_blocks(N, 256), 1),
-                              (hipStream_t)stream);
+                              false, (hipStream_t)stream);
 }
 
 VT_EXPORT int vt_iaf_encode_par(const void *input, int in_dtype, long long input_ld, int N,
@@ -373,7 +476,8 @@ VT_EXPORT int vt_iaf_encode_par(const void *input, int in_dtype, long long input
     VT_CHECK_ARG((input || size == 0) && input_ld >= N && (in_dtype == 0 || in_dtype == 1));
     VT_CHECK_ARG(ws_bytes >= iafp_ws_bytes(N, size));
     hipStream_t s = (hipStream_t)stream;
-    dim3 g = iafp_grid(N, size, 1), gs(vt_blocks(N, 256), 1);
+    IafpGrid g = iafp_grid(N, size, 1);
+    dim3 gs(vt_blocks(N, 256), 1);
     if (in_dtype == 0)
         iafp_sampled_launch<double>(d, g, gs, s);
     else
@@ -384,16 +488,19 @@ VT_EXPORT int vt_iaf_encode_par(const void *input, int in_dtype, long long input
 
 // Batched trig encode: one descriptor per block (device array), launched together.  The
 // host passes the maxima of N, size and Dt over the descriptors for the grid; workgroups
-// beyond a block's own extent exit at once.
+// beyond a block's own extent exit at once.  flags bit 0: every descriptor carries ybuf, so
+// pass 1 stores the input and pass 2 reads it instead of evaluating it again.
 VT_EXPORT int vt_iaf_encode_trig_batched(const void *descs, int nd, int maxN, int maxSize,
-                                         int maxDt, void *stream)
+                                         int maxDt, int flags, void *stream)
 {
     VT_CHECK_ARG(nd >= 0 && maxN >= 0 && maxSize >= 0 && maxDt >= 1);
     VT_CHECK_ARG(descs || nd == 0);
+    VT_CHECK_ARG((flags & ~1) == 0);
     if (nd == 0 || maxN == 0) return VT_OK;
     IafParDesc d0{};
     return iafp_trig_dispatch((const IafParDesc *)descs, d0, maxDt, iafp_grid(maxN, maxSize, nd),
-                              dim3(vt_blocks(maxN, 256), (unsigned)nd), (hipStream_t)stream);
+                              dim3(vt_blocks(maxN, 256), (unsigned)nd), (flags & 1) != 0,
+                              (hipStream_t)stream);
 }
 
 VT_EXPORT int vt_iaf_par_desc_size(int *bytes)

@@ -193,8 +193,13 @@ class TiledCodec:
         return out
 
     def decode(self, spikes, decs: Sequence[Decoder], precision: str = "kpcg",
-               max_batch: int = 64) -> torch.Tensor:
-        """The blended video (frames, Py, Px) from the tiles' spike trains."""
+               max_batch: int = 64, cg_iters: int = 160, **solver) -> torch.Tensor:
+        """The blended video (frames, Py, Px) from the tiles' spike trains.  cg_iters and
+        `solver` go to the BatchDecoder (retry_iters, tol, ...): a window of a tile whose
+        neurons fall silent for part of it (high contrast gain) is further from the
+        Kronecker preconditioner's model and takes more CG iterations than a served block.
+        last_stats counts the blocks the guard solved again (kpcg_retries) and those it
+        redid in FP64."""
         p = self.plan
         T, W = p.tile, p.window
         jobs, where = [], []
@@ -202,11 +207,15 @@ class TiledCodec:
             for it, f0 in enumerate(p.f0):
                 jobs.append(BlockJob(decs[k], spikes[k], t0=f0 / self.fps, tag=(iy, ix, it)))
                 where.append((iy, ix, it))
-        bd = BatchDecoder(precision=precision)
+        bd = BatchDecoder(precision=precision, cg_iters=cg_iters, **solver)
         frames = []
+        fallbacks = retries = 0
         for b0 in range(0, len(jobs), max_batch):
             res = bd.decode(jobs[b0:b0 + max_batch])
+            fallbacks += bd.last_stats.get("fallbacks", 0)
+            retries += bd.last_stats.get("kpcg_retries", 0)
             frames += [r.frames.view(W, T, T) for r in res]
+        self.last_stats = {"blocks": len(jobs), "fallbacks": fallbacks, "kpcg_retries": retries}
         out = torch.zeros((p.frames, p.Py, p.Px), dtype=F64, device=self.device)
         keep = []
         # blocks that share a parity in every axis never overlap: 8 accumulation passes
