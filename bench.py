@@ -77,6 +77,7 @@ def parse():
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--max-batch", type=int, default=32)
     p.add_argument("--streams", type=int, default=2)
+    p.add_argument("--rnn-lanes", type=int, default=1, help="streams of the RNN lane")
     p.add_argument("--form", default="kpcg", help="solver form of the chosen served path")
     p.add_argument("--compare", default="bf16x3", help="second served form measured ('' = none)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -408,6 +409,7 @@ def main():
 
     def run_form(form):
         server = Server(max_batch=args.max_batch, streams=args.streams, precision=form,
+                        rnn_lanes=args.rnn_lanes,
                         adaptive=form != "kpcg")
         for _ in range(args.warmup):           # allocator, code objects, precision policy
             server.serve(make_requests(items, zeros, form))
@@ -484,7 +486,8 @@ def main():
         "config": {"workload": "serving form: blocks {32,48,64,96}^2 px x {0.5,1,2} s, "
                                "N=round(2D/9), Poisson arrivals above capacity",
                    "requests_per_step": R, "frames_per_step": frames, "D_min": int(D.min()),
-                   "D_max": int(D.max()), "max_batch": args.max_batch, "streams": args.streams},
+                   "D_max": int(D.max()), "max_batch": args.max_batch, "streams": args.streams,
+                              "rnn_lanes": args.rnn_lanes},
         "chosen": {"form": args.form, "frames_per_s": round(value, 2),
                    "p50_ms": round(stats[-1]["p50_ms"], 3), "p99_ms": round(stats[-1]["p99_ms"], 3),
                    "batches": stats[-1]["batches"], "fp64_fallbacks": stats[-1]["fallbacks"]},

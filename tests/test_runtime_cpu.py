@@ -103,9 +103,9 @@ def test_iaf_par_host_queries():
 
 def test_batch_formation_and_rnn_lane_routing():
     """Batches take requests in arrival order up to the request cap and the free HBM;
-    RNN-decoded requests are routed to their own lane."""
+    RNN-decoded requests are routed to their own lanes, which split the waiting ones."""
     import collections
-    from vtem_amd.serve import Request, form_batch, is_rnn
+    from vtem_amd.serve import Request, form_batch, is_rnn, split_cap
 
     class _J:
         def __init__(self, method):
@@ -121,6 +121,10 @@ def test_batch_formation_and_rnn_lane_routing():
     assert [r.rid for r in b] == [3, 4] and nb == 90
     b, nb = form_batch(q, 10, 10)                  # nothing fits
     assert b == [] and nb == 0 and len(q) == 7
+    # several free RNN lanes split the waiting RNN requests evenly, within the cap
+    assert split_cap(13, 64, 1) == 13
+    assert split_cap(13, 64, 2) == 7 and split_cap(6, 64, 2) == 3
+    assert split_cap(13, 4, 2) == 4 and split_cap(0, 64, 3) == 1
 
 
 def test_spike_buffer_rate_rows():

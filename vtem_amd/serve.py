@@ -14,9 +14,9 @@ The server:
 * launches the whole batch on that stream without a host synchronisation -- the batched
   encode (filter products and time-parallel IAF of every block) and then the batched
   decode -- so batches formed from requests that arrive while others are on the GPU run
-  concurrently on the device; the RNN queue feeds one lane of its own, whose batch takes
-  every RNN request waiting (up to rnn_max_batch), so RNN requests share one sequence of
-  iteration launches;
+  concurrently on the device; the RNN queue feeds lanes of its own (rnn_lanes), whose
+  batches split the waiting RNN requests between the free RNN lanes (up to
+  rnn_max_batch each), so RNN requests share a few sequences of iteration launches;
 * retires a batch when its completion event fires: re-runs any block whose spike buffer
   overflowed (with an exact buffer) or whose low-precision solve missed tolerance,
   releases its memory and stamps each request's completion time.
@@ -117,7 +117,7 @@ class PrecisionPolicy:
 class Server:
     def __init__(self, hbm_budget: int = 64 << 30, max_batch: int = 32, streams: int = 2,
                  precision: str = "bf16x3", verify: bool = True, adaptive: bool = True,
-                 rnn_lane: bool = True, rnn_max_batch: int = 64):
+                 rnn_lane: bool = True, rnn_max_batch: int = 64, rnn_lanes: int = 1):
         self.hbm_budget = int(hbm_budget)
         self.max_batch = int(max_batch)
         self.n_streams = int(streams)
@@ -126,6 +126,7 @@ class Server:
         # up, so their iterations share launches instead of each batch running its own loop
         self.rnn_lane = bool(rnn_lane)
         self.rnn_max_batch = int(rnn_max_batch)
+        self.rnn_lanes = max(1, int(rnn_lanes))
         self.precision = precision
         self.verify = verify
         self.policy = PrecisionPolicy(precision) if adaptive else None
@@ -147,9 +148,11 @@ class Server:
         # lanes: (stream, queue, max batch); lane queues hold arrived, not yet launched requests
         ready: collections.deque = collections.deque()
         ready_rnn: collections.deque = collections.deque()
-        lanes = [(torch.cuda.Stream(), ready, self.max_batch) for _ in range(self.n_streams)]
-        if self.rnn_lane:
-            lanes.append((torch.cuda.Stream(), ready_rnn, self.rnn_max_batch))
+        # The RNN lane comes first: when several lanes are free, its batch (the longest chain
+        # of dependent launches) is enqueued before the host spends its time on the others.
+        nr = self.rnn_lanes if self.rnn_lane else 0
+        lanes = [(torch.cuda.Stream(), ready_rnn, self.rnn_max_batch) for _ in range(nr)]
+        lanes += [(torch.cuda.Stream(), ready, self.max_batch) for _ in range(self.n_streams)]
         streams = [ln[0] for ln in lanes]
         busy = [None] * len(lanes)              # (event, handle, reqs, bytes, overflow flags)
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
@@ -191,6 +194,8 @@ class Server:
             for k, (_, queue, cap) in enumerate(lanes):
                 if busy[k] is not None or not queue:
                     continue
+                if k < nr:
+                    cap = split_cap(len(queue), cap, sum(1 for i in range(k, nr) if busy[i] is None))
                 batch, nbytes = form_batch(queue, cap, self.hbm_budget - in_use)
                 if not batch:
                     continue
@@ -297,6 +302,12 @@ class Server:
 def is_rnn(r: Request) -> bool:
     """Requests decoded by the gradient-flow RNN go to the server's RNN lane."""
     return r.job.method == "rnn"
+
+
+def split_cap(waiting: int, cap: int, free_lanes: int) -> int:
+    """Batch cap of one of `free_lanes` free lanes that share a queue of `waiting`
+    requests: an even split, at most `cap`."""
+    return max(1, min(cap, -(-waiting // max(1, free_lanes))))
 
 
 def form_batch(queue: collections.deque, cap: int, free_bytes: int):
