@@ -267,7 +267,7 @@ def precision_report(seed=0):
             "intervals": n, "forms": forms}
 
 
-def stitched_c3(seed=0, precision="kpcg", max_batch=64):
+def stitched_c3(seed=0, precision="kpcg", max_batch=1024):
     """Batching form: a 512 x 512 px, 4 s video (trigonometric polynomial with the frame's
     and the duration's periods, orders 64 / 32, i.e. the blocks' bandwidth) in 64 px tiles
     with 16 px overlap and 1 s windows with 0.25 s overlap: 11 x 11 tiles x 5 windows =

@@ -406,6 +406,9 @@ class BatchDecoder:
                  ptr(offsets[Ntot:]), cap, s)
         Tm = torch.empty((Ntot, Dt, Dt), dtype=F64, device=dev)
         tv = torch.empty((Ntot, Dt), dtype=F64, device=dev)
+        if Dt > 17:
+            raise ValueError(f"temporal order Lt = {(Dt - 1) // 2} above 8: the T_j kernel holds "
+                             "at most 17 x 17 moments per neuron")
         tj = np.zeros(B, TJ_DTYPE)
         for b, d in enumerate(decs):
             tj[b] = (nbase[b], N[b], 0, d.geom.Lt, 0, d.geom.W, nbase[b], 0)

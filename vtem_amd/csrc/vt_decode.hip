@@ -104,9 +104,20 @@ neuron_T_kernel(double *__restrict__ Tm, double *__restrict__ tv, const double *
     const int lane = threadIdx.x;
     const int k0 = offsets[j], k1 = offsets[j + 1];
     const double w = norm ? norm[j] : 1.0;
-    double accT[5] = {0, 0, 0, 0, 0};   // entries lane, lane+64, ... < Dt*Dt (Dt <= 17)
+    // T_j is symmetric: each lane sums the upper-triangle entries u = lane, lane + 64, ...
+    // (p <= pp; at most 3 for Dt <= 17) and writes both (p, pp) and (pp, p).  The product
+    // al[r][p] * al[r][pp] and the order over r are those of the mirrored entry.
+    const int DTU = Dt * (Dt + 1) / 2;
+    int tp[3], tq[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        int u = lane + 64 * e, p = 0;
+        while (p < Dt && u >= Dt - p) { u -= Dt - p; ++p; }
+        tp[e] = p;
+        tq[e] = p + u;
+    }
+    double accT[3] = {0, 0, 0};
     double acct = 0.0;
-    const int DT2 = Dt * Dt;
     for (int base = k0; base < k1; base += 64) {
         const int k = base + lane;
         if (k < k1) {
@@ -120,10 +131,9 @@ neuron_T_kernel(double *__restrict__ Tm, double *__restrict__ tv, const double *
         __syncthreads();
         const int cnt = min(64, k1 - base);
 #pragma unroll
-        for (int e = 0; e < 5; ++e) {
-            int idx = lane + 64 * e;
-            if (idx < DT2) {
-                int p = idx / Dt, pp = idx % Dt;
+        for (int e = 0; e < 3; ++e) {
+            if (lane + 64 * e < DTU) {
+                const int p = tp[e], pp = tq[e];
                 double s = 0.0;
                 for (int r = 0; r < cnt; ++r) s += al[r][p] * al[r][pp];
                 accT[e] += s;
@@ -138,9 +148,12 @@ neuron_T_kernel(double *__restrict__ Tm, double *__restrict__ tv, const double *
     }
     const long long ob = (long long)(d.T_base + jl);
 #pragma unroll
-    for (int e = 0; e < 5; ++e) {
-        int idx = lane + 64 * e;
-        if (idx < DT2) Tm[ob * Dt * Dt + idx] = accT[e];
+    for (int e = 0; e < 3; ++e) {
+        if (lane + 64 * e < DTU) {
+            const int p = tp[e], pp = tq[e];
+            Tm[ob * Dt * Dt + p * Dt + pp] = accT[e];
+            if (pp != p) Tm[ob * Dt * Dt + pp * Dt + p] = accT[e];
+        }
     }
     if (lane < Dt) tv[ob * Dt + lane] = acct;
 }

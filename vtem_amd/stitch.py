@@ -193,13 +193,15 @@ class TiledCodec:
         return out
 
     def decode(self, spikes, decs: Sequence[Decoder], precision: str = "kpcg",
-               max_batch: int = 64, cg_iters: int = 160, **solver) -> torch.Tensor:
+               max_batch: int = 1024, cg_iters: int = 160, **solver) -> torch.Tensor:
         """The blended video (frames, Py, Px) from the tiles' spike trains.  cg_iters and
         `solver` go to the BatchDecoder (retry_iters, tol, ...): a window of a tile whose
         neurons fall silent for part of it (high contrast gain) is further from the
         Kronecker preconditioner's model and takes more CG iterations than a served block.
-        last_stats counts the blocks the guard solved again (kpcg_retries) and those it
-        redid in FP64."""
+        The blocks go through the decoder max_batch at a time: one joint solve of many blocks
+        shares each iteration's launches (about 10 MB of device memory per 64 x 64 x 1 s
+        block).  last_stats counts the blocks the guard solved again (kpcg_retries) and
+        those it redid in FP64."""
         p = self.plan
         T, W = p.tile, p.window
         jobs, where = [], []
