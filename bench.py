@@ -273,10 +273,11 @@ def stitched_c3(seed=0, precision="kpcg", max_batch=1024):
     with 16 px overlap and 1 s windows with 0.25 s overlap: 11 x 11 tiles x 5 windows =
     605 blocks of Lx = Ly = 8, Lt = 8, each tile with 2048 neurons of the config-1 laws, IAF
     b = 2.5, delta ~ N(0.04, 0.004^2), and a contrast gain log-uniform in [0.25, 4].  Timed
-    (video, RF bank and decoders resident): the batched encode of all tiles, the batched
-    decode of all blocks and the raised-cosine blending.  SNR over the full video and its
-    interior (16 px border and the first / last 0.25 s excluded); the same codec on a video
-    with the blocks' own periods (64 px, 1 s), untimed, checks the machinery."""
+    (video, RF bank, IAF parameters and decoders resident): the batched encode of all
+    tiles, the batched decode of all blocks and the raised-cosine blending.  SNR over the
+    full video and its interior (16 px border and the first / last 0.25 s excluded); the
+    same codec on a video with the blocks' own periods (64 px, 1 s), untimed, checks the
+    machinery."""
     import torch
     from vtem_amd import synthetic as syn
     from vtem_amd.stitch import TiledCodec
@@ -288,12 +289,13 @@ def stitched_c3(seed=0, precision="kpcg", max_batch=1024):
     iaf = [syn.iaf_params(N, 2.5, 1.0, 0.04, 0.004, rng) for _ in range(n)]
     contrast = np.exp(rng.uniform(math.log(0.25), math.log(4.0), n))
     decs = codec.decoders(iaf, contrast)
+    devs = codec.iaf_devices(iaf, contrast)
 
     def run(frames):
         V = torch.tensor(frames, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        spikes = codec.encode(V, iaf, contrast)
+        spikes = codec.encode(V, iaf, contrast, devices=devs)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         out = codec.decode(spikes, decs, precision=precision, max_batch=max_batch)

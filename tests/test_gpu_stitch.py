@@ -37,6 +37,14 @@ def test_stitched_codec_matches_plain_blocks():
     contrast = np.exp(rng.uniform(math.log(0.25), math.log(4.0), ntile))
     V = torch.tensor(frames, dtype=torch.float64, device="cuda")
     spikes = codec.encode(V, iaf, contrast)
+    # resident IAF parameters (built once) encode the same spikes
+    again = codec.encode(V, iaf, contrast, devices=codec.iaf_devices(iaf, contrast))
+    for a, b in zip(spikes, again):
+        ca = a.counts.cpu().numpy()
+        np.testing.assert_array_equal(ca, b.counts.cpu().numpy())
+        sa, sb = a.spikes.cpu().numpy(), b.spikes.cpu().numpy()
+        for j in range(ca.size):
+            np.testing.assert_array_equal(sa[:ca[j], j], sb[:ca[j], j])
     decs = codec.decoders(iaf, contrast)
     got = codec.decode(spikes, decs).cpu().numpy()
     counts = [int(s.counts.sum().item()) for s in spikes]

@@ -139,14 +139,30 @@ class TiledCodec:
         self.wy = [dev(w) for w in self.plan.wy]
         self.wt = [dev(w) for w in self.plan.wt]
 
+    @staticmethod
+    def tile_params(iaf: Sequence[IAFParams], contrast: Sequence[float]) -> List[IAFParams]:
+        """Each tile's IAF parameters with its input gain c folded in: the neuron integrates
+        (c v + b) / kappa = (v + b/c) / (kappa/c)."""
+        return [dataclasses.replace(p_, kappa=p_.kappa / float(c), bias=p_.bias / float(c))
+                for p_, c in zip(iaf, contrast)]
+
+    def iaf_devices(self, iaf: Sequence[IAFParams], contrast: Sequence[float]) -> List[IAFDevice]:
+        """The tiles' IAF parameters and state resident on the device (gain folded in), for
+        encode(devices=...): built once, like the RF bank and the decoders."""
+        return [IAFDevice.from_params(p_, self.device) for p_ in self.tile_params(iaf, contrast)]
+
     def encode(self, V: torch.Tensor, iaf: Sequence[IAFParams], contrast: Sequence[float],
-               group: int = 32):
+               group: int = 32, devices: Optional[Sequence[IAFDevice]] = None):
         """Spike trains per tile (plan.tiles order).  V: (frames, Py, Px) float64 on the GPU.
         Tiles are encoded `group` at a time; a tile whose buffer overflows is encoded again
-        with a larger one."""
+        with a larger one.  `devices` (from iaf_devices) are the resident IAF parameters;
+        without them they are copied to the device here.  The state is not advanced."""
         p = self.plan
         T = p.tile
         be = BatchEncoder(self.device)
+        pks = self.tile_params(iaf, contrast)
+        if devices is None:
+            devices = [IAFDevice.from_params(pk, self.device) for pk in pks]
         out = []
         tiles = p.tiles
         for g0 in range(0, len(tiles), group):
@@ -156,14 +172,11 @@ class TiledCodec:
                 y0, x0 = p.y0[iy], p.x0[ix]
                 # the tile's frames (a copy: the product's operand has one pixel stride)
                 tv = V[:, y0:y0 + T, x0:x0 + T].reshape(V.shape[0], T * T).contiguous()
-                # input gain c: the neuron integrates (c v + b) / kappa = (v + b/c) / (kappa/c)
-                c = float(contrast[k])
-                pk = dataclasses.replace(iaf[k], kappa=iaf[k].kappa / c, bias=iaf[k].bias / c)
                 # rows for the bias-driven rate, widened for the input's swing c |v| (|v| of
                 # order 1 for unit-peak videos); an overflow is redone below
-                m = 1.5 * (1.0 + 0.6 * c)
-                jobs.append(EncodeJob(self.encoder, tv, IAFDevice.from_params(pk, self.device),
-                                      max_spikes_rate(self.enc_geom, pk, margin=m)))
+                m = 1.5 * (1.0 + 0.6 * float(contrast[k]))
+                jobs.append(EncodeJob(self.encoder, tv, devices[k],
+                                      max_spikes_rate(self.enc_geom, pks[k], margin=m)))
                 margins.append(m)
             flags = torch.zeros(len(jobs), dtype=torch.int32, device=self.device)
             sts = be.encode(jobs, flags, carry=False)
@@ -172,7 +185,7 @@ class TiledCodec:
                 m = margins[i]
                 while True:
                     m *= 2.0
-                    j2 = dataclasses.replace(jobs[i], max_spike=max_spikes_rate(self.enc_geom, iaf[g0 + i],
+                    j2 = dataclasses.replace(jobs[i], max_spike=max_spikes_rate(self.enc_geom, pks[g0 + i],
                                                                                 margin=m))
                     fl = torch.zeros(1, dtype=torch.int32, device=self.device)
                     st = be.encode([j2], fl, carry=False)[0]
