@@ -14,10 +14,36 @@
 // with 2*N*D^2 instead of 2*n*D^2 flops.
 //
 // A workgroup owns a pair of spatial blocks (b1 >= b2) of BP = 8 spatial indices each,
-// i.e. the 64 rows (ps in b1, ps' in b2) x all Dt^2 columns; that covers the D-space
-// block rows [b1*8*Dt, ...) x cols [b2*8*Dt, ...), so only the lower block triangle of A
-// is computed (diagonal blocks in full).  4 waves, wave w owns row tile w.
+// i.e. the 64 rows (ps in b1, ps' in b2) x the Dt (Dt + 1) / 2 temporal columns; that
+// covers the D-space block rows [b1*8*Dt, ...) x cols [b2*8*Dt, ...), so only the lower
+// block triangle of A is computed (diagonal blocks in full).  4 waves, wave w owns row
+// tile w.
 #include "vt_common.hpp"
+
+// T_j is symmetric, so the columns (pt, pt') and (pt', pt) of every row are equal.  The
+// kernels compute only the Dt (Dt + 1) / 2 columns pt <= pt' (numbered row by row) and
+// write each result at both positions: 153 of 289 columns at Dt = 17, the same products in
+// the same order as the mirrored column, so the matrix is unchanged bit for bit.
+__host__ __device__ __forceinline__ int gram_tcols(int Dt) { return Dt * (Dt + 1) / 2; }
+
+// column c < gram_tcols(Dt) of the upper triangle -> (pt, pt')
+__device__ __forceinline__ void gram_tpair(int c, int Dt, int &pt, int &ptp)
+{
+    int p = 0;
+    while (c >= Dt - p) {
+        c -= Dt - p;
+        ++p;
+    }
+    pt = p;
+    ptp = p + c;
+}
+
+// column tiles of 16 for a temporal order Dt (the kernels' template buckets)
+__host__ __device__ __forceinline__ int gram_nct(int Dt)
+{
+    const int nct = (gram_tcols(Dt) + 15) / 16;
+    return nct <= 1 ? 1 : (nct <= 3 ? 3 : (nct <= 6 ? 6 : (nct <= 10 ? 10 : 0)));
+}
 
 #define BP 8
 #define KC 16
@@ -39,6 +65,7 @@ gram_f64_kernel(const Gram64Desc *__restrict__ descs, Gram64Desc d0)
     __shared__ double S1[KC][BP];
     __shared__ double S2[KC][BP];
     __shared__ double Tt[KC][TLD];
+    __shared__ int toff[NC];              // column -> offset pt * Dt + pt' in T_j, -1 past the end
 
     const Gram64Desc d = descs ? descs[blockIdx.y] : d0;
     const double *__restrict__ S = d.S;
@@ -56,13 +83,19 @@ gram_f64_kernel(const Gram64Desc *__restrict__ descs, Gram64Desc d0)
     const int b2 = pid - b1 * (b1 + 1) / 2;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int DT2 = Dt * Dt;
+    const int DT2 = Dt * Dt, DTU = gram_tcols(Dt);
+    for (int c = tid; c < NC; c += 256) {
+        int pt = 0, ptp = 0;
+        if (c < DTU) gram_tpair(c, Dt, pt, ptp);
+        toff[c] = c < DTU ? pt * Dt + ptp : -1;
+    }
     v4d acc[NCT];
 #pragma unroll
     for (int c = 0; c < NCT; ++c) acc[c] = (v4d){0, 0, 0, 0};
 
     const int r = wave * 16 + (lane & 15);   // row of this lane's A operand
     const int i1 = r >> 3, i2 = r & 7;
+    __syncthreads();
 
     for (int j0 = 0; j0 < N; j0 += KC) {
         // stage S tiles
@@ -72,11 +105,12 @@ gram_f64_kernel(const Gram64Desc *__restrict__ descs, Gram64Desc d0)
             S1[kk][i] = (j < N && p1 < Ds) ? S[(long long)j * lds + p1] : 0.0;
             S2[kk][i] = (j < N && p2 < Ds) ? S[(long long)j * lds + p2] : 0.0;
         }
-        // stage T tile (contiguous KC*Dt^2 doubles in global)
+        // stage the upper-triangle columns of the T tile
         const double *Tg = Tm + (long long)j0 * DT2;
         for (int e = tid; e < KC * NC; e += 256) {
             int kk = e / NC, c = e % NC;
-            Tt[kk][c] = (c < DT2 && j0 + kk < N) ? Tg[(long long)kk * DT2 + c] : 0.0;
+            const int o = toff[c];
+            Tt[kk][c] = (o >= 0 && j0 + kk < N) ? Tg[(long long)kk * DT2 + o] : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -94,14 +128,16 @@ gram_f64_kernel(const Gram64Desc *__restrict__ descs, Gram64Desc d0)
 #pragma unroll
     for (int c = 0; c < NCT; ++c) {
         const int col = c * 16 + mfma_col(lane);
-        if (col >= DT2) continue;
-        const int pt = col / Dt, ptp = col % Dt;
+        if (col >= DTU) continue;
+        const int pt = toff[col] / Dt, ptp = toff[col] % Dt;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int rr = wave * 16 + mfma_row_f64(lane, q);
             const int ps = b1 * BP + (rr >> 3), psp = b2 * BP + (rr & 7);
-            if (ps < Ds && psp < Ds)
+            if (ps < Ds && psp < Ds) {
                 A[(long long)(ps * Dt + pt) * lda + psp * Dt + ptp] = acc[c][q];
+                if (ptp != pt) A[(long long)(ps * Dt + ptp) * lda + psp * Dt + pt] = acc[c][q];
+            }
         }
     }
 }
@@ -111,10 +147,11 @@ static int launch_gram_f64(const Gram64Desc *descs, const Gram64Desc &d0, int ba
 {
     const int nbp = (max_Ds + BP - 1) / BP;
     dim3 grid(nbp * (nbp + 1) / 2, batch);
-    const int nct = (Dt * Dt + 15) / 16;
-    if (nct <= 2) gram_f64_kernel<2><<<grid, 256, 0, s>>>(descs, d0);
-    else if (nct <= 6) gram_f64_kernel<6><<<grid, 256, 0, s>>>(descs, d0);
-    else if (nct <= 19) gram_f64_kernel<19><<<grid, 256, 0, s>>>(descs, d0);
+    const int nct = gram_nct(Dt);
+    if (nct == 1) gram_f64_kernel<1><<<grid, 256, 0, s>>>(descs, d0);
+    else if (nct == 3) gram_f64_kernel<3><<<grid, 256, 0, s>>>(descs, d0);
+    else if (nct == 6) gram_f64_kernel<6><<<grid, 256, 0, s>>>(descs, d0);
+    else if (nct == 10) gram_f64_kernel<10><<<grid, 256, 0, s>>>(descs, d0);
     else return VT_ERANGE;
     VT_LAUNCH_CHECK();
     return VT_OK;
@@ -177,8 +214,9 @@ VT_EXPORT int vt_symmetrize(void *A, int dtype, long long lda, int D, void *stre
 // FP32 registers and split per k-step.  The LDS stage is double-buffered: chunk k+1 is
 // copied while chunk k feeds the MFMAs, one barrier per chunk.
 // A workgroup owns a 16 x 16 block of spatial pairs (256 pair rows, so T_j is streamed
-// once per 256 rows) and all Dt^2 columns; its 8 waves form a 4 (64-row groups) x 2
-// (column halves) grid, so every B operand read from LDS feeds 4 row tiles.
+// once per 256 rows) and the Dt (Dt + 1) / 2 temporal columns; its 8 waves form a 4
+// (64-row groups) x 2 (column halves) grid, so every B operand read from LDS feeds 4 row
+// tiles.
 // ------------------------------------------------------------------------------------
 struct GramDesc {
     const double *S;      // N x Ds (row stride lds)
@@ -194,12 +232,6 @@ struct GramDesc {
 #define LP_BP 16          // spatial indices per block: a workgroup owns 16 x 16 = 256 pair rows
 #define LP_WAVES 8        // 4 row groups (64 pair rows each) x 2 column groups
 
-// columns of the tiled T layout for a temporal order Dt (the Gram kernels' column tiles)
-__host__ __device__ __forceinline__ int gram_lp_nct(int Dt)
-{
-    const int nct = (Dt * Dt + 15) / 16;
-    return nct <= 2 ? 2 : (nct <= 6 ? 6 : (nct <= 19 ? 19 : 0));
-}
 
 template <int NCT, int MODE>
 __global__ void __launch_bounds__(64 * LP_WAVES)
@@ -226,7 +258,7 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rg = wave & 3, cg = wave >> 2;            // row group (64 rows), column group
     const int c0 = cg * NCT2;                            // first column tile of the group
-    const int Dt = d.Dt, DT2 = Dt * Dt, N = d.N;
+    const int Dt = d.Dt, N = d.N;
     const float sW = MODE == 1 ? d.scales[1] : 1.0f;
     const int g = lane >> 4;
     const int nsteps = (N + KL - 1) / KL;
@@ -344,20 +376,25 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
         __syncthreads();
     }
     const float unscale = MODE == 1 ? d.scales[2] : 1.0f;
+    const int DTU = gram_tcols(Dt);
 #pragma unroll
     for (int c = 0; c < NCT2; ++c) {
         if (c0 + c >= NCT) break;
         const int col = (c0 + c) * 16 + mfma_col(lane);
-        if (col >= DT2) continue;
-        const int pt = col / Dt, ptp = col % Dt;
+        if (col >= DTU) continue;
+        int pt, ptp;
+        gram_tpair(col, Dt, pt, ptp);
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int rr = rg * 64 + t * 16 + mfma_row(lane, q);
                 const int ps = b1 * LP_BP + (rr >> 4), psp = b2 * LP_BP + (rr & 15);
-                if (ps < d.Ds && psp < d.Ds)
-                    d.A[(long long)(ps * Dt + pt) * d.lda + psp * Dt + ptp] = acc[t][c][q] * unscale;
+                if (ps < d.Ds && psp < d.Ds) {
+                    const float v = acc[t][c][q] * unscale;
+                    d.A[(long long)(ps * Dt + pt) * d.lda + psp * Dt + ptp] = v;
+                    if (ptp != pt) d.A[(long long)(ps * Dt + ptp) * d.lda + psp * Dt + pt] = v;
+                }
             }
     }
 }
@@ -365,10 +402,20 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
 // Split T_j (N x Dt^2 doubles) into the tiled 16-bit layout of the Gram kernels: chunk k of
 // 32 neurons is a contiguous [NC][KP] tile (column-major within the chunk, rows padded from
 // 32 to KP elements with zeros, as the kernels keep it in LDS), so the kernels copy it to
-// LDS with 16-byte direct loads.  mode 0: hi / lo bf16; mode 1: fp16 of T*scales[0].
-__global__ void split_T_kernel(const double *__restrict__ T, int N, int Dt, int NC, long long total,
-                               unsigned short *hi, unsigned short *lo, int mode, const float *scales)
+// LDS with 16-byte direct loads.  Tile column c holds T_j[pt][pt'] for the c-th upper-
+// triangle pair.  mode 0: hi / lo bf16; mode 1: fp16 of T*scales[0].
+__global__ void __launch_bounds__(256)
+split_T_kernel(const double *__restrict__ T, int N, int Dt, int NC, long long total,
+               unsigned short *hi, unsigned short *lo, int mode, const float *scales)
 {
+    __shared__ int toff[256];
+    const int DTU = gram_tcols(Dt);
+    for (int c = threadIdx.x; c < NC; c += 256) {
+        int pt = 0, ptp = 0;
+        if (c < DTU) gram_tpair(c, Dt, pt, ptp);
+        toff[c] = c < DTU ? pt * Dt + ptp : -1;
+    }
+    __syncthreads();
     const float s = mode == 1 ? scales[0] : 1.0f;
     const int DT2 = Dt * Dt;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
@@ -377,7 +424,8 @@ __global__ void split_T_kernel(const double *__restrict__ T, int N, int Dt, int 
         const long long r = i / KP;
         const int c = (int)(r % NC);
         const int j = (int)(r / NC) * KL + kk;
-        const float t = (kk < KL && j < N && c < DT2) ? (float)T[(long long)j * DT2 + c] : 0.f;
+        const int o = toff[c];
+        const float t = (kk < KL && j < N && o >= 0) ? (float)T[(long long)j * DT2 + o] : 0.f;
         if (mode == 0) {
             const __bf16 h = (__bf16)t;
             const __bf16 l = (__bf16)(t - (float)h);
@@ -436,7 +484,7 @@ __global__ void zero_bits_kernel(unsigned int *bits) { bits[0] = 0u; bits[1] = 0
 VT_EXPORT int vt_gram_split_T_size(int N, int Dt, long long *elems_host)
 {
     VT_CHECK_ARG(N >= 0 && Dt >= 1 && elems_host);
-    const int nct = gram_lp_nct(Dt);
+    const int nct = gram_nct(Dt);
     if (!nct) return VT_ERANGE;
     *elems_host = (long long)((N + KL - 1) / KL) * nct * 16 * KP;
     return VT_OK;
@@ -447,7 +495,7 @@ VT_EXPORT int vt_gram_split_T(const double *Tm, int N, int Dt, void *hi, void *l
 {
     VT_CHECK_ARG(Tm && hi && N >= 1 && Dt >= 1 && (mode == 0 || mode == 1));
     VT_CHECK_ARG(mode == 1 ? (S && scales && nS > 0) : (lo != nullptr));
-    const int nct = gram_lp_nct(Dt);
+    const int nct = gram_nct(Dt);
     if (!nct) return VT_ERANGE;
     hipStream_t s = (hipStream_t)stream;
     if (mode == 1) {
@@ -473,7 +521,7 @@ VT_EXPORT int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int m
                  (mode == 0 || mode == 1));
     const int nbp = (max_Ds + LP_BP - 1) / LP_BP;
     dim3 grid(nbp * (nbp + 1) / 2, batch);
-    const int nct = gram_lp_nct(Dt);
+    const int nct = gram_nct(Dt);
     hipStream_t s = (hipStream_t)stream;
     const GramDesc *dd = (const GramDesc *)descs;
 #define LAUNCH_LP(NCTV)                                                                       \
@@ -481,9 +529,10 @@ VT_EXPORT int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int m
         if (mode == 0) gram_lp_kernel<NCTV, 0><<<grid, 64 * LP_WAVES, 0, s>>>(dd);            \
         else gram_lp_kernel<NCTV, 1><<<grid, 64 * LP_WAVES, 0, s>>>(dd);                      \
     } while (0)
-    if (nct == 2) LAUNCH_LP(2);
+    if (nct == 1) LAUNCH_LP(1);
+    else if (nct == 3) LAUNCH_LP(3);
     else if (nct == 6) LAUNCH_LP(6);
-    else if (nct == 19) LAUNCH_LP(19);
+    else if (nct == 10) LAUNCH_LP(10);
     else return VT_ERANGE;
 #undef LAUNCH_LP
     VT_LAUNCH_CHECK();
