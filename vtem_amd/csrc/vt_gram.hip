@@ -214,9 +214,8 @@ VT_EXPORT int vt_symmetrize(void *A, int dtype, long long lda, int D, void *stre
 // FP32 registers and split per k-step.  The LDS stage is double-buffered: chunk k+1 is
 // copied while chunk k feeds the MFMAs, one barrier per chunk.
 // A workgroup owns a 16 x 16 block of spatial pairs (256 pair rows, so T_j is streamed
-// once per 256 rows) and the Dt (Dt + 1) / 2 temporal columns; its 8 waves form a 4
-// (64-row groups) x 2 (column halves) grid, so every B operand read from LDS feeds 4 row
-// tiles.
+// once per 256 rows) and the Dt (Dt + 1) / 2 temporal columns, split over its 8 waves by
+// rows (split-BF16: 8 x 32 rows) or by rows and column halves (FP16: 4 x 64 rows x 2).
 // ------------------------------------------------------------------------------------
 struct GramDesc {
     const double *S;      // N x Ds (row stride lds)
@@ -230,7 +229,8 @@ struct GramDesc {
 #define KL 32
 #define KP (KL + 8)           // padded k stride (16-bit elements) of a T tile row, in HBM and LDS
 #define LP_BP 16          // spatial indices per block: a workgroup owns 16 x 16 = 256 pair rows
-#define LP_WAVES 8        // 4 row groups (64 pair rows each) x 2 column groups
+
+#define LP_WAVES 8        // waves per workgroup
 
 
 template <int NCT, int MODE>
@@ -238,8 +238,12 @@ __global__ void __launch_bounds__(64 * LP_WAVES)
 gram_lp_kernel(const GramDesc *__restrict__ descs)
 {
     constexpr int NC = NCT * 16;
-    constexpr int NCT2 = (NCT + 1) / 2;      // column tiles per column group
-    constexpr int NV = NC * KL / 8;          // 16-byte vectors per T tile
+    // Row tiles per wave.  Split-BF16: 2 (8 row groups, every wave all columns), so each W
+    // element is formed once per workgroup (its hi/lo split is the costly part); FP16: 4
+    // (4 row groups x 2 column halves), so each B fragment read from LDS feeds 4 row tiles.
+    constexpr int RT = MODE == 0 ? 2 : 4;
+    constexpr int NCG = RT == 4 ? 2 : 1;     // column groups
+    constexpr int NCT2 = (NCT + NCG - 1) / NCG;  // column tiles per column group
     // rows padded to LP_BP + 2 floats: the four 16-lane groups of a wave read rows 8 apart,
     // which a 16-float stride maps onto the same banks
     __shared__ float S1[2][KL][LP_BP + 2];
@@ -256,7 +260,8 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
     while (b1 * (b1 + 1) / 2 > pid) --b1;
     const int b2 = pid - b1 * (b1 + 1) / 2;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int rg = wave & 3, cg = wave >> 2;            // row group (64 rows), column group
+    const int rg = RT == 4 ? (wave & 3) : wave;         // row group (16 RT rows)
+    const int cg = RT == 4 ? (wave >> 2) : 0;           // column group
     const int c0 = cg * NCT2;                            // first column tile of the group
     const int Dt = d.Dt, N = d.N;
     const float sW = MODE == 1 ? d.scales[1] : 1.0f;
@@ -306,9 +311,9 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
         }
     };
 
-    v4f acc[4][NCT2];
+    v4f acc[RT][NCT2];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < RT; ++t)
 #pragma unroll
         for (int c = 0; c < NCT2; ++c) acc[t][c] = (v4f){0, 0, 0, 0};
 
@@ -325,10 +330,10 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
         }
         // A operands: W = S1 S2 for this wave's 4 row tiles, neurons 8g..8g+7 of the chunk
         if (MODE == 0) {
-            v8bf wh[4], wl[4];
+            v8bf wh[RT], wl[RT];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int r = rg * 64 + t * 16 + (lane & 15);
+            for (int t = 0; t < RT; ++t) {
+                const int r = rg * 16 * RT + t * 16 + (lane & 15);
                 const int i1 = r >> 4, i2 = r & 15;
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
@@ -345,17 +350,17 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
                 const v8bf bh = *(const v8bf *)&Th[bf][cc][8 * g];
                 const v8bf bl = *(const v8bf *)&Tl[bf][cc][8 * g];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
+                for (int t = 0; t < RT; ++t) {
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[t], bh, acc[t][c], 0, 0, 0);
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[t], bl, acc[t][c], 0, 0, 0);
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[t], bh, acc[t][c], 0, 0, 0);
                 }
             }
         } else {
-            v8h wh[4];
+            v8h wh[RT];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int r = rg * 64 + t * 16 + (lane & 15);
+            for (int t = 0; t < RT; ++t) {
+                const int r = rg * 16 * RT + t * 16 + (lane & 15);
                 const int i1 = r >> 4, i2 = r & 15;
 #pragma unroll
                 for (int i = 0; i < 8; ++i)
@@ -367,7 +372,7 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
                 const int cc = (c0 + c) * 16 + (lane & 15);
                 const v8h bh = *(const v8h *)&Th[bf][cc][8 * g];
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
+                for (int t = 0; t < RT; ++t)
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wh[t], bh, acc[t][c], 0, 0, 0);
             }
         }
@@ -385,10 +390,10 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
         int pt, ptp;
         gram_tpair(col, Dt, pt, ptp);
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < RT; ++t)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int rr = rg * 64 + t * 16 + mfma_row(lane, q);
+                const int rr = rg * 16 * RT + t * 16 + mfma_row(lane, q);
                 const int ps = b1 * LP_BP + (rr >> 4), psp = b2 * LP_BP + (rr & 15);
                 if (ps < d.Ds && psp < d.Ds) {
                     const float v = acc[t][c][q] * unscale;
