@@ -620,6 +620,39 @@ def test_kpcg_batch(torch_mod, oracle):
                                       err_msg=f"block {i} {specs[i]}")
 
 
+def test_kpcg_retry_guard(torch_mod, oracle):
+    """With too few CG iterations every block misses the 1e-10 guard: the guard solves each
+    again with retry_iters iterations (no FP64 redo), and the result equals that of a solve
+    run with retry_iters from the start, bit for bit; with the retry switched off the same
+    blocks go to the batched FP64 path and equal its direct result."""
+    torch = torch_mod
+    from vtem_amd import synthetic as syn
+    from vtem_amd.batch import BatchDecoder, BlockJob
+    _banner("kpcg retry guard")
+    specs = [(32, 1.0), (48, 0.5), (48, 1.0)]
+    blocks = [syn.serve_block(s, d, seed=900 + i) for i, (s, d) in enumerate(specs)]
+    jobs = []
+    for blk in blocks:
+        geom, st, dec = _encode_block(torch, blk)
+        jobs.append(BlockJob(dec, st))
+    bd = BatchDecoder(precision="kpcg", cg_iters=2, retry_iters=64)
+    res = bd.decode(jobs)
+    assert bd.last_stats["kpcg_retries"] == len(jobs) and bd.last_stats["fallbacks"] == 0
+    ref = BatchDecoder(precision="kpcg", cg_iters=64).decode(jobs)
+    for i, (r, q) in enumerate(zip(res, ref)):
+        assert not r.fallback
+        np.testing.assert_array_equal(r.frames.cpu().numpy(), q.frames.cpu().numpy(),
+                                      err_msg=f"block {i} {specs[i]}")
+    bd0 = BatchDecoder(precision="kpcg", cg_iters=2, retry_iters=2)
+    res0 = bd0.decode(jobs)
+    assert bd0.last_stats["kpcg_retries"] == 0 and bd0.last_stats["fallbacks"] == len(jobs)
+    f64 = BatchDecoder(precision="fp64").decode(jobs)
+    for i, (r, q) in enumerate(zip(res0, f64)):
+        assert r.fallback
+        np.testing.assert_array_equal(r.frames.cpu().numpy(), q.frames.cpu().numpy(),
+                                      err_msg=f"block {i} {specs[i]}")
+
+
 def test_rnn_batch(torch_mod, oracle):
     """Batched gradient-flow decoder (blocks of different temporal orders in one joint
     launch sequence) against the per-block path and the oracle's coefficient-space RNN
