@@ -75,9 +75,11 @@ VT_EXPORT int vt_temporal_factors(double *alpha, const double *tk1, const double
     return VT_OK;
 }
 
-// One 64-lane workgroup per neuron: intervals in chunks of 64, alpha staged in LDS, each
-// lane owns a strided set of (pt, pt') entries of T_j.  Batched over independent problems
-// through per-problem descriptors (offsets into shared arrays).
+// One wave per neuron, TJ_NPW neurons per workgroup: intervals in chunks of 64, alpha
+// staged in LDS, each lane owns a strided set of (pt, pt') entries of T_j.  All waves of a
+// workgroup run the largest chunk count among its neurons (a wave past its own intervals
+// adds nothing), so every barrier is reached by the whole workgroup.  Batched over
+// independent problems through per-problem descriptors (offsets into shared arrays).
 struct TjDesc {
     int neuron_base;      // first neuron of this problem in the global neuron numbering
     int num_neurons;
@@ -87,23 +89,28 @@ struct TjDesc {
     int T_base;           // offset (in neurons) into T/t outputs
 };
 
-__global__ void __launch_bounds__(64)
+#define TJ_NPW 4          // neurons (waves) per workgroup
+
+__global__ void __launch_bounds__(64 * TJ_NPW)
 neuron_T_kernel(double *__restrict__ Tm, double *__restrict__ tv, const double *__restrict__ tk1,
                 const double *__restrict__ tk2, const double *__restrict__ q,
                 const double *__restrict__ norm, const int *__restrict__ offsets,
                 const TjDesc *__restrict__ descs, TjDesc d0)
 {
-    __shared__ double al[64][66];
-    __shared__ double qs[64];
-    // locate problem: blockIdx.y = problem, blockIdx.x = neuron within problem
+    __shared__ double al[TJ_NPW][64][18];    // Dt <= 17
+    __shared__ double qs[TJ_NPW][64];
+    __shared__ int trips[TJ_NPW];
+    // locate problem: blockIdx.y = problem, blockIdx.x = group of TJ_NPW neurons
     const TjDesc d = descs ? descs[blockIdx.y] : d0;
-    const int jl = blockIdx.x;
-    if (jl >= d.num_neurons) return;
+    if (blockIdx.x * TJ_NPW >= d.num_neurons) return;        // uniform over the workgroup
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int jl = blockIdx.x * TJ_NPW + wave;
+    const bool valid = jl < d.num_neurons;
     const int j = d.neuron_base + jl;
     const int Dt = 2 * d.Lt + 1;
-    const int lane = threadIdx.x;
-    const int k0 = offsets[j], k1 = offsets[j + 1];
-    const double w = norm ? norm[j] : 1.0;
+    const int k0 = valid ? offsets[j] : 0, k1 = valid ? offsets[j + 1] : 0;
+    const double w = (norm && valid) ? norm[j] : 1.0;
+    if (lane == 0) trips[wave] = (k1 - k0 + 63) / 64;
     // T_j is symmetric: each lane sums the upper-triangle entries u = lane, lane + 64, ...
     // (p <= pp; at most 3 for Dt <= 17) and writes both (p, pp) and (pp, p).  The product
     // al[r][p] * al[r][pp] and the order over r are those of the mirrored entry.
@@ -118,34 +125,43 @@ neuron_T_kernel(double *__restrict__ Tm, double *__restrict__ tv, const double *
     }
     double accT[3] = {0, 0, 0};
     double acct = 0.0;
-    for (int base = k0; base < k1; base += 64) {
+    __syncthreads();
+    int nmax = 0;
+#pragma unroll
+    for (int v = 0; v < TJ_NPW; ++v) nmax = max(nmax, trips[v]);
+    double (*a)[18] = al[wave];
+    for (int it = 0; it < nmax; ++it) {
+        const int base = k0 + 64 * it;
         const int k = base + lane;
         if (k < k1) {
             const long long kk = d.t_base + k;
-            temporal_factor(tk1[kk], tk2[kk], d.Lt, d.W, w, &al[lane][0]);
-            qs[lane] = q[kk];
+            temporal_factor(tk1[kk], tk2[kk], d.Lt, d.W, w, &a[lane][0]);
+            qs[wave][lane] = q[kk];
         } else {
-            for (int p = 0; p < Dt; ++p) al[lane][p] = 0.0;
-            qs[lane] = 0.0;
+            for (int p = 0; p < Dt; ++p) a[lane][p] = 0.0;
+            qs[wave][lane] = 0.0;
         }
         __syncthreads();
         const int cnt = min(64, k1 - base);
+        if (cnt > 0) {
 #pragma unroll
-        for (int e = 0; e < 3; ++e) {
-            if (lane + 64 * e < DTU) {
-                const int p = tp[e], pp = tq[e];
-                double s = 0.0;
-                for (int r = 0; r < cnt; ++r) s += al[r][p] * al[r][pp];
-                accT[e] += s;
+            for (int e = 0; e < 3; ++e) {
+                if (lane + 64 * e < DTU) {
+                    const int p = tp[e], pp = tq[e];
+                    double s = 0.0;
+                    for (int r = 0; r < cnt; ++r) s += a[r][p] * a[r][pp];
+                    accT[e] += s;
+                }
             }
-        }
-        if (lane < Dt) {
-            double s = 0.0;
-            for (int r = 0; r < cnt; ++r) s += al[r][lane] * qs[r];
-            acct += s;
+            if (lane < Dt) {
+                double s = 0.0;
+                for (int r = 0; r < cnt; ++r) s += a[r][lane] * qs[wave][r];
+                acct += s;
+            }
         }
         __syncthreads();
     }
+    if (!valid) return;
     const long long ob = (long long)(d.T_base + jl);
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
@@ -166,7 +182,8 @@ VT_EXPORT int vt_neuron_T(double *Tm, double *tv, const double *tk1, const doubl
     VT_CHECK_ARG(Tm && tv && tk1 && tk2 && q && offsets && N >= 1 && Lt >= 0 && Lt <= 8 && W > 0);
     TjDesc d{0, N, 0, Lt, W, 0};
     hipStream_t s = (hipStream_t)stream;
-    neuron_T_kernel<<<dim3(N, 1), 64, 0, s>>>(Tm, tv, tk1, tk2, q, norm, offsets, nullptr, d);
+    neuron_T_kernel<<<dim3((N + TJ_NPW - 1) / TJ_NPW, 1), 64 * TJ_NPW, 0, s>>>(Tm, tv, tk1, tk2, q, norm,
+                                                                            offsets, nullptr, d);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
@@ -181,8 +198,9 @@ VT_EXPORT int vt_neuron_T_batched(double *Tm, double *tv, const double *tk1, con
     VT_CHECK_ARG(Tm && tv && tk1 && tk2 && q && offsets && descs_dev && n_desc >= 1 &&
                  max_neurons >= 1);
     TjDesc d0{0, 0, 0, 0, 1.0, 0};
-    neuron_T_kernel<<<dim3(max_neurons, n_desc), 64, 0, (hipStream_t)stream>>>(
-        Tm, tv, tk1, tk2, q, norm, offsets, (const TjDesc *)descs_dev, d0);
+    neuron_T_kernel<<<dim3((max_neurons + TJ_NPW - 1) / TJ_NPW, n_desc), 64 * TJ_NPW, 0,
+                      (hipStream_t)stream>>>(Tm, tv, tk1, tk2, q, norm, offsets,
+                                             (const TjDesc *)descs_dev, d0);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
