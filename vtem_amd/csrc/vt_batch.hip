@@ -66,15 +66,24 @@ __global__ void batch_interval_write(const SpikeDesc *__restrict__ sd, const int
     }
 }
 
+// Exclusive scan of n counts in one workgroup: thread t owns a contiguous chunk (a multiple
+// of 4 elements, so its reads are 16-byte vectors), the chunk sums are scanned in LDS, then
+// each thread writes its chunk's offsets.
 __global__ void __launch_bounds__(1024) batch_scan_kernel(const int *__restrict__ in, int n,
                                                          int *__restrict__ offsets)
 {
     __shared__ long long part[1024];
     const int t = threadIdx.x;
-    const int chunk = (n + 1023) / 1024;
-    const int b = t * chunk, e = min(n, b + chunk);
+    const int chunk = ((n + 1023) / 1024 + 3) & ~3;
+    const int b = min(n, t * chunk), e = min(n, b + chunk);
+    typedef int i4a4 __attribute__((ext_vector_type(4), aligned(4)));   // 4-byte aligned
     long long s = 0;
-    for (int i = b; i < e; ++i) s += in[i];
+    int i = b;
+    for (; i + 4 <= e; i += 4) {
+        const i4a4 v = *(const i4a4 *)(in + i);
+        s += (long long)v[0] + v[1] + v[2] + v[3];
+    }
+    for (; i < e; ++i) s += in[i];
     part[t] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -84,7 +93,21 @@ __global__ void __launch_bounds__(1024) batch_scan_kernel(const int *__restrict_
         __syncthreads();
     }
     long long run = t > 0 ? part[t - 1] : 0;
-    for (int i = b; i < e; ++i) {
+    i = b;
+    for (; i + 4 <= e; i += 4) {
+        const i4a4 v = *(const i4a4 *)(in + i);
+        i4a4 o;
+        o[0] = (int)run;
+        run += v[0];
+        o[1] = (int)run;
+        run += v[1];
+        o[2] = (int)run;
+        run += v[2];
+        o[3] = (int)run;
+        run += v[3];
+        *(i4a4 *)(offsets + i) = o;
+    }
+    for (; i < e; ++i) {
         offsets[i] = (int)run;
         run += in[i];
     }
