@@ -28,7 +28,7 @@ from __future__ import annotations
 import ctypes
 import math
 import os
-from typing import List, Sequence, Tuple
+from typing import Sequence, Tuple
 
 import numpy as np
 import scipy.linalg as sla

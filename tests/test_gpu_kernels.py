@@ -3,7 +3,6 @@
 Integer/index outputs must match exactly; floating-point tolerances are written per test.
 """
 
-import math
 
 import numpy as np
 import pytest

@@ -2,7 +2,6 @@
 policy, configuration laws."""
 
 import numpy as np
-import pytest
 
 from vtem_amd import synthetic as syn
 

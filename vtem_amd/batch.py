@@ -28,7 +28,7 @@ import numpy as np
 import torch
 
 from ._lib import lib, ptr, stream_ptr
-from .block import (F64, I32, Decoder, DecodeResult, SpikeTrains, gram_descs, mat_descs,
+from .block import (F64, I32, Decoder, DecodeResult, SpikeTrains, mat_descs,
                     split_T_size)
 
 SPIKE_DTYPE = np.dtype([("spikes", np.uint64), ("counts", np.uint64), ("ld", np.int64),

@@ -16,7 +16,7 @@ does (vtem/ensemble_encode.py:64-65, :214-219); pass rng= for reproducible draws
 from __future__ import annotations
 
 import math
-from typing import List, Optional, Tuple
+from typing import List, Tuple
 
 import numpy as np
 import torch

@@ -21,7 +21,6 @@ reference's tiling of decode_video and windowing of VTDM (vtem/vtdm.py:179-309,
 from __future__ import annotations
 
 import dataclasses
-import math
 from typing import List, Optional, Sequence
 
 import numpy as np

@@ -20,7 +20,6 @@ translations on a (2t+1)^2 grid of spacing b0 alpha0^m b0scalebase^-m.
 from __future__ import annotations
 
 import math
-from typing import Optional
 
 import numpy as np
 import torch

@@ -15,11 +15,9 @@ archive when output_filename is given).
 from __future__ import annotations
 
 import dataclasses
-import math
-from typing import Any, List, Optional
+from typing import Any, List
 
 import numpy as np
-import torch
 
 from . import ensemble_encode as ee
 from . import vrf as vrf_mod
