@@ -452,7 +452,7 @@ VT_EXPORT int vt_iaf_encode_trig_par(const double *coef, long long coef_ld, cons
 {
     IafParDesc d{coef, coef_ld, 1, basis, nullptr, 0, spikes, spike_ld, v0, time_count, kappa,
                  bias, delta, spike_count, (double2 *)ws, nullptr, dt, N, size, Dt, max_spike,
-                 root_mode, 0};
+                 root_mode, 0, nullptr, 0};
     int r = iafp_check(d);
     if (r || N == 0) return r;
     VT_CHECK_ARG(coef && basis && Dt >= 1 && coef_ld >= N);
@@ -470,7 +470,7 @@ VT_EXPORT int vt_iaf_encode_par(const void *input, int in_dtype, long long input
 {
     IafParDesc d{nullptr, 0, 1, nullptr, input, input_ld, spikes, spike_ld, v0, time_count,
                  kappa, bias, delta, spike_count, (double2 *)ws, nullptr, dt, N, size, 0,
-                 max_spike, root_mode, in_dtype};
+                 max_spike, root_mode, in_dtype, nullptr, 0};
     int r = iafp_check(d);
     if (r || N == 0) return r;
     VT_CHECK_ARG((input || size == 0) && input_ld >= N && (in_dtype == 0 || in_dtype == 1));
