@@ -78,6 +78,8 @@ def parse():
     p.add_argument("--max-batch", type=int, default=32)
     p.add_argument("--streams", type=int, default=2)
     p.add_argument("--rnn-lanes", type=int, default=1, help="streams of the RNN lane")
+    p.add_argument("--graph-chunk", type=int, default=25,
+                   help="RNN / power iterations per replayed HIP graph (0 = launch each)")
     p.add_argument("--form", default="kpcg", help="solver form of the chosen served path")
     p.add_argument("--compare", default="bf16x3", help="second served form measured ('' = none)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -425,7 +427,7 @@ def main():
 
     def run_form(form):
         server = Server(max_batch=args.max_batch, streams=args.streams, precision=form,
-                        rnn_lanes=args.rnn_lanes,
+                        rnn_lanes=args.rnn_lanes, graph_chunk=args.graph_chunk,
                         adaptive=form != "kpcg")
         for _ in range(args.warmup):           # allocator, code objects, precision policy
             server.serve(make_requests(items, zeros, form))
@@ -468,7 +470,7 @@ def main():
     # roofline of the dominant stage (HIP events on the launch stream, one instrumented wave)
     # one stream, so that a stage's events bracket only its own kernels
     iserver = Server(max_batch=args.max_batch, streams=1, precision=args.form,
-                     adaptive=args.form != "kpcg")
+                     graph_chunk=args.graph_chunk, adaptive=args.form != "kpcg")
     iserver.policy = server.policy
     timers = iserver.serve(make_requests(items, zeros), instrument=True)["timers"]
     torch.cuda.synchronize()
@@ -503,10 +505,13 @@ def main():
                                "N=round(2D/9), Poisson arrivals above capacity",
                    "requests_per_step": R, "frames_per_step": frames, "D_min": int(D.min()),
                    "D_max": int(D.max()), "max_batch": args.max_batch, "streams": args.streams,
-                              "rnn_lanes": args.rnn_lanes},
+                              "rnn_lanes": args.rnn_lanes, "graph_chunk": args.graph_chunk},
         "chosen": {"form": args.form, "frames_per_s": round(value, 2),
                    "p50_ms": round(stats[-1]["p50_ms"], 3), "p99_ms": round(stats[-1]["p99_ms"], 3),
-                   "batches": stats[-1]["batches"], "fp64_fallbacks": stats[-1]["fallbacks"]},
+                   "batches": stats[-1]["batches"], "fp64_fallbacks": stats[-1]["fallbacks"],
+                   "host_launch_ms": round(stats[-1]["host_launch_s"] * 1e3, 2),
+                   "host_finalize_ms": round(stats[-1]["host_finalize_s"] * 1e3, 2),
+                   "wave_ms": round(stats[-1]["wall_s"] * 1e3, 2)},
         "plain": {"frames_per_s": round(plain["frames_per_s"], 2),
                   "p50_ms": round(plain["p50_ms"], 3), "p99_ms": round(plain["p99_ms"], 3)},
         "speedup_vs_plain": round(value / plain["frames_per_s"], 3),

@@ -698,6 +698,32 @@ def test_rnn_batch(torch_mod, oracle):
         assert abs(s_b - s_a) < 0.1
 
 
+def test_rnn_graph_replay(torch_mod):
+    """RNN and power loops replayed as HIP graphs of 8 iterations (61 and 50 iterations:
+    replays plus a directly launched remainder) give bit-identical iterates to the
+    launch-by-launch loop, on every one of three decodes with eager work between them."""
+    torch = torch_mod
+    from vtem_amd import synthetic as syn
+    from vtem_amd.batch import BatchDecoder, BlockJob
+    _banner("rnn graph replay")
+    specs = [(32, 0.5), (32, 2.0), (48, 1.0)]
+    jobs = []
+    for i, (sd, du) in enumerate(specs):
+        blk = syn.serve_block(sd, du, seed=720 + i)
+        geom, st, dec = _encode_block(torch, blk)
+        jobs.append(BlockJob(dec, st, method="rnn", rnn_steps=60))
+    ref = BatchDecoder(precision="kpcg", graph_chunk=0).decode(jobs)
+    bd = BatchDecoder(precision="kpcg", graph_chunk=8)
+    for rep in range(3):
+        res = bd.decode(jobs)
+        torch.cuda.synchronize()
+        for a, b in zip(ref, res):
+            assert torch.equal(a.X, b.X), f"replay {rep}"
+            assert torch.equal(a.state, b.state)
+        # eager work on the same stream between the graph launches
+        torch.ones(1 << 20, dtype=torch.float64, device="cuda").sum().item()
+
+
 def test_server_alone_equals_mix(torch_mod):
     """Requests served with Poisson arrivals on 2 streams (continuous batching) give the
     same frames as each request decoded alone with the same precision decision."""

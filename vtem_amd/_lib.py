@@ -93,7 +93,7 @@ SIGNATURES = {
     "vt_kpcg_step_a": [P, P, P, P, P, P, I, P, P, I, I, D, P],
     "vt_kpcg_step_b": [P, P, P, P, I, P, P],
     "vt_kpcg_iterate": [P, I, I, P, I, I, P, I, I, I, I, P, P, P, P, P, P, P, P, P, I, I, D, P],
-    "vt_kron_power": [P, I, I, P, I, I, I, I, P, P, P, P, I, P, P],
+    "vt_kron_power": [P, I, I, P, I, I, I, I, P, P, P, P, I, P, I, P],
     "vt_power_step": [P, P, P, P, I, P, P],
     # spatial Dirichlet side, pseudoinverse, stitching
     "vt_dirichlet_table": [P, L, I, I, D, D, D, D, I, I, D, D, P],
@@ -102,7 +102,7 @@ SIGNATURES = {
     "vt_eig_sym": [P, I, I, P, P, P, P],
     "vt_eig_filter": [P, P, I, D, P],
     "vt_stitch_accumulate": [P, L, L, P, I, L, P],
-    "vt_kron_rnn": [P, I, I, P, I, I, I, I, P, P, P, P, P, P, L, I, P],
+    "vt_kron_rnn": [P, I, I, P, I, I, I, I, P, P, P, P, P, P, L, I, I, P],
 }
 
 

@@ -165,11 +165,13 @@ class BatchDecoder:
     the non-FP64 forms that fail the guard (relative residual of the FP64 system above
     `tol`, or a factorisation that lost positive definiteness) are redone together as
     one batched FP64 group; kpcg blocks first get one more CG solve with retry_iters
-    iterations."""
+    iterations.  The RNN and power-iteration loops are replayed as HIP graphs of
+    graph_chunk iterations (0: every kernel launched by the host; same results)."""
 
     def __init__(self, device="cuda", precision: str = "bf16x3", lam_rel: float = 1e-6,
                  refine_iters: int = 4, tol: float = 1e-10, verify: bool = True,
-                 cg_iters: int = 32, cg_stop: float = 1e-12, retry_iters: Optional[int] = None):
+                 cg_iters: int = 32, cg_stop: float = 1e-12, retry_iters: Optional[int] = None,
+                 graph_chunk: int = 25):
         if precision not in PRECISIONS:
             raise ValueError(f"batched precision must be one of {PRECISIONS}")
         self.device = device
@@ -183,6 +185,8 @@ class BatchDecoder:
         # kpcg blocks that fail the guard are solved again with this many iterations
         # before the FP64 redo (CG that ran out of iterations on a harder system)
         self.retry_iters = 4 * cg_iters if retry_iters is None else retry_iters
+        # RNN and power loops: replay a HIP graph of this many iterations (0: launch each)
+        self.graph_chunk = graph_chunk
         self.last_stats: Dict[str, Any] = {}
         self.timers: Optional[Dict[str, list]] = None     # stage -> [(ev0, ev1, work)]
 
@@ -833,9 +837,9 @@ class BatchDecoder:
         eta = torch.empty(Bt, dtype=F64, device=dev)
         maxNc = int(Dt.max())
         lib.vt_kron_power(*fv.args(), *bv.args(), Bt, maxNc, ptr(V), ptr(Q), ptr(lam), ptr(seg), POWER_ITERS,
-                          ptr(eta), s)
+                          ptr(eta), self.graph_chunk, s)
         lib.vt_kron_rnn(*fx.args(), *bx.args(), Bt, maxNc, ptr(X), ptr(r), ptr(Q), ptr(lam), ptr(eta),
-                        ptr(elem_job), Dtot, steps, s)
+                        ptr(elem_job), Dtot, steps, self.graph_chunk, s)
         sv = st.view(Bt, 4)
         sv[:, 0] = eta
         sv[:, 3] = float(steps + 1)

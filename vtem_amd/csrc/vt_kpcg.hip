@@ -21,6 +21,7 @@
 // (vtem/utils/linalg.py:591-631) and has no iterative Tikhonov solver.
 #include "vt_common.hpp"
 #include "vt_skinny.hpp"
+#include "vt_graph.hpp"
 
 #define KP_CHUNKS 16
 #define KP_MAX_DT 33
@@ -457,41 +458,48 @@ VT_EXPORT int vt_power_step(double *v, const double *q, const double *lam, const
 }
 
 // eta (batch doubles) from `iters` power steps started at v (overwritten); the operator
-// launches read v and write q.
+// launches read v and write q.  graph_chunk >= 2: the steps are replayed as a HIP graph of
+// that many steps (vt_graph_loop); 0: launched one by one.  Same results either way.
 VT_EXPORT int vt_kron_power(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                             int batch, int max_Nc, double *v, double *q, const double *lam,
-                            const long long *seg, int iters, double *eta, void *stream)
+                            const long long *seg, int iters, double *eta, int graph_chunk,
+                            void *stream)
 {
-    VT_CHECK_ARG(fwd && bwd && v && q && lam && seg && eta && batch >= 1 && iters >= 1);
+    VT_CHECK_ARG(fwd && bwd && v && q && lam && seg && eta && batch >= 1 && iters >= 1 &&
+                 graph_chunk >= 0);
     const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
-    hipStream_t s = (hipStream_t)stream;
-    for (int it = 0; it < iters; ++it) {
-        VT_TRY(skinny_run(F, batch, max_Nc, stream));
-        VT_TRY(skinny_run(Bk, batch, max_Nc, stream));
-        power_step_kernel<<<batch, 256, 0, s>>>(v, q, lam, seg, eta);
+    auto step = [&](hipStream_t st) -> int {
+        VT_TRY(skinny_run(F, batch, max_Nc, st));
+        VT_TRY(skinny_run(Bk, batch, max_Nc, st));
+        power_step_kernel<<<batch, 256, 0, st>>>(v, q, lam, seg, eta);
         VT_LAUNCH_CHECK();
-    }
-    return VT_OK;
+        return VT_OK;
+    };
+    return vt_graph_loop((hipStream_t)stream, iters, graph_chunk, step);
 }
 
-// steps+1 gradient-flow iterations from x = 0 (the operator launches read x, write q)
+// steps+1 gradient-flow iterations from x = 0 (the operator launches read x, write q);
+// graph_chunk as for vt_kron_power.
 VT_EXPORT int vt_kron_rnn(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                           int batch, int max_Nc, double *x, const double *r, double *q,
                           const double *lam, const double *eta, const int *elem_job, long long n,
-                          int steps, void *stream)
+                          int steps, int graph_chunk, void *stream)
 {
-    VT_CHECK_ARG(fwd && bwd && x && r && q && lam && eta && elem_job && n >= 0 && steps >= 0);
+    VT_CHECK_ARG(fwd && bwd && x && r && q && lam && eta && elem_job && n >= 0 && steps >= 0 &&
+                 graph_chunk >= 0);
     const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
     hipStream_t s = (hipStream_t)stream;
     unsigned g = vt_blocks(n, 256);
     if (g > 4096) g = 4096;
     zero_kernel<<<g, 256, 0, s>>>(x, n);   // a kernel, not a memset node, if captured
-    for (int it = 0; it <= steps; ++it) {
-        VT_TRY(skinny_run(F, batch, max_Nc, stream));
-        if (Bk.max_M > 0) VT_TRY(vt_skinny_first_pass(Bk.descs, batch, Bk.max_M, Bk.max_K, max_Nc, s));
-        rnn_reduce_update_kernel<<<g, 256, 0, s>>>(x, r, q, (const SkDesc *)Bk.descs, lam, eta,
-                                                   elem_job, n);
+    VT_LAUNCH_CHECK();
+    auto step = [&](hipStream_t st) -> int {
+        VT_TRY(skinny_run(F, batch, max_Nc, st));
+        if (Bk.max_M > 0) VT_TRY(vt_skinny_first_pass(Bk.descs, batch, Bk.max_M, Bk.max_K, max_Nc, st));
+        rnn_reduce_update_kernel<<<g, 256, 0, st>>>(x, r, q, (const SkDesc *)Bk.descs, lam, eta,
+                                                    elem_job, n);
         VT_LAUNCH_CHECK();
-    }
-    return VT_OK;
+        return VT_OK;
+    };
+    return vt_graph_loop(s, steps + 1, graph_chunk, step);
 }

@@ -117,7 +117,8 @@ class PrecisionPolicy:
 class Server:
     def __init__(self, hbm_budget: int = 64 << 30, max_batch: int = 32, streams: int = 2,
                  precision: str = "bf16x3", verify: bool = True, adaptive: bool = True,
-                 rnn_lane: bool = True, rnn_max_batch: int = 64, rnn_lanes: int = 1):
+                 rnn_lane: bool = True, rnn_max_batch: int = 64, rnn_lanes: int = 1,
+                 graph_chunk: int = 25):
         self.hbm_budget = int(hbm_budget)
         self.max_batch = int(max_batch)
         self.n_streams = int(streams)
@@ -129,6 +130,8 @@ class Server:
         self.rnn_lanes = max(1, int(rnn_lanes))
         self.precision = precision
         self.verify = verify
+        # RNN and power loops replayed as HIP graphs of this many iterations (0: no graph)
+        self.graph_chunk = int(graph_chunk)
         self.policy = PrecisionPolicy(precision) if adaptive else None
 
     # ----------------------------------------------------------------------------------
@@ -141,7 +144,7 @@ class Server:
         for r in requests:
             if r.footprint > self.hbm_budget:
                 raise ValueError(f"request {r.rid} needs {r.footprint} B, above the HBM budget")
-        bd = BatchDecoder(precision=self.precision, verify=self.verify)
+        bd = BatchDecoder(precision=self.precision, verify=self.verify, graph_chunk=self.graph_chunk)
         bd.instrument(instrument)
         be = BatchEncoder()
         overflows = 0
@@ -159,6 +162,7 @@ class Server:
         in_use = 0
         nbatch = 0
         fallbacks = 0
+        host_launch = host_finalize = 0.0      # host seconds spent enqueueing / retiring
         torch.cuda.synchronize()
         t0 = clock()
         while pending or ready or ready_rnn or any(b is not None for b in busy):
@@ -170,6 +174,7 @@ class Server:
             for k, b in enumerate(busy):
                 if b is not None and b[0].query():
                     ev, h, reqs, nbytes, flags = b
+                    tf = clock()
                     with torch.cuda.stream(streams[k]):
                         res = bd.finalize(h)
                         if flags is not None:
@@ -185,6 +190,7 @@ class Server:
                         for i, (r, pr) in enumerate(zip(reqs, h.precisions)):
                             self.policy.record(r.job, pr, i in failed)
                     tdone = clock() - t0
+                    host_finalize += tdone + t0 - tf
                     for r, rr in zip(reqs, res):
                         r.result, r.done = rr, tdone
                     in_use -= nbytes
@@ -207,6 +213,7 @@ class Server:
                     h = bd.launch([r.job for r in batch], precs)
                     ev = torch.cuda.Event()
                     ev.record()
+                host_launch += clock() - t0 - tstart
                 for r in batch:
                     r.start, r.batch = tstart, nbatch
                 nbatch += 1
@@ -222,7 +229,8 @@ class Server:
                     time.sleep(min(nxt, 5e-3))
         torch.cuda.synchronize()
         wall = clock() - t0
-        extra = {"batches": nbatch, "fallbacks": fallbacks, "overflows": overflows}
+        extra = {"batches": nbatch, "fallbacks": fallbacks, "overflows": overflows,
+                 "host_launch_s": host_launch, "host_finalize_s": host_finalize}
         if instrument:
             extra["timers"] = bd.timers
         return self._stats(requests, wall, extra)
