@@ -206,6 +206,7 @@ def precision_report(seed=0):
     import torch
     from vtem_amd import synthetic as syn
     from vtem_amd.basis import Geometry
+    from vtem_amd.autotune import gram_tuner, tune_key
     from vtem_amd.batch import BatchDecoder, BlockJob
     from vtem_amd.block import Decoder, Encoder, dev, spatial_coeffs
     geom = Geometry(128, 128, 12, 12, 8, 2.0)
@@ -255,6 +256,12 @@ def precision_report(seed=0):
                        "peak_tflops": round(peak, 1),
                        "mfma_executed_tflops": round(exec_lp * (3 if mode == 0 else 1) / t / 1e9, 2),
                        "mfma_executed_frac": round(exec_lp * (3 if mode == 0 else 1) / t / 1e9 / 2500.0, 4)}
+        # the autotuner's tiling for this shape (row tiles per wave) and, when it was tuned
+        # in this process, the time of every tiling (ms, Gram kernel alone)
+        key = tune_key(name, geom.Dt, geom.Ds, 1, N)
+        forms[name]["tuned_tile"] = gram_tuner().choices.get(key)
+        if key in gram_tuner().timings:
+            forms[name]["tile_ms"] = {str(k): round(v, 3) for k, v in gram_tuner().timings[key].items()}
         del A
     del A64
     torch.cuda.empty_cache()

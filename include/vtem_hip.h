@@ -198,8 +198,11 @@ int vt_gram_split_T_size(int N, int Dt, long long *elems_host);
 /* Batched low-precision Gram (mode 0 split-BF16 3-product, mode 1 FP16) on the 16x16x32
  * MFMAs with fp32 accumulation into fp32 A (lower block triangle).  descs: device array of
  * {const double *S; const u16 *Th, *Tl; float *A; long long lds, lda; int N, Ds, Dt, pad;
- * const float *scales;} (72 bytes); all problems share Dt.  Same role as vt_gram_f64. */
-int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int mode, void *stream);
+ * const float *scales;} (72 bytes); all problems share Dt.  Same role as vt_gram_f64.
+ * tile: row tiles of 16 per wave (2, 4 or 8; 0 = 2 for split-BF16, 4 for FP16).  Every
+ * tiling gives a bit-identical A; vtem_amd/autotune.py picks the fastest per shape. */
+int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int mode, int tile,
+               void *stream);
 /* s2[j] = |S_j|^2 (once per neuron set). */
 int vt_row_sumsq(const double *S, long long lds, int N, int Ds, double *s2, void *stream);
 /* dst (cols x rows, row stride ldd) = src^T (rows x cols, row stride lds).  Makes the

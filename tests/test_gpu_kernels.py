@@ -400,6 +400,59 @@ def test_gram_lowprec(torch_mod, oracle, mode, tol):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gram_tilings_bit_identical(torch_mod, oracle, mode, tmp_path):
+    """The three tilings of the low-precision Gram (2, 4, 8 row tiles per wave) write the
+    same bits, for temporal orders filling 1, 3, 6 and 10 column tiles and a spatial order
+    that leaves partial 16-blocks; the autotuner's launch leaves the same Gram again, and
+    one tiling agrees with the FP64 oracle Gram."""
+    torch = torch_mod
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    from vtem_amd.autotune import GramTuner, TILES
+    from vtem_amd.block import dev, gram_descs, split_T_size
+    _banner(f"gram tilings mode={mode}")
+    rng = np.random.default_rng(23)
+    N, Ds = 150, 75
+    tuner = GramTuner(path=str(tmp_path / "tiles.json"), enabled=True, reps=1)
+    for Dt in (5, 9, 13, 17):
+        D = Ds * Dt
+        S = rng.standard_normal((N, Ds))
+        al = rng.standard_normal((2000, Dt)) * 0.1
+        nid = np.sort(rng.integers(0, N, 2000)).astype(np.int32)
+        Tm, _ = oracle.neuron_T(al, np.zeros(2000), nid, N)
+        Sd, Td = dev(S), dev(Tm)
+        n = split_T_size(N, Dt)
+        hi = torch.empty(n, dtype=torch.int16, device="cuda")
+        lo = torch.empty(n if mode == 0 else 1, dtype=torch.int16, device="cuda")
+        scales = torch.ones(5, dtype=torch.float32, device="cuda")
+        s = stream_ptr()
+        lib.vt_gram_split_T(ptr(Td), N, Dt, ptr(hi), ptr(lo), mode, ptr(Sd), Sd.numel(), ptr(scales), s)
+        outs = []
+        for tile in TILES + (0,):
+            A = torch.full((D, D), float("nan"), dtype=torch.float32, device="cuda")
+            desc = gram_descs([(Sd, hi, lo, A, N, Ds, Dt, scales)], "cuda")
+            lib.vt_gram_lp(ptr(desc), 1, Ds, Dt, mode, tile, s)
+            outs.append((A, desc))
+        A = torch.full((D, D), float("nan"), dtype=torch.float32, device="cuda")
+        desc = gram_descs([(Sd, hi, lo, A, N, Ds, Dt, scales)], "cuda")
+        chosen = tuner.launch(lambda t: lib.vt_gram_lp(ptr(desc), 1, Ds, Dt, mode, t, s),
+                              "bf16x3" if mode == 0 else "fp16", Dt, Ds, 1, N)
+        outs.append((A, desc))
+        torch.cuda.synchronize()
+        bits = [a.view(torch.int32) for a, _ in outs]
+        for b in bits[1:]:
+            assert torch.equal(bits[0], b), f"Dt={Dt}"
+        assert chosen in TILES
+        ref = np.tril(oracle.gram_kron(S, Tm))
+        got = np.tril(np.nan_to_num(outs[0][0].cpu().numpy().astype(np.float64)))
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        print(f"Dt={Dt}: tilings identical, tuned tile {chosen} "
+              f"({', '.join(f'{t}: {ms:.3f} ms' for t, ms in tuner.timings[next(iter(tuner.timings))].items())}), "
+              f"rel Frobenius error {err:.2e}")
+        tuner.timings.clear()
+        assert err < (2e-5 if mode == 0 else 3e-3)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed", "bf16x3", "fp16"])
 def test_decode_c0_end_to_end(torch_mod, oracle, precision):
     """Config-0 decode on the GPU vs the oracle: SNR within 0.1 dB, coefficients close."""

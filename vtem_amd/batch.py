@@ -28,6 +28,7 @@ import numpy as np
 import torch
 
 from ._lib import lib, ptr, stream_ptr
+from .autotune import gram_tuner
 from .block import (F64, I32, Decoder, DecodeResult, SpikeTrains, mat_descs,
                     split_T_size)
 
@@ -514,7 +515,8 @@ class BatchDecoder:
                          scales.data_ptr() + 20 * b)
             gd_d = _to_dev(gd, dev)
             e0 = self._tic()
-            lib.vt_gram_lp(ptr(gd_d), B, int(Ds.max()), Dt, mode, s)
+            gram_tuner().launch(lambda tile: lib.vt_gram_lp(ptr(gd_d), B, int(Ds.max()), Dt, mode, tile, s),
+                                prec, Dt, int(Ds.max()), B, int(N.max()))
             self._toc("gram_lp", e0, gflops)
             extra_keep += [Th, Tl, scales, gd_d]
         else:

@@ -28,6 +28,7 @@ import numpy as np
 import torch
 
 from ._lib import lib, ptr, stream_ptr
+from .autotune import gram_tuner
 from .basis import Geometry
 from .synthetic import Bank, IAFParams
 
@@ -364,7 +365,9 @@ class Decoder:
                             ptr(scales), stream_ptr())
         A = torch.empty((D, D), dtype=torch.float32, device=self.device) if out is None else out
         desc = gram_descs([(self.S, hi, lo, A, N, g.Ds, g.Dt, scales)], self.device)
-        lib.vt_gram_lp(ptr(desc), 1, g.Ds, g.Dt, mode, stream_ptr())
+        s = stream_ptr()
+        gram_tuner().launch(lambda tile: lib.vt_gram_lp(ptr(desc), 1, g.Ds, g.Dt, mode, tile, s),
+                            "bf16x3" if mode == 0 else "fp16", g.Dt, g.Ds, 1, N)
         self._keep = (hi, lo, scales, desc)   # alive until the next decode
         return A
 

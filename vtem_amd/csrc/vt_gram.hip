@@ -233,16 +233,20 @@ struct GramDesc {
 #define LP_WAVES 8        // waves per workgroup
 
 
-template <int NCT, int MODE>
+// MODE 0: split-BF16 3-product; MODE 1: FP16.  RT: row tiles of 16 per wave, so the
+// workgroup's 256 pair rows form 16/RT row groups and its 8 waves RT/2 column groups.
+// Every tiling issues the same MFMAs in the same order for each output element (the
+// result is bit-identical); they differ in how often an operand is formed or read:
+// RT = 2 forms each W element once per workgroup (every wave takes all columns; split-BF16
+// pays most for W), RT = 4 and 8 let each B fragment read from LDS feed more row tiles.
+template <int NCT, int MODE, int RT>
 __global__ void __launch_bounds__(64 * LP_WAVES)
 gram_lp_kernel(const GramDesc *__restrict__ descs)
 {
     constexpr int NC = NCT * 16;
-    // Row tiles per wave.  Split-BF16: 2 (8 row groups, every wave all columns), so each W
-    // element is formed once per workgroup (its hi/lo split is the costly part); FP16: 4
-    // (4 row groups x 2 column halves), so each B fragment read from LDS feeds 4 row tiles.
-    constexpr int RT = MODE == 0 ? 2 : 4;
-    constexpr int NCG = RT == 4 ? 2 : 1;     // column groups
+    static_assert(RT == 2 || RT == 4 || RT == 8, "row tiles per wave");
+    constexpr int NRG = 16 / RT;             // row groups
+    constexpr int NCG = LP_WAVES / NRG;      // column groups
     constexpr int NCT2 = (NCT + NCG - 1) / NCG;  // column tiles per column group
     // rows padded to LP_BP + 2 floats: the four 16-lane groups of a wave read rows 8 apart,
     // which a 16-float stride maps onto the same banks
@@ -260,8 +264,8 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
     while (b1 * (b1 + 1) / 2 > pid) --b1;
     const int b2 = pid - b1 * (b1 + 1) / 2;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int rg = RT == 4 ? (wave & 3) : wave;         // row group (16 RT rows)
-    const int cg = RT == 4 ? (wave >> 2) : 0;           // column group
+    const int rg = wave % NRG;                           // row group (16 RT rows)
+    const int cg = wave / NRG;                           // column group
     const int c0 = cg * NCT2;                            // first column tile of the group
     const int Dt = d.Dt, N = d.N;
     const float sW = MODE == 1 ? d.scales[1] : 1.0f;
@@ -520,19 +524,28 @@ VT_EXPORT int vt_gram_split_T(const double *Tm, int N, int Dt, void *hi, void *l
 
 // Batched low-precision Gram.  descs: device array of GramDesc (80 bytes each); every
 // problem must have the same Dt (column tiling), max_Ds bounds the grid.
-VT_EXPORT int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int mode, void *stream)
+VT_EXPORT int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int mode, int tile,
+                         void *stream)
 {
     VT_CHECK_ARG(descs && batch >= 1 && batch <= 65535 && max_Ds >= 1 && Dt >= 1 &&
-                 (mode == 0 || mode == 1));
+                 (mode == 0 || mode == 1) && (tile == 0 || tile == 2 || tile == 4 || tile == 8));
+    // default tiling per form (measured): split-BF16 2 row tiles per wave, FP16 4
+    if (tile == 0) tile = mode == 0 ? 2 : 4;
     const int nbp = (max_Ds + LP_BP - 1) / LP_BP;
     dim3 grid(nbp * (nbp + 1) / 2, batch);
     const int nct = gram_nct(Dt);
     hipStream_t s = (hipStream_t)stream;
     const GramDesc *dd = (const GramDesc *)descs;
+#define LAUNCH_LP_T(NCTV, M)                                                                  \
+    do {                                                                                      \
+        if (tile == 2) gram_lp_kernel<NCTV, M, 2><<<grid, 64 * LP_WAVES, 0, s>>>(dd);         \
+        else if (tile == 4) gram_lp_kernel<NCTV, M, 4><<<grid, 64 * LP_WAVES, 0, s>>>(dd);    \
+        else gram_lp_kernel<NCTV, M, 8><<<grid, 64 * LP_WAVES, 0, s>>>(dd);                   \
+    } while (0)
 #define LAUNCH_LP(NCTV)                                                                       \
     do {                                                                                      \
-        if (mode == 0) gram_lp_kernel<NCTV, 0><<<grid, 64 * LP_WAVES, 0, s>>>(dd);            \
-        else gram_lp_kernel<NCTV, 1><<<grid, 64 * LP_WAVES, 0, s>>>(dd);                      \
+        if (mode == 0) LAUNCH_LP_T(NCTV, 0);                                                  \
+        else LAUNCH_LP_T(NCTV, 1);                                                            \
     } while (0)
     if (nct == 1) LAUNCH_LP(1);
     else if (nct == 3) LAUNCH_LP(3);
@@ -540,6 +553,7 @@ VT_EXPORT int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int m
     else if (nct == 10) LAUNCH_LP(10);
     else return VT_ERANGE;
 #undef LAUNCH_LP
+#undef LAUNCH_LP_T
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
