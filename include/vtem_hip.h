@@ -200,9 +200,16 @@ int vt_gram_split_T_size(int N, int Dt, long long *elems_host);
  * {const double *S; const u16 *Th, *Tl; float *A; long long lds, lda; int N, Ds, Dt, pad;
  * const float *scales;} (72 bytes); all problems share Dt.  Same role as vt_gram_f64.
  * tile: row tiles of 16 per wave (2, 4 or 8; 0 = 2 for split-BF16, 4 for FP16).  Every
- * tiling gives a bit-identical A; vtem_amd/autotune.py picks the fastest per shape. */
+ * tiling gives a bit-identical A; vtem_amd/autotune.py picks the fastest per shape.
+ * ws (ws_bytes >= vt_gram_lp_ws_size): a single-problem launch whose last round of pair
+ * blocks would leave most CUs idle splits those blocks over up to 8 neuron ranges, FP32
+ * partials in ws summed in order by a second kernel (A then differs from the unsplit one
+ * by FP32 reassociation only, and not between tilings); null: no split. */
 int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int mode, int tile,
-               void *stream);
+               void *ws, long long ws_bytes, void *stream);
+/* Bytes of workspace the tail split of such a launch needs (0: it does not split),
+ * written to the host pointer. */
+int vt_gram_lp_ws_size(int batch, int max_Ds, int Dt, int mode, int tile, long long *bytes_host);
 /* s2[j] = |S_j|^2 (once per neuron set). */
 int vt_row_sumsq(const double *S, long long lds, int N, int Ds, double *s2, void *stream);
 /* dst (cols x rows, row stride ldd) = src^T (rows x cols, row stride lds).  Makes the

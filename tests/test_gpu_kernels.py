@@ -431,11 +431,11 @@ def test_gram_tilings_bit_identical(torch_mod, oracle, mode, tmp_path):
         for tile in TILES + (0,):
             A = torch.full((D, D), float("nan"), dtype=torch.float32, device="cuda")
             desc = gram_descs([(Sd, hi, lo, A, N, Ds, Dt, scales)], "cuda")
-            lib.vt_gram_lp(ptr(desc), 1, Ds, Dt, mode, tile, s)
+            lib.vt_gram_lp(ptr(desc), 1, Ds, Dt, mode, tile, 0, 0, s)
             outs.append((A, desc))
         A = torch.full((D, D), float("nan"), dtype=torch.float32, device="cuda")
         desc = gram_descs([(Sd, hi, lo, A, N, Ds, Dt, scales)], "cuda")
-        chosen = tuner.launch(lambda t: lib.vt_gram_lp(ptr(desc), 1, Ds, Dt, mode, t, s),
+        chosen = tuner.launch(lambda t: lib.vt_gram_lp(ptr(desc), 1, Ds, Dt, mode, t, 0, 0, s),
                               "bf16x3" if mode == 0 else "fp16", Dt, Ds, 1, N)
         outs.append((A, desc))
         torch.cuda.synchronize()
@@ -451,6 +451,52 @@ def test_gram_tilings_bit_identical(torch_mod, oracle, mode, tmp_path):
               f"rel Frobenius error {err:.2e}")
         tuner.timings.clear()
         assert err < (2e-5 if mode == 0 else 3e-3)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gram_tail_split(torch_mod, oracle, mode):
+    """A single-problem Gram whose last round of pair blocks would leave most CUs idle
+    splits those blocks over neuron ranges (FP32 partials summed in order by a second
+    kernel).  For the first spatial orders whose plan splits on this device: every element
+    of the lower triangle is written, and A equals the unsplit Gram (itself pinned against
+    the FP64 oracle) within FP32 reassociation, for N whose ranges are all non-empty and
+    for N with empty ranges."""
+    torch = torch_mod
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    from vtem_amd.block import dev, gram_descs, gram_lp_ws_bytes, split_T_size
+    _banner(f"gram tail split mode={mode}")
+    rng = np.random.default_rng(41)
+    Dt = 9
+    tile = 2 if mode == 0 else 4
+    Ds = next((d for d in range(16 * 20 + 8, 16 * 64, 16) if gram_lp_ws_bytes(d, Dt, mode, tile) > 0), None)
+    assert Ds is not None, "no spatial order up to 1024 splits"
+    for N in (1000, 300):
+        D = Ds * Dt
+        S = rng.standard_normal((N, Ds))
+        al = rng.standard_normal((4 * N, Dt)) * 0.1
+        nid = np.sort(rng.integers(0, N, 4 * N)).astype(np.int32)
+        Tm, _ = oracle.neuron_T(al, np.zeros(4 * N), nid, N)
+        Sd, Td = dev(S), dev(Tm)
+        n = split_T_size(N, Dt)
+        hi = torch.empty(n, dtype=torch.int16, device="cuda")
+        lo = torch.empty(n if mode == 0 else 1, dtype=torch.int16, device="cuda")
+        scales = torch.ones(5, dtype=torch.float32, device="cuda")
+        s = stream_ptr()
+        lib.vt_gram_split_T(ptr(Td), N, Dt, ptr(hi), ptr(lo), mode, ptr(Sd), Sd.numel(), ptr(scales), s)
+        wsb = gram_lp_ws_bytes(Ds, Dt, mode, tile)
+        ws = torch.empty(wsb // 4, dtype=torch.float32, device="cuda")
+        outs = []
+        for w, wb in ((0, 0), (ptr(ws), wsb)):
+            A = torch.full((D, D), float("nan"), dtype=torch.float32, device="cuda")
+            desc = gram_descs([(Sd, hi, lo, A, N, Ds, Dt, scales)], "cuda")
+            lib.vt_gram_lp(ptr(desc), 1, Ds, Dt, mode, tile, w, wb, s)
+            outs.append((A, desc))
+        torch.cuda.synchronize()
+        a, b = (torch.tril(x.double()) for x, _ in outs)
+        assert not torch.isnan(b).any() and not torch.isnan(a).any()
+        err = float((a - b).abs().max() / a.abs().max())
+        print(f"Ds={Ds} N={N}: workspace {wsb / 2**20:.1f} MiB, split vs unsplit max rel diff {err:.2e}")
+        assert err < 2e-6
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed", "bf16x3", "fp16"])

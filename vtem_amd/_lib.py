@@ -63,7 +63,8 @@ SIGNATURES = {
     "vt_gram_f64_batched": [P, I, I, I, P],
     "vt_gram_split_T": [P, I, I, P, P, I, P, L, P, P],
     "vt_gram_split_T_size": [I, I, P],
-    "vt_gram_lp": [P, I, I, I, I, I, P],
+    "vt_gram_lp": [P, I, I, I, I, I, P, L, P],
+    "vt_gram_lp_ws_size": [I, I, I, I, I, P],
     "vt_kron_trace": [P, P, I, I, P, D, P, P],
     "vt_kron_trace_batched": [P, P, P, P, P, I, P, I, D, P, I, P],
     "vt_row_sumsq": [P, L, I, I, P, P],

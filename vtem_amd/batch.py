@@ -515,7 +515,7 @@ class BatchDecoder:
                          scales.data_ptr() + 20 * b)
             gd_d = _to_dev(gd, dev)
             e0 = self._tic()
-            gram_tuner().launch(lambda tile: lib.vt_gram_lp(ptr(gd_d), B, int(Ds.max()), Dt, mode, tile, s),
+            gram_tuner().launch(lambda tile: lib.vt_gram_lp(ptr(gd_d), B, int(Ds.max()), Dt, mode, tile, 0, 0, s),
                                 prec, Dt, int(Ds.max()), B, int(N.max()))
             self._toc("gram_lp", e0, gflops)
             extra_keep += [Th, Tl, scales, gd_d]

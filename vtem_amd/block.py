@@ -60,6 +60,13 @@ def gram_descs(items, device) -> torch.Tensor:
     return torch.from_numpy(arr.view(np.uint8).copy()).to(device)
 
 
+def gram_lp_ws_bytes(Ds: int, Dt: int, mode: int, tile: int) -> int:
+    """Workspace bytes of a single-problem low-precision Gram's tail split (0: no split)."""
+    out = ctypes.c_longlong(0)
+    lib.vt_gram_lp_ws_size(1, int(Ds), int(Dt), int(mode), int(tile), ctypes.addressof(out))
+    return int(out.value)
+
+
 def split_T_size(N: int, Dt: int) -> int:
     """16-bit elements of one block's tiled low-precision T (queried from the library)."""
     out = ctypes.c_longlong(0)
@@ -366,9 +373,19 @@ class Decoder:
         A = torch.empty((D, D), dtype=torch.float32, device=self.device) if out is None else out
         desc = gram_descs([(self.S, hi, lo, A, N, g.Ds, g.Dt, scales)], self.device)
         s = stream_ptr()
-        gram_tuner().launch(lambda tile: lib.vt_gram_lp(ptr(desc), 1, g.Ds, g.Dt, mode, tile, s),
-                            "bf16x3" if mode == 0 else "fp16", g.Dt, g.Ds, 1, N)
-        self._keep = (hi, lo, scales, desc)   # alive until the next decode
+        wss = {}
+
+        def run(tile):
+            # workspace of the tail split (a thin last round of pair blocks spread over
+            # the chip), per tiling: the plan depends on the kernel's occupancy
+            ws = wss.get(tile)
+            if ws is None:
+                ws = wss[tile] = torch.empty(max(1, gram_lp_ws_bytes(g.Ds, g.Dt, mode, tile) // 4),
+                                             dtype=torch.float32, device=self.device)
+            lib.vt_gram_lp(ptr(desc), 1, g.Ds, g.Dt, mode, tile, ptr(ws), 4 * ws.numel(), s)
+
+        gram_tuner().launch(run, "bf16x3" if mode == 0 else "fp16", g.Dt, g.Ds, 1, N)
+        self._keep = (hi, lo, scales, desc, wss)   # alive until the next decode
         return A
 
     def solve_tikhonov(self, Tm, r, n_dev, lam_rel=1e-6, precision="fp64", refine_iters=4,

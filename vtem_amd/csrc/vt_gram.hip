@@ -239,9 +239,20 @@ struct GramDesc {
 // result is bit-identical); they differ in how often an operand is formed or read:
 // RT = 2 forms each W element once per workgroup (every wave takes all columns; split-BF16
 // pays most for W), RT = 4 and 8 let each B fragment read from LDS feed more row tiles.
+// Tail split of a single-problem launch.  A problem of P pair blocks at one workgroup per
+// CU fills the chip for P / CUs rounds; a thin last round would leave most CUs idle for a
+// whole round.  Its ntail blocks are instead split over f neuron ranges each: workgroup
+// nfull + t f + part computes block nfull + t over chunk range `part` and writes its FP32
+// partial tile (before unscaling) to ws[((t f + part) * 256 + row) * NC + col];
+// gram_lp_fixup_kernel adds the f partials in part order and writes the block.
+struct GramSplit {
+    int nfull, ntail, f, pad;
+    float *ws;
+};
+
 template <int NCT, int MODE, int RT>
 __global__ void __launch_bounds__(64 * LP_WAVES)
-gram_lp_kernel(const GramDesc *__restrict__ descs)
+gram_lp_kernel(const GramDesc *__restrict__ descs, const GramSplit sp)
 {
     constexpr int NC = NCT * 16;
     static_assert(RT == 2 || RT == 4 || RT == 8, "row tiles per wave");
@@ -257,7 +268,18 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
 
     const GramDesc d = descs[blockIdx.y];
     const int nbp = (d.Ds + LP_BP - 1) / LP_BP;
-    const int pid = blockIdx.x;
+    const int nsteps = (d.N + KL - 1) / KL;
+    int pid = blockIdx.x, kb = 0, ke = nsteps, part = -1, tb = 0;
+    if (sp.f > 1 && pid >= sp.nfull) {
+        const int it = pid - sp.nfull;
+        tb = it / sp.f;
+        part = it % sp.f;
+        if (tb >= sp.ntail) return;
+        pid = sp.nfull + tb;
+        const int per = (nsteps + sp.f - 1) / sp.f;
+        kb = min(nsteps, part * per);
+        ke = min(nsteps, kb + per);
+    }
     if (pid >= nbp * (nbp + 1) / 2) return;
     int b1 = (int)((sqrt(8.0 * pid + 1.0) - 1.0) * 0.5);
     while ((b1 + 1) * (b1 + 2) / 2 <= pid) ++b1;
@@ -270,7 +292,6 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
     const int Dt = d.Dt, N = d.N;
     const float sW = MODE == 1 ? d.scales[1] : 1.0f;
     const int g = lane >> 4;
-    const int nsteps = (N + KL - 1) / KL;
 
     // Chunk k (neurons 32k..32k+31).  The pre-tiled T rows go from HBM straight into LDS
     // (global_load_lds, 16 bytes per lane at the wave's LDS base + 16 * lane; the tile's
@@ -321,14 +342,16 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
 #pragma unroll
         for (int c = 0; c < NCT2; ++c) acc[t][c] = (v4f){0, 0, 0, 0};
 
-    load_S(0);
-    store_S(0);
-    dma_T(0, 0);
-    __builtin_amdgcn_s_waitcnt(0);                      // this wave's tile loads have landed
-    __syncthreads();
-    for (int k = 0; k < nsteps; ++k) {
-        const int bf = k & 1;
-        if (k + 1 < nsteps) {                           // the other buffer: free since the last barrier
+    if (kb < ke) {                                      // uniform (an empty split range adds zeros)
+        load_S(kb);
+        store_S(0);
+        dma_T(kb, 0);
+        __builtin_amdgcn_s_waitcnt(0);                  // this wave's tile loads have landed
+        __syncthreads();
+    }
+    for (int k = kb; k < ke; ++k) {
+        const int bf = (k - kb) & 1;
+        if (k + 1 < ke) {                               // the other buffer: free since the last barrier
             load_S(k + 1);
             dma_T(k + 1, bf ^ 1);
         }
@@ -380,9 +403,23 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wh[t], bh, acc[t][c], 0, 0, 0);
             }
         }
-        if (k + 1 < nsteps) store_S(bf ^ 1);
+        if (k + 1 < ke) store_S(bf ^ 1);
         __builtin_amdgcn_s_waitcnt(0);                  // the next chunk's tile loads have landed
         __syncthreads();
+    }
+    if (part >= 0) {
+        float *W = sp.ws + (long long)(tb * sp.f + part) * 256 * NC;
+#pragma unroll
+        for (int c = 0; c < NCT2; ++c) {
+            if (c0 + c >= NCT) break;
+            const int col = (c0 + c) * 16 + mfma_col(lane);
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    W[(rg * 16 * RT + t * 16 + mfma_row(lane, q)) * NC + col] = acc[t][c][q];
+        }
+        return;
     }
     const float unscale = MODE == 1 ? d.scales[2] : 1.0f;
     const int DTU = gram_tcols(Dt);
@@ -405,6 +442,36 @@ gram_lp_kernel(const GramDesc *__restrict__ descs)
                     if (ptp != pt) d.A[(long long)(ps * Dt + ptp) * d.lda + psp * Dt + pt] = v;
                 }
             }
+    }
+}
+
+// Tail blocks of a split launch: sum the f partial tiles in part order, unscale, write the
+// block's pairs (pt <= pt') at both mirrored places, as the Gram kernel's epilogue does.
+__global__ void __launch_bounds__(256)
+gram_lp_fixup_kernel(const GramDesc *__restrict__ descs, const GramSplit sp, int NC, int mode)
+{
+    const GramDesc d = descs[0];
+    const int tb = blockIdx.x;
+    const int pid = sp.nfull + tb;
+    int b1 = (int)((sqrt(8.0 * pid + 1.0) - 1.0) * 0.5);
+    while ((b1 + 1) * (b1 + 2) / 2 <= pid) ++b1;
+    while (b1 * (b1 + 1) / 2 > pid) --b1;
+    const int b2 = pid - b1 * (b1 + 1) / 2;
+    const int Dt = d.Dt, DTU = gram_tcols(Dt);
+    const float unscale = mode == 1 ? d.scales[2] : 1.0f;
+    const float *W = sp.ws + (long long)tb * sp.f * 256 * NC;
+    for (int e = blockIdx.y * 256 + threadIdx.x; e < 256 * DTU; e += 256 * gridDim.y) {
+        const int rr = e / DTU, col = e % DTU;
+        float v = 0.f;
+        for (int p = 0; p < sp.f; ++p) v += W[((long long)p * 256 + rr) * NC + col];
+        v *= unscale;
+        int pt, ptp;
+        gram_tpair(col, Dt, pt, ptp);
+        const int ps = b1 * LP_BP + (rr >> 4), psp = b2 * LP_BP + (rr & 15);
+        if (ps < d.Ds && psp < d.Ds) {
+            d.A[(long long)(ps * Dt + pt) * d.lda + psp * Dt + ptp] = v;
+            if (ptp != pt) d.A[(long long)(ps * Dt + ptp) * d.lda + psp * Dt + pt] = v;
+        }
     }
 }
 
@@ -522,25 +589,99 @@ VT_EXPORT int vt_gram_split_T(const double *Tm, int N, int Dt, void *hi, void *l
     return VT_OK;
 }
 
-// Batched low-precision Gram.  descs: device array of GramDesc (80 bytes each); every
-// problem must have the same Dt (column tiling), max_Ds bounds the grid.
+// The kernel instance of a launch: nct column tiles, mode, tile (row tiles per wave).
+static const void *gram_lp_fn(int nct, int mode, int tile)
+{
+#define LP_FN_T(NCTV, M)                                                                       \
+    return tile == 2 ? (const void *)gram_lp_kernel<NCTV, M, 2>                                \
+           : tile == 4 ? (const void *)gram_lp_kernel<NCTV, M, 4>                              \
+                       : (const void *)gram_lp_kernel<NCTV, M, 8>
+#define LP_FN(NCTV)                                                                            \
+    do {                                                                                       \
+        if (mode == 0) { LP_FN_T(NCTV, 0); }                                                   \
+        else { LP_FN_T(NCTV, 1); }                                                             \
+    } while (0)
+    if (nct == 1) LP_FN(1);
+    if (nct == 3) LP_FN(3);
+    if (nct == 6) LP_FN(6);
+    if (nct == 10) LP_FN(10);
+#undef LP_FN
+#undef LP_FN_T
+    return nullptr;
+}
+
+// Tail-split plan of a launch (f = 1: none).  Only single-problem launches split: in a
+// batch, a block's result must not depend on the other blocks.  ws_floats: workspace the
+// plan needs.
+static void gram_lp_plan(int batch, int max_Ds, int nct, const void *fn, GramSplit *sp,
+                         long long *ws_floats)
+{
+    const int nbp = (max_Ds + LP_BP - 1) / LP_BP;
+    const int P = nbp * (nbp + 1) / 2;
+    *sp = GramSplit{P, 0, 1, 0, nullptr};
+    *ws_floats = 0;
+    if (batch != 1 || !fn) return;
+    int dev = 0, cus = 0, occ = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        return;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64 * LP_WAVES, 0) != hipSuccess || occ < 1)
+        occ = 1;
+    const int W = cus * occ;                 // workgroups in flight
+    const int rounds = P / W, tail = P % W;
+    if (rounds < 1 || tail == 0 || 2 * tail > W) return;
+    const int f = W / tail > 8 ? 8 : W / tail;
+    if (f < 2) return;
+    sp->nfull = P - tail;
+    sp->ntail = tail;
+    sp->f = f;
+    *ws_floats = (long long)tail * f * 256 * nct * 16;
+}
+
+VT_EXPORT int vt_gram_lp_ws_size(int batch, int max_Ds, int Dt, int mode, int tile, long long *bytes_host)
+{
+    VT_CHECK_ARG(bytes_host && batch >= 1 && max_Ds >= 1 && Dt >= 1 && (mode == 0 || mode == 1) &&
+                 (tile == 0 || tile == 2 || tile == 4 || tile == 8));
+    if (tile == 0) tile = mode == 0 ? 2 : 4;
+    const int nct = gram_nct(Dt);
+    if (!nct) return VT_ERANGE;
+    GramSplit sp;
+    long long wf;
+    gram_lp_plan(batch, max_Ds, nct, gram_lp_fn(nct, mode, tile), &sp, &wf);
+    *bytes_host = wf * (long long)sizeof(float);
+    return VT_OK;
+}
+
+// Batched low-precision Gram.  descs: device array of GramDesc; every problem must have
+// the same Dt (column tiling), max_Ds bounds the grid.  ws (ws_bytes): workspace of the
+// tail split (single-problem launches; vt_gram_lp_ws_size), or null: no split.
 VT_EXPORT int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int mode, int tile,
-                         void *stream)
+                         void *ws, long long ws_bytes, void *stream)
 {
     VT_CHECK_ARG(descs && batch >= 1 && batch <= 65535 && max_Ds >= 1 && Dt >= 1 &&
-                 (mode == 0 || mode == 1) && (tile == 0 || tile == 2 || tile == 4 || tile == 8));
+                 (mode == 0 || mode == 1) && (tile == 0 || tile == 2 || tile == 4 || tile == 8) &&
+                 ws_bytes >= 0);
     // default tiling per form (measured): split-BF16 2 row tiles per wave, FP16 4
     if (tile == 0) tile = mode == 0 ? 2 : 4;
-    const int nbp = (max_Ds + LP_BP - 1) / LP_BP;
-    dim3 grid(nbp * (nbp + 1) / 2, batch);
     const int nct = gram_nct(Dt);
+    if (!nct) return VT_ERANGE;
+    GramSplit sp;
+    long long wf = 0;
+    if (ws) gram_lp_plan(batch, max_Ds, nct, gram_lp_fn(nct, mode, tile), &sp, &wf);
+    if (!ws || sp.f < 2 || ws_bytes < wf * (long long)sizeof(float)) {
+        const int nbp = (max_Ds + LP_BP - 1) / LP_BP;
+        sp = GramSplit{nbp * (nbp + 1) / 2, 0, 1, 0, nullptr};
+    } else {
+        sp.ws = (float *)ws;
+    }
+    dim3 grid(sp.nfull + sp.ntail * sp.f, batch);
     hipStream_t s = (hipStream_t)stream;
     const GramDesc *dd = (const GramDesc *)descs;
 #define LAUNCH_LP_T(NCTV, M)                                                                  \
     do {                                                                                      \
-        if (tile == 2) gram_lp_kernel<NCTV, M, 2><<<grid, 64 * LP_WAVES, 0, s>>>(dd);         \
-        else if (tile == 4) gram_lp_kernel<NCTV, M, 4><<<grid, 64 * LP_WAVES, 0, s>>>(dd);    \
-        else gram_lp_kernel<NCTV, M, 8><<<grid, 64 * LP_WAVES, 0, s>>>(dd);                   \
+        if (tile == 2) gram_lp_kernel<NCTV, M, 2><<<grid, 64 * LP_WAVES, 0, s>>>(dd, sp);     \
+        else if (tile == 4) gram_lp_kernel<NCTV, M, 4><<<grid, 64 * LP_WAVES, 0, s>>>(dd, sp); \
+        else gram_lp_kernel<NCTV, M, 8><<<grid, 64 * LP_WAVES, 0, s>>>(dd, sp);               \
     } while (0)
 #define LAUNCH_LP(NCTV)                                                                       \
     do {                                                                                      \
@@ -550,10 +691,13 @@ VT_EXPORT int vt_gram_lp(const void *descs, int batch, int max_Ds, int Dt, int m
     if (nct == 1) LAUNCH_LP(1);
     else if (nct == 3) LAUNCH_LP(3);
     else if (nct == 6) LAUNCH_LP(6);
-    else if (nct == 10) LAUNCH_LP(10);
-    else return VT_ERANGE;
+    else LAUNCH_LP(10);
 #undef LAUNCH_LP
 #undef LAUNCH_LP_T
     VT_LAUNCH_CHECK();
+    if (sp.f > 1) {
+        gram_lp_fixup_kernel<<<dim3(sp.ntail, 8), 256, 0, s>>>(dd, sp, nct * 16, mode);
+        VT_LAUNCH_CHECK();
+    }
     return VT_OK;
 }
