@@ -30,7 +30,7 @@ import json
 import math
 import os
 import threading
-from typing import Callable, Dict, List, Optional, Tuple
+from typing import Callable, Dict, List, Optional
 
 TILES = (2, 4, 8)                     # row tiles of 16 per wave
 LP_BP = 16                            # spatial indices per workgroup block (256 pair rows)
@@ -143,9 +143,3 @@ def gram_tuner() -> GramTuner:
         if _TUNER is None:
             _TUNER = GramTuner()
         return _TUNER
-
-
-def timed_keys() -> Dict[str, Tuple[int, Dict[int, float]]]:
-    """Keys tuned in this process: key -> (chosen tiling, {tiling: ms})."""
-    t = gram_tuner()
-    return {k: (t.choices[k], dict(v)) for k, v in t.timings.items()}
