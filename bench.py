@@ -232,15 +232,19 @@ def precision_report(seed=0):
     exec_lp = nb16 * (nb16 + 1) / 2 * 256 * nct * 16 * 2.0 * (-(-N // 32) * 32)
     exec_64 = nb8 * (nb8 + 1) / 2 * 64 * nct * 16 * 2.0 * (-(-N // 16) * 16)
 
-    def timed(fn):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        fn()                                   # warm
+    def timed(fn, reps=5):
+        """Median of reps timed calls (HIP events), after one warm call."""
+        out = fn()                             # warm (and, for a new shape, the tile tuning)
         torch.cuda.synchronize()
-        e0.record()
-        out = fn()
-        e1.record()
-        torch.cuda.synchronize()
-        return out, e0.elapsed_time(e1)
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            out = fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return out, sorted(ts)[reps // 2]
 
     A64, t64 = timed(lambda: dec.gram(Tm))
     L64 = torch.tril(A64)
