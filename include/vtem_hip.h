@@ -362,6 +362,8 @@ int vt_kpcg_place_diag(double *X, const long long *xoff, const int *dim, int bat
  * <r,z>/rz, p = z + beta p.  Stopped blocks are left unchanged by later steps. */
 int vt_kpcg_init(const double *r, const double *z, double *p, double *x, const long long *seg,
                  int batch, double *st, double *hist, int hist_stride, void *stream);
+/* it = -1: each block takes its iteration index from its own count st[4b+3] (equal to
+ * the loop index while it is active), so every iteration has the same arguments. */
 int vt_kpcg_step_a(double *x, double *r, const double *p, double *q, const double *lam,
                    const long long *seg, int batch, double *st, double *hist, int hist_stride,
                    int it, double stop_rel2, void *stream);
@@ -371,12 +373,14 @@ int vt_kpcg_step_b(const double *r, const double *z, double *p, const long long 
 /* `iters` CG iterations after vt_kpcg_init, enqueued by one call: per iteration the
  * operator (skinny products fwd then bwd, q = A p), step_a, the preconditioner (skinny
  * product pre, z = Mi r Ti) and step_b.  Each product is given as its descriptors and
- * its max_M / max_K; max_Nc bounds every problem's column count. */
+ * its max_M / max_K; max_Nc bounds every problem's column count.  graph_chunk >= 2
+ * replays the iterations as a HIP graph of up to graph_chunk (at most iters/2)
+ * iterations, as vt_kron_rnn does; 0 launches each.  Same results either way. */
 int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                     const void *pre, int pre_M, int pre_K, int batch, int max_Nc, double *x,
                     double *r, double *p, double *q, double *z, const double *lam,
                     const long long *seg, double *st, double *hist, int hist_stride, int iters,
-                    double stop_rel2, void *stream);
+                    double stop_rel2, int graph_chunk, void *stream);
 
 /* ----------------------------------------- gradient-flow (RNN) decoder, batched -- */
 /* Coefficient-space image of neural_network.rnn3 (vtem/neural_network.py:15-99, update

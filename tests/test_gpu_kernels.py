@@ -823,6 +823,36 @@ def test_rnn_graph_replay(torch_mod):
         torch.ones(1 << 20, dtype=torch.float64, device="cuda").sum().item()
 
 
+def test_kpcg_graph_replay(torch_mod):
+    """The CG loop replayed as HIP graphs (each block's iteration index from its own
+    count) gives bit-identical iterates, residual histories and iteration counts to the
+    launch-by-launch loop, on three decodes in a row, with blocks that stop at different
+    iterations and with a retry solve (more iterations) for blocks that miss the guard."""
+    torch = torch_mod
+    from vtem_amd import synthetic as syn
+    from vtem_amd.batch import BatchDecoder, BlockJob
+    _banner("kpcg graph replay")
+    specs = [(32, 0.5), (32, 2.0), (48, 1.0), (64, 2.0)]
+    jobs = []
+    for i, (sd, du) in enumerate(specs):
+        blk = syn.serve_block(sd, du, seed=760 + i)
+        geom, st, dec = _encode_block(torch, blk)
+        jobs.append(BlockJob(dec, st))
+    for iters, retry in ((32, None), (3, 40)):
+        ref = BatchDecoder(precision="kpcg", graph_chunk=0, cg_iters=iters, retry_iters=retry).decode(jobs)
+        bd = BatchDecoder(precision="kpcg", graph_chunk=8, cg_iters=iters, retry_iters=retry)
+        for rep in range(3):
+            res = bd.decode(jobs)
+            torch.cuda.synchronize()
+            for a, b in zip(ref, res):
+                assert torch.equal(a.X, b.X), (iters, rep)
+                assert torch.equal(a.state, b.state)
+                assert torch.equal(a.resid, b.resid)
+            torch.ones(1 << 20, dtype=torch.float64, device="cuda").sum().item()
+        print(f"cg_iters={iters}: iterations {[int(r.state.cpu().numpy()[3]) for r in ref]}, "
+              f"retries {bd.last_stats.get('kpcg_retries')}")
+
+
 def test_server_alone_equals_mix(torch_mod):
     """Requests served with Poisson arrivals on 2 streams (continuous batching) give the
     same frames as each request decoded alone with the same precision decision."""

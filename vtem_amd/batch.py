@@ -166,7 +166,7 @@ class BatchDecoder:
     the non-FP64 forms that fail the guard (relative residual of the FP64 system above
     `tol`, or a factorisation that lost positive definiteness) are redone together as
     one batched FP64 group; kpcg blocks first get one more CG solve with retry_iters
-    iterations.  The RNN and power-iteration loops are replayed as HIP graphs of
+    iterations.  The CG, RNN and power-iteration loops are replayed as HIP graphs of
     graph_chunk iterations (0: every kernel launched by the host; same results)."""
 
     def __init__(self, device="cuda", precision: str = "bf16x3", lam_rel: float = 1e-6,
@@ -186,7 +186,7 @@ class BatchDecoder:
         # kpcg blocks that fail the guard are solved again with this many iterations
         # before the FP64 redo (CG that ran out of iterations on a harder system)
         self.retry_iters = 4 * cg_iters if retry_iters is None else retry_iters
-        # RNN and power loops: replay a HIP graph of this many iterations (0: launch each)
+        # CG, RNN and power loops: replay a HIP graph of this many iterations (0: launch each)
         self.graph_chunk = graph_chunk
         self.last_stats: Dict[str, Any] = {}
         self.timers: Optional[Dict[str, list]] = None     # stage -> [(ev0, ev1, work)]
@@ -777,7 +777,7 @@ class BatchDecoder:
         lib.vt_kpcg_init(ptr(R), ptr(Zp), ptr(Pv), ptr(X), ptr(seg), Bt, ptr(st), ptr(hist), hs, s)
         lib.vt_kpcg_iterate(*fp.args(), *bp.args(), *pre.args(), Bt, maxNc, ptr(X), ptr(R), ptr(Pv),
                             ptr(Q), ptr(Zp), ptr(lam), ptr(seg), ptr(st), ptr(hist), hs, iters,
-                            float(self.cg_stop) ** 2, s)
+                            float(self.cg_stop) ** 2, self.graph_chunk, s)
         # true FP64 residual of the final iterate (the guard's input)
         nrm = torch.zeros((Bt, 2), dtype=F64, device=dev)
         lib.vt_seg_sumsq(ptr(r), ptr(seg), Bt, ptr(nrm), 2, s)

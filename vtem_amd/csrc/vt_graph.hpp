@@ -72,12 +72,14 @@ inline hipStream_t vt_capture_stream(hipError_t *err)
 }
 
 // body(stream) enqueues one iteration on `stream` and returns a status (VT_OK = 0).
-// Runs `total` iterations on s: with 2 <= chunk <= total/2, total/chunk replays of a
-// captured chunk and the remainder launched directly; otherwise all directly.
+// Runs `total` iterations on s: with chunk >= 2, total/chunk replays of a captured chunk
+// of at most total/2 iterations and the remainder launched directly; chunk 0 or a chunk
+// that would hold fewer than 2 iterations: all directly.
 template <class Body>
 int vt_graph_loop(hipStream_t s, int total, int chunk, Body &&body)
 {
-    if (chunk < 2 || total < 2 * chunk) {
+    if (chunk >= 2 && total < 2 * chunk) chunk = total / 2;
+    if (chunk < 2) {
         for (int i = 0; i < total; ++i) {
             const int st = body(s);
             if (st != VT_OK) return st;

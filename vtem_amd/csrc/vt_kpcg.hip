@@ -295,8 +295,12 @@ kpcg_step_a_kernel(double *__restrict__ x, double *__restrict__ r, const double 
     }
     rr = block_sum(rr, red);
     if (threadIdx.x == 0) {
-        hist[(long long)blockIdx.x * hs + it + 1] = rr;
-        s[3] = it + 1;
+        // it < 0: the block's own count of completed iterations (s[3], which equals the
+        // loop index while the block is active), so every iteration launches the same
+        // kernel arguments and a loop of them can be replayed as a graph
+        const int itx = it >= 0 ? it : (int)s[3];
+        if (itx + 1 < hs) hist[(long long)blockIdx.x * hs + itx + 1] = rr;
+        s[3] = itx + 1;
         if (rr <= stop_rel2 * s[1]) s[2] = 0.0;
     }
 }
@@ -305,7 +309,7 @@ VT_EXPORT int vt_kpcg_step_a(double *x, double *r, const double *p, double *q, c
                              const long long *seg, int batch, double *st, double *hist,
                              int hist_stride, int it, double stop_rel2, void *stream)
 {
-    VT_CHECK_ARG(x && r && p && q && lam && seg && st && hist && batch >= 1 && it >= 0 &&
+    VT_CHECK_ARG(x && r && p && q && lam && seg && st && hist && batch >= 1 && it >= -1 &&
                  it + 1 < hist_stride);
     kpcg_step_a_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(x, r, p, q, lam, seg, st, hist,
                                                                hist_stride, it, stop_rel2);
@@ -373,18 +377,20 @@ VT_EXPORT int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void 
                               int bwd_K, const void *pre, int pre_M, int pre_K, int batch, int max_Nc,
                               double *x, double *r, double *p, double *q, double *z, const double *lam,
                               const long long *seg, double *st, double *hist, int hist_stride,
-                              int iters, double stop_rel2, void *stream)
+                              int iters, double stop_rel2, int graph_chunk, void *stream)
 {
-    VT_CHECK_ARG(fwd && bwd && pre && iters >= 0 && iters < hist_stride);
+    VT_CHECK_ARG(fwd && bwd && pre && iters >= 0 && iters < hist_stride && graph_chunk >= 0);
     const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K}, Pr{pre, pre_M, pre_K};
-    for (int it = 0; it < iters; ++it) {
-        VT_TRY(skinny_run(F, batch, max_Nc, stream));
-        VT_TRY(skinny_run(Bk, batch, max_Nc, stream));
-        VT_TRY(vt_kpcg_step_a(x, r, p, q, lam, seg, batch, st, hist, hist_stride, it, stop_rel2, stream));
-        VT_TRY(skinny_run(Pr, batch, max_Nc, stream));
-        VT_TRY(vt_kpcg_step_b(r, z, p, seg, batch, st, stream));
-    }
-    return VT_OK;
+    // the iteration index comes from each block's count (it = -1): identical iterations
+    auto step = [&](hipStream_t sq) -> int {
+        VT_TRY(skinny_run(F, batch, max_Nc, sq));
+        VT_TRY(skinny_run(Bk, batch, max_Nc, sq));
+        VT_TRY(vt_kpcg_step_a(x, r, p, q, lam, seg, batch, st, hist, hist_stride, -1, stop_rel2, sq));
+        VT_TRY(skinny_run(Pr, batch, max_Nc, sq));
+        VT_TRY(vt_kpcg_step_b(r, z, p, seg, batch, st, sq));
+        return VT_OK;
+    };
+    return vt_graph_loop((hipStream_t)stream, iters, graph_chunk, step);
 }
 
 // ------------------------------------------------------------------------------------
