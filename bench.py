@@ -520,7 +520,7 @@ def main():
     iserver.policy = server.policy
     timers = iserver.serve(make_requests(items, zeros), instrument=True)["timers"]
     torch.cuda.synchronize()
-    stage = {k: (sum(a.elapsed_time(b) for a, b, _ in v), sum(w for _, _, w in v), len(v))
+    stage = {k: (sum(a.elapsed_time(b) for a, b, _ in v), sum(w() if callable(w) else w for _, _, w in v), len(v))
              for k, v in timers.items()}
     dom = max(stage, key=lambda k: stage[k][0])
     t_ms, work, nlaunch = stage[dom]

@@ -203,6 +203,8 @@ class BatchDecoder:
         return e
 
     def _toc(self, name, e0, work):
+        """Close a stage: work is its algorithmic work, or a callable giving it once the
+        stream has run (for work that depends on device state, such as CG iterations)."""
         if e0 is None:
             return
         e1 = torch.cuda.Event(enable_timing=True)
@@ -762,7 +764,6 @@ class BatchDecoder:
                              Zp.data_ptr() + 8 * int(dbj[b]), TiT_ptrs[b], act[b],
                              Ds[b], 1, Dt[b], Dt[b], 0, Ds[b], Ds[b], Dt[b]) for b in range(Bt)])
         self._toc("kpcg_setup", e_setup, float(np.sum(2.0 * N * Ds * Ds + 2.0 * Ds.astype(np.float64) ** 3)))
-        e_cg = self._tic()
         X = torch.empty(Dtot, dtype=F64, device=dev)
         Pv = torch.empty(Dtot, dtype=F64, device=dev)
         Q = torch.empty(Dtot, dtype=F64, device=dev)
@@ -773,6 +774,7 @@ class BatchDecoder:
         hs = iters + 1
         hist = torch.zeros((Bt, hs), dtype=F64, device=dev)
         maxNc = int(Dt.max())
+        e_cg = self._tic()                      # the solve's launches only (host work before)
         pre.run(s)
         lib.vt_kpcg_init(ptr(R), ptr(Zp), ptr(Pv), ptr(X), ptr(seg), Bt, ptr(st), ptr(hist), hs, s)
         lib.vt_kpcg_iterate(*fp.args(), *bp.args(), *pre.args(), Bt, maxNc, ptr(X), ptr(R), ptr(Pv),
@@ -786,10 +788,12 @@ class BatchDecoder:
         res = torch.empty(Dtot, dtype=F64, device=dev)
         lib.vt_residual_batched(ptr(res), ptr(r), ptr(Q), ptr(X), ptr(lam), ptr(elem_job), Dtot, s)
         lib.vt_seg_sumsq(ptr(res), ptr(seg), Bt, ptr(nrm[:, 1:]), 2, s)
-        # algorithmic HBM bytes per iteration: S twice, T_j once, Z written and read, Mi
-        # once, about a dozen passes over the D-space vectors
-        nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + Ds * Ds + 12.0 * D)
-        self._toc("kpcg_iterate", e_cg, float(nbytes) * iters)
+        # algorithmic HBM bytes per iteration and block: S twice, T_j once, Z written and
+        # read, Mi once, about a dozen passes over the D-space vectors; a block that has
+        # converged skips its products, so each block counts its own iterations (st[4b+3])
+        per = 8.0 * (2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + Ds * Ds + 12.0 * D)
+        self._toc("kpcg_iterate", e_cg,
+                  lambda: float(np.sum(per * st.view(Bt, 4)[:, 3].cpu().numpy())))
         keep += [r, lam, seg, elem_job, Mi, mkeep, R, Zp, pre, X, Pv, Q, Z, fp, bp, fx, bx, st, hist, res]
         return X, nrm, info, st, boff, keep
 

@@ -138,7 +138,8 @@ class Server:
     def serve(self, requests: List[Request], mode: str = "served",
               clock=time.perf_counter, instrument: bool = False) -> Dict[str, Any]:
         """Serve the requests; with `instrument`, stats["timers"] holds HIP events of the
-        batched stages (name -> [(start, end, algorithmic work)])."""
+        batched stages (name -> [(start, end, algorithmic work)]; the work is a number, or a
+        callable giving it once the stream has run)."""
         if mode == "plain":
             return self._serve_plain(requests, clock)
         for r in requests:
