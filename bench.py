@@ -55,15 +55,18 @@ PEAKS = {"gram_lp": (2500.0, "TFLOP/s", "mfma"), "gram_f64": (78.6, "TFLOP/s", "
          "solve_refine": (157.3, "TFLOP/s", "mfma"), "kpcg_setup": (78.6, "TFLOP/s", "mfma"),
          "kpcg_mgram": (78.6, "TFLOP/s", "mfma"), "kpcg_potrf": (78.6, "TFLOP/s", "mfma"),
          "kpcg_inverse": (78.6, "TFLOP/s", "mfma"), "prologue": (8000.0, "GB/s", "hbm"),
-         "synthesis": (8000.0, "GB/s", "hbm"), "rnn_iterate": (8000.0, "GB/s", "hbm"),
+         "synthesis": (78.6, "TFLOP/s", "mfma"), "rnn_iterate": (8000.0, "GB/s", "hbm"),
          "kpcg_iterate": (8000.0, "GB/s", "hbm"), "encode_coeffs": (8000.0, "GB/s", "hbm"),
          "iaf": (78.6, "TFLOP/s", "fp64")}
 WORK_NOTES = {"encode_coeffs": "algorithmic HBM bytes 8*(F*P + N*P + 2*(P+N)*Dt) per block (frames, "
                                "RF bank, the two reassociated coefficient products)",
               "iaf": "algorithmic FP64 flops 2*Dt*N*(T_f-1) per block (the trigonometric input's "
                      "evaluation; the walk's second pass and the step arithmetic are extra)",
-              "kpcg_iterate": "algorithmic HBM bytes per CG iteration: 8*(2*N*Ds + N*Dt^2 + 4*N*Dt "
-                              "+ Ds^2 + 12*D) per block (S twice, T_j, Y/Z, Mi, vectors)",
+              "kpcg_iterate": "algorithmic HBM bytes per CG iteration: 8*(2*N*Ds + N*Dt^2 + 2*N*Dt "
+                              "+ Ds^2 + 12*D) per block (S twice, T_j, Y/Z, Mi, vectors), times "
+                              "the block's own iteration count",
+              "synthesis": "algorithmic FP64 flops 2*T_out*Ds*(Dt + P) per block (Theta X^T, then "
+                           "times B_s^T)",
               "gram_lp": "algorithmic flops N*D*(D+1) per block",
               "potrf_f32": "algorithmic flops D^3/3 per block"}
 

@@ -468,7 +468,8 @@ class BatchDecoder:
         e_syn = self._tic()                     # the two launches only (host work before)
         lib.vt_gemm_grouped(ptr(gs1_d), B, int(T_out.max()), int(Ds.max()), s)
         lib.vt_gemm_grouped(ptr(gs2_d), B, int(T_out.max()), int(P.max()), s)
-        self._toc("synthesis", e_syn, float(np.sum(8.0 * T_out * P)))
+        # algorithmic FP64 flops of the two products (Theta X^T, then times B_s^T)
+        self._toc("synthesis", e_syn, float(np.sum(2.0 * T_out * Ds * (Dt + P))))
         results = []
         for b in range(B):
             Xb = X[int(dbase[b]):int(dbase[b + 1])]
