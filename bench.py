@@ -325,21 +325,23 @@ def precision_report(seed=0):
 
 def stitched_c3(seed=0, precision="kpcg", max_batch=1024):
     """Batching form: a 512 x 512 px, 4 s video (trigonometric polynomial with the frame's
-    and the duration's periods, orders 64 / 32, i.e. the blocks' bandwidth) in 64 px tiles
-    with 16 px overlap and 1 s windows with 0.25 s overlap: 11 x 11 tiles x 5 windows =
-    605 blocks of Lx = Ly = 8, Lt = 8, each tile with 2048 neurons of the config-1 laws, IAF
-    b = 2.5, delta ~ N(0.04, 0.004^2), and a contrast gain log-uniform in [0.25, 4].  Timed
-    (video, RF bank, IAF parameters and decoders resident): the batched encode of all
-    tiles, the batched decode of all blocks and the raised-cosine blending.  SNR over the
-    full video and its interior (16 px border and the first / last 0.25 s excluded); the
-    same codec on a video with the blocks' own periods (64 px, 1 s), untimed, checks the
-    machinery."""
+    and the duration's periods, orders 48 / 24: bandwidths Wx = 2 pi 48/512 rad/px and
+    Wt = 2 pi 24/4 rad/s) in 64 px tiles with 16 px overlap and 1 s windows with 0.25 s
+    overlap: 11 x 11 tiles x 5 windows = 605 blocks of Lx = Ly = 8, Lt = 8, each decoded in
+    the Dirichlet space whose periods follow from the orders and the bandwidth,
+    2 pi M / W = 85.3 px and 1.33 s (the reference's rule), each tile with 2048 neurons of
+    the config-1 laws, IAF b = 2.5, delta ~ N(0.04, 0.004^2), and a contrast gain
+    log-uniform in [0.25, 4].  Timed (video, RF bank, IAF parameters and decoders
+    resident): the batched encode of all tiles, the batched decode of all blocks and the
+    raised-cosine blending.  SNR over the full video and its interior (16 px border and the
+    first / last 0.25 s excluded)."""
     import torch
     from vtem_amd import synthetic as syn
     from vtem_amd.stitch import TiledCodec
-    vid = syn.make_video(512, 512, 4.0, 100.0, 64, 64, 32, seed=seed)
+    vid = syn.make_video(512, 512, 4.0, 100.0, 48, 48, 24, seed=seed)
     rng = np.random.default_rng(seed + 3)
-    codec = TiledCodec(512, 512, 4.0, 32, syn.bank_dyadic(64.0, 64.0))
+    codec = TiledCodec(512, 512, 4.0, 24, syn.bank_dyadic(64.0, 64.0),
+                       Wx=2 * math.pi * 48 / 512, Wt=2 * math.pi * 24 / 4.0)
     n = len(codec.plan.tiles)
     N = codec.encoder.RF.shape[0]
     iaf = [syn.iaf_params(N, 2.5, 1.0, 0.04, 0.004, rng) for _ in range(n)]
@@ -367,8 +369,9 @@ def stitched_c3(seed=0, precision="kpcg", max_batch=1024):
     u, te, td, nsp, dst = run(vid.frames)
     x = vid.frames
     inner = (slice(25, -25), slice(16, -16), slice(16, -16))
-    res = {"workload": "512x512 px, 4 s; 64 px tiles / 16 px overlap x 1 s windows / 0.25 s overlap; "
-                       "Lx=Ly=8, Lt=8, 2048 neurons per tile",
+    res = {"workload": "512x512 px, 4 s (video orders 48/24); 64 px tiles / 16 px overlap x 1 s windows / "
+                       "0.25 s overlap; Lx=Ly=8, Lt=8 with periods 2 pi M/W = 85.3 px, 1.33 s; 2048 neurons "
+                       "per tile",
            "blocks": len(codec.plan.blocks), "tiles": n, "windows": len(codec.plan.f0),
            "precision": precision, "spikes": nsp, "kpcg_retries": dst["kpcg_retries"],
            "fp64_fallbacks": dst["fallbacks"],
@@ -376,11 +379,6 @@ def stitched_c3(seed=0, precision="kpcg", max_batch=1024):
            "frames_per_s_512x512": round(400 / (te + td), 2),
            "block_frames_per_s": round(len(codec.plan.blocks) * codec.plan.window / (te + td), 1),
            "snr_db": round(snr_db(x, u), 3), "snr_interior_db": round(snr_db(x[inner], u[inner]), 3)}
-    c = syn.video_coefficients(8, 8, 8, np.random.default_rng(seed + 5))
-    per = syn.evaluate_video(c, 512, 512, np.arange(400) / 100.0, Sx=64.0, Sy=64.0, T=1.0)
-    per /= np.abs(per).max()
-    up, *_ = run(per)
-    res["snr_db_block_periodic_video"] = round(snr_db(per, up), 3)
     return res
 
 

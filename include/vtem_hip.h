@@ -427,8 +427,8 @@ int vt_scale_outer(double *W, long long ldw, int n, const double *norm, void *st
  * doubles of workspace; info (device int) > 0 if the iteration did not converge.  -2 if
  * no rocBLAS handle could be created. */
 int vt_eig_sym(double *A, int lda, int n, double *w, double *E, int *info, void *stream);
-/* y_k <- y_k / w_k where |w_k| > rcond max|w|, else 0: the truncation of solve_eq_sym
- * (vtem/utils/linalg.py:591-631, sq_Kernel :790-831). */
+/* y_k <- y_k / |w_k| where |w_k| > rcond max|w|, else 0: the truncation of solve_eq_sym
+ * (vtem/utils/linalg.py:591-631, sq_Kernel :790-831 divides by the singular values). */
 int vt_eig_filter(double *y, const double *w, int n, double rcond, void *stream);
 /* Stitching: for each block descriptor {const double *blk, *wt, *wy, *wx; int T, Py, Px,
  * t0, y0, x0;} (56 bytes), out[t0+t, y0+y, x0+x] += wt[t] wy[y] wx[x] blk[t, y, x].  Blocks

@@ -330,6 +330,26 @@ def select_intervals(spike_lists: Sequence[np.ndarray], window: Tuple[float, flo
     return np.zeros(0), np.zeros(0), np.zeros(0, np.int32), np.array(counts, np.int32)
 
 
+def select_neurons(h_x0, h_y0, domain, tol: float = 0.01):
+    """Neurons to decode for a domain [x_start, x_end, y_start, y_end]
+    (vtem/utils/spike_store.py:243-277): indices whose RF centre satisfies
+    x_start - 0.01 < x0 < x_end + 0.01 and the same in y (strict, :266), with the centres
+    shifted by the domain's centre x_start + Sx/2, y_start + Sy/2 (:260-271).
+    Returns (choose, x0 shifted, y0 shifted)."""
+    h_x0 = np.asarray(h_x0, np.float64)
+    h_y0 = np.asarray(h_y0, np.float64)
+    Sx = domain[1] - domain[0]
+    Sy = domain[3] - domain[2]
+    shiftx = domain[0] + Sx / 2
+    shifty = domain[2] + Sy / 2
+    choose = []
+    for j in range(h_x0.size):
+        if domain[0] - tol < h_x0[j] < domain[1] + tol and domain[2] - tol < h_y0[j] < domain[3] + tol:
+            choose.append(j)
+    choose = np.array(choose, np.int64)
+    return choose, h_x0[choose] - shiftx, h_y0[choose] - shifty
+
+
 def decode_norm(kappa, sigma) -> np.ndarray:
     """norm_j = 1/(kappa sigma) for sigma != 0, else 1, divided by the max
     (vtem/vtdm.py:634-638)."""
@@ -424,15 +444,18 @@ def add_diag(G: np.ndarray, lamb: float) -> np.ndarray:
 
 
 def solve_eq_sym(G: np.ndarray, q: np.ndarray, rcond: float = 1e-8) -> np.ndarray:
-    """Truncated pseudoinverse c = U diag(1/s_i if s_i > rcond s_0 else 0) U^T q
-    (vtem/utils/linalg.py:591-631, sq_Kernel :790-831).  For symmetric G the singular
-    values are |eigenvalues| and the right singular vectors are the eigenvectors times
-    sign(eigenvalue), so the pseudoinverse is U diag(1/w_i) U^T over the kept |w_i|."""
+    """c = U diag(1/s_i if s_i > rcond s_0 else 0) U^T q with U the LEFT singular vectors
+    and s the singular values of G, used on both sides (vtem/utils/linalg.py:622-630:
+    svd(G, compute_v=0), then dot(U, q, opa='c'), sq_Kernel :790-831 divides by s, then
+    dot(U, qq)).  For symmetric G = sum w_i u_i u_i^T the left singular vectors are the
+    eigenvectors and s_i = |w_i|, so this is the truncated pinv of |G| = sum |w_i| u_i u_i^T:
+    the pinv of G itself when G is positive semidefinite (every Gram), and a different
+    matrix for an indefinite G."""
     w, U = np.linalg.eigh(G)
     s = np.abs(w)
     qq = U.T @ q
     keep = s > rcond * s.max()
-    qq = np.where(keep, qq / np.where(keep, w, 1.0), 0.0)
+    qq = np.where(keep, qq / np.where(keep, s, 1.0), 0.0)
     return U @ qq
 
 

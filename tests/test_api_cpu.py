@@ -78,3 +78,39 @@ def test_tile_schedule_reproduces_a_common_field(size, interval):
     u = O.decode_tiles(tile, dom, dx, dy, interval, 2)
     X, Y = np.meshgrid(np.arange(Px), np.arange(Py))
     np.testing.assert_allclose(u[1], f(X, Y), atol=1e-12)
+
+
+def test_select_neurons_matches_oracle():
+    """vtdm.select_neurons (host) equals the oracle's restatement of
+    vtem/utils/spike_store.py:243-277, bit for bit, on random centres plus centres placed
+    exactly on the widened bounds (excluded: the comparisons are strict)."""
+    from types import SimpleNamespace
+    from vtem_amd import vtdm
+    rng = np.random.default_rng(2)
+    dom = [-20.0, 12.5, -7.25, 30.0]
+    tol = 0.01
+    x0 = np.concatenate([rng.uniform(-40, 40, 300), [dom[0] - tol, dom[1] + tol, 0.0, 0.0]])
+    y0 = np.concatenate([rng.uniform(-40, 40, 300), [0.0, 0.0, dom[2] - tol, dom[3] + tol]])
+    sel, xs, ys = vtdm.select_neurons(SimpleNamespace(h_x0=x0, h_y0=y0), dom)
+    ch, xo, yo = O.select_neurons(x0, y0, dom)
+    np.testing.assert_array_equal(sel, ch)
+    np.testing.assert_array_equal(xs, xo)
+    np.testing.assert_array_equal(ys, yo)
+    assert not set(range(300, 304)) & set(sel.tolist())
+
+
+def test_decode_norm_is_over_all_neurons():
+    """The decoder's weights 1/(kappa sigma) are normalised by their maximum over every
+    neuron before the selection (vtem/vtdm.py:408-413), so a subset's weights do not
+    change with the subset."""
+    from types import SimpleNamespace
+    from vtem_amd import vtdm
+    rng = np.random.default_rng(3)
+    N = 50
+    kappa = rng.uniform(0.5, 2.0, N)
+    sigma = np.where(rng.random(N) < 0.8, rng.uniform(1e-3, 5e-3, N), 0.0)
+    iaf = SimpleNamespace(h_kappa=kappa, h_sigma=sigma)
+    sel = np.arange(5, 40, 3)
+    got = vtdm.decode_norm(iaf, sel)
+    np.testing.assert_array_equal(got, O.decode_norm(kappa, sigma)[sel])
+    assert got.max() < 1.0 or np.argmax(O.decode_norm(kappa, sigma)) in sel

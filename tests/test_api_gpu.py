@@ -226,8 +226,8 @@ def test_dirichlet_pinv_rnn_api(torch_mod, oracle):
     assert _rel(u, u_o) < 1e-11
 
 
-def _oracle_window_decoder(oracle, lists, sel, kappa, delta, bias, Dsw, psi, Wt, Mt, dt):
-    norm = np.ones(len(sel))
+def _oracle_window_decoder(oracle, lists, sel, kappa, delta, bias, Dsw, psi, Wt, Mt, dt, norm=None):
+    norm = np.ones(len(sel)) if norm is None else norm
 
     def decode(start, end):
         ls_ = [lists[j] for j in sel]
@@ -302,7 +302,7 @@ def test_decode_video_tiles_vs_oracle(torch_mod, oracle):
                           spatial_interval=[32, 32])
 
     def tile(dom):
-        sel, x0, y0 = vtdm.select_neurons(g, dom)
+        sel, x0, y0 = oracle.select_neurons(bank.h_x0, bank.h_y0, dom)
         rf_o = oracle.rf_gabor(bank.h_alpha[sel], bank.h_l[sel], x0, y0, bank.h_ab[sel], bank.KAPPA, 32, 32,
                                32.0, 32.0)
         Ds_o = oracle.compute_Ds_fft(rf_o, 1.0, 1.0, Mx, Mx, "ellipse")
@@ -315,4 +315,64 @@ def test_decode_video_tiles_vs_oracle(torch_mod, oracle):
     u_o = oracle.decode_tiles(tile, [-24.0, 24.0, -24.0, 24.0], 1.0, 1.0, [32, 32], 30)
     print(f"decode_video tiles: rel diff {_rel(u, u_o):.2e} SNR gpu {oracle.snr_db(vid.frames, u):.3f} "
           f"oracle {oracle.snr_db(vid.frames, u_o):.3f}")
+    assert np.linalg.norm(u - u_o) <= 1e-6 * np.linalg.norm(u_o)
+
+
+def test_iaf_rt_encode_and_weighted_decode(torch_mod, oracle):
+    """Random thresholds end to end.  IAF_encode_rt (sigma != 0, per-neuron kappa) equals
+    the oracle's restatement of vtem/ensemble_encode.py:555-674 on the same N(0,1) draws:
+    counts identical, times within 1e-9 s (bit for bit in fact, same loop).  Then VTDM_prep
+    + VTDM over a sub-domain decode those spikes with the weights 1/(kappa sigma) normalised
+    over ALL neurons and baked into Dsw (vtem/vtdm.py:408-413, :444) and applied to q and
+    the reconstruction (:634-638): equal to the oracle's window decode with the same
+    weights."""
+    _banner("IAF_encode_rt + weighted decode")
+    from vtem_amd import vtdm
+    from vtem_amd.ensemble_encode import IAF_encode_rt
+    from vtem_amd.vtem import VTEM_IAF
+    vid, bank = _c0()
+    g = _gabor(bank)
+    N = bank.num_neurons
+    rng = np.random.default_rng(12)
+    kappa = rng.uniform(0.8, 1.25, N)
+    bias = np.full(N, 2.0) * kappa
+    mu = np.full(N, 0.04)
+    sigma = rng.uniform(0.002, 0.006, N)
+    dv0 = mu + sigma * rng.standard_normal(N)
+    v0 = rng.uniform(0, 1, N) * dv0
+    # encoder API vs the oracle on one call of filtered frames
+    v = oracle.filter_video(vid.frames, oracle.rf_bank(bank, 32, 32))
+    iaf = IAF_encode_rt(N, 0.01, rng=np.random.default_rng(40))
+    iaf.load_parameters(kappa, bias, mu, sigma, dv0, h_v0=v0)
+    lists, cnt = iaf.encode(v)
+    draws = np.random.default_rng(40).normal(size=(int(np.ceil(v.shape[0] / 0.1)), N))
+    lo, co, v0o, tco, dvo = oracle.iaf_encode_rt(v, 0.01, kappa, bias, mu, sigma, dv0, v0, draws)
+    np.testing.assert_array_equal(cnt, co)
+    assert max(np.abs(a - b).max(initial=0) for a, b in zip(lists, lo)) <= 1e-9
+    np.testing.assert_allclose(iaf.d_delta_value.cpu().numpy(), dvo, rtol=0, atol=0)
+    # VTEM_IAF with the rt population, then a weighted decode of a sub-domain
+    iaf2 = IAF_encode_rt(N, 0.01, rng=np.random.default_rng(41))
+    iaf2.load_parameters(kappa, bias, mu, sigma, dv0, h_v0=v0)
+    W = 2 * math.pi * 5 / 32
+    Wt = 2 * math.pi * 4
+    enc = VTEM_IAF(vid.frames, None, Wt, g, W, W, dx=1.0, dy=1.0, npinput=True, iaf=iaf2)
+    dom = [-16.0, 8.0, -16.0, 16.0]
+    Dsw, dirich, sel = vtdm.VTDM_prep(enc, None, None, 5, 5, domain=dom)
+    ch, x0, y0 = oracle.select_neurons(bank.h_x0, bank.h_y0, dom)
+    np.testing.assert_array_equal(sel, ch)
+    assert 0 < sel.size < N
+    u = vtdm.VTDM(enc, Dsw, dirich, 0.0, 1.0, 0.01, 5, 5, Mt=4, domain=dom, neurons=sel)
+    norm = oracle.decode_norm(kappa, sigma)[sel]
+    assert norm.max() < 1.0 or np.argmax(oracle.decode_norm(kappa, sigma)) in sel
+    rf_o = oracle.rf_gabor(bank.h_alpha[sel], bank.h_l[sel], x0, y0, bank.h_ab[sel], bank.KAPPA,
+                           32, 32, 32.0, 32.0)
+    Ds_o = oracle.compute_Ds_fft(rf_o, 1.0, 1.0, 5, 5, "ellipse")
+    Dsw_o = oracle.compute_Dsw(Ds_o, norm)
+    assert _rel(Dsw.cpu().numpy(), Dsw_o) < 1e-12
+    psi_o = oracle.compute_psi(Ds_o, 24, 32, 24.0, 32.0, W / 5, W / 5)
+    assert _rel(dirich.cpu().numpy(), psi_o) < 1e-11
+    dec = _oracle_window_decoder(oracle, enc.spikes, sel, kappa, mu, bias, Dsw_o, psi_o, Wt, 4, 0.01,
+                                 norm=norm)
+    u_o = oracle.vtdm_windows(dec, 0.0, 1.0, 0.01)
+    print(f"rt decode: rel diff {_rel(u, u_o):.2e}", flush=True)
     assert np.linalg.norm(u - u_o) <= 1e-6 * np.linalg.norm(u_o)

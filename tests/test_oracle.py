@@ -352,3 +352,116 @@ def test_video_generator_hermitian_and_scaled():
     c = vid.coeffs
     np.testing.assert_allclose(c, np.conj(c[::-1, ::-1, ::-1]), atol=1e-15)
     assert abs(c[2, 3, 3].imag) < 1e-15
+
+
+# --------------------------------------------------------------------------------------
+# Pins added for the random-threshold encoder, the pinv sign rule, q and selection
+# --------------------------------------------------------------------------------------
+
+def test_iaf_rt_constant_input_closed_form(oracle):
+    """Constant input y = (c + b)/kappa, one spike per step at most.  The reference's
+    rt loop (vtem/ensemble_encode.py:606-668) reads threshold delta_{k+1} = r_k sigma + mu
+    from slot k when spike k fires, writes the spike time into that slot, and subtracts
+    the NEW threshold from the step's remaining area (:623-627).  So right after spike k
+    the voltage sits at -(delta_{k+1} - delta_k), and spike k+1 needs an integral of
+    2 delta_{k+1} - delta_k:
+
+        t_0 = (delta_0 - v0) / y,   t_{k+1} = t_k + (2 delta_{k+1} - delta_k) / y.
+
+    A restatement that subtracted the old threshold (t_{k+1} = t_k + delta_{k+1}/y), read
+    the draw of the next slot, or wrote before reading fails this."""
+    rng = np.random.default_rng(11)
+    N, T, dt = 4, 20001, 1e-4
+    c = np.array([0.3, -0.2, 0.0, 0.7])
+    b = np.array([0.9, 1.1, 1.0, 1.3])
+    kap = np.array([1.0, 1.5, 0.8, 2.0])
+    mu = np.array([0.02, 0.025, 0.018, 0.03])
+    sig = np.array([0.002, 0.0025, 0.001, 0.004])
+    d0 = mu + sig * rng.standard_normal(N)
+    v0 = rng.uniform(0, 1, N) * d0
+    u = np.tile(c, (T, 1))
+    rnd = rng.standard_normal((2000, N))
+    lists, cnt, v0o, tco, dv = oracle.iaf_encode_rt(u, dt, kap, b, mu, sig, d0, v0, rnd)
+    y = (c + b) / kap
+    for j in range(N):
+        deltas = np.concatenate([[d0[j]], rnd[:, j] * sig[j] + mu[j]])
+        t = [(d0[j] - v0[j]) / y[j]]
+        while True:
+            k = len(t) - 1
+            nxt = t[-1] + (2 * deltas[k + 1] - deltas[k]) / y[j]
+            if nxt > (T - 1) * dt:
+                break
+            t.append(nxt)
+        t = np.array(t)
+        assert cnt[j] == t.size, (j, cnt[j], t.size)
+        np.testing.assert_allclose(lists[j], t, rtol=0, atol=1e-10)
+        assert dv[j] == deltas[cnt[j]]         # the carried threshold is the last one drawn
+        assert abs(tco[j] - (T - 1) * dt) < 1e-9
+
+
+def test_solve_eq_sym_is_the_reference_svd_formula(oracle):
+    """c = U diag(1/s) U^T q over the kept singular values, U the left singular vectors
+    (vtem/utils/linalg.py:622-630), on an indefinite symmetric matrix with a truncated
+    part: dividing by the signed eigenvalue (pinv(G)) fails here."""
+    rng = np.random.default_rng(21)
+    n = 40
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    w = np.concatenate([rng.uniform(0.5, 3.0, 20), -rng.uniform(0.5, 3.0, 12), 1e-12 * rng.standard_normal(8)])
+    G = (Q * w) @ Q.T
+    G = (G + G.T) / 2
+    q = rng.standard_normal(n)
+    U, s, _ = np.linalg.svd(G)
+    keep = s > 1e-8 * s[0]
+    ref = U[:, keep] @ ((U[:, keep].T @ q) / s[keep])
+    c = oracle.solve_eq_sym(G, q, rcond=1e-8)
+    np.testing.assert_allclose(c, ref, atol=1e-10 * np.abs(ref).max())
+    assert np.linalg.norm(c - np.linalg.pinv(G, rcond=1e-8, hermitian=True) @ q) > 1e-3 * np.linalg.norm(c)
+
+
+def test_decode_through_q_recovers_the_video(oracle):
+    """q from the t-transform, q_k = kappa delta - b (tk2 - tk1) (vtem/dirichlet.py:216),
+    then the D-space Tikhonov decode of a video inside the decoder's space: the
+    reconstruction reaches > 60 dB.  A wrong sign or term in q, or a misaligned interval,
+    leaves it near 0 dB."""
+    P, L, Lt, dur = 16, 2, 2, 1.0
+    vid = syn.make_video(P, P, dur, 100.0, L, L, Lt, seed=4)
+    bank = syn.Bank.concat([syn.gabor_grid_bank(16.0, 16.0, 2.0, 4), syn.gabor_grid_bank(16.0, 16.0, 4.0, 2)])
+    N = bank.num_neurons
+    rng = np.random.default_rng(5)
+    kap, b = np.ones(N), np.full(N, 2.5)
+    dl = rng.normal(0.04, 0.004, N)
+    v0 = rng.uniform(0, 1, N) * dl
+    RF = oracle.rf_bank(bank, P, P)
+    t_f = np.arange(int(dur / 1e-4) + 1) * 1e-4
+    vu = oracle.upsample(oracle.filter_video(vid.frames, RF), Lt, dur, 100.0, t_f)
+    lists, *_ = oracle.iaf_encode(vu, 1e-4, kap, b, dl, v0, max_spike=400)
+    tk1, tk2, nid, _ = oracle.select_intervals(lists, (0.0, dur))
+    q = oracle.compute_q(tk1, tk2, nid, kap, dl, b, np.ones(N))
+    x, y = oracle._grid(P, P, float(P), float(P))
+    B = oracle.spatial_basis(x, y, oracle.spatial_modes(L, L), 2 * np.pi / P, 2 * np.pi / P)
+    S = oracle.spatial_coeffs(RF, B)
+    Tm, tv = oracle.neuron_T(oracle.temporal_factors(tk1, tk2, Lt, 2 * np.pi / dur), q, nid, N)
+    A = oracle.gram_kron(S, Tm)
+    X = oracle.solve_tikhonov(A, oracle.rhs_kron(S, tv), oracle.tikhonov_lambda(A, tk1.size, 1e-12))
+    u = oracle.synthesize(X, B, oracle.temporal_basis(np.arange(100) / 100.0, Lt, 2 * np.pi / dur))
+    snr = oracle.snr_db(vid.frames, u.reshape(vid.frames.shape))
+    assert snr > 60.0, snr
+    qbad = oracle.compute_q(tk1, tk2, nid, kap, dl, -b, np.ones(N))
+    _, tvb = oracle.neuron_T(oracle.temporal_factors(tk1, tk2, Lt, 2 * np.pi / dur), qbad, nid, N)
+    Xb = oracle.solve_tikhonov(A, oracle.rhs_kron(S, tvb), oracle.tikhonov_lambda(A, tk1.size, 1e-12))
+    ub = oracle.synthesize(Xb, B, oracle.temporal_basis(np.arange(100) / 100.0, Lt, 2 * np.pi / dur))
+    assert oracle.snr_db(vid.frames, ub.reshape(vid.frames.shape)) < 10.0
+
+
+def test_select_neurons_strict_bounds(oracle):
+    """Centres exactly at domain -+ 0.01 are left out, centres just inside are kept
+    (vtem/utils/spike_store.py:266 uses strict comparisons); shift by the domain centre."""
+    dom = [-16.0, 8.0, -4.0, 12.0]
+    tol = 0.01
+    x0 = np.array([dom[0] - tol, np.nextafter(dom[0] - tol, 0), 0.0, dom[1] + tol,
+                   np.nextafter(dom[1] + tol, 0), 3.0, 3.0, 3.0])
+    y0 = np.array([0.0, 0.0, dom[2] - tol, 0.0, 0.0, np.nextafter(dom[2] - tol, 0), dom[3] + tol, 5.0])
+    ch, xs, ys = oracle.select_neurons(x0, y0, dom)
+    assert ch.tolist() == [1, 4, 5, 7]
+    np.testing.assert_array_equal(xs, x0[ch] - (-4.0))
+    np.testing.assert_array_equal(ys, y0[ch] - 4.0)

@@ -136,3 +136,25 @@ def test_spike_buffer_rate_rows():
     p = IAFParams(np.ones(3), np.full(3, 2.5), np.array([0.05, 0.025, 0.04]), np.zeros(3), np.zeros(3))
     assert max_spikes_rate(g, p, margin=2.0) == int(np.ceil(2.0 * (2.5 / 0.025) * 2.0)) + 16
     assert max_spikes_rate(g, p, margin=1.0) == 200 + 16
+
+
+def test_guard_reads_the_triangular_solve_expiry_word():
+    """The batched decoder's guard reads word 1 of each direct solve's flag workspace: an
+    expired bounded wait sends a low-precision group's blocks to the FP64 redo, and
+    raises for an FP64 group (its solutions are invalid).  Forced from the host with CPU
+    tensors; no kernel runs."""
+    import pytest
+    import torch
+    from vtem_amd._lib import VtemError
+    from vtem_amd.batch import BatchDecoder
+    bd = BatchDecoder(device="cpu", precision="bf16x3")
+    nrm = torch.tensor([[1.0, 1e-30], [1.0, 1e-30]], dtype=torch.float64)
+    info = torch.zeros(2, dtype=torch.int32)
+    ok = torch.zeros(6, dtype=torch.int32)
+    bad = ok.clone()
+    bad[1] = 1
+    assert bd._guard([([3, 4], (nrm, info, ok), "bf16x3")]) == ([], 0)
+    assert bd._guard([([3, 4], (nrm, info, bad), "bf16x3")]) == ([(3, "bf16x3"), (4, "bf16x3")], 0)
+    assert bd._guard([([3, 4], (nrm, info, None), "kpcg")]) == ([], 0)
+    with pytest.raises(VtemError):
+        bd._guard([([3, 4], (nrm, info, bad), "fp64")])

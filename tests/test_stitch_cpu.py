@@ -5,11 +5,12 @@ every pixel (the property the reference's Hann ramps rely on, vtem/vtdm.py:252-2
 import numpy as np
 import pytest
 
-from vtem_amd.stitch import StitchPlan, blend_weights, tile_origins
+from vtem_amd.stitch import StitchPlan, blend_weights, coverage, tile_origins
 
 
 @pytest.mark.parametrize("P,B,O", [(512, 64, 16), (128, 64, 16), (400, 100, 25), (64, 64, 16),
-                                   (100, 64, 16), (300, 96, 20)])
+                                   (100, 64, 16), (300, 96, 20), (115, 64, 16), (1080, 64, 16),
+                                   (410, 100, 25)])
 def test_tiles_cover_and_blend_to_one(P, B, O):
     org = tile_origins(P, B, O)
     assert org[0] == 0 and org[-1] + B == max(P, B)
@@ -20,6 +21,27 @@ def test_tiles_cover_and_blend_to_one(P, B, O):
         assert np.all(w > 0) and np.all(w <= 1)
         acc[a:a + B] += w
     np.testing.assert_allclose(acc[:P], 1.0, atol=1e-15)
+    # interleaved passes: tiles i and i + k (k = coverage) never overlap
+    k = coverage(org, B)
+    for i in range(len(org) - k):
+        assert org[i + k] >= org[i] + B
+
+
+@pytest.mark.parametrize("Px,Py,F", [(115, 1080, 410), (512, 512, 400), (100, 300, 250)])
+def test_plan_passes_are_disjoint_and_complete(Px, Py, F):
+    """Every block is in exactly one pass, and no two blocks of a pass overlap in all
+    three axes (so the plain += of the accumulation kernel has one writer per element)."""
+    p = StitchPlan(Px, Py, F, 64, 16, 100, 25)
+    seen = []
+    for grp in p.passes():
+        seen += grp
+        for a in range(len(grp)):
+            for b in range(a + 1, len(grp)):
+                (y1, x1, t1), (y2, x2, t2) = grp[a], grp[b]
+                ov = (abs(p.y0[y1] - p.y0[y2]) < 64 and abs(p.x0[x1] - p.x0[x2]) < 64
+                      and abs(p.f0[t1] - p.f0[t2]) < 100)
+                assert not ov, (grp[a], grp[b])
+    assert sorted(seen) == sorted(p.blocks)
 
 
 def test_config3_plan():

@@ -78,11 +78,17 @@ class Geometry:
     """One decode block: pixel grid, Dirichlet orders, periods and time grids.
 
     Px, Py, dx, dy    pixel grid of the block (RFs are evaluated on it);
-    Lx, Ly, Lt        Dirichlet orders; the spatial periods are the block extent and
-                      the temporal period is `period` (seconds);
+    Lx, Ly, Lt        Dirichlet orders; the spatial periods are `period_x`, `period_y`
+                      (default: the block extent) and the temporal period is `period`
+                      (seconds; default: the duration);
     fps               frame rate of the video and of the reconstruction;
     dt                IAF integration step;
     duration          length of the encoded / decoded window (seconds).
+
+    A block decoded inside a longer period (the reference's rule: period 2 pi M / W from
+    the orders and the bandwidth, independent of the block's extent, vtem/vtdm.py:385-388,
+    :964-967) keeps its pixel grid centred at 0 and only changes the basis frequencies
+    2 pi / period.
     """
     Px: int
     Py: int
@@ -96,6 +102,8 @@ class Geometry:
     dy: float = 1.0
     period: float = None
     mask: str = "rect"
+    period_x: float = None
+    period_y: float = None
 
     @property
     def T_period(self) -> float:
@@ -115,11 +123,11 @@ class Geometry:
 
     @property
     def Ox(self) -> float:
-        return 2 * math.pi / self.Sx
+        return 2 * math.pi / (self.Sx if self.period_x is None else self.period_x)
 
     @property
     def Oy(self) -> float:
-        return 2 * math.pi / self.Sy
+        return 2 * math.pi / (self.Sy if self.period_y is None else self.period_y)
 
     @property
     def W(self) -> float:

@@ -27,7 +27,7 @@ from typing import Any, Dict, List, Optional
 import numpy as np
 import torch
 
-from ._lib import lib, ptr, stream_ptr
+from ._lib import VtemError, lib, ptr, stream_ptr
 from .autotune import gram_tuner
 from .block import (F64, I32, Decoder, DecodeResult, SpikeTrains, mat_descs,
                     split_T_size)
@@ -260,7 +260,7 @@ class BatchDecoder:
             for i, r in zip(idx, res):
                 h.results[i] = r
             h.keep.append((g.keep, skeep, keep))
-            h.checks.append((idx, (nrm, info), pr))
+            h.checks.append((idx, (nrm, info, skeep[4]), pr))
         if kp:
             gs = [g for _, g in kp]
             X, nrm, info, st, boff, skeep = self._solve_kpcg(gs)
@@ -273,7 +273,7 @@ class BatchDecoder:
                 h.keep.append((g.keep, keep))
                 all_idx += idx
             h.keep.append(skeep)
-            h.checks.append((all_idx, (nrm, info), "kpcg"))
+            h.checks.append((all_idx, (nrm, info, None), "kpcg"))
         if side is not None:
             join = torch.cuda.Event()
             join.record(side)
@@ -313,17 +313,22 @@ class BatchDecoder:
 
     def _guard(self, checks):
         """(job index, precision) of every non-FP64 result that failed the guard (factor
-        lost positive definiteness, or FP64 relative residual above tol), and the number of
-        FP64 factorisations that failed."""
+        lost positive definiteness, FP64 relative residual above tol, or a triangular
+        solve whose bounded wait expired), and the number of FP64 factorisations that
+        failed.  An expired wait in an FP64 group raises: its results are invalid."""
         redo, failed_fp64 = [], 0
-        for idx, (nrm, info), pr in checks:
+        for idx, (nrm, info, sync), pr in checks:
             hn = nrm.cpu().numpy()
             inf = info.cpu().numpy()
+            expired = sync is not None and int(sync[1].item()) != 0
+            if expired and pr == "fp64":
+                raise VtemError("a triangular solve's bounded wait expired in the FP64 path: "
+                                "its solutions are invalid")
             for k, i in enumerate(idx):
                 if pr == "fp64":
                     failed_fp64 += int(inf[k] != 0)
                     continue
-                ok = inf[k] == 0 and np.isfinite(hn[k]).all() and \
+                ok = not expired and inf[k] == 0 and np.isfinite(hn[k]).all() and \
                     math.sqrt(hn[k, -1]) <= self.tol * math.sqrt(hn[k, 0])
                 if not ok:
                     redo.append((i, pr))
@@ -335,6 +340,10 @@ class BatchDecoder:
         whose refinement did not reach `tol`, are redone as one batched FP64 group
         (flagged)."""
         redo, failed_fp64, retried = [], 0, []
+        if not self.verify:
+            for _, (_, _, sync), _ in h.checks:
+                if sync is not None and int(sync[1].item()) != 0:
+                    raise VtemError("a triangular solve's bounded wait expired: its solutions are invalid")
         if self.verify:
             redo, failed_fp64 = self._guard(h.checks)
             kp = [i for i, pr in redo if pr == "kpcg"]
@@ -354,6 +363,7 @@ class BatchDecoder:
                 retried = kp
                 redo = [(i, pr) for i, pr in redo if pr != "kpcg"] + [(kp[j], "kpcg") for j in sorted(bad)]
         redo = [i for i, _ in redo]
+        h2 = None
         if redo:
             h2 = self.launch([h.jobs[i] for i in redo], ["fp64"] * len(redo))
             for i, rr in zip(redo, h2.results):
@@ -364,6 +374,11 @@ class BatchDecoder:
                            "fp64_not_pd": failed_fp64, "fallback_index": redo,
                            "kpcg_retries": len(retried)}
         torch.cuda.current_stream().synchronize()
+        if h2 is not None:
+            for _, (_, _, sync), _ in h2.checks:
+                if sync is not None and int(sync[1].item()) != 0:
+                    raise VtemError("a triangular solve's bounded wait expired in the FP64 redo: "
+                                    "its solutions are invalid")
         h.keep = []
         return h.results
 

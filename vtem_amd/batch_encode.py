@@ -118,9 +118,17 @@ class BatchEncoder:
     input, C_j = (1/F) Theta_f^T (dx dy V RF_j^T), are formed as RF (V^T Theta_f dx dy / F):
     the same product reassociated, so the F x P frames are contracted to Dt x P before the
     receptive fields are applied (Dt / F of the filtering flops).  Both products are grouped
-    skinny GEMMs over all blocks; the IAF is the batched time-parallel encoder."""
+    skinny GEMMs over all blocks; the IAF is the batched time-parallel encoder.
 
-    def __init__(self, device="cuda", root_mode: int = 0):
+    root_mode 1 (default) takes each crossing time from the cancellation-free form
+    2 dv / (sqrt(y1^2 + 2 dv a) + y1) of the reference's root (sqrt(y1^2 + 2 dv a) - y1) / a
+    (vtem/ensemble_encode.py:501, :521); the two are algebraically identical.  The
+    reference's form subtracts two nearly equal numbers when |a| is just above its 1e-12
+    switch, so input values 1e-15 apart (the reassociated coefficient products here
+    against frame-by-frame filtering) can move a spike by ~1e-8 s; with root mode 1 the
+    times of both agree to ~1e-12 s.  root_mode 0 keeps the reference's formula."""
+
+    def __init__(self, device="cuda", root_mode: int = 1):
         self.device = device
         self.root_mode = int(root_mode)
 

@@ -27,7 +27,7 @@ from typing import Optional
 import numpy as np
 import torch
 
-from ._lib import lib, ptr, stream_ptr
+from ._lib import VtemError, lib, ptr, stream_ptr
 from .autotune import gram_tuner
 from .basis import Geometry
 from .synthetic import Bank, IAFParams
@@ -352,6 +352,12 @@ class Decoder:
             ws = torch.zeros(need, dtype=I32, device=self.device)
             self._sync = ws
         return ws
+    def solve_expired(self) -> bool:
+        """True when a bounded wait of the triangular solves expired since the flag
+        workspace was zeroed (word 1 of the workspace, sticky; a host synchronisation)."""
+        ws = getattr(self, "_sync", None)
+        return ws is not None and int(ws[1].item()) != 0
+
     def lam_exact(self, Tm, n_dev, lam_rel):
         """[lam, tr(A)] with tr(A) = sum_j |S_j|^2 tr(T_j) in FP64 (no Gram needed)."""
         g = self.geom
@@ -495,10 +501,13 @@ class Decoder:
             if verify and precision != "fp64":
                 h = nrm.cpu().numpy()
                 ok = int(info.item()) == 0 and np.isfinite(h).all() and \
-                    math.sqrt(h[-1]) <= tol * math.sqrt(h[0])
+                    math.sqrt(h[-1]) <= tol * math.sqrt(h[0]) and not self.solve_expired()
                 if not ok:
+                    self._sync.zero_()
                     X, lam, info, nrm = self.solve_tikhonov(Tm, r, n_dev, lam_rel, "fp64")
                     fallback = True
+            if verify and self.solve_expired():
+                raise VtemError("a triangular solve's bounded wait expired: the solution is invalid")
         elif method == "rnn":
             lam = self.lam_exact(Tm, n_dev, lam_rel)
             eta = self.power_eta(Tm, lam) if rnn_eta is None else rnn_eta

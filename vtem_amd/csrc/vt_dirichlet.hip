@@ -138,14 +138,14 @@ VT_EXPORT int vt_eig_sym(double *A, int lda, int n, double *w, double *E, int *i
     return VT_OK;
 }
 
-// y_k <- y_k / w_k for |w_k| > rcond max|w|, else 0 (the truncation of solve_eq_sym,
-// vtem/utils/linalg.py:622-630 with singular values |w| of the symmetric G)
+// y_k <- y_k / |w_k| for |w_k| > rcond max|w|, else 0 (the truncation of solve_eq_sym,
+// vtem/utils/linalg.py:622-630: it divides by the singular values, |w| for a symmetric G)
 __global__ void eig_filter_kernel(double *__restrict__ y, const double *__restrict__ w, int n, double rcond)
 {
     const double wmax = fmax(fabs(w[0]), fabs(w[n - 1]));
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const double a = fabs(w[k]);
-        y[k] = (a > rcond * wmax) ? y[k] / w[k] : 0.0;
+        const double a = fabs(w[k]);       // the singular value of a symmetric matrix
+        y[k] = (a > rcond * wmax) ? y[k] / a : 0.0;
     }
 }
 

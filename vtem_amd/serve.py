@@ -23,7 +23,8 @@ The server:
 
 Latency is arrival -> result resident in HBM.  The plain mode is the per-block FP64
 path: one request at a time, in arrival order, no batching: the reference's encoder
-(frame filtering, serial IAF loop) and the FP64 Gram + Cholesky decode.
+(frame filtering, serial IAF loop) and the FP64 Gram + Cholesky decode.  Both modes take
+crossing times from the cancellation-free root (root mode 1, see BatchEncoder).
 
 The reference has no serving layer; its decode drivers run one spike file at a time
 (vtem/vtdm.py:18-309).
@@ -285,7 +286,7 @@ class Server:
                 # the reference's encoder per block: every frame filtered, then the
                 # temporal coefficients, then the serial one-lane-per-neuron IAF loop
                 e = r.encode
-                spikes = e.encoder.encode(e.frames, r.iaf, parallel=False)
+                spikes = e.encoder.encode(e.frames, r.iaf, parallel=False, root_mode=1)
                 r.spikes = spikes
             r.result = jb.decoder.decode(spikes, jb.t0, precision="fp64", method=jb.method,
                                          rnn_steps=jb.rnn_steps)

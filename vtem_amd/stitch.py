@@ -53,10 +53,22 @@ def tile_origins(P: int, B: int, overlap: int) -> List[int]:
     return out
 
 
+def coverage(origins: Sequence[int], B: int) -> int:
+    """The largest number of length-B tiles that cover one pixel.  With k = coverage,
+    tiles i and i + k never overlap (otherwise the start of tile i + k would be covered
+    k + 1 times), so k interleaved passes write disjoint tiles."""
+    cnt = np.zeros(max(origins) + B, np.int64)
+    for a in origins:
+        cnt[a:a + B] += 1
+    return int(cnt.max())
+
+
 def blend_weights(origins: Sequence[int], B: int) -> List[np.ndarray]:
     """Per tile a length-B weight: 1 inside, a raised-cosine ramp over the overlap with each
-    neighbour.  At a pixel covered by tiles i and i+1 the weights are cos^2 and sin^2 of
-    the same angle, so every pixel's weights sum to one."""
+    neighbour.  Where at most two tiles cover a pixel (every plan whose last tile is not
+    clamped into a third tile) the weights are cos^2 and sin^2 of the same angle and sum to
+    one.  Where a clamped last tile makes three tiles cover a pixel, each tile's ramps are
+    divided by the sum of the ramps of all tiles covering it, so they sum to one there too."""
     n = len(origins)
     ws = []
     for i, a in enumerate(origins):
@@ -70,6 +82,11 @@ def blend_weights(origins: Sequence[int], B: int) -> List[np.ndarray]:
             k = np.arange(L)
             w[B - L:] *= np.cos(0.5 * np.pi * (k + 0.5) / L) ** 2
         ws.append(w)
+    if n > 2 and coverage(origins, B) > 2:
+        tot = np.zeros(max(origins) + B)
+        for a, w in zip(origins, ws):
+            tot[a:a + B] += w
+        ws = [w / tot[a:a + B] for a, w in zip(origins, ws)]
     return ws
 
 
@@ -90,6 +107,21 @@ class StitchPlan:
         self.wx = blend_weights(self.x0, self.tile)
         self.wy = blend_weights(self.y0, self.tile)
         self.wt = blend_weights(self.f0, self.window)
+        # interleaved passes per axis: tiles i and i + k never overlap
+        self.kx = coverage(self.x0, self.tile)
+        self.ky = coverage(self.y0, self.tile)
+        self.kt = coverage(self.f0, self.window)
+
+    def passes(self):
+        """Groups of blocks (iy, ix, it) of which no two overlap: blocks whose indices agree
+        modulo each axis' coverage, one group per residue triple."""
+        out = []
+        for ry in range(self.ky):
+            for rx in range(self.kx):
+                for rt in range(self.kt):
+                    out.append([(iy, ix, it) for (iy, ix, it) in self.blocks
+                                if iy % self.ky == ry and ix % self.kx == rx and it % self.kt == rt])
+        return [g for g in out if g]
 
     @property
     def tiles(self):
@@ -121,7 +153,13 @@ class TiledCodec:
     def __init__(self, Px: int, Py: int, duration: float, Lt_video: int, bank: Bank,
                  tile: int = 64, overlap: int = 16, window: float = 1.0,
                  window_overlap: float = 0.25, L: int = 8, Lt: int = 8, fps: float = 100.0,
-                 device="cuda"):
+                 Wx: Optional[float] = None, Wt: Optional[float] = None, device="cuda"):
+        """Wx (rad per pixel) and Wt (rad/s) are the video's bandwidths.  Each block is
+        decoded in the Dirichlet space of orders (L, L, Lt) whose periods follow from the
+        orders and the bandwidths, 2 pi L / Wx pixels and 2 pi Lt / Wt seconds, as the
+        reference sets a decoder's periods (vtem/vtdm.py:385-388, :964-967) independently of
+        the tile it decodes (:212-217); a block then sits inside one period with margin.
+        Without a bandwidth the period is the tile's own extent (or window)."""
         self.device = device
         frames = int(round(duration * fps))
         self.plan = StitchPlan(Px, Py, frames, tile, overlap, int(round(window * fps)),
@@ -129,9 +167,17 @@ class TiledCodec:
         self.fps = fps
         # the encoder integrates a tile over the whole duration (the video's period)
         self.enc_geom = Geometry(tile, tile, L, L, Lt_video, duration, fps=fps)
+        self.bank = bank
         self.encoder = Encoder(self.enc_geom, bank, device=device)
-        # every window is a block of the decoder's own periodic model
-        self.dec_geom = Geometry(tile, tile, L, L, Lt, window, fps=fps)
+        # every window is a block of the decoder's periodic model, periods from M / W
+        per_x = None if Wx is None else 2 * np.pi * L / float(Wx)
+        per_t = None if Wt is None else 2 * np.pi * Lt / float(Wt)
+        if per_x is not None and per_x < tile:
+            raise ValueError("the decoder's spatial period 2 pi L / Wx is shorter than the tile")
+        if per_t is not None and per_t < window:
+            raise ValueError("the decoder's temporal period 2 pi Lt / Wt is shorter than the window")
+        self.dec_geom = Geometry(tile, tile, L, L, Lt, window, fps=fps, period=per_t,
+                                 period_x=per_x, period_y=per_x)
         self.S = spatial_coeffs(self.encoder.RF, self.dec_geom)
         dev = lambda w: torch.as_tensor(np.ascontiguousarray(w), dtype=F64, device=device)
         self.wx = [dev(w) for w in self.plan.wx]
@@ -232,13 +278,12 @@ class TiledCodec:
         self.last_stats = {"blocks": len(jobs), "fallbacks": fallbacks, "kpcg_retries": retries}
         out = torch.zeros((p.frames, p.Py, p.Px), dtype=F64, device=self.device)
         keep = []
-        # blocks that share a parity in every axis never overlap: 8 accumulation passes
-        for par in range(8):
-            items = []
-            for (iy, ix, it), fr in zip(where, frames):
-                if (iy & 1) == (par & 1) and (ix & 1) == ((par >> 1) & 1) and (it & 1) == ((par >> 2) & 1):
-                    items.append((fr, self.wt[it], self.wy[iy], self.wx[ix], p.f0[it], p.y0[iy], p.x0[ix]))
-            if items:
-                keep.append(stitch_accumulate(out, items))
+        # blocks whose indices agree modulo each axis' coverage never overlap: one
+        # accumulation per group (8 for plans covered at most twice), in a fixed order
+        fr_of = dict(zip(where, frames))
+        for grp in p.passes():
+            items = [(fr_of[(iy, ix, it)], self.wt[it], self.wy[iy], self.wx[ix], p.f0[it], p.y0[iy], p.x0[ix])
+                     for (iy, ix, it) in grp]
+            keep.append(stitch_accumulate(out, items))
         self._keep = keep
         return out

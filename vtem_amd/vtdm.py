@@ -64,14 +64,31 @@ def _hann_ramp(n: int) -> np.ndarray:
 
 
 def select_neurons(rf, domain, tol: float = 0.01):
-    """Neurons whose RF centre lies in the domain [xs, xe, ys, ye] (+- tol), with centres
-    shifted to the domain's centre (vtem/utils/spike_store.py:243-277).  Returns (indices,
-    x0 shifted, y0 shifted)."""
+    """Neurons whose RF centre lies strictly inside the domain [xs, xe, ys, ye] widened by
+    tol on every side, with centres shifted to the domain's centre
+    (vtem/utils/spike_store.py:243-277; strict comparisons as at :266, shift as at
+    :260-271).  Returns (indices, x0 shifted, y0 shifted)."""
     xs, xe, ys, ye = domain
     x0, y0 = np.asarray(rf.h_x0), np.asarray(rf.h_y0)
-    sel = np.nonzero((x0 >= xs - tol) & (x0 <= xe + tol) & (y0 >= ys - tol) & (y0 <= ye + tol))[0]
-    cx, cy = (xs + xe) / 2, (ys + ye) / 2
+    sel = np.nonzero((x0 > xs - tol) & (x0 < xe + tol) & (y0 > ys - tol) & (y0 < ye + tol))[0]
+    cx, cy = xs + (xe - xs) / 2, ys + (ye - ys) / 2
     return sel, x0[sel] - cx, y0[sel] - cy
+
+
+def decode_norm(iaf, neurons) -> np.ndarray:
+    """Per selected neuron the decoder's weight: 1/(kappa sigma) for a random-threshold
+    neuron (sigma != 0), else 1, divided by its maximum over ALL neurons, then taken at
+    `neurons` (vtem/vtdm.py:408-413).  VTDM uses the same array for q and the
+    reconstruction (vtem/vtdm.py:634-638 means the same rule; there the reference indexes
+    the selected kappa with full-population indices, which only agrees when every neuron
+    is selected or kappa is uniform; the full kappa is used here)."""
+    kappa = np.asarray(iaf.h_kappa, np.float64)
+    sigma = np.asarray(getattr(iaf, "h_sigma", np.zeros_like(kappa)), np.float64)
+    h = np.ones(kappa.size)
+    nz = sigma != 0
+    h[nz] = 1.0 / (kappa[nz] * sigma[nz])
+    h = h / h.max()
+    return h[np.asarray(neurons, np.int64)]
 
 
 def _sub_rf(rf, sel, x0, y0, shape, dx, dy, scale):
@@ -115,8 +132,7 @@ def VTDM_prep(spikefile, Dswfilename=None, dirichfilename=None, Mx=None, My=None
     rfPy = int(round(2 * math.pi * My / Wy / enc.dy))
     period_rf = _sub_rf(enc.rf, sel, x0, y0, (rfPy, rfPx), enc.dx, enc.dy, scale)
     Ds = period_rf.compute_Ds(Mx, My, mask=mask)
-    norm = np.ones(sel.size)
-    Dsw = period_rf.compute_Dsw(Ds, Mx, My, norm)
+    Dsw = period_rf.compute_Dsw(Ds, Mx, My, decode_norm(enc.iaf, sel))
     decSx, decSy = dom[1] - dom[0], dom[3] - dom[2]
     decPx, decPy = int(round(decSx / enc.dx)), int(round(decSy / enc.dy))
     dirich = period_rf.compute_dirich_space(Ds, Mx, My, decPx, decPy, decSx, decSy, Wx, Wy)
@@ -178,11 +194,7 @@ def VTDM(spikefile, Dswfilename, dirichfilename, start_time, end_time, dt, Mx, M
     kappa = np.asarray(enc.iaf.h_kappa)[neurons]
     delta = np.asarray(enc.iaf.h_delta)[neurons]
     bias = np.asarray(enc.iaf.h_bias)[neurons]
-    sigma = np.asarray(getattr(enc.iaf, "h_sigma", np.zeros_like(enc.iaf.h_kappa)))[neurons]
-    norm = np.ones(len(neurons))
-    nz = sigma != 0
-    norm[nz] = 1.0 / (kappa[nz] * sigma[nz])
-    norm = norm / norm.max()
+    norm = decode_norm(enc.iaf, neurons)
     end_time = float(end_time)
     if stitching:
         stitch_interval = 20 * dt if stitch_interval is None else float(stitch_interval)
