@@ -118,6 +118,19 @@ int vt_iaf_encode_par(const void *input, int in_dtype, long long input_ld, int N
                       long long ws_bytes, void *stream);
 /* Host query: workspace bytes of one time-parallel encoder call. */
 int vt_iaf_par_workspace_size(int N, int size, long long *bytes_host);
+/* vt_iaf_encode_par without the workspace argument: the segment workspace is a static
+ * device array of vt_noalloc_scratch_bytes() bytes per GPU (64 MiB), so the call allocates
+ * nothing.  No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Returns -5 when vt_iaf_par_workspace_size(N, size) exceeds the array.  All
+ * calls on one GPU share its array: calls that overlap on one GPU race. */
+int vt_iaf_encode_par_noalloc(const void *input, int in_dtype, long long input_ld, int N,
+                              int size, double *spikes, long long spike_ld, int max_spike,
+                              double *v0, const double *kappa, const double *bias,
+                              const double *delta, double *time_count, int *spike_count,
+                              double dt, int root_mode, void *stream);
+/* Host query: bytes of the per-GPU array of vt_iaf_encode_par_noalloc. */
+int vt_noalloc_scratch_bytes(long long *bytes_host);
 /* Batched time-parallel trig encoder: nd descriptors (device array) of the layout
  *   struct { const double *coef; long long coef_ld, coef_ns; const double *basis;
  *            const void *input; long long input_ld;          (unused here)

@@ -34,6 +34,8 @@ SIGNATURES = {
     "vt_iaf_encode_par": [P, I, L, I, I, P, L, I, P, P, P, P, P, P, D, I, P, L, P],
     "vt_iaf_encode_trig_batched": [P, I, I, I, I, I, P],
     "vt_iaf_par_workspace_size": [I, I, P],
+    "vt_iaf_encode_par_noalloc": [P, I, L, I, I, P, L, I, P, P, P, P, P, P, D, I, P],
+    "vt_noalloc_scratch_bytes": [P],
     "vt_iaf_par_desc_size": [P],
     "vt_iaf_encode_rt": [P, L, I, I, P, L, I, P, P, P, P, P, P, D, P, P, I, P],
     "vt_intervals": [P, L, I, P, I, D, D, P, P, P, P, P, I, P],

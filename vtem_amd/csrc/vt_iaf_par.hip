@@ -24,6 +24,7 @@
 // size / IAFP_SEG segments, and the launch holds N x size / IAFP_SEG lanes.
 #include "vt_common.hpp"
 #include "vt_iaf.hpp"
+#include <mutex>
 
 #define IAFP_SEG 64        // sample intervals per segment
 #define IAFP_WAVES 4       // segments per workgroup (one wave each, 64 neurons per wave)
@@ -484,6 +485,67 @@ VT_EXPORT int vt_iaf_encode_par(const void *input, int in_dtype, long long input
         iafp_sampled_launch<float>(d, g, gs, s);
     VT_LAUNCH_CHECK();
     return VT_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// The same encoder for callers that cannot allocate device memory (code under HIP graph
+// capture, tight serving loops, harnesses that forbid hipMalloc): the segment workspace
+// lives in a file-scope device array of VT_NOALLOC_SCRATCH_BYTES, one per GPU, held until
+// the process ends.  Its address is taken once per device with hipGetSymbolAddress (the
+// array's name in host code is only a stand-in) and used from then on.  Every call on a
+// GPU shares that GPU's array: calls that overlap on one GPU (two streams, two threads)
+// race on it.  Pass 1 writes every workspace entry the scan and the walk read, so the
+// contents a previous call left need no clearing.
+// ------------------------------------------------------------------------------------
+#define VT_NOALLOC_SCRATCH_BYTES (64LL << 20)
+#define VT_MAX_DEVICES 64
+
+static __device__ __align__(256) unsigned char scratch[VT_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mutex;
+static void *g_scratch_ptr[VT_MAX_DEVICES];
+
+// Device address of the current GPU's scratch array (looked up on the first call per device).
+static int scratch_address(void **out)
+{
+    int dev = 0;
+    VT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= VT_MAX_DEVICES) return VT_EUNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (!ptr) return VT_EUNSUPPORTED;
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return VT_OK;
+}
+
+VT_EXPORT int vt_noalloc_scratch_bytes(long long *bytes)
+{
+    VT_CHECK_ARG(bytes);
+    *bytes = VT_NOALLOC_SCRATCH_BYTES;
+    return VT_OK;
+}
+
+VT_EXPORT int vt_iaf_encode_par_noalloc(const void *input, int in_dtype, long long input_ld,
+                                        int N, int size, double *spikes, long long spike_ld,
+                                        int max_spike, double *v0, const double *kappa,
+                                        const double *bias, const double *delta,
+                                        double *time_count, int *spike_count, double dt,
+                                        int root_mode, void *stream)
+{
+    VT_CHECK_ARG(N >= 0 && size >= 0);
+    const long long need = iafp_ws_bytes(N, size);
+    if (need > VT_NOALLOC_SCRATCH_BYTES) return VT_ERANGE;
+    void *ws = nullptr;
+    int r = scratch_address(&ws);
+    if (r) return r;
+    return vt_iaf_encode_par(input, in_dtype, input_ld, N, size, spikes, spike_ld, max_spike, v0,
+                             kappa, bias, delta, time_count, spike_count, dt, root_mode, ws,
+                             need, stream);
 }
 
 // Batched trig encode: one descriptor per block (device array), launched together.  The
