@@ -9,8 +9,9 @@ IAF parameters and decoder data (spatial coefficients) are resident in HBM when 
 starts; serving it filters and IAF-encodes the frames into spike trains and decodes them
 into 100 fps frames over its duration.
 
-One step = one wave of R requests with Poisson arrivals at an offered rate above the
-server's capacity (default), served by
+One step = one wave of R requests with seeded Poisson arrivals at `--overload` (default 2)
+times the server's capacity, the capacity being measured on a closed batch of the same R
+requests just before; served by
   chosen:     the served path (continuous batching on 2 streams, HBM-budget admission):
               batched encode (reassociated filter products + time-parallel IAF), then the
               batched Kronecker-preconditioned CG solver: FP64 conjugate gradients
@@ -22,8 +23,10 @@ server's capacity (default), served by
               (every frame filtered, serial one-lane-per-neuron IAF) and FP64 Gram + FP64
               Cholesky.
 `value` = chosen decoded frames/s (frames of the wave / time from first arrival to last
-result).  A latency sweep at sub-saturation rates gives p50/p99 (arrival -> result in
-HBM).  SNR of every request is checked against its plain FP64 decode (|dSNR| < 0.1 dB).
+result), encode + decode.  The closed-batch figure, a decode-only chosen-vs-plain figure
+(spikes resident to frames resident) and a latency sweep at 0.25/0.5/0.75/1.0 of capacity
+(p50/p99, arrival -> result in HBM) are reported beside it.  SNR of every request is
+checked against its plain FP64 decode (|dSNR| < 0.1 dB).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--requests R] [--no-cpu]
 """
@@ -88,6 +91,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-sweep", action="store_true", help="skip the latency sweep")
     p.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
+    p.add_argument("--overload", type=float, default=2.0,
+                   help="Poisson arrival rate of the timed steps, in multiples of the closed-batch capacity")
     p.add_argument("--no-precision", action="store_true", help="skip the 128x128 precision report")
     p.add_argument("--precision-only", action="store_true", help="only the 128x128 precision report")
     p.add_argument("--no-c3", action="store_true", help="skip the 512x512 stitched batching form")
@@ -382,6 +387,20 @@ def stitched_c3(seed=0, precision="kpcg", max_batch=1024):
     return res
 
 
+KERNEL_SOURCES = ("vtem_amd/csrc/vt_batch.hip", "vtem_amd/csrc/vt_skinny.hpp", "vtem_amd/csrc/vt_common.hpp")
+
+
+def kernel_source_sha16() -> str:
+    """Hash of the dominant kernel's sources: the committed HBM-traffic figure is used only
+    while these files are unchanged since it was measured."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def copy_calibration(nbytes=512 << 20, reps=5):
     """A plain device copy of known size (the scale reference for HBM byte counters)."""
     import torch
@@ -470,28 +489,37 @@ def main():
     R = len(items)
     frames = sum(g.frames for _, g, _ in items)
     D = np.array([g.D for _, g, _ in items])
-    zeros = np.zeros(R)                       # offered load above capacity
+    zeros = np.zeros(R)                       # a closed batch: every request at t = 0
 
     def run_form(form):
+        """Warm-up waves as closed batches; one more closed batch gives the capacity
+        (requests/s); the timed steps are Poisson arrivals at args.overload x that capacity
+        (a fresh seeded arrival sequence per step), served by continuous batching."""
         server = Server(max_batch=args.max_batch, streams=args.streams, precision=form,
                         rnn_lanes=args.rnn_lanes, graph_chunk=args.graph_chunk,
                         adaptive=form != "kpcg")
         for _ in range(args.warmup):           # allocator, code objects, precision policy
             server.serve(make_requests(items, zeros, form))
+        torch.cuda.synchronize()
+        closed = server.serve(make_requests(items, zeros, form))
+        cap = R / closed["wall_s"]
+        rate = args.overload * cap
         stats, reqs = [], None
-        for _ in range(args.steps):
-            reqs = make_requests(items, zeros, form)
+        for k in range(args.steps):
+            arr = poisson_arrivals(R, rate, np.random.default_rng(args.seed + 1000 + k))
+            reqs = make_requests(items, arr, form)
             torch.cuda.synchronize()
             stats.append(server.serve(reqs))
         wall = sum(st["wall_s"] for st in stats)
-        return server, stats, reqs, wall
+        return server, stats, reqs, wall, {"frames_per_s": round(frames / closed["wall_s"], 2),
+                                           "capacity_rps": round(cap, 2), "offered_rps": round(rate, 2)}
 
     # plain (per-block FP64) on the same wave
     plain_reqs = make_requests(items, zeros)
     Server().serve(plain_reqs, mode="plain")              # warm
     plain_reqs = make_requests(items, zeros)
     plain = Server().serve(plain_reqs, mode="plain")
-    server, stats, chosen_reqs, wall = run_form(args.form)
+    server, stats, chosen_reqs, wall, closed = run_form(args.form)
     value = frames * args.steps / wall
 
     def snr_diffs(reqs):
@@ -507,13 +535,28 @@ def main():
             for (blk, geom, _), rp in zip(items, plain_reqs)]
     compare = None
     if args.compare:
-        _, cst, creqs, cwall = run_form(args.compare)
+        _, cst, creqs, cwall, _ = run_form(args.compare)
         cd = snr_diffs(creqs)
         compare = {"form": args.compare, "frames_per_s": round(frames * args.steps / cwall, 2),
                    "p50_ms": round(cst[-1]["p50_ms"], 3), "p99_ms": round(cst[-1]["p99_ms"], 3),
                    "fp64_fallbacks": cst[-1]["fallbacks"],
                    "speedup_vs_plain": round(frames * args.steps / cwall / plain["frames_per_s"], 3),
                    "snr_max_abs_diff_vs_plain_db": float(f"{max(cd):.3g}")}
+    # decode only (spikes resident -> frames resident), chosen vs plain, closed batches
+    def decode_requests(form):
+        from vtem_amd.serve import Request, footprint_bytes
+        return [Request(i, 0.0, it.job, it.geom.frames, footprint_bytes(it.job, form))
+                for i, it in enumerate(items)]
+    dserver = Server(max_batch=args.max_batch, streams=args.streams, precision=args.form,
+                     rnn_lanes=args.rnn_lanes, graph_chunk=args.graph_chunk, adaptive=args.form != "kpcg")
+    dserver.serve(decode_requests(args.form))                     # warm
+    dwall = sum(dserver.serve(decode_requests(args.form))["wall_s"] for _ in range(args.steps))
+    Server().serve(decode_requests("fp64"), mode="plain")         # warm
+    dplain = Server().serve(decode_requests("fp64"), mode="plain")
+    decode_only = {"chosen_frames_per_s": round(frames * args.steps / dwall, 2),
+                   "plain_frames_per_s": round(dplain["frames_per_s"], 2),
+                   "speedup_vs_plain": round(frames * args.steps / dwall / dplain["frames_per_s"], 3),
+                   "note": "spikes resident to frames resident, closed batch of the same requests"}
     # roofline of the dominant stage (HIP events on the launch stream, one instrumented wave)
     # one stream, so that a stage's events bracket only its own kernels
     iserver = Server(max_batch=args.max_batch, streams=1, precision=args.form,
@@ -533,7 +576,8 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("blocks") == dk["blocks"] and tr.get("requests") == R and tr.get("seed") == args.seed:
+        if tr.get("blocks") == dk["blocks"] and tr.get("requests") == R and tr.get("seed") == args.seed \
+                and tr.get("source_sha16") == kernel_source_sha16():
             traffic = tr.get("traffic_bytes")
     line = {
         "metric": "decoded frames/s at fixed SNR (chosen vs plain form) + p50/p99 block latency",
@@ -549,7 +593,8 @@ def main():
         "dtype": "f64" if args.form == "kpcg" else "bf16x3+f32+f64",
         "data": "synthetic (seeded bandlimited videos resident in HBM; encode and decode timed)",
         "config": {"workload": "serving form: blocks {32,48,64,96}^2 px x {0.5,1,2} s, "
-                               "N=round(2D/9), Poisson arrivals above capacity",
+                               f"N=round(2D/9), Poisson arrivals at {args.overload:g}x the closed-batch "
+                               "capacity, continuous batching",
                    "requests_per_step": R, "frames_per_step": frames, "D_min": int(D.min()),
                    "D_max": int(D.max()), "max_batch": args.max_batch, "streams": args.streams,
                               "rnn_lanes": args.rnn_lanes, "graph_chunk": args.graph_chunk},
@@ -558,7 +603,9 @@ def main():
                    "batches": stats[-1]["batches"], "fp64_fallbacks": stats[-1]["fallbacks"],
                    "host_launch_ms": round(stats[-1]["host_launch_s"] * 1e3, 2),
                    "host_finalize_ms": round(stats[-1]["host_finalize_s"] * 1e3, 2),
-                   "wave_ms": round(stats[-1]["wall_s"] * 1e3, 2)},
+                   "wave_ms": round(stats[-1]["wall_s"] * 1e3, 2),
+                   "closed_batch": closed},
+        "decode_only": decode_only,
         "plain": {"frames_per_s": round(plain["frames_per_s"], 2),
                   "p50_ms": round(plain["p50_ms"], 3), "p99_ms": round(plain["p99_ms"], 3)},
         "speedup_vs_plain": round(value / plain["frames_per_s"], 3),
@@ -581,10 +628,10 @@ def main():
     }
     if not args.no_sweep:
         # p50/p99 over LAT_REQUESTS requests per rate point: the prepared blocks, cycled
-        cap_rps = R / (wall / args.steps)
+        cap_rps = closed["capacity_rps"]
         sweep = []
         cyc = [items[i % R] for i in range(LAT_REQUESTS)]
-        for frac in (0.5, 0.8):
+        for frac in (0.25, 0.5, 0.75, 1.0):
             rate = frac * cap_rps
             arr = poisson_arrivals(LAT_REQUESTS, rate, np.random.default_rng(args.seed + 17))
             st = server.serve(make_requests(cyc, arr))
@@ -598,11 +645,17 @@ def main():
         line["stitched_c3"] = stitched_c3(args.seed)
     if not args.no_cpu:
         fps, used, t = cpu_baseline(items, args.cpu_budget)
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=1):
+            fps1, used1, t1 = cpu_baseline(items, args.cpu_budget / 2)
         line["cpu_baseline"] = {"value": round(fps, 3), "unit": "frames/s", "cores": blas_threads(),
-                                "kind": "port",
+                                "host_cpus": os.cpu_count(), "kind": "port",
                                 "sample": f"float64 oracle encode (RF bank, frame filtering, "
                                           f"upsampling, serial IAF) + decode of the first {used} "
-                                          f"requests ({t:.1f} s)"}
+                                          f"requests ({t:.1f} s); cores = the BLAS pool's threads "
+                                          f"(the serial IAF loop uses one)",
+                                "one_thread": {"value": round(fps1, 3), "cores": 1,
+                                               "sample": f"the first {used1} requests ({t1:.1f} s)"}}
     print(json.dumps(line), flush=True)
 
 

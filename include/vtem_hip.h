@@ -289,7 +289,9 @@ int vt_lower_to_f32(float *dst, long long ldd, const double *src, long long lds,
 
 /* ---------------------------------------------------------------- spike-space path --- */
 
-/* G (n x n) by the reference's separable closed form.  Replaces compute_G_Kernel
+/* G (n x n) = Gt(k,l) Dsw[j(k), j(l)], Gt the double integral of the temporal Dirichlet
+ * kernel over intervals k and l, formed as the dot product of the two intervals' integrals
+ * of the real temporal basis (Mt <= 1024, else -5).  Replaces compute_G_Kernel
  * (vtem/dirichlet.py:272-330). */
 int vt_G_spike(double *G, long long ldg, const double *tk1, const double *tk2, double Wt,
                int Mt, const double *Dsw, long long ldw, const int *neuron_ind, int n,

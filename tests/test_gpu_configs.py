@@ -1,0 +1,158 @@
+"""The plain-method and precision configurations at their full sizes against the oracle.
+
+* 64 x 64 px, 2 s, Lx = Ly = Lt = 8 (D = 4913), 2048 neurons (dyadic Gabor + DoG), IAF
+  b = 2.5, delta ~ N(0.04, 0.004^2): the RNN gradient decoder for 500 iterations through
+  the served batched decoder, eta = 1/lambda_max from 50 power steps, against the
+  oracle's D-space iterates (vtem/neural_network.py:78-92, :102-128) with its own power
+  iteration: eta and the iterate within 1e-9 relative, SNR within 0.1 dB.
+* 128 x 128 px, 2 s, Lx = Ly = 12, Lt = 8 (D = 10625), 8192 neurons: the encode against
+  the oracle (counts identical, spike times within 1e-9 s, cancellation-free root); the
+  FP64, split-BF16 x3 and FP16 Grams against the oracle's Kronecker Gram on 48 sampled
+  rows (relative Frobenius error: FP64 <= 1e-10, split-BF16 <= 1e-5, FP16 <= 1e-3); and every
+  solver form's SNR within 0.1 dB of the oracle's FP64 Tikhonov solve."""
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _banner(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def _oracle_power_eta(A, lam, iters=50):
+    """1/rho with rho the Rayleigh quotient of A + lam I at the last of `iters` normalised
+    power steps from the all-ones vector."""
+    v = np.ones(A.shape[0])
+    eta = 0.0
+    for _ in range(iters):
+        w = A @ v + lam * v
+        eta = float(v @ v) / float(v @ w)
+        v = w / np.linalg.norm(w)
+    return eta
+
+
+def test_config1_rnn500_vs_oracle(oracle):
+    import torch
+    from vtem_amd._lib import require_gpu
+    require_gpu()
+    from vtem_amd import synthetic as syn
+    from vtem_amd.basis import Geometry
+    from vtem_amd.batch import BatchDecoder, BlockJob
+    from vtem_amd.block import Decoder, Encoder, dev, spatial_coeffs
+    _banner("config 1: RNN decoder, 500 iterations")
+    geom = Geometry(64, 64, 8, 8, 8, 2.0)
+    vid = syn.make_video(64, 64, 2.0, 100.0, 8, 8, 8, seed=0)
+    bank = syn.bank_dyadic(64.0, 64.0)
+    iafp = syn.iaf_params(bank.num_neurons, 2.5, 1.0, 0.04, 0.004, np.random.default_rng(1))
+    enc = Encoder(geom, bank)
+    st = enc.encode(dev(vid.frames.reshape(geom.frames, -1)), iafp, root_mode=1)
+    dec = Decoder(geom, spatial_coeffs(enc.RF, geom), iafp.kappa, iafp.delta, iafp.bias)
+    res = BatchDecoder(precision="kpcg").decode([BlockJob(dec, st, method="rnn", rnn_steps=500)])[0]
+    torch.cuda.synchronize()
+    eta_g = float(res.state.cpu().numpy()[0])
+    X_g = res.X.cpu().numpy()
+    fr = res.frames.cpu().numpy().reshape(vid.frames.shape)
+    # oracle on the same spikes
+    N = bank.num_neurons
+    lists = st.to_lists()
+    RF = oracle.rf_bank(bank, 64, 64)
+    B = geom.spatial().reshape(64, 64, -1)
+    S = oracle.spatial_coeffs(RF, B)
+    tk1, tk2, nid, _ = oracle.select_intervals(lists, (0.0, 2.0))
+    q = oracle.compute_q(tk1, tk2, nid, iafp.kappa, iafp.delta, iafp.bias, np.ones(N))
+    Tm, tv = oracle.neuron_T(oracle.temporal_factors(tk1, tk2, 8, geom.W), q, nid, N)
+    A = oracle.gram_kron(S, Tm)
+    A = np.tril(A) + np.tril(A, -1).T
+    lam = oracle.tikhonov_lambda(A, tk1.size)
+    eta = _oracle_power_eta(A, lam)
+    X_o = oracle.rnn_dspace(A, oracle.rhs_kron(S, tv), eta, 500, lam)
+    u_o = oracle.synthesize(X_o, B, oracle.temporal_basis(geom.frame_times(), 8, geom.W))
+    snr_o, snr_g = oracle.snr_db(vid.frames, u_o), oracle.snr_db(vid.frames, fr)
+    rel = float(np.linalg.norm(X_g - X_o) / np.linalg.norm(X_o))
+    print(f"eta gpu {eta_g:.12e} oracle {eta:.12e}; iterate rel diff {rel:.2e}; "
+          f"SNR gpu {snr_g:.4f} oracle {snr_o:.4f} dB", flush=True)
+    assert abs(eta_g - eta) <= 1e-9 * eta
+    assert rel <= 1e-9
+    assert abs(snr_g - snr_o) < 0.1
+
+
+def test_config2_precision_form_vs_oracle(oracle):
+    import torch
+    from vtem_amd._lib import require_gpu
+    require_gpu()
+    from vtem_amd import synthetic as syn
+    from vtem_amd.basis import Geometry
+    from vtem_amd.batch import BatchDecoder, BlockJob
+    from vtem_amd.block import Decoder, Encoder, dev, spatial_coeffs
+    _banner("config 2: 128^2 x 2 s, D = 10625, 8192 neurons")
+    geom = Geometry(128, 128, 12, 12, 8, 2.0)
+    vid = syn.make_video(128, 128, 2.0, 100.0, 12, 12, 8, seed=0)
+    bank = syn.bank_dyadic(128.0, 128.0)
+    N = bank.num_neurons
+    assert N == 8192 and geom.D == 10625
+    iafp = syn.iaf_params(N, 2.5, 1.0, 0.04, 0.004, np.random.default_rng(1))
+    enc = Encoder(geom, bank)
+    st = enc.encode(dev(vid.frames.reshape(geom.frames, -1)), iafp, root_mode=1)
+    torch.cuda.synchronize()
+    # encode vs the oracle
+    RF = oracle.rf_bank(bank, 128, 128)
+    v = oracle.filter_video(vid.frames.reshape(geom.frames, -1), RF)
+    vu = oracle.upsample(v, 8, 2.0, 100.0, geom.iaf_times())
+    del v
+    lists, cnt, *_ = oracle.iaf_encode(vu, geom.dt, iafp.kappa, iafp.bias, iafp.delta, iafp.v0,
+                                       root_mode=1, max_spike=4 * st.spikes.shape[0])
+    del vu
+    gc, gs = st.counts.cpu().numpy(), st.spikes.cpu().numpy()
+    np.testing.assert_array_equal(gc, cnt)
+    err = max(float(np.abs(gs[:cnt[j], j] - lists[j]).max()) for j in range(N) if cnt[j])
+    print(f"encode: {int(cnt.sum())} spikes, counts identical, max |dt| {err:.2e} s", flush=True)
+    assert err <= 1e-9
+    # Grams on sampled rows
+    dec = Decoder(geom, spatial_coeffs(enc.RF, geom), iafp.kappa, iafp.delta, iafp.bias)
+    tk1g, tk2g, nidg, off = dec.intervals(st)
+    qg, Tmg, tvg = dec.moments(tk1g, tk2g, nidg, off)
+    B = geom.spatial().reshape(128, 128, -1)
+    S = oracle.spatial_coeffs(RF, B)
+    del RF
+    tk1, tk2, nid, _ = oracle.select_intervals(lists, (0.0, 2.0))
+    q = oracle.compute_q(tk1, tk2, nid, iafp.kappa, iafp.delta, iafp.bias, np.ones(N))
+    Tm, tv = oracle.neuron_T(oracle.temporal_factors(tk1, tk2, 8, geom.W), q, nid, N)
+    D, Dt = geom.D, geom.Dt
+    rows = np.sort(np.random.default_rng(4).choice(D, 48, replace=False))
+    ps, pt = rows // Dt, rows % Dt
+    # oracle rows: A[d, d'] = sum_j S[j, ps] S[j, ps'] T_j[pt, pt']
+    Ao = np.stack([((S[:, a:a + 1] * S).T @ Tm[:, b, :]).reshape(D) for a, b in zip(ps, pt)])
+    lower = np.arange(D)[None, :] <= rows[:, None]
+
+    def rel_rows(Ag):
+        Gr = Ag[rows].double().cpu().numpy()
+        return float(np.linalg.norm((Gr - Ao)[lower]) / np.linalg.norm(Ao[lower]))
+    e64 = rel_rows(dec.gram(Tmg))
+    torch.cuda.empty_cache()
+    e_bf = rel_rows(dec.gram_lp(Tmg, 0))
+    e_h = rel_rows(dec.gram_lp(Tmg, 1))
+    torch.cuda.synchronize()
+    print(f"Gram rows rel Frobenius error: fp64 {e64:.2e} bf16x3 {e_bf:.2e} fp16 {e_h:.2e}", flush=True)
+    assert e64 <= 1e-10 and e_bf <= 1e-5 and e_h <= 1e-3
+    # the oracle's FP64 Tikhonov solve, then every form's SNR
+    A = oracle.gram_kron(S, Tm)
+    X = oracle.solve_tikhonov(A, oracle.rhs_kron(S, tv), oracle.tikhonov_lambda(A, tk1.size))
+    del A
+    u_o = oracle.synthesize(X, B, oracle.temporal_basis(geom.frame_times(), 8, geom.W))
+    snr_o = oracle.snr_db(vid.frames, u_o)
+    line = [f"oracle {snr_o:.4f}"]
+    for prec in ("fp64", "bf16x3", "fp16"):
+        r = dec.decode(st, precision=prec, refine_iters=6)
+        s_ = oracle.snr_db(vid.frames, r.frames.cpu().numpy().reshape(vid.frames.shape))
+        line.append(f"{prec} {s_:.4f}")
+        assert abs(s_ - snr_o) < 0.1, (prec, s_, snr_o)
+        torch.cuda.empty_cache()
+    r = BatchDecoder(precision="kpcg").decode([BlockJob(dec, st)])[0]
+    s_ = oracle.snr_db(vid.frames, r.frames.cpu().numpy().reshape(vid.frames.shape))
+    line.append(f"kpcg {s_:.4f}")
+    print("SNR dB: " + ", ".join(line), flush=True)
+    assert not r.fallback and abs(s_ - snr_o) < 0.1

@@ -7,8 +7,9 @@ the orders and the video's bandwidth, 2 pi M / W (vtem/vtdm.py:385-388, :964-967
 the tile and of the window (85.3 px, 1.33 s), so a block sits inside one period with
 margin.
 
-* 128 x 128 px, 2 s, a video inside every block's space (orders 8, 8, 8 with the decoder's
-  periods).  The blocks decoded together by the batched KPCG decoder and blended equal
+* 128 x 128 px, 4 s, a video inside every block's space (orders 8, 8, 8 with the decoder's
+  periods; 3 temporal periods over the 4 s, so the encoder's trigonometric input of order
+  24 over 4 s holds it exactly).  The blocks decoded together by the batched KPCG decoder and blended equal
   every block decoded by the plain per-block FP64 path and blended the same way (within
   0.1 dB), and the blend reconstructs the video (SNR floor).
 * The full 512 x 512 px, 4 s plan (605 blocks) of a video with orders 48, 48, 24 (the
@@ -48,14 +49,14 @@ def test_stitched_codec_matches_plain_blocks():
     require_gpu()
     from vtem_amd import synthetic as syn
     from vtem_amd.stitch import TiledCodec
-    print("\n[case] stitched codec 128^2 x 2 s, 27 blocks", flush=True)
+    print("\n[case] stitched codec 128^2 x 4 s, 45 blocks", flush=True)
     rng = np.random.default_rng(9)
-    # periods 256/3 px and 4/3 s: 8 cycles of the space's top frequency per period; over
-    # the 2 s encode that is temporal order 24
+    # periods 256/3 px and 4/3 s (the decoder's); over the 4 s encode the temporal
+    # harmonics m 0.75 Hz are the order-24 harmonics of 0.25 Hz
     c = syn.video_coefficients(8, 8, 8, rng)
-    frames = syn.evaluate_video(c, 128, 128, np.arange(200) / 100.0, Sx=256 / 3, Sy=256 / 3, T=4 / 3)
+    frames = syn.evaluate_video(c, 128, 128, np.arange(400) / 100.0, Sx=256 / 3, Sy=256 / 3, T=4 / 3)
     frames /= np.abs(frames).max()
-    codec = TiledCodec(128, 128, 2.0, 24, syn.bank_dyadic(64.0, 64.0),
+    codec = TiledCodec(128, 128, 4.0, 24, syn.bank_dyadic(64.0, 64.0),
                        Wx=2 * math.pi * 8 / (256 / 3), Wt=2 * math.pi * 8 / (4 / 3))
     assert abs(codec.dec_geom.Ox - 2 * math.pi * 3 / 256) < 1e-15 and abs(codec.dec_geom.W - 1.5 * math.pi) < 1e-14
     ntile = len(codec.plan.tiles)

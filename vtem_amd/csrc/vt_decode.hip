@@ -300,30 +300,44 @@ VT_EXPORT int vt_trace_lambda(const void *A, int dtype, long long lda, int D, co
 // Spike-space reference kernels
 // ------------------------------------------------------------------------------------
 
-// G_kl = Gt(k,l) * Dsw[j(k), j(l)]  (dirichlet.py:274-285, :316-323).  One row per block.
-__device__ __forceinline__ double G_dirichlet_time(double tk1, double tk2, double tl1, double tl2,
-                                                   int M, double WM)
-{
-    double sum = 0;
-    for (int m = 1; m <= M; ++m)
-        sum += (cos(m * WM * (tk2 - tl2)) - cos(m * WM * (tk2 - tl1)) -
-                cos(m * WM * (tk1 - tl2)) + cos(m * WM * (tk1 - tl1))) / (m * m);
-    sum = sum * 2 / (WM * WM) + (tk2 - tk1) * (tl2 - tl1);
-    return sum;
-}
+// Spike-space Gram G_kl = Gt(k,l) Dsw[j(k), j(l)], the matrix the reference builds at
+// vtem/dirichlet.py:62-85 (its kernels :274-285, :316-323).  Gt(k,l) is the double integral
+// of the temporal reproducing kernel 1 + 2 sum_{m<=M} cos(m W (t - s)) over the two
+// intervals.  That kernel is sum_p theta_p(t) theta_p(s) for the real orthonormal basis
+// theta_0 = 1, theta_{2m-1} = sqrt2 cos(m W t), theta_{2m} = sqrt2 sin(m W t), so
+// Gt(k,l) = alpha_k . alpha_l with alpha_k[p] the integral of theta_p over interval k:
+//   alpha[0] = length,  alpha[2m-1] + i alpha[2m] = 2 sqrt2 sin(m W h) e^{i m W c} / (m W)
+// (h the half-length, c the midpoint).  One workgroup per row: the row's alpha in LDS,
+// each thread forms its column's alpha term by term.  W = Wt / Mt.
+#define VT_GSPIKE_MAX_M 1024
 
 __global__ void __launch_bounds__(256)
 G_spike_kernel(double *__restrict__ G, long long ldg, const double *__restrict__ tk1,
                const double *__restrict__ tk2, double Wt, int Mt, const double *__restrict__ Dsw,
                long long ldw, const int *__restrict__ nid, int n)
 {
+    __shared__ double ac[VT_GSPIKE_MAX_M], as[VT_GSPIKE_MAX_M];
     const int row = blockIdx.y;
+    const double W = Wt / Mt;
+    const double hk = 0.5 * (tk2[row] - tk1[row]), ck = 0.5 * (tk1[row] + tk2[row]);
+    for (int m = threadIdx.x + 1; m <= Mt; m += blockDim.x) {
+        const double w = m * W;
+        const double amp = 2.0 * sin(w * hk) / w;     // the sqrt2 of both factors: a 2 below
+        ac[m - 1] = amp * cos(w * ck);
+        as[m - 1] = amp * sin(w * ck);
+    }
+    __syncthreads();
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= n || col >= n) return;
-    const double a1 = tk1[row], a2 = tk2[row];
-    const int j1 = nid[row];
-    G[(long long)row * ldg + col] =
-        G_dirichlet_time(a1, a2, tk1[col], tk2[col], Mt, Wt / Mt) * Dsw[(long long)j1 * ldw + nid[col]];
+    const double hl = 0.5 * (tk2[col] - tk1[col]), cl = 0.5 * (tk1[col] + tk2[col]);
+    double dot = 0.0;
+    for (int m = 1; m <= Mt; ++m) {
+        const double w = m * W;
+        const double amp = 2.0 * sin(w * hl) / w;
+        dot += amp * (ac[m - 1] * cos(w * cl) + as[m - 1] * sin(w * cl));
+    }
+    const double gt = 4.0 * hk * hl + 2.0 * dot;
+    G[(long long)row * ldg + col] = gt * Dsw[(long long)nid[row] * ldw + nid[col]];
 }
 
 VT_EXPORT int vt_G_spike(double *G, long long ldg, const double *tk1, const double *tk2, double Wt,
@@ -331,6 +345,7 @@ VT_EXPORT int vt_G_spike(double *G, long long ldg, const double *tk1, const doub
                          void *stream)
 {
     VT_CHECK_ARG(G && tk1 && tk2 && Dsw && neuron_ind && n >= 1 && ldg >= n && Mt >= 1);
+    if (Mt > VT_GSPIKE_MAX_M) return VT_ERANGE;
     VT_CHECK_ARG(n <= 65535);
     G_spike_kernel<<<dim3(vt_blocks(n, 256), n), 256, 0, (hipStream_t)stream>>>(G, ldg, tk1, tk2, Wt,
                                                                               Mt, Dsw, ldw,
