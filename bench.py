@@ -91,6 +91,10 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-sweep", action="store_true", help="skip the latency sweep")
     p.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
+    p.add_argument("--max-delay-ms", type=float, default=0.0,
+                   help="dynamic batching: a pseudoinverse lane holds a partial batch this long")
+    p.add_argument("--rnn-max-delay-ms", type=float, default=0.0,
+                   help="dynamic batching: the RNN lane holds a partial batch this long")
     p.add_argument("--overload", type=float, default=2.0,
                    help="Poisson arrival rate of the timed steps, in multiples of the closed-batch capacity")
     p.add_argument("--no-precision", action="store_true", help="skip the 128x128 precision report")
@@ -497,7 +501,8 @@ def main():
         (a fresh seeded arrival sequence per step), served by continuous batching."""
         server = Server(max_batch=args.max_batch, streams=args.streams, precision=form,
                         rnn_lanes=args.rnn_lanes, graph_chunk=args.graph_chunk,
-                        adaptive=form != "kpcg")
+                        adaptive=form != "kpcg", max_delay=args.max_delay_ms * 1e-3,
+                        rnn_max_delay=args.rnn_max_delay_ms * 1e-3)
         for _ in range(args.warmup):           # allocator, code objects, precision policy
             server.serve(make_requests(items, zeros, form))
         torch.cuda.synchronize()
@@ -595,6 +600,7 @@ def main():
         "config": {"workload": "serving form: blocks {32,48,64,96}^2 px x {0.5,1,2} s, "
                                f"N=round(2D/9), Poisson arrivals at {args.overload:g}x the closed-batch "
                                "capacity, continuous batching",
+                   "max_delay_ms": args.max_delay_ms, "rnn_max_delay_ms": args.rnn_max_delay_ms,
                    "requests_per_step": R, "frames_per_step": frames, "D_min": int(D.min()),
                    "D_max": int(D.max()), "max_batch": args.max_batch, "streams": args.streams,
                               "rnn_lanes": args.rnn_lanes, "graph_chunk": args.graph_chunk},

@@ -158,3 +158,13 @@ def test_guard_reads_the_triangular_solve_expiry_word():
     assert bd._guard([([3, 4], (nrm, info, None), "kpcg")]) == ([], 0)
     with pytest.raises(VtemError):
         bd._guard([([3, 4], (nrm, info, bad), "fp64")])
+
+
+def test_dynamic_batching_rule():
+    from vtem_amd.serve import should_launch
+    assert not should_launch(0, 8, 1.0, 0.0, False)           # nothing queued
+    assert should_launch(3, 8, 0.0, 0.0, True)                # no delay: launch at once
+    assert not should_launch(3, 8, 0.002, 0.005, True)        # hold: not full, young, more due
+    assert should_launch(8, 8, 0.0, 0.005, True)              # full batch
+    assert should_launch(3, 8, 0.005, 0.005, True)            # oldest waited the delay
+    assert should_launch(3, 8, 0.0, 0.005, False)             # no further arrivals

@@ -26,6 +26,153 @@ __host__ __device__ __forceinline__ int skinny_splits(int K)
     return ks < 1 ? 1 : (ks > 8 ? 8 : ks);
 }
 
+#ifndef SK_WAVES
+#define SK_WAVES 8
+#endif
+#ifndef SK_INFLIGHT
+#define SK_INFLIGHT 2      // chunks each wave has in flight
+#endif
+
+// operands of one 16-wide K chunk for lane (r, g); chunks at or beyond c_end load zeros
+template <int NT, int RT>
+__device__ __forceinline__ void skinny_load(const SkDesc &d, int m0, int c, int c_end, int r, int g,
+                                            int Nc, double (&a)[RT][4], double (&bv)[NT][4])
+{
+    const int kb = c * 16 + 4 * g;
+    const bool cin = c < c_end;
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        const int m = m0 + 16 * t + r;
+        if (d.sak == 1 && cin && m < d.M && kb + 3 < d.K) {
+            // row-major A: the lane's 4 consecutive k values as two 16-byte loads
+            typedef double d2a8 __attribute__((ext_vector_type(2), aligned(8)));
+            const double *p = d.A + (long long)m * d.sam + kb;
+            const d2a8 x = *(const d2a8 *)p, y = *(const d2a8 *)(p + 2);
+            a[t][0] = x[0];
+            a[t][1] = x[1];
+            a[t][2] = y[0];
+            a[t][3] = y[1];
+            continue;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = kb + s;
+            a[t][s] = (cin && m < d.M && k < d.K) ? d.A[(long long)m * d.sam + (long long)k * d.sak] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int n = 16 * j + r;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = kb + s;
+            bv[j][s] = (cin && n < Nc && k < d.K) ? d.B[(long long)k * d.ldb + n] : 0.0;
+        }
+    }
+}
+
+template <int NT, int RT>
+__device__ __forceinline__ void skinny_mma(v4d (&acc)[RT][NT], const double (&a)[RT][4],
+                                           const double (&bv)[NT][4])
+{
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                acc[t][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][s], bv[j][s], acc[t][j], 0, 0, 0);
+}
+
+__device__ __forceinline__ double skinny_row_factor(const SkDesc &d, int m, int col, int Nc,
+                                                    const double *row)
+{
+    const double *Tr = d.T + (long long)m * d.tstride + (long long)col * Nc;
+    double v = 0.0;
+    for (int k = 0; k < Nc; ++k) v += Tr[k] * row[k];
+    return v;
+}
+
+// One workgroup of SK_WAVES waves computes rows m0 .. m0 + 16 RT - 1 of K split z of
+// problem d (RT row tiles of 16: the row tiling does not change any sum's order, so a
+// launch may pick it from the batch size).  The waves split K and are reduced in LDS in a
+// fixed tree order; red is the workgroup's LDS scratch.
+template <int NT, int RT>
+__device__ __forceinline__ void skinny_tile(const SkDesc &d, int m0, int z,
+                                            double (&red)[SK_WAVES / 2][16 * RT][33])
+{
+    constexpr int R = 16 * RT;
+    const int ks = skinny_splits(d.K), Nc = d.Nc;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    v4d acc[RT][NT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[t][j] = (v4d){0, 0, 0, 0};
+    const int nch = (d.K + 15) / 16;
+    const int per = (nch + ks - 1) / ks;
+    const int c_beg = z * per, c_end = min(nch, c_beg + per);
+    // SK_INFLIGHT chunks in flight per wave: c, c + SK_WAVES, ... (added in that order)
+    for (int c = c_beg + wave; c < c_end; c += SK_INFLIGHT * SK_WAVES) {
+        double a[SK_INFLIGHT][RT][4], b[SK_INFLIGHT][NT][4];
+#pragma unroll
+        for (int u = 0; u < SK_INFLIGHT; ++u)
+            skinny_load<NT, RT>(d, m0, c + u * SK_WAVES, c_end, r, g, Nc, a[u], b[u]);
+#pragma unroll
+        for (int u = 0; u < SK_INFLIGHT; ++u) skinny_mma<NT, RT>(acc, a[u], b[u]);
+    }
+    // cross-wave tree reduction: at each level the upper half adds into the lower half
+#pragma unroll
+    for (int half = SK_WAVES / 2; half >= 1; half >>= 1) {
+        if (wave >= half && wave < 2 * half) {
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        red[wave - half][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)] = acc[t][j][i];
+        }
+        __syncthreads();
+        if (wave < half) {
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        acc[t][j][i] += red[wave][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)];
+        }
+        __syncthreads();
+    }
+    if (wave == 0) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    red[0][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)] = acc[t][j][i];
+    }
+    __syncthreads();
+    const int rows = min(R, d.M - m0);
+    if (ks > 1) {
+        double *W = d.ws + (long long)z * d.M * Nc;
+        for (int e = tid; e < rows * Nc; e += 64 * SK_WAVES) {
+            const int row = e / Nc, col = e % Nc;
+            W[(long long)(m0 + row) * Nc + col] = red[0][row][col];
+        }
+        return;
+    }
+    for (int e = tid; e < rows * Nc; e += 64 * SK_WAVES) {
+        const int row = e / Nc, col = e % Nc;
+        const double v = d.T ? skinny_row_factor(d, m0 + row, col, Nc, red[0][row]) : red[0][row][col];
+        d.C[(long long)(m0 + row) * d.ldc + col] = v;
+    }
+}
+
+
 // First pass only (library-internal): problems with ks > 1 leave their ks partial tiles in
 // ws (partial z of row m, column c at ws[(z * M + m) * Nc + c]) for a fused second pass.
 int vt_skinny_first_pass(const void *descs, int batch, int max_M, int max_K, int max_Nc,

@@ -119,7 +119,7 @@ class Server:
     def __init__(self, hbm_budget: int = 64 << 30, max_batch: int = 32, streams: int = 2,
                  precision: str = "bf16x3", verify: bool = True, adaptive: bool = True,
                  rnn_lane: bool = True, rnn_max_batch: int = 64, rnn_lanes: int = 1,
-                 graph_chunk: int = 25):
+                 graph_chunk: int = 25, max_delay: float = 0.0, rnn_max_delay: float = 0.0):
         self.hbm_budget = int(hbm_budget)
         self.max_batch = int(max_batch)
         self.n_streams = int(streams)
@@ -134,6 +134,10 @@ class Server:
         # RNN and power loops replayed as HIP graphs of this many iterations (0: no graph)
         self.graph_chunk = int(graph_chunk)
         self.policy = PrecisionPolicy(precision) if adaptive else None
+        # dynamic batching: a free lane holds its queue until the batch is full, its
+        # oldest request has waited this long (seconds), or no further arrivals are due
+        self.max_delay = float(max_delay)
+        self.rnn_max_delay = float(rnn_max_delay)
 
     # ----------------------------------------------------------------------------------
     def serve(self, requests: List[Request], mode: str = "served",
@@ -199,11 +203,17 @@ class Server:
                     busy[k] = None
             # launch on a free lane
             launched = False
+            wake = math.inf                     # earliest batching deadline of a held queue
             for k, (_, queue, cap) in enumerate(lanes):
                 if busy[k] is not None or not queue:
                     continue
                 if k < nr:
                     cap = split_cap(len(queue), cap, sum(1 for i in range(k, nr) if busy[i] is None))
+                delay = self.rnn_max_delay if k < nr else self.max_delay
+                now = clock() - t0
+                if not should_launch(len(queue), cap, now - queue[0].arrival, delay, bool(pending)):
+                    wake = min(wake, queue[0].arrival + delay)
+                    continue
                 batch, nbytes = form_batch(queue, cap, self.hbm_budget - in_use)
                 if not batch:
                     continue
@@ -223,11 +233,13 @@ class Server:
                 busy[k] = (ev, h, batch, nbytes, flags)
                 launched = True
             if not launched:
-                # nothing to launch: wait for the next arrival or a completion
-                nxt = pending[0].arrival - (clock() - t0) if pending else 0.0
+                # nothing to launch: wait for the next arrival, a batching deadline or a
+                # completion
+                now = clock() - t0
+                nxt = min(pending[0].arrival if pending else math.inf, wake) - now
                 if any(b is not None for b in busy):
                     time.sleep(min(max(nxt, 0.0), 2e-4))
-                elif nxt > 0:
+                elif nxt > 0 and nxt < math.inf:
                     time.sleep(min(nxt, 5e-3))
         torch.cuda.synchronize()
         wall = clock() - t0
@@ -318,6 +330,13 @@ def split_cap(waiting: int, cap: int, free_lanes: int) -> int:
     """Batch cap of one of `free_lanes` free lanes that share a queue of `waiting`
     requests: an even split, at most `cap`."""
     return max(1, min(cap, -(-waiting // max(1, free_lanes))))
+
+
+def should_launch(queued: int, cap: int, oldest_wait: float, max_delay: float,
+                  more_due: bool) -> bool:
+    """Dynamic batching rule of a free lane: launch when the queue fills a batch, when its
+    oldest request has waited max_delay seconds, or when no further arrivals are due."""
+    return queued > 0 and (queued >= cap or oldest_wait >= max_delay or not more_due)
 
 
 def form_batch(queue: collections.deque, cap: int, free_bytes: int):
