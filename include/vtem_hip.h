@@ -437,14 +437,29 @@ int vt_ds_assemble(double *Ds, const double *C, long long ldc, int N, int Mx, in
                    double scale, void *stream);
 /* W[i,j] *= norm[i] norm[j] (put_norm of compute_Dsw, vtem/vrf.py:529-533, :540-559). */
 int vt_scale_outer(double *W, long long ldw, int n, const double *norm, void *stream);
-/* Symmetric eigendecomposition by rocSOLVER (the counterpart of the reference's CULA call,
- * vtem/utils/linalg.py:366-523): w ascending; A's rows become the eigenvectors.  E: n
- * doubles of workspace; info (device int) > 0 if the iteration did not converge.  -2 if
- * no rocBLAS handle could be created. */
-int vt_eig_sym(double *A, int lda, int n, double *w, double *E, int *info, void *stream);
-/* y_k <- y_k / |w_k| where |w_k| > rcond max|w|, else 0: the truncation of solve_eq_sym
- * (vtem/utils/linalg.py:591-631, sq_Kernel :790-831 divides by the singular values). */
-int vt_eig_filter(double *y, const double *w, int n, double rcond, void *stream);
+/* One-sided Jacobi (FP64) on the rows of A (n x n; initially the symmetric G, whose rows
+ * are its columns): one sweep of n'-1 rounds (n' = n rounded up to even) of disjoint row
+ * pairs, a pair rotated when |a_p.a_q| > tol |a_p| |a_q| and both |a|^2 > small2 (rows at
+ * or below it are numerically zero and left alone); V (optional, null to skip)
+ * accumulates the rotations.  *rotations (device int) counts the rotations applied; the
+ * caller zeroes it (vt_zero_int) and sweeps until a sweep leaves it at 0.  At convergence
+ * A = (G V)^T with rows a_j = s_j u_j: the singular values and left singular vectors of
+ * G, which the reference gets from CULA's SVD (vtem/utils/linalg.py:366-523). */
+int vt_svd_jacobi_sweep(double *A, long long lda, double *V, long long ldv, int n, double tol,
+                        double small2, int *rotations, void *stream);
+/* *x = 0 (device int). */
+int vt_zero_int(int *x, void *stream);
+/* c = sum_{s_j > rcond s_max} u_j (u_j.q) / s_j from the converged rows of A: the
+ * truncated-SVD solve of solve_eq_sym (vtem/utils/linalg.py:591-631, sq_Kernel :790-831).
+ * ws: 2n doubles. */
+int vt_svd_pinv_apply(const double *A, long long lda, int n, const double *q, double rcond,
+                      double *c, double *ws, void *stream);
+/* Eigendecomposition from the converged rows of A = G + shift I (shift > |G|, so A is
+ * positive definite) and the accumulated V: eigenvalues w_j = a_j.v_j - shift ascending in
+ * w, eigenvector k in row k of E (row stride lde).  Replaces eig_sym
+ * (vtem/utils/linalg.py:634-718).  ws: n doubles. */
+int vt_svd_eig_finish(const double *A, long long lda, const double *V, long long ldv, int n,
+                      double shift, double *w, double *E, long long lde, double *ws, void *stream);
 /* Stitching: for each block descriptor {const double *blk, *wt, *wy, *wx; int T, Py, Px,
  * t0, y0, x0;} (56 bytes), out[t0+t, y0+y, x0+x] += wt[t] wy[y] wx[x] blk[t, y, x].  Blocks
  * of one call must not overlap.  Replaces the tile / window accumulation of decode_video

@@ -102,8 +102,10 @@ SIGNATURES = {
     "vt_dirichlet_table": [P, L, I, I, D, D, D, D, I, I, D, D, P],
     "vt_ds_assemble": [P, P, L, I, I, I, I, D, P],
     "vt_scale_outer": [P, L, I, P, P],
-    "vt_eig_sym": [P, I, I, P, P, P, P],
-    "vt_eig_filter": [P, P, I, D, P],
+    "vt_svd_jacobi_sweep": [P, L, P, L, I, D, D, P, P],
+    "vt_zero_int": [P, P],
+    "vt_svd_pinv_apply": [P, L, I, P, D, P, P, P],
+    "vt_svd_eig_finish": [P, L, P, L, I, D, P, P, L, P, P],
     "vt_stitch_accumulate": [P, L, L, P, I, L, P],
     "vt_kron_rnn": [P, I, I, P, I, I, I, I, P, P, P, P, P, P, L, I, I, P],
 }

@@ -486,7 +486,7 @@ class Decoder:
     def decode(self, st: SpikeTrains, t0: float = 0.0, method: str = "tikhonov",
                precision: str = "fp64", lam_rel: float = 1e-6, rnn_steps: int = 500,
                rnn_eta: Optional[float] = None, refine_iters: int = 4,
-               verify: bool = True, tol: float = 1e-10) -> DecodeResult:
+               verify: bool = True, tol: float = 1e-10, pinv_rcond: float = 1e-8) -> DecodeResult:
         """Decode one window.  For the low-precision forms with `verify`, a factorisation
         that lost positive definiteness or a refinement that did not reach `tol` relative
         residual is redone on the FP64 path (result.fallback = True); this costs one host
@@ -508,6 +508,16 @@ class Decoder:
                     fallback = True
             if verify and self.solve_expired():
                 raise VtemError("a triangular solve's bounded wait expired: the solution is invalid")
+        elif method == "pinv":
+            # the reference's decode: truncated-SVD pseudoinverse (rcond 1e-8, no lambda) of
+            # the Gram (vtem/vtdm.py:451, vtem/utils/linalg.py:591-631).  In D-space: the
+            # nonzero spectra of Phi Phi^T and Phi^T Phi coincide, so the truncation and
+            # X = pinv(A) r equal Phi^T pinv(G) q
+            from .utils.linalg import solve_eq_sym
+            A = self.gram(Tm)
+            lib.vt_symmetrize(ptr(A), 0, A.stride(0), self.geom.D, stream_ptr())
+            X = solve_eq_sym(A, r, rcond=pinv_rcond)
+            lam, info, nrm = None, None, None
         elif method == "rnn":
             lam = self.lam_exact(Tm, n_dev, lam_rel)
             eta = self.power_eta(Tm, lam) if rnn_eta is None else rnn_eta

@@ -22,9 +22,8 @@ LIB = os.path.join(HERE, "libvtem_hip.so")
 ARCH = os.environ.get("VTEM_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["vt_encode.hip", "vt_gemm.hip", "vt_decode.hip", "vt_gram.hip", "vt_chol.hip",
-           "vt_kpcg.hip", "vt_batch.hip", "vt_dirichlet.hip", "vt_iaf_par.hip"]
-# rocSOLVER (symmetric eigendecomposition of the truncated pseudoinverse) and its rocBLAS
-LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+           "vt_kpcg.hip", "vt_batch.hip", "vt_dirichlet.hip", "vt_iaf_par.hip", "vt_svd.hip"]
+LIBS = []
 
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
          "-Xclang", "-packed-fp32-ops", "-fvisibility=hidden", "-Wno-unused-result",

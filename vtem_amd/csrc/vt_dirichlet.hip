@@ -1,15 +1,10 @@
-// Spatial Dirichlet coefficients, the decode-space functions psi, the truncated-eigen
-// pseudoinverse and block stitching: the spike-space side of the reference's decoder and
-// its window / tile drivers.
+// Spatial Dirichlet coefficients, the decode-space functions psi and block stitching: the
+// spike-space side of the reference's decoder and its window / tile drivers.
 //
 // Pixel grid (vtem/vrf.py:1254-1257, origin :323-327): x_p = x_start + p dx,
 // y_q = -(y_start + q dy).  Mode m = (mx, my), |mx| <= Mx, |my| <= My, is column
 // m = (my + My)(2Mx + 1) + (mx + Mx) of the table; theta = mx Ox x_p + my Oy y_q.
 #include "vt_common.hpp"
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
-#include <mutex>
-#include <map>
 
 // table (P x 2M, row stride ld): [cos theta | -sin theta], row q*Px + p.  RF (N x P) times
 // it gives sum RF e^{-i theta} split in real / imaginary halves; [Re Ds | Im Ds] times its
@@ -94,66 +89,6 @@ VT_EXPORT int vt_scale_outer(double *W, long long ldw, int n, const double *norm
     VT_CHECK_ARG(W && norm && n >= 1 && n <= 65535 && ldw >= n);
     dim3 g(vt_blocks(n, 256) < 16 ? vt_blocks(n, 256) : 16, n);
     scale_outer_kernel<<<g, 256, 0, (hipStream_t)stream>>>(W, ldw, n, norm);
-    VT_LAUNCH_CHECK();
-    return VT_OK;
-}
-
-// ------------------------------------------------------------------------------------
-// Symmetric eigendecomposition (rocSOLVER divide and conquer, the library counterpart of
-// the reference's CULA call) and the truncated-eigen pseudoinverse filter.
-// One rocBLAS handle per device, created on first use and kept for the process.
-// ------------------------------------------------------------------------------------
-static std::mutex g_handle_mutex;
-static std::map<int, rocblas_handle> g_handles;
-
-static rocblas_handle device_handle()
-{
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-    std::lock_guard<std::mutex> lock(g_handle_mutex);
-    auto it = g_handles.find(dev);
-    if (it != g_handles.end()) return it->second;
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
-    g_handles[dev] = h;
-    return h;
-}
-
-// A (n x n symmetric, row stride lda) -> eigenvalues w ascending; A's rows become the
-// eigenvectors (row k = eigenvector of w[k]).  info (device int) > 0: no convergence.
-// E: workspace of n doubles.  Calls on one device are serialised by the handle lock.
-VT_EXPORT int vt_eig_sym(double *A, int lda, int n, double *w, double *E, int *info, void *stream)
-{
-    VT_CHECK_ARG(A && w && E && info && n >= 1 && lda >= n);
-    rocblas_handle h = device_handle();
-    if (!h) return VT_EUNSUPPORTED;
-    std::lock_guard<std::mutex> lock(g_handle_mutex);
-    if (rocblas_set_stream(h, (hipStream_t)stream) != rocblas_status_success) return VT_EUNSUPPORTED;
-    // row-major symmetric A is its own column-major transpose; the column-major eigenvector
-    // matrix read row-major has one eigenvector per row
-    rocblas_status st = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, lda, w, E,
-                                         info);
-    if (st != rocblas_status_success) return VT_EUNSUPPORTED;
-    VT_LAUNCH_CHECK();
-    return VT_OK;
-}
-
-// y_k <- y_k / |w_k| for |w_k| > rcond max|w|, else 0 (the truncation of solve_eq_sym,
-// vtem/utils/linalg.py:622-630: it divides by the singular values, |w| for a symmetric G)
-__global__ void eig_filter_kernel(double *__restrict__ y, const double *__restrict__ w, int n, double rcond)
-{
-    const double wmax = fmax(fabs(w[0]), fabs(w[n - 1]));
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const double a = fabs(w[k]);       // the singular value of a symmetric matrix
-        y[k] = (a > rcond * wmax) ? y[k] / a : 0.0;
-    }
-}
-
-VT_EXPORT int vt_eig_filter(double *y, const double *w, int n, double rcond, void *stream)
-{
-    VT_CHECK_ARG(y && w && n >= 1);
-    eig_filter_kernel<<<vt_blocks(n, 256) < 1024 ? vt_blocks(n, 256) : 1024, 256, 0,
-                        (hipStream_t)stream>>>(y, w, n, rcond);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
