@@ -382,8 +382,8 @@ def stitched_c3(seed=0, precision="kpcg", max_batch=1024):
                        "0.25 s overlap; Lx=Ly=8, Lt=8 with periods 2 pi M/W = 85.3 px, 1.33 s; 2048 neurons "
                        "per tile",
            "blocks": len(codec.plan.blocks), "tiles": n, "windows": len(codec.plan.f0),
-           "precision": precision, "spikes": nsp, "kpcg_retries": dst["kpcg_retries"],
-           "fp64_fallbacks": dst["fallbacks"],
+           "precision": precision, "preconditioner": dst.get("preconditioner"), "spikes": nsp,
+           "kpcg_retries": dst["kpcg_retries"], "fp64_fallbacks": dst["fallbacks"],
            "encode_ms": round(te * 1e3, 2), "decode_blend_ms": round(td * 1e3, 2),
            "frames_per_s_512x512": round(400 / (te + td), 2),
            "block_frames_per_s": round(len(codec.plan.blocks) * codec.plan.window / (te + td), 1),

@@ -392,10 +392,23 @@ int vt_kpcg_step_b(const double *r, const double *z, double *p, const long long 
  * replays the iterations as a HIP graph of up to graph_chunk (at most iters/2)
  * iterations, as vt_kron_rnn does; 0 launches each.  Same results either way. */
 int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
-                    const void *pre, int pre_M, int pre_K, int batch, int max_Nc, double *x,
+                    const void *pre, int pre_n, int pre_M, int pre_K, int pre_Nc, const void *pre2,
+                    int pre2_n, int pre2_M, int pre2_K, int pre2_Nc, const void *pre3, int pre3_n,
+                    int pre3_M, int pre3_K, int pre3_Nc, int batch, int max_Nc, double *x,
                     double *r, double *p, double *q, double *z, const double *lam,
                     const long long *seg, double *st, double *hist, int hist_stride, int iters,
                     double stop_rel2, int graph_chunk, void *stream);
+/* Kronecker-sum preconditioner (S^T C S) (x) Tbar + lam I for blocks whose decode period
+ * is longer than their window: c_j as vt_kpcg_tbar, and per block Tbar's eigenvalues theta
+ * (batch x Dt) and eigenvectors U (eigenvector k in column k of the Dt x Dt block; UT its
+ * transpose), by two-sided cyclic Jacobi.  Then P^-1 R = sum_k (theta_k K + lam I)^-1
+ * (R u_k) u_k^T; vt_kpcg_shift_pool forms the Dt matrices theta_k K_b + lam_b I of every
+ * block (K_b at kpool + kbase[b], matrix k of block b at opool + obase[b] + k Ds_b^2). */
+int vt_kpcg_tbar_eig(const double *Tm, const int *nbase, int Dt, int batch, double *ws,
+                     double *c, double *theta, double *U, double *UT, void *stream);
+int vt_kpcg_shift_pool(const double *kpool, const long long *kbase, const long long *obase,
+                       const int *ds, const double *theta, const double *lam, int Dt, int batch,
+                       int max_Ds, double *opool, void *stream);
 
 /* ----------------------------------------- gradient-flow (RNN) decoder, batched -- */
 /* Coefficient-space image of neural_network.rnn3 (vtem/neural_network.py:15-99, update

@@ -4,6 +4,9 @@ Integer/index outputs must match exactly; floating-point tolerances are written 
 """
 
 
+import dataclasses
+import math
+
 import numpy as np
 import pytest
 
@@ -684,23 +687,26 @@ def test_batch_equals_alone(torch_mod, oracle):
         assert abs(s_b - s_64) < 0.1
 
 
-def test_kpcg_batch(torch_mod, oracle):
+@pytest.mark.parametrize("pre", ["kron", "kron_sum"])
+def test_kpcg_batch(torch_mod, oracle, pre):
     """Kronecker-preconditioned CG over a ragged batch, including short blocks whose FP32
     Cholesky fails: every block reaches 1e-10 relative residual of the FP64 system, its
     SNR is within 0.1 dB of the FP64 Cholesky decode, and its frames are bit-identical
-    alone and in the batch."""
+    alone and in the batch -- with the Kronecker-product preconditioner and with the
+    Kronecker-sum one (S^T C S) kron Tbar + lam I."""
     torch = torch_mod
     from vtem_amd import synthetic as syn
     from vtem_amd.batch import BatchDecoder, BlockJob
-    _banner("kpcg batch")
+    _banner(f"kpcg batch, preconditioner {pre}")
     specs = [(32, 0.5), (48, 1.0), (32, 1.0), (48, 0.5), (32, 2.0), (48, 2.0), (64, 0.5)]
     blocks = [syn.serve_block(s, d, seed=300 + i) for i, (s, d) in enumerate(specs)]
     jobs = []
     for blk in blocks:
         geom, st, dec = _encode_block(torch, blk)
         jobs.append(BlockJob(dec, st))
-    bd = BatchDecoder(precision="kpcg", verify=False)
+    bd = BatchDecoder(precision="kpcg", verify=False, preconditioner=pre)
     res = bd.decode(jobs)
+    assert bd.last_stats["preconditioner"] == pre
     its = [float(r.state.cpu().numpy()[3]) for r in res]
     for i, (blk, jb, r) in enumerate(zip(blocks, jobs, res)):
         h = r.resid.cpu().numpy()
@@ -719,10 +725,12 @@ def test_kpcg_batch(torch_mod, oracle):
 
 
 def test_kpcg_retry_guard(torch_mod, oracle):
-    """With too few CG iterations every block misses the 1e-10 guard: the guard solves each
-    again with retry_iters iterations (no FP64 redo), and the result equals that of a solve
-    run with retry_iters from the start, bit for bit; with the retry switched off the same
-    blocks go to the batched FP64 path and equal its direct result."""
+    """With too few CG iterations every block misses the 1e-10 guard: the guard gives each
+    a correction solve of retry_iters iterations from its iterate (no FP64 redo), which
+    reaches the guard, equals -- bit for bit -- a solve warm-started by hand from the short
+    solve's coefficients, and decodes within 0.1 dB of the FP64 path; with the retry
+    switched off (retry_iters 0) the same blocks go to the batched FP64 path and equal its
+    direct result."""
     torch = torch_mod
     from vtem_amd import synthetic as syn
     from vtem_amd.batch import BatchDecoder, BlockJob
@@ -736,15 +744,21 @@ def test_kpcg_retry_guard(torch_mod, oracle):
     bd = BatchDecoder(precision="kpcg", cg_iters=2, retry_iters=64)
     res = bd.decode(jobs)
     assert bd.last_stats["kpcg_retries"] == len(jobs) and bd.last_stats["fallbacks"] == 0
-    ref = BatchDecoder(precision="kpcg", cg_iters=64).decode(jobs)
-    for i, (r, q) in enumerate(zip(res, ref)):
+    short = BatchDecoder(precision="kpcg", cg_iters=2, verify=False).decode(jobs)
+    warm = [dataclasses.replace(jb, x0=r.X.clone()) for jb, r in zip(jobs, short)]
+    ref = BatchDecoder(precision="kpcg", cg_iters=64, verify=False).decode(warm)
+    f64 = BatchDecoder(precision="fp64").decode(jobs)
+    for i, (r, q, f) in enumerate(zip(res, ref, f64)):
         assert not r.fallback
+        h = r.resid.cpu().numpy()
+        assert math.sqrt(h[1] / h[0]) <= 1e-10
         np.testing.assert_array_equal(r.frames.cpu().numpy(), q.frames.cpu().numpy(),
                                       err_msg=f"block {i} {specs[i]}")
-    bd0 = BatchDecoder(precision="kpcg", cg_iters=2, retry_iters=2)
+        u = blocks[i].video.frames.reshape(f.frames.shape)
+        assert abs(oracle.snr_db(u, r.frames.cpu().numpy()) - oracle.snr_db(u, f.frames.cpu().numpy())) < 0.1
+    bd0 = BatchDecoder(precision="kpcg", cg_iters=2, retry_iters=0)
     res0 = bd0.decode(jobs)
     assert bd0.last_stats["kpcg_retries"] == 0 and bd0.last_stats["fallbacks"] == len(jobs)
-    f64 = BatchDecoder(precision="fp64").decode(jobs)
     for i, (r, q) in enumerate(zip(res0, f64)):
         assert r.fallback
         np.testing.assert_array_equal(r.frames.cpu().numpy(), q.frames.cpu().numpy(),

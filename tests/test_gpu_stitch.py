@@ -75,6 +75,8 @@ def test_stitched_codec_matches_plain_blocks():
     decs = codec.decoders(iaf, contrast)
     got = codec.decode(spikes, decs).cpu().numpy()
     print(f"decode stats {codec.last_stats}", flush=True)
+    # blocks decoded inside a longer temporal period get the Kronecker-sum preconditioner
+    assert codec.last_stats["preconditioner"] == "kron_sum"
     counts = [int(s.counts.sum().item()) for s in spikes]
     print(f"spikes per tile {min(counts)}..{max(counts)}", flush=True)
     # plain per-block FP64 decodes, blended the same way
@@ -150,6 +152,7 @@ def test_config3_full_plan_vs_oracle(oracle):
     s_all, s_in = _snr(vid.frames, got), _snr(vid.frames[inner], got[inner])
     print(f"stitched SNR {s_all:.2f} dB, interior {s_in:.2f} dB", flush=True)
     assert s_in > 30.0
+    assert st["preconditioner"] == "kron_sum"
     # two tiles' encodes (a corner tile with the clamped last column, an interior tile)
     for (iy, ix) in [(0, len(p.x0) - 1), (5, 4)]:
         k = p.tiles.index((iy, ix))

@@ -82,6 +82,20 @@ def test_skinny_workspace_query():
     assert skinny_workspace(3, 10 ** 6, 5) == 8 * 3 * 5
 
 
+def test_skinny_workspace_array_matches_the_library_query():
+    """The vectorised split rule the descriptor builders use gives the library's own answer
+    on every problem shape, across the split boundaries."""
+    from vtem_amd.batch import skinny_workspace, skinny_workspace_array
+    rng = np.random.default_rng(5)
+    K = np.concatenate([np.arange(1, 40), [1007, 1008, 1009, 1024, 1025, 1535, 1536, 1537, 4095, 4096, 4097,
+                                           8191, 8192, 8193, 10 ** 6], rng.integers(1, 20000, 60)])
+    M = rng.integers(1, 700, K.size)
+    Nc = rng.integers(1, 33, K.size)
+    got = skinny_workspace_array(M, K, Nc)
+    want = [skinny_workspace(m, k, n) for m, k, n in zip(M, K, Nc)]
+    assert got.tolist() == want
+
+
 def test_iaf_par_host_queries():
     """Host-only entries of the time-parallel encoder: the descriptor layout the batched
     launch reads matches the host dtype, and the workspace holds (segments + 2) x N pairs

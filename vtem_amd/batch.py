@@ -29,7 +29,7 @@ import torch
 
 from ._lib import VtemError, lib, ptr, stream_ptr
 from .autotune import gram_tuner
-from .block import (F64, I32, Decoder, DecodeResult, SpikeTrains, mat_descs,
+from .block import (F64, I32, MAT_DTYPE, Decoder, DecodeResult, SpikeTrains, mat_descs,
                     split_T_size)
 
 SPIKE_DTYPE = np.dtype([("spikes", np.uint64), ("counts", np.uint64), ("ld", np.int64),
@@ -85,18 +85,29 @@ def skinny_workspace(M: int, K: int, Nc: int) -> int:
     return int(out.value)
 
 
+def skinny_workspace_array(M, K, Nc) -> np.ndarray:
+    """skinny_workspace for arrays of problems: the library's split rule (one split per 32
+    chunks of 16 along K, between 1 and 8; a workspace of ks M Nc doubles when ks > 1)."""
+    M, K, Nc = (np.asarray(v, np.int64) for v in (M, K, Nc))
+    ks = np.clip(((K + 15) // 16) // 32, 1, 8)
+    return np.where(ks > 1, ks * M * Nc, 0)
+
+
 @dataclasses.dataclass
 class BlockJob:
     """One block to decode: a prepared Decoder (geometry, S, IAF parameters) and its
     resident spike trains; window starts at t0 and lasts the decoder's duration.
     method "tikhonov" (pseudoinverse, regularised) or "rnn" (rnn_steps + 1 gradient-flow
-    iterations with step 1/lambda_max)."""
+    iterations with step 1/lambda_max).  x0: a starting iterate for a "kpcg" solve (the
+    D-space coefficients of an earlier solve of the same block): the solve then runs on
+    the residual r - (A + lam I) x0 and adds its correction."""
     decoder: Decoder
     spikes: SpikeTrains
     t0: float = 0.0
     tag: Any = None
     method: str = "tikhonov"
     rnn_steps: int = 500
+    x0: Optional[torch.Tensor] = None
 
 
 @dataclasses.dataclass
@@ -126,6 +137,17 @@ class _Prologue:
     keep: tuple
 
 
+def _fill_gemm(arr, A, B, C, sam, sak, sbk, sbn, ldc, M, N, K, alpha):
+    """Fill a GEMM_DTYPE descriptor array column by column (scalars or per-row arrays)."""
+    for name, v in (("A", A), ("B", B), ("C", C)):
+        arr[name] = np.asarray(v, np.int64).astype(np.uint64)
+    for name, v in (("sam", sam), ("sak", sak), ("sbk", sbk), ("sbn", sbn), ("ldc", ldc), ("M", M), ("N", N),
+                    ("K", K)):
+        arr[name] = v
+    arr["alpha"] = alpha
+    arr["beta"] = 0.0
+
+
 @dataclasses.dataclass
 class _Skinny:
     """Launch record of one skinny grouped product."""
@@ -141,6 +163,9 @@ class _Skinny:
 
     def args(self):
         return ptr(self.descs), self.max_M, self.max_K
+
+    def args5(self):
+        return ptr(self.descs), self.batch, self.max_M, self.max_K, self.max_Nc
 
 
 @dataclasses.dataclass
@@ -165,14 +190,17 @@ class BatchDecoder:
     exact Kronecker operator, preconditioned by (S^T C S + lam I) kron Tbar).  Blocks of
     the non-FP64 forms that fail the guard (relative residual of the FP64 system above
     `tol`, or a factorisation that lost positive definiteness) are redone together as
-    one batched FP64 group; kpcg blocks first get one more CG solve with retry_iters
-    iterations.  The CG, RNN and power-iteration loops are replayed as HIP graphs of
+    one batched FP64 group; kpcg blocks first get one more CG solve of retry_iters
+    iterations (0: none) that starts from their first iterate and solves for its
+    correction (in finite precision the recursively updated CG residual drifts from the
+    true one on ill-conditioned blocks; the correction solve restarts from the true
+    residual).  The CG, RNN and power-iteration loops are replayed as HIP graphs of
     graph_chunk iterations (0: every kernel launched by the host; same results)."""
 
     def __init__(self, device="cuda", precision: str = "bf16x3", lam_rel: float = 1e-6,
                  refine_iters: int = 4, tol: float = 1e-10, verify: bool = True,
                  cg_iters: int = 32, cg_stop: float = 1e-12, retry_iters: Optional[int] = None,
-                 graph_chunk: int = 25):
+                 graph_chunk: int = 25, preconditioner: str = "auto"):
         if precision not in PRECISIONS:
             raise ValueError(f"batched precision must be one of {PRECISIONS}")
         self.device = device
@@ -188,6 +216,13 @@ class BatchDecoder:
         self.retry_iters = 4 * cg_iters if retry_iters is None else retry_iters
         # CG, RNN and power loops: replay a HIP graph of this many iterations (0: launch each)
         self.graph_chunk = graph_chunk
+        # KPCG preconditioner: "kron" (S^T C S + lam I) kron Tbar, "kron_sum"
+        # (S^T C S) kron Tbar + lam I (exact in lam), "auto" (kron_sum for blocks decoded in a
+        # period longer than their window)
+        if preconditioner not in ("auto", "kron", "kron_sum"):
+            raise ValueError("preconditioner must be auto, kron or kron_sum")
+        self.preconditioner = preconditioner
+        self._pre_used = None                  # the last KPCG solve's preconditioner
         self.last_stats: Dict[str, Any] = {}
         self.timers: Optional[Dict[str, list]] = None     # stage -> [(ev0, ev1, work)]
 
@@ -335,8 +370,8 @@ class BatchDecoder:
         return redo, failed_fp64
 
     def finalize(self, h: "BatchHandle") -> List[DecodeResult]:
-        """Guard: kpcg blocks that did not reach `tol` are solved again with retry_iters CG
-        iterations; blocks still failing, and low-precision blocks whose factor failed or
+        """Guard: kpcg blocks that did not reach `tol` get a correction solve of retry_iters
+        CG iterations from their iterate; blocks still failing, and low-precision blocks whose factor failed or
         whose refinement did not reach `tol`, are redone as one batched FP64 group
         (flagged)."""
         redo, failed_fp64, retried = [], 0, []
@@ -347,11 +382,12 @@ class BatchDecoder:
         if self.verify:
             redo, failed_fp64 = self._guard(h.checks)
             kp = [i for i, pr in redo if pr == "kpcg"]
-            if kp and self.retry_iters > self.cg_iters:
+            if kp and self.retry_iters > 0:
                 base = self.cg_iters
                 self.cg_iters = self.retry_iters
                 try:
-                    h1 = self.launch([h.jobs[i] for i in kp], ["kpcg"] * len(kp))
+                    h1 = self.launch([dataclasses.replace(h.jobs[i], x0=h.results[i].X) for i in kp],
+                                     ["kpcg"] * len(kp))
                 finally:
                     self.cg_iters = base
                 bad, _ = self._guard(h1.checks)
@@ -372,7 +408,7 @@ class BatchDecoder:
             h.keep.append(h2.keep)
         self.last_stats = {"groups": len(h.checks), "fallbacks": len(redo),
                            "fp64_not_pd": failed_fp64, "fallback_index": redo,
-                           "kpcg_retries": len(retried)}
+                           "kpcg_retries": len(retried), "preconditioner": self._pre_used}
         torch.cuda.current_stream().synchronize()
         if h2 is not None:
             for _, (_, _, sync), _ in h2.checks:
@@ -624,17 +660,19 @@ class BatchDecoder:
         """Skinny grouped product: rows of (A, B, C, T, active, sam, sak, ldb, ldc, tstride,
         M, K, Nc) with pointers as ints (T = 0: no row factor; active = 0: always run).
         Returns the launch record (descriptors, max_M, max_K, max_Nc, split-K workspace)."""
-        wsz = [skinny_workspace(int(r[10]), int(r[11]), int(r[12])) for r in rows]
-        ws = torch.empty(max(1, sum(wsz)), dtype=F64, device=self.device)
+        R = np.array(rows, dtype=np.int64).reshape(len(rows), 13)
+        wsz = skinny_workspace_array(R[:, 10], R[:, 11], R[:, 12])
+        ws = torch.empty(max(1, int(wsz.sum())), dtype=F64, device=self.device)
+        woff = np.concatenate([[0], np.cumsum(wsz)[:-1]])
         d = np.zeros(len(rows), SK_DTYPE)
-        off = 0
-        for i, r in enumerate(rows):
-            A, Bp, C, T, act, sam, sak, ldb, ldc, tst, M, K, Nc = r
-            d[i] = (A, Bp, C, T, ws.data_ptr() + 8 * off if wsz[i] else 0, act, sam, sak, ldb, ldc, tst,
-                    M, K, Nc, 0)
-            off += wsz[i]
-        return _Skinny(_to_dev(d, self.device), len(rows), max(int(r[10]) for r in rows),
-                       max(int(r[11]) for r in rows), max(int(r[12]) for r in rows), ws)
+        for j, name in enumerate(("A", "B", "C", "T")):
+            d[name] = R[:, j].astype(np.uint64)
+        d["ws"] = np.where(wsz > 0, ws.data_ptr() + 8 * woff, 0).astype(np.uint64)
+        d["active"] = R[:, 4].astype(np.uint64)
+        for j, name in zip(range(5, 13), ("sam", "sak", "ldb", "ldc", "tstride", "M", "K", "Nc")):
+            d[name] = R[:, j]
+        return _Skinny(_to_dev(d, self.device), len(rows), int(R[:, 10].max()), int(R[:, 11].max()),
+                       int(R[:, 12].max()), ws)
 
     def _operator(self, blocks, src, dst, Z, dbj, nzj, active=None):
         """dst_b = A_b src_b for every block (blocks: (S, Tm_ptr, N, Ds, Dt)) as two skinny
@@ -660,72 +698,86 @@ class BatchDecoder:
         bwd = self._skinny(rows)
         return fwd, bwd
 
-    def _spatial_inverse(self, Ss, N, Ds, c_ptrs, lam, s):
-        """Mi_b = (S_b^T diag(c_b) S_b + lam_b I)^-1 for every block, FP64: Cholesky, then
-        L^-1 by blocked forward substitution (64-row block steps, two grouped GEMMs each,
-        diagonal blocks from the factor's block inverses), then Mi = L^-T L^-1."""
+    def _mgram(self, Ss, N, Ds, c_ptrs, s):
+        """K_b = S_b^T diag(c_b) S_b (Ds x Ds, FP64) for every block into one pool.  Returns
+        (pool, offsets, keep)."""
         dev = self.device
         B = len(Ss)
-        maxDs = int(Ds.max())
         mbase = np.concatenate([[0], np.cumsum(Ds * Ds)])
-        Mpool = torch.empty(int(mbase[-1]), dtype=F64, device=dev)
+        Kpool = torch.empty(int(mbase[-1]), dtype=F64, device=dev)
         kd = np.zeros(B, KPM_DTYPE)
-        dd = np.zeros(B, DIAG_DTYPE)
-        mat_items = []
         for b, S in enumerate(Ss):
-            mp = Mpool.data_ptr() + 8 * int(mbase[b])
-            kd[b] = (S.data_ptr(), c_ptrs[b], mp, S.stride(0), Ds[b], N[b], Ds[b])
-            dd[b] = (mp, Ds[b], Ds[b], 0)
-            mat_items.append((mp, int(Ds[b]), int(Ds[b])))
-        kd_d, dd_d = _to_dev(kd, dev), _to_dev(dd, dev)
+            kd[b] = (S.data_ptr(), c_ptrs[b], Kpool.data_ptr() + 8 * int(mbase[b]), S.stride(0), Ds[b], N[b], Ds[b])
+        kd_d = _to_dev(kd, dev)
         e0 = self._tic()
-        lib.vt_kpcg_mgram(ptr(kd_d), B, maxDs, s)
+        lib.vt_kpcg_mgram(ptr(kd_d), B, int(Ds.max()), s)
         self._toc("kpcg_mgram", e0, float(np.sum(2.0 * N * Ds * Ds)))
-        lib.vt_diag_add_batched(ptr(dd_d), B, 0, ptr(lam), maxDs, s)
+        return Kpool, mbase, [kd_d]
+
+    def _spd_inverse(self, pool, mbase, dims, s):
+        """Every matrix of the pool (matrix i: dims[i] x dims[i] at pool + mbase[i],
+        symmetric positive definite, lower triangle read) replaced by its full inverse, FP64:
+        Cholesky, then L^-1 by blocked forward substitution (64-row block steps, two grouped
+        GEMMs each, diagonal blocks from the factor's block inverses), then L^-T L^-1.
+        Returns (info, keep)."""
+        dev = self.device
+        B = len(dims)
+        dims = np.asarray(dims, np.int64)
+        maxDs = int(dims.max())
         nblk = (maxDs + 63) // 64
+        base = pool.data_ptr() + 8 * np.asarray(mbase[:-1], np.int64)
         inv = torch.empty(B * nblk * 4096, dtype=F64, device=dev)
         info = torch.zeros(B, dtype=I32, device=dev)
-        mats = mat_descs(mat_items, dev)
+        md = np.zeros(B, MAT_DTYPE)
+        md["A"], md["lda"], md["D"] = base.astype(np.uint64), dims, dims
+        mats = _to_dev(md, dev)
         e0 = self._tic()
         lib.vt_potrf_f64(ptr(mats), B, maxDs, ptr(inv), nblk * 4096, ptr(info), s)
-        self._toc("kpcg_potrf", e0, float(np.sum(Ds.astype(np.float64) ** 3 / 3)))
+        self._toc("kpcg_potrf", e0, float(np.sum(dims.astype(np.float64) ** 3 / 3)))
         e0 = self._tic()
         # L^-1 (lower triangular, zero above the diagonal)
         Lpool = torch.zeros(int(mbase[-1]), dtype=F64, device=dev)
-        xoff = _h2d(mbase[:-1].astype(np.int64), dev)
-        dim = _h2d(Ds.astype(np.int32), dev)
+        xoff = _h2d(np.asarray(mbase[:-1], np.int64), dev)
+        dim = _h2d(dims.astype(np.int32), dev)
         lib.vt_kpcg_place_diag(ptr(Lpool), ptr(xoff), ptr(dim), B, maxDs, ptr(inv), nblk * 4096, s)
         Tb = torch.empty(B * 64 * maxDs, dtype=F64, device=dev)
-        keep = [Mpool, kd_d, dd_d, inv, mats, Lpool, xoff, dim, Tb]
+        keep = [inv, mats, Lpool, xoff, dim, Tb]
+        lbase = Lpool.data_ptr() + 8 * np.asarray(mbase[:-1], np.int64)
+        tp = Tb.data_ptr() + 8 * np.arange(B, dtype=np.int64) * 64 * maxDs
         for i in range(1, nblk):
+            rows = np.clip(dims - 64 * i, 0, 64)
+            # T = L[i, :i] Linv[:i, :i];  Linv[i, :i] = -Dinv_i T
             g1 = np.zeros(B, GEMM_DTYPE)
             g2 = np.zeros(B, GEMM_DTYPE)
-            for b in range(B):
-                D = int(Ds[b])
-                rows = max(0, min(64, D - 64 * i))
-                Mb = Mpool.data_ptr() + 8 * int(mbase[b])
-                Lb = Lpool.data_ptr() + 8 * int(mbase[b])
-                Tp = Tb.data_ptr() + 8 * b * 64 * maxDs
-                # T = L[i, :i] Linv[:i, :i];  Linv[i, :i] = -Dinv_i T
-                g1[b] = (Mb + 8 * 64 * i * D, Lb, Tp, D, 1, D, 1, maxDs, rows, 64 * i, 64 * i, 0, 1.0, 0.0)
-                # K = rows: T's rows beyond the block's own were never written
-                g2[b] = (inv.data_ptr() + 8 * (b * nblk + i) * 4096, Tp, Lb + 8 * 64 * i * D, 64, 1, maxDs,
-                         1, D, rows, 64 * i, rows, 0, -1.0, 0.0)
+            _fill_gemm(g1, base + 8 * 64 * i * dims, lbase, tp, dims, 1, dims, 1, maxDs, rows, 64 * i, 64 * i,
+                       1.0)
+            # K = rows: T's rows beyond the block's own were never written
+            _fill_gemm(g2, inv.data_ptr() + 8 * (np.arange(B, dtype=np.int64) * nblk + i) * 4096, tp,
+                       lbase + 8 * 64 * i * dims, 64, 1, maxDs, 1, dims, rows, 64 * i, rows, -1.0)
             g1_d, g2_d = _to_dev(g1, dev), _to_dev(g2, dev)
             lib.vt_gemm_grouped(ptr(g1_d), B, 64, 64 * i, s)
             lib.vt_gemm_grouped(ptr(g2_d), B, 64, 64 * i, s)
             keep += [g1_d, g2_d]
-        # Mi = Linv^T Linv (into Mpool: the factor is no longer needed)
+        # inverse = Linv^T Linv (into the pool: the factor is no longer needed)
         g3 = np.zeros(B, GEMM_DTYPE)
-        for b in range(B):
-            D = int(Ds[b])
-            Lb = Lpool.data_ptr() + 8 * int(mbase[b])
-            g3[b] = (Lb, Lb, Mpool.data_ptr() + 8 * int(mbase[b]), 1, D, D, 1, D, D, D, D, 0, 1.0, 0.0)
+        _fill_gemm(g3, lbase, lbase, base, 1, dims, dims, 1, dims, dims, dims, dims, 1.0)
         g3_d = _to_dev(g3, dev)
         lib.vt_gemm_grouped(ptr(g3_d), B, maxDs, maxDs, s)
-        self._toc("kpcg_inverse", e0, float(np.sum(Ds.astype(np.float64) ** 3)))
+        self._toc("kpcg_inverse", e0, float(np.sum(dims.astype(np.float64) ** 3)))
         keep.append(g3_d)
-        return Mpool, mbase, info, keep
+        return info, keep
+
+    def _spatial_inverse(self, Ss, N, Ds, c_ptrs, lam, s):
+        """Mi_b = (S_b^T diag(c_b) S_b + lam_b I)^-1 for every block, FP64."""
+        B = len(Ss)
+        Mpool, mbase, keep = self._mgram(Ss, N, Ds, c_ptrs, s)
+        dd = np.zeros(B, DIAG_DTYPE)
+        for b in range(B):
+            dd[b] = (Mpool.data_ptr() + 8 * int(mbase[b]), Ds[b], Ds[b], 0)
+        dd_d = _to_dev(dd, self.device)
+        lib.vt_diag_add_batched(ptr(dd_d), B, 0, ptr(lam), int(Ds.max()), s)
+        info, ikeep = self._spd_inverse(Mpool, mbase, Ds, s)
+        return Mpool, mbase, info, keep + [dd_d] + ikeep
 
     def _solve_kpcg(self, gs: List[_Prologue]):
         """Kronecker-preconditioned CG on (A + lam I) X = r for every block of the groups,
@@ -741,20 +793,33 @@ class BatchDecoder:
         dev = self.device
         s = stream_ptr()
         e_setup = self._tic()
+        kron_sum = self._kron_sum(gs)
+        self._pre_used = "kron_sum" if kron_sum else "kron"
         blocks, c_ptrs, TiT_ptrs, keep = [], [], [], []
+        eig = []                                 # Kronecker sum: per group (theta, U, UT)
         boff, b0, e0 = [], 0, 0
         for g in gs:
             E = g.Dt * g.Dt
             c = torch.empty(g.Ntot, dtype=F64, device=dev)
-            TiT = torch.empty(g.B * E, dtype=F64, device=dev)
             tws = torch.empty(16 * g.B * E, dtype=F64, device=dev)
-            lib.vt_kpcg_tbar(ptr(g.Tm), ptr(g.nbase_d), g.Dt, g.B, ptr(tws), ptr(c), ptr(TiT), s)
-            keep += [c, TiT, tws]
+            if kron_sum:
+                th = torch.empty(g.B * g.Dt, dtype=F64, device=dev)
+                U = torch.empty(g.B * E, dtype=F64, device=dev)
+                UT = torch.empty(g.B * E, dtype=F64, device=dev)
+                lib.vt_kpcg_tbar_eig(ptr(g.Tm), ptr(g.nbase_d), g.Dt, g.B, ptr(tws), ptr(c), ptr(th), ptr(U),
+                                     ptr(UT), s)
+                eig.append((th, U, UT))
+                keep += [c, tws, th, U, UT]
+            else:
+                TiT = torch.empty(g.B * E, dtype=F64, device=dev)
+                lib.vt_kpcg_tbar(ptr(g.Tm), ptr(g.nbase_d), g.Dt, g.B, ptr(tws), ptr(c), ptr(TiT), s)
+                keep += [c, TiT, tws]
             for b, d in enumerate(g.decs):
                 blocks.append((d.S, g.Tm.data_ptr() + 8 * int(g.nbase[b]) * E, int(g.N[b]), int(g.Ds[b]), g.Dt,
                                d.St))
                 c_ptrs.append(c.data_ptr() + 8 * int(g.nbase[b]))
-                TiT_ptrs.append(TiT.data_ptr() + 8 * b * E)
+                if not kron_sum:
+                    TiT_ptrs.append(TiT.data_ptr() + 8 * b * E)
             boff.append((b0, e0))
             b0 += g.B
             e0 += g.Dtot
@@ -770,16 +835,21 @@ class BatchDecoder:
         lam = torch.cat([g.lam for g in gs]) if len(gs) > 1 else gs[0].lam
         seg = _h2d(dbj.astype(np.int64), dev)
         elem_job = _h2d(np.repeat(np.arange(Bt, dtype=np.int32), D), dev)
-        Mi, mbase, info, mkeep = self._spatial_inverse([b[0] for b in blocks], N, Ds, c_ptrs, lam, s)
         R = r.clone()
         Zp = torch.empty(Dtot, dtype=F64, device=dev)
         st = torch.zeros(4 * Bt, dtype=F64, device=dev)
         st.view(Bt, 4)[:, 2] = 1.0          # active for the first preconditioner application
         act = [st.data_ptr() + 8 * (4 * b + 2) for b in range(Bt)]
-        pre = self._skinny([(Mi.data_ptr() + 8 * int(mbase[b]), R.data_ptr() + 8 * int(dbj[b]),
-                             Zp.data_ptr() + 8 * int(dbj[b]), TiT_ptrs[b], act[b],
-                             Ds[b], 1, Dt[b], Dt[b], 0, Ds[b], Ds[b], Dt[b]) for b in range(Bt)])
-        self._toc("kpcg_setup", e_setup, float(np.sum(2.0 * N * Ds * Ds + 2.0 * Ds.astype(np.float64) ** 3)))
+        if not kron_sum:
+            Mi, mbase, info, mkeep = self._spatial_inverse([b[0] for b in blocks], N, Ds, c_ptrs, lam, s)
+            pres = [self._skinny([(Mi.data_ptr() + 8 * int(mbase[b]), R.data_ptr() + 8 * int(dbj[b]),
+                                   Zp.data_ptr() + 8 * int(dbj[b]), TiT_ptrs[b], act[b],
+                                   Ds[b], 1, Dt[b], Dt[b], 0, Ds[b], Ds[b], Dt[b]) for b in range(Bt)])]
+            setup_flops = np.sum(2.0 * N * Ds * Ds + 2.0 * Ds.astype(np.float64) ** 3)
+        else:
+            Mi, info, mkeep, pres = self._kron_sum_pre(gs, eig, blocks, N, Ds, Dt, dbj, c_ptrs, lam, R, Zp, act, s)
+            setup_flops = np.sum(2.0 * N * Ds * Ds + 2.0 * Dt * Ds.astype(np.float64) ** 3)
+        self._toc("kpcg_setup", e_setup, float(setup_flops))
         X = torch.empty(Dtot, dtype=F64, device=dev)
         Pv = torch.empty(Dtot, dtype=F64, device=dev)
         Q = torch.empty(Dtot, dtype=F64, device=dev)
@@ -790,12 +860,30 @@ class BatchDecoder:
         hs = iters + 1
         hist = torch.zeros((Bt, hs), dtype=F64, device=dev)
         maxNc = int(Dt.max())
+        x0s = [jb.x0 for g in gs for jb in g.jobs]
+        X0 = None
+        if any(x is not None for x in x0s):
+            # warm start: solve (A + lam I) E = r - (A + lam I) x0, then X = x0 + E
+            X0 = torch.zeros(Dtot, dtype=F64, device=dev)
+            for b, x in enumerate(x0s):
+                if x is not None:
+                    if x.numel() != int(D[b]):
+                        raise ValueError("x0 does not match its block's coefficient count")
+                    X0[int(dbj[b]):int(dbj[b]) + int(D[b])].copy_(x)
+            X.copy_(X0)
+            fx.run(s)
+            bx.run(s)
+            lib.vt_residual_batched(ptr(R), ptr(r), ptr(Q), ptr(X), ptr(lam), ptr(elem_job), Dtot, s)
         e_cg = self._tic()                      # the solve's launches only (host work before)
-        pre.run(s)
+        for pr in pres:
+            pr.run(s)
         lib.vt_kpcg_init(ptr(R), ptr(Zp), ptr(Pv), ptr(X), ptr(seg), Bt, ptr(st), ptr(hist), hs, s)
-        lib.vt_kpcg_iterate(*fp.args(), *bp.args(), *pre.args(), Bt, maxNc, ptr(X), ptr(R), ptr(Pv),
-                            ptr(Q), ptr(Zp), ptr(lam), ptr(seg), ptr(st), ptr(hist), hs, iters,
+        pargs = [pr.args5() for pr in pres] + [(0, 0, 0, 0, 0)] * (3 - len(pres))
+        lib.vt_kpcg_iterate(*fp.args(), *bp.args(), *pargs[0], *pargs[1], *pargs[2], Bt, maxNc, ptr(X), ptr(R),
+                            ptr(Pv), ptr(Q), ptr(Zp), ptr(lam), ptr(seg), ptr(st), ptr(hist), hs, iters,
                             float(self.cg_stop) ** 2, self.graph_chunk, s)
+        if X0 is not None:
+            lib.vt_add_inplace(ptr(X), ptr(X0), Dtot, s)
         # true FP64 residual of the final iterate (the guard's input)
         nrm = torch.zeros((Bt, 2), dtype=F64, device=dev)
         lib.vt_seg_sumsq(ptr(r), ptr(seg), Bt, ptr(nrm), 2, s)
@@ -807,11 +895,67 @@ class BatchDecoder:
         # algorithmic HBM bytes per iteration and block: S twice, T_j once, Z written and
         # read, Mi once, about a dozen passes over the D-space vectors; a block that has
         # converged skips its products, so each block counts its own iterations (st[4b+3])
-        per = 8.0 * (2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + Ds * Ds + 12.0 * D)
+        per = 8.0 * (2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + (Dt if kron_sum else 1) * Ds * Ds + 12.0 * D)
         self._toc("kpcg_iterate", e_cg,
                   lambda: float(np.sum(per * st.view(Bt, 4)[:, 3].cpu().numpy())))
-        keep += [r, lam, seg, elem_job, Mi, mkeep, R, Zp, pre, X, Pv, Q, Z, fp, bp, fx, bx, st, hist, res]
+        keep += [r, lam, seg, elem_job, Mi, mkeep, R, Zp, pres, X, Pv, Q, Z, fp, bp, fx, bx, st, hist, res, X0]
         return X, nrm, info, st, boff, keep
+
+    def _kron_sum(self, gs) -> bool:
+        """The Kronecker-sum preconditioner for this solve?  "auto": when a block is
+        decoded in a temporal period longer than its window (its Tbar is then badly
+        conditioned and the Kronecker product misjudges lam)."""
+        if self.preconditioner in ("kron", "kron_sum"):
+            return self.preconditioner == "kron_sum"
+        return any(d.geom.T_period > d.geom.duration * (1 + 1e-12) for g in gs for d in g.decs)
+
+    def _kron_sum_pre(self, gs, eig, blocks, N, Ds, Dt, dbj, c_ptrs, lam, R, Zp, act, s):
+        """P^-1 R = sum_k (theta_k K + lam I)^-1 (R u_k) u_k^T per block, Tbar = U diag(theta)
+        U^T, K = S^T diag(c) S: the Dt shifted matrices of every block inverted in one batch,
+        and three skinny launches per application (Rt = R U; column k of Zt = Mi_k column k
+        of Rt; Z = Zt U^T).  Returns (the inverses' pool, factor info, keep, launches)."""
+        dev = self.device
+        Bt = len(blocks)
+        Kpool, kb, kkeep = self._mgram([b[0] for b in blocks], N, Ds, c_ptrs, s)
+        nmat = Ds * Ds * Dt                      # per block: Dt matrices of Ds^2
+        obase = np.concatenate([[0], np.cumsum(nmat)])
+        Opool = torch.empty(int(obase[-1]), dtype=F64, device=dev)
+        keep = [Kpool, kkeep]
+        b0 = 0
+        for g, (th, U, UT) in zip(gs, eig):
+            idx = slice(b0, b0 + g.B)
+            kb_d = _h2d(np.asarray(kb[:-1][idx], np.int64), dev)
+            ob_d = _h2d(np.asarray(obase[:-1][idx], np.int64), dev)
+            ds_d = _h2d(Ds[idx].astype(np.int32), dev)
+            lib.vt_kpcg_shift_pool(ptr(Kpool), ptr(kb_d), ptr(ob_d), ptr(ds_d), ptr(th),
+                                   ptr(lam) + 16 * b0, g.Dt, g.B, int(Ds[idx].max()), ptr(Opool), s)
+            keep += [kb_d, ob_d, ds_d]
+            b0 += g.B
+        mats = np.concatenate([obase[b] + np.arange(Dt[b], dtype=np.int64) * Ds[b] * Ds[b] for b in range(Bt)])
+        dims = np.repeat(Ds, Dt)
+        info_all, ikeep = self._spd_inverse(Opool, np.concatenate([mats, [obase[-1]]]), dims, s)
+        keep += ikeep
+        Rt = torch.empty_like(R)
+        Zt = torch.empty_like(R)
+        keep += [Rt, Zt]
+        Uptr, UTptr = [], []
+        for g, (th, U, UT) in zip(gs, eig):
+            E = g.Dt * g.Dt
+            Uptr += [U.data_ptr() + 8 * b * E for b in range(g.B)]
+            UTptr += [UT.data_ptr() + 8 * b * E for b in range(g.B)]
+        rp, rtp, ztp, zp = (R.data_ptr(), Rt.data_ptr(), Zt.data_ptr(), Zp.data_ptr())
+        pa = self._skinny([(rp + 8 * int(dbj[b]), Uptr[b], rtp + 8 * int(dbj[b]), 0, act[b],
+                            Dt[b], 1, Dt[b], Dt[b], 0, Ds[b], Dt[b], Dt[b]) for b in range(Bt)])
+        pb = self._skinny([(Opool.data_ptr() + 8 * int(obase[b] + k * Ds[b] * Ds[b]), rtp + 8 * (int(dbj[b]) + k),
+                            ztp + 8 * (int(dbj[b]) + k), 0, act[b], Ds[b], 1, Dt[b], Dt[b], 0, Ds[b], Ds[b], 1)
+                           for b in range(Bt) for k in range(int(Dt[b]))])
+        pc = self._skinny([(ztp + 8 * int(dbj[b]), UTptr[b], zp + 8 * int(dbj[b]), 0, act[b],
+                            Dt[b], 1, Dt[b], Dt[b], 0, Ds[b], Dt[b], Dt[b]) for b in range(Bt)])
+        # a block's factor info: the first failing of its Dt matrices
+        owner = _h2d(np.repeat(np.arange(Bt, dtype=np.int64), Dt), dev)
+        info = torch.zeros(Bt, dtype=I32, device=dev).scatter_reduce_(0, owner, info_all.view(-1), "amax")
+        keep.append(owner)
+        return Opool, info, keep, [pa, pb, pc]
 
     def _joint_blocks(self, gs: List[_Prologue]):
         """Blocks of several prologue groups in one joint numbering: (S, T_j pointer, N, Ds,

@@ -97,6 +97,167 @@ kpcg_tbar_final(const double *__restrict__ ws, int Dt, double *__restrict__ Ti)
         Ti[(long long)b * E + e] = G[e % Dt][Dt + e / Dt];
 }
 
+// ------------------------------------------------------------------------------------
+// Kronecker-sum preconditioner P = (S^T C S) (x) Tbar + lam I.  For blocks decoded in a
+// period longer than their window, Tbar is badly conditioned (the basis functions that are
+// small inside the window have tiny moments), and lam, not (S^T C S) (x) Tbar, dominates A
+// there; the Kronecker-product preconditioner above then misjudges those directions by up
+// to 1/theta_min.  With Tbar = U diag(theta) U^T, P = (I (x) U) blockdiag_k(theta_k K +
+// lam I) (I (x) U)^T, so P^-1 R = sum_k (theta_k K + lam I)^-1 (R u_k) u_k^T for the
+// residual R (Ds x Dt).  The eigendecomposition of Tbar (Dt <= 33) is a two-sided cyclic
+// Jacobi in LDS, one workgroup per block, rounds of disjoint pairs, fixed order.
+// ------------------------------------------------------------------------------------
+#define KP_EIG_SWEEPS 30
+
+__global__ void __launch_bounds__(256)
+kpcg_tbar_eig_kernel(const double *__restrict__ ws, int Dt, double *__restrict__ theta,
+                     double *__restrict__ U, double *__restrict__ UT)
+{
+    __shared__ double G[KP_MAX_DT + 1][KP_MAX_DT + 1];
+    __shared__ double V[KP_MAX_DT + 1][KP_MAX_DT + 1];
+    __shared__ double cs[KP_MAX_DT + 1][2];
+    __shared__ int rotated;
+    const int b = blockIdx.x, E = Dt * Dt, tid = threadIdx.x;
+    for (int e = tid; e < E; e += blockDim.x) {
+        double t = 0.0;
+        for (int g = 0; g < KP_CHUNKS; ++g) t += ws[((long long)b * KP_CHUNKS + g) * E + e];
+        G[e / Dt][e % Dt] = t;
+        V[e / Dt][e % Dt] = (e / Dt == e % Dt) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    __shared__ double tr;
+    if (tid == 0) {
+        double t = 0.0;
+        for (int a = 0; a < Dt; ++a) t += G[a][a];
+        tr = t;
+    }
+    __syncthreads();
+    const double scale = Dt / tr;             // trace Dt, as the Kronecker-product form
+    for (int e = tid; e < E; e += blockDim.x) G[e / Dt][e % Dt] *= scale;
+    __syncthreads();
+    const int m = Dt + (Dt & 1), half = m / 2;
+    const double tol = 4.0 * Dt * 2.220446049250313e-16;
+    for (int sweep = 0; sweep < KP_EIG_SWEEPS; ++sweep) {
+        if (tid == 0) rotated = 0;
+        __syncthreads();
+        for (int r = 0; r < m - 1; ++r) {
+            // angles of the round's disjoint pairs (c = 1, s = 0: not rotated)
+            if (tid < half) {
+                int p, q;
+                if (tid == 0) {
+                    p = r;
+                    q = m - 1;
+                } else {
+                    p = (r + tid) % (m - 1);
+                    q = (r - tid + (m - 1)) % (m - 1);
+                }
+                if (p > q) { const int t = p; p = q; q = t; }
+                double c = 1.0, sn = 0.0;
+                if (q < Dt) {
+                    const double app = G[p][p], aqq = G[q][q], apq = G[p][q];
+                    if (fabs(apq) > tol * sqrt(fabs(app) * fabs(aqq)) && apq != 0.0) {
+                        const double zeta = (aqq - app) / (2.0 * apq);
+                        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        sn = c * t;
+                        rotated = 1;
+                    }
+                }
+                cs[tid][0] = c;
+                cs[tid][1] = sn;
+            }
+            __syncthreads();
+            // rows p, q of G <- J^T G ; then columns (G J) and V J
+            for (int e = tid; e < half * Dt; e += blockDim.x) {
+                const int i = e / Dt, k = e % Dt;
+                int p, q;
+                if (i == 0) { p = r; q = m - 1; } else { p = (r + i) % (m - 1); q = (r - i + (m - 1)) % (m - 1); }
+                if (p > q) { const int t = p; p = q; q = t; }
+                if (q >= Dt) continue;
+                const double c = cs[i][0], sn = cs[i][1], x = G[p][k], y = G[q][k];
+                G[p][k] = c * x - sn * y;
+                G[q][k] = sn * x + c * y;
+            }
+            __syncthreads();
+            for (int e = tid; e < half * Dt; e += blockDim.x) {
+                const int i = e / Dt, k = e % Dt;
+                int p, q;
+                if (i == 0) { p = r; q = m - 1; } else { p = (r + i) % (m - 1); q = (r - i + (m - 1)) % (m - 1); }
+                if (p > q) { const int t = p; p = q; q = t; }
+                if (q >= Dt) continue;
+                const double c = cs[i][0], sn = cs[i][1];
+                const double x = G[k][p], y = G[k][q];
+                G[k][p] = c * x - sn * y;
+                G[k][q] = sn * x + c * y;
+                const double vx = V[k][p], vy = V[k][q];
+                V[k][p] = c * vx - sn * vy;
+                V[k][q] = sn * vx + c * vy;
+            }
+            __syncthreads();
+        }
+        const int done = !rotated;
+        __syncthreads();
+        if (done) break;
+    }
+    // theta_k = G[k][k]; U[i][k] = V[i][k] (eigenvector k in column k), UT = U^T
+    for (int e = tid; e < E; e += blockDim.x) {
+        const int i = e / Dt, k = e % Dt;
+        U[(long long)b * E + e] = V[i][k];
+        UT[(long long)b * E + e] = V[k][i];
+    }
+    for (int k = tid; k < Dt; k += blockDim.x) theta[(long long)b * Dt + k] = G[k][k];
+}
+
+// c_j, and per block Tbar's eigenvalues theta (batch x Dt) and eigenvectors U (batch x Dt x
+// Dt, eigenvector k in column k; UT = U^T), Tbar scaled to trace Dt.
+VT_EXPORT int vt_kpcg_tbar_eig(const double *Tm, const int *nbase, int Dt, int batch, double *ws,
+                               double *c, double *theta, double *U, double *UT, void *stream)
+{
+    VT_CHECK_ARG(Tm && nbase && ws && c && theta && U && UT && batch >= 1 && batch <= 65535 && Dt >= 1);
+    if (Dt > KP_MAX_DT) return VT_ERANGE;
+    hipStream_t s = (hipStream_t)stream;
+    kpcg_tbar_partial<<<dim3(KP_CHUNKS, batch), 256, 0, s>>>(Tm, nbase, Dt, ws, c);
+    kpcg_tbar_eig_kernel<<<batch, 256, 0, s>>>(ws, Dt, theta, U, UT);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// out_{b,k} = theta[b*Dt + k] K_b + lam_b I (Ds_b x Ds_b each) for k < Dt: K_b at
+// kpool + kbase[b], out_{b,k} at opool + obase[b] + k Ds_b^2.  Grid (chunks, batch*Dt).
+__global__ void __launch_bounds__(256)
+kpcg_shift_pool_kernel(const double *__restrict__ kpool, const long long *__restrict__ kbase,
+                       const long long *__restrict__ obase, const int *__restrict__ ds,
+                       const double *__restrict__ theta, const double *__restrict__ lam, int Dt,
+                       double *__restrict__ opool)
+{
+    const int b = blockIdx.y / Dt, k = blockIdx.y % Dt;
+    const int D = ds[b];
+    const long long n = (long long)D * D;
+    const double th = theta[(long long)b * Dt + k], l = lam[2 * b];
+    const double *K = kpool + kbase[b];
+    double *O = opool + obase[b] + (long long)k * n;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long i = e / D, j = e % D;
+        O[e] = th * K[e] + (i == j ? l : 0.0);
+    }
+}
+
+VT_EXPORT int vt_kpcg_shift_pool(const double *kpool, const long long *kbase, const long long *obase,
+                                 const int *ds, const double *theta, const double *lam, int Dt,
+                                 int batch, int max_Ds, double *opool, void *stream)
+{
+    VT_CHECK_ARG(kpool && kbase && obase && ds && theta && lam && opool && Dt >= 1 && batch >= 1 &&
+                 max_Ds >= 1);
+    VT_CHECK_ARG((long long)batch * Dt <= 65535);
+    long long g = ((long long)max_Ds * max_Ds + 255) / 256;
+    if (g > 64) g = 64;
+    kpcg_shift_pool_kernel<<<dim3((unsigned)g, batch * Dt), 256, 0, (hipStream_t)stream>>>(
+        kpool, kbase, obase, ds, theta, lam, Dt, opool);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
 VT_EXPORT int vt_kpcg_tbar(const double *Tm, const int *nbase, int Dt, int batch, double *ws,
                            double *c, double *Ti, void *stream)
 {
@@ -371,22 +532,31 @@ static int skinny_run(const VtSkinnyLaunch &l, int batch, int max_Nc, void *stre
         if (st__ != VT_OK) return st__; \
     } while (0)
 
-// `iters` CG iterations (after vt_kpcg_init): launches = {fwd, bwd, pre} as
-// {descs, max_M, max_K} triples for the operator halves and the preconditioner.
+// `iters` CG iterations (after vt_kpcg_init): launches = {fwd, bwd} as {descs, max_M, max_K}
+// triples for the operator halves (batch problems each, max_Nc columns), and the
+// preconditioner as up to three skinny launches run in order, each {descs, count, max_M,
+// max_K, max_Nc} (a null descs skips it).
 VT_EXPORT int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M,
-                              int bwd_K, const void *pre, int pre_M, int pre_K, int batch, int max_Nc,
-                              double *x, double *r, double *p, double *q, double *z, const double *lam,
-                              const long long *seg, double *st, double *hist, int hist_stride,
-                              int iters, double stop_rel2, int graph_chunk, void *stream)
+                              int bwd_K, const void *pre, int pre_n, int pre_M, int pre_K, int pre_Nc,
+                              const void *pre2, int pre2_n, int pre2_M, int pre2_K, int pre2_Nc,
+                              const void *pre3, int pre3_n, int pre3_M, int pre3_K, int pre3_Nc,
+                              int batch, int max_Nc, double *x, double *r, double *p, double *q,
+                              double *z, const double *lam, const long long *seg, double *st,
+                              double *hist, int hist_stride, int iters, double stop_rel2,
+                              int graph_chunk, void *stream)
 {
     VT_CHECK_ARG(fwd && bwd && pre && iters >= 0 && iters < hist_stride && graph_chunk >= 0);
-    const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K}, Pr{pre, pre_M, pre_K};
+    const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
+    struct PreLaunch { const void *d; int n, M, K, Nc; };
+    const PreLaunch P[3] = {{pre, pre_n, pre_M, pre_K, pre_Nc}, {pre2, pre2_n, pre2_M, pre2_K, pre2_Nc},
+                            {pre3, pre3_n, pre3_M, pre3_K, pre3_Nc}};
     // the iteration index comes from each block's count (it = -1): identical iterations
     auto step = [&](hipStream_t sq) -> int {
         VT_TRY(skinny_run(F, batch, max_Nc, sq));
         VT_TRY(skinny_run(Bk, batch, max_Nc, sq));
         VT_TRY(vt_kpcg_step_a(x, r, p, q, lam, seg, batch, st, hist, hist_stride, -1, stop_rel2, sq));
-        VT_TRY(skinny_run(Pr, batch, max_Nc, sq));
+        for (int i = 0; i < 3; ++i)
+            if (P[i].d) VT_TRY(vt_gemm_skinny(P[i].d, P[i].n, P[i].M, P[i].K, P[i].Nc, sq));
         VT_TRY(vt_kpcg_step_b(r, z, p, seg, batch, st, sq));
         return VT_OK;
     };
