@@ -263,7 +263,7 @@ def test_vtdm_windows_vs_oracle(torch_mod, oracle):
     Dsw, dirich, sel = vtdm.VTDM_prep(enc, None, None, 5, 5)
     assert sel.size == N
     u = vtdm.VTDM(enc, Dsw, dirich, 0.0, 1.0, 0.01, 5, 5, Mt=4, stitching=True, stitch_interval=0.4,
-                  neurons=sel)
+                  output_format=2, neurons=sel)
     rf_o = oracle.rf_gabor(bank.h_alpha[sel_all], bank.h_l[sel_all], bank.h_x0[sel_all],
                            bank.h_y0[sel_all], bank.h_ab[sel_all], bank.KAPPA, 32, 32, 32.0, 32.0)
     Ds_o = oracle.compute_Ds_fft(rf_o, 1.0, 1.0, 5, 5, "ellipse")
@@ -299,7 +299,7 @@ def test_decode_video_tiles_vs_oracle(torch_mod, oracle):
     W = 2 * math.pi * Mx / 32
     enc = VTEM_IAF(vid.frames, None, Wt, g, W, W, dx=1.0, dy=1.0, npinput=True, iaf=iaf)
     u = vtdm.decode_video(enc, None, None, 0.0, 0.3, 0.01, Mx, Mt=Mt, spatial_stitching=True,
-                          spatial_interval=[32, 32])
+                          spatial_interval=[32, 32], output_format=2)
 
     def tile(dom):
         sel, x0, y0 = oracle.select_neurons(bank.h_x0, bank.h_y0, dom)
@@ -361,7 +361,7 @@ def test_iaf_rt_encode_and_weighted_decode(torch_mod, oracle):
     ch, x0, y0 = oracle.select_neurons(bank.h_x0, bank.h_y0, dom)
     np.testing.assert_array_equal(sel, ch)
     assert 0 < sel.size < N
-    u = vtdm.VTDM(enc, Dsw, dirich, 0.0, 1.0, 0.01, 5, 5, Mt=4, domain=dom, neurons=sel)
+    u = vtdm.VTDM(enc, Dsw, dirich, 0.0, 1.0, 0.01, 5, 5, Mt=4, domain=dom, output_format=2, neurons=sel)
     norm = oracle.decode_norm(kappa, sigma)[sel]
     assert norm.max() < 1.0 or np.argmax(oracle.decode_norm(kappa, sigma)) in sel
     rf_o = oracle.rf_gabor(bank.h_alpha[sel], bank.h_l[sel], x0, y0, bank.h_ab[sel], bank.KAPPA,

@@ -5,8 +5,9 @@ the intervals of the selected neurons; compute_q, compute_G(Dsw, lamb) and
 reconstruct(dirich, time_frame, dt) run vt_q, vt_G_spike (+ vt_diag_add) and
 vt_reconstruct_spike.  As in the reference, reconstruct reads the solution c from
 self.d_q (the caller overwrites it with the solver's output, e.g. solve_eq_sym or
-rnn3).  The Dsw / dirich arguments, file names in the reference, are the arrays
-themselves (the outputs of vrf.compute_Dsw / compute_dirich_space).
+rnn3).  The Dsw / Ds / dirich arguments, file names in the reference, are file names
+(array files, utils.simpleio) or the arrays themselves (the outputs of vrf.compute_Dsw /
+compute_Ds / compute_dirich_space).
 """
 
 from __future__ import annotations
@@ -15,11 +16,14 @@ import numpy as np
 import torch
 
 from ._lib import lib, ptr, stream_ptr
+from .utils import simpleio
 
 F64 = torch.float64
 
 
 def _dev(a, dtype=F64, device="cuda"):
+    if isinstance(a, (str, bytes)) or hasattr(a, "__fspath__"):
+        a = simpleio.read_file(a)
     if isinstance(a, torch.Tensor):
         return a.to(device=device, dtype=dtype).contiguous()
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
@@ -62,6 +66,14 @@ class dirichlet:
         if lamb != 0:
             lib.vt_diag_add(ptr(self.d_G), 0, self.size, self.size, float(lamb), 0, stream_ptr())
         return self.d_G
+
+    def compute_Gb(self, Ds, lamb=0.0):
+        """G from the RFs' Dirichlet coefficients Ds (N x modes, complex): Dsw =
+        Re(Ds Ds^H) * norm norm^T with this decoder's norm, then compute_G
+        (vtem/dirichlet.py:87-127)."""
+        from .vrf import weighted_gram
+        D = _dev(Ds, torch.complex128, self.device)
+        return self.compute_G(weighted_gram(D.reshape(D.shape[0], -1), self.d_norm, self.device), lamb)
 
     def reconstruct(self, dirich, time_frame, dt):
         """u(t, x, y) = sum_k c_k norm_j psi_j(x, y) Phi_k(t) at t = arange(t0, t1, dt),

@@ -137,17 +137,7 @@ class vrf:
 
     def compute_Dsw(self, d_Ds, Mx=None, My=None, h_norm=None):
         """Dsw = Re(Ds Ds^H) * norm norm^T (N x N float64)."""
-        Ds = _dev(d_Ds, torch.complex128, self.device)
-        N = Ds.shape[0]
-        M = Ds[0].numel()
-        R = torch.view_as_real(Ds).reshape(N, 2 * M)      # interleaved re, im
-        W = torch.empty((N, N), dtype=F64, device=self.device)
-        # Re Re^T + Im Im^T = sum over the interleaved pairs
-        _gemm(N, N, 2 * M, 1.0, R, 2 * M, 1, R, 1, 2 * M, 0.0, W, N)
-        if h_norm is not None:
-            nrm = _dev(h_norm, F64, self.device)
-            lib.vt_scale_outer(ptr(W), N, N, ptr(nrm), stream_ptr())
-        return W
+        return weighted_gram(d_Ds, h_norm, self.device)
 
     def compute_dirich_space(self, d_Ds, Mx, My, Px, Py, Sx, Sy, Wx, Wy, x_start=None, y_start=None):
         """psi_j(x, y) = Re sum_m Ds_j(m) e^{i(mx Wx/Mx x + my Wy/My y)} on a Px x Py grid
@@ -175,6 +165,23 @@ class vrf:
         """The same functions as compute_dirich_space (the reference computes them by an
         inverse FFT; both forms agree)."""
         return self.compute_dirich_space(d_Ds, Mx, My, Px, Py, Sx, Sy, Wx, Wy)
+
+
+def weighted_gram(d_Ds, h_norm=None, device="cuda"):
+    """Re(Ds Ds^H) * norm norm^T (N x N float64) for Dirichlet coefficients Ds (N x
+    modes, complex): the RFs' inner products in the Dirichlet space, weighted
+    (vtem/vtdm.py:440-446 with put_norm; vtem/dirichlet.py:101-115)."""
+    Ds = _dev(d_Ds, torch.complex128, device)
+    N = Ds.shape[0]
+    M = Ds[0].numel()
+    R = torch.view_as_real(Ds).reshape(N, 2 * M)      # interleaved re, im
+    W = torch.empty((N, N), dtype=F64, device=device)
+    # Re Re^T + Im Im^T = sum over the interleaved pairs
+    _gemm(N, N, 2 * M, 1.0, R, 2 * M, 1, R, 1, 2 * M, 0.0, W, N)
+    if h_norm is not None:
+        nrm = _dev(h_norm, F64, device)
+        lib.vt_scale_outer(ptr(W), N, N, ptr(nrm), stream_ptr())
+    return W
 
 
 def _grid_law(S, dilations, alpha0, b0, b0scalebase):

@@ -8,9 +8,13 @@ the HIP library (vrf.compute_Ds / compute_Dsw / compute_dirich_space, vt_interva
 dirichlet.*, utils.linalg.solve_eq_sym, neural_network.rnn3, vt_stitch_accumulate).
 
 The reference passes spikes and prepared data through HDF5 files (spike file, dsw.h5,
-dirich.h5).  Here the spike source is the EncodedVideo that vtem.VTEM_IAF returns and
-VTDM_prep returns (Dsw, dirich) instead of writing them; the file-name arguments are
-accepted and ignored when the arrays are given.
+dirich.h5).  Here `spikefile` is a spike file name (utils.spike_store) or the
+EncodedVideo that vtem.VTEM_IAF returns; VTDM_prep / VTDM_prepb write Dsw (or Ds) and
+dirich to the given file names (utils.simpleio) and also return them, and VTDM / VTDMb
+take either the file names or the arrays.  With precompute=False, decode_video runs
+VTDM_prepb / VTDMb: the prepared file holds each RF's Dirichlet coefficients Ds and the
+decoder forms Dsw = Re(Ds Ds^H) with the neurons' weights itself (vtem/vtdm.py:699-1071,
+vtem/dirichlet.py:87-127).
 
 Spatial Dirichlet coefficients are computed over one period on the encoder's pixel grid
 (scale = 1 by default: Ds is then exactly the DFT the encoder's filtering applied); the
@@ -20,6 +24,7 @@ reference samples the RFs at scale 4 (vtem/vtdm.py:424-425), available through `
 from __future__ import annotations
 
 import math
+import os
 from typing import Optional, Sequence
 
 import numpy as np
@@ -30,6 +35,7 @@ from . import neural_network as nn
 from . import vrf as vrf_mod
 from ._lib import lib, ptr, stream_ptr
 from .utils import linalg as ls
+from .utils import simpleio, videoio
 
 F64 = torch.float64
 STITCH_DTYPE = np.dtype([("blk", np.uint64), ("wt", np.uint64), ("wy", np.uint64), ("wx", np.uint64),
@@ -114,11 +120,20 @@ def _full_domain(enc, domain):
             max(float(domain[2]), -Sy / 2), min(float(domain[3]), Sy / 2)]
 
 
-def VTDM_prep(spikefile, Dswfilename=None, dirichfilename=None, Mx=None, My=None, domain=None, Wx=None,
-              Wy=None, dtype=np.complex128, scale: int = 1, mask: str = "ellipse"):
-    """Dsw (N_sel x N_sel) and dirich (N_sel x decPy x decPx) of the neurons selected by
-    `domain` (vtem/vtdm.py:311-447).  Returns (Dsw, dirich, selected neuron indices)."""
-    enc = spikefile
+def _encoded(spikefile):
+    """The EncodedVideo behind a spike-file name or an EncodedVideo."""
+    if isinstance(spikefile, (str, bytes, os.PathLike)):
+        from .vtem import load_encoded
+        return load_encoded(spikefile)
+    return spikefile
+
+
+def _is_name(x) -> bool:
+    return isinstance(x, (str, bytes, os.PathLike))
+
+
+def _period_rf(enc, Mx, My, Wx, Wy, domain, scale):
+    """(rf over one spatial period for the selected neurons, selection, domain)."""
     Mx = int(Mx)
     My = Mx if My is None else int(My)
     Wx = enc.Wx if Wx is None else float(Wx)
@@ -130,13 +145,45 @@ def VTDM_prep(spikefile, Dswfilename=None, dirichfilename=None, Mx=None, My=None
     # one spatial period 2 pi M / W on the encoder's grid
     rfPx = int(round(2 * math.pi * Mx / Wx / enc.dx))
     rfPy = int(round(2 * math.pi * My / Wy / enc.dy))
-    period_rf = _sub_rf(enc.rf, sel, x0, y0, (rfPy, rfPx), enc.dx, enc.dy, scale)
+    return _sub_rf(enc.rf, sel, x0, y0, (rfPy, rfPx), enc.dx, enc.dy, scale), sel, dom, Mx, My, Wx, Wy
+
+
+def VTDM_prep(spikefile, Dswfilename=None, dirichfilename=None, Mx=None, My=None, domain=None, Wx=None,
+              Wy=None, dtype=np.complex128, scale: int = 1, mask: str = "ellipse"):
+    """Dsw (N_sel x N_sel) and dirich (N_sel x decPy x decPx) of the neurons selected by
+    `domain` (vtem/vtdm.py:311-447), written to Dswfilename / dirichfilename when given.
+    Returns (Dsw, dirich, selected neuron indices)."""
+    enc = _encoded(spikefile)
+    period_rf, sel, dom, Mx, My, Wx, Wy = _period_rf(enc, Mx, My, Wx, Wy, domain, scale)
     Ds = period_rf.compute_Ds(Mx, My, mask=mask)
     Dsw = period_rf.compute_Dsw(Ds, Mx, My, decode_norm(enc.iaf, sel))
     decSx, decSy = dom[1] - dom[0], dom[3] - dom[2]
     decPx, decPy = int(round(decSx / enc.dx)), int(round(decSy / enc.dy))
     dirich = period_rf.compute_dirich_space(Ds, Mx, My, decPx, decPy, decSx, decSy, Wx, Wy)
+    if _is_name(Dswfilename):
+        simpleio.write_memory_to_file(Dsw, Dswfilename)
+    if _is_name(dirichfilename):
+        simpleio.write_memory_to_file(dirich, dirichfilename)
     return Dsw, dirich, sel
+
+
+def VTDM_prepb(spikefile, Dsfilename=None, dirichfilename=None, Mx=None, My=None, domain=None, Wx=None,
+               Wy=None, dtype=np.complex128, scale: int = 1, mask: str = "ellipse"):
+    """The Dirichlet coefficients Ds (N_sel x (2My+1) x (2Mx+1), complex) and dirich of the
+    selected neurons (vtem/vtdm.py:699-813), written to Dsfilename / dirichfilename when
+    given; VTDMb forms the weighted Gram of the RFs from Ds.  Returns (Ds, dirich, selected
+    neuron indices)."""
+    enc = _encoded(spikefile)
+    period_rf, sel, dom, Mx, My, Wx, Wy = _period_rf(enc, Mx, My, Wx, Wy, domain, scale)
+    Ds = period_rf.compute_Ds(Mx, My, mask=mask)
+    decSx, decSy = dom[1] - dom[0], dom[3] - dom[2]
+    decPx, decPy = int(round(decSx / enc.dx)), int(round(decSy / enc.dy))
+    dirich = period_rf.compute_dirich_space(Ds, Mx, My, decPx, decPy, decSx, decSy, Wx, Wy)
+    if _is_name(Dsfilename):
+        simpleio.write_memory_to_file(Ds, Dsfilename)
+    if _is_name(dirichfilename):
+        simpleio.write_memory_to_file(dirich, dirichfilename)
+    return Ds, dirich, sel
 
 
 class _SpikeSource:
@@ -175,17 +222,28 @@ class _SpikeSource:
         return tk1[:n], tk2[:n], nid[:n], n
 
 
+def _write_output(u, output_format, name):
+    """output_format 0: an AVI file, 1: an array file, else nothing."""
+    if output_format == 0:
+        videoio.write_video(u, name)
+    elif output_format == 1:
+        simpleio.write_memory_to_file(u, name)
+
+
 def VTDM(spikefile, Dswfilename, dirichfilename, start_time, end_time, dt, Mx, My=None, Mt=None,
          domain=None, Wx=None, Wy=None, Wt=None, lamb=0.0, dtype=np.float64, rnn=False, alpha=50000,
-         steps=4000, stitching=False, stitch_interval=None, precompute=True, output_format=2,
-         output="rec", neurons=None, source: Optional[_SpikeSource] = None, keep_device: bool = False):
+         steps=4000, stitching=False, stitch_interval=None, output_format=0, output="rec", *,
+         neurons=None, source: Optional[_SpikeSource] = None, keep_device: bool = False, _gram_from_ds=False):
     """Spike-space decode of [start_time, end_time) at frame step dt, in temporal windows
     of stitch_interval overlapping by 20 % (at least 4 frames when the window has 8 or
     more) and crossfaded with Hann ramps (vtem/vtdm.py:450-692).  Dswfilename /
-    dirichfilename are the arrays from VTDM_prep; `neurons` the indices it selected.
-    Returns the video (T, decPy, decPx) as a float64 ndarray."""
-    enc = spikefile
-    Dsw, dirich = Dswfilename, dirichfilename
+    dirichfilename are VTDM_prep's file names or arrays; `neurons` the indices it selected
+    (selected again from `domain` when not given).  output_format 0 writes output + ".avi",
+    1 the array file output + ".h5", anything else nothing.  Returns the video (T, decPy,
+    decPx) as a float64 ndarray (a device tensor with keep_device)."""
+    enc = _encoded(spikefile)
+    Dsw = simpleio.read_file(Dswfilename) if _is_name(Dswfilename) else Dswfilename
+    dirich = simpleio.read_file(dirichfilename) if _is_name(dirichfilename) else dirichfilename
     if neurons is None:
         neurons = select_neurons(enc.rf, _full_domain(enc, domain))[0]
     Wt = enc.Wt if Wt is None else float(Wt)
@@ -223,7 +281,10 @@ def VTDM(spikefile, Dswfilename, dirichfilename, start_time, end_time, dt, Mx, M
         tk1, tk2, nid, n = src.select(neurons, start, end)
         cp = dr.dirichlet(len(neurons), tk1, tk2, nid, kappa, delta, bias, norm, Wt, Mt)
         cp.compute_q()
-        cp.compute_G(Dsw, float(lamb))
+        if _gram_from_ds:
+            cp.compute_Gb(Dsw, float(lamb))
+        else:
+            cp.compute_G(Dsw, float(lamb))
         if rnn:
             cp.d_q = nn.rnn3(cp.d_G, cp.d_q, alpha=alpha, steps=steps)
         else:
@@ -254,6 +315,23 @@ def VTDM(spikefile, Dswfilename, dirichfilename, start_time, end_time, dt, Mx, M
         if end_time - end < overlap:
             end = end_time
         i += 1
+    if output_format in (0, 1):
+        _write_output(u, output_format, output + (".avi" if output_format == 0 else ".h5"))
+    return u if keep_device else u.cpu().numpy()
+
+
+def VTDMb(spikefile, Dsfilename, dirichfilename, start_time, end_time, dt, Mx, My=None, Mt=None,
+          domain=None, Wx=None, Wy=None, Wt=None, lamb=0.0, dtype=np.float64, rnn=False, alpha=50000,
+          steps=4000, stitching=False, stitch_interval=None, output_format=0, output="rec", *,
+          neurons=None, source: Optional[_SpikeSource] = None, keep_device: bool = False):
+    """VTDM from VTDM_prepb's Dirichlet coefficients (vtem/vtdm.py:816-1071): each
+    window's G comes from Dsw = Re(Ds Ds^H) * norm norm^T formed on the device
+    (dirichlet.compute_Gb), the rest as VTDM.  output_format 0 / 1 write the file named
+    `output` as given (no extension added, as the reference's VTDMb)."""
+    u = VTDM(spikefile, Dsfilename, dirichfilename, start_time, end_time, dt, Mx, My, Mt, domain, Wx, Wy, Wt,
+             lamb, dtype, rnn, alpha, steps, stitching, stitch_interval, 2, output, neurons=neurons,
+             source=source, keep_device=True, _gram_from_ds=True)
+    _write_output(u, output_format, output)
     return u if keep_device else u.cpu().numpy()
 
 
@@ -261,21 +339,31 @@ def decode_video(spikefile, Dswfilename=None, dirichfilename=None, start_time=0.
                  dt=0.01, Mx=None, My=None, Mt=None, domain=None, Wx=None, Wy=None, Wt=None, lamb=0.0,
                  dtype=np.float64, rnn=False, alpha=50000, steps=4000, stitching=False,
                  stitch_interval=None, spatial_stitching=False, spatial_interval=(96, 96),
-                 precompute=True, write_blocks=False, output_format=2, output="rec", scale: int = 1,
+                 precompute=True, write_blocks=False, output_format=0, output="rec", scale: int = 1,
                  mask: str = "ellipse"):
-    """Decode a VTEM_IAF result (vtem/vtdm.py:18-309): one VTDM over the domain, or, with
-    spatial_stitching, tiles of spatial_interval pixels overlapping by 20 % placed from the
-    right / top edges, each decoded with its own VTDM_prep and accumulated with Hann-ramp
-    windows.  Returns the video (T, Py, Px) float64."""
-    enc = spikefile
-    end_time = enc.num_frames / enc.fps if end_time is None else end_time
+    """Decode a VTEM_IAF result, a spike file name or an EncodedVideo (vtem/vtdm.py:18-309):
+    one VTDM over the domain, or, with spatial_stitching, tiles of spatial_interval pixels
+    overlapping by 20 % placed from the right / top edges, each decoded with its own
+    VTDM_prep and accumulated with Hann-ramp windows.  precompute=False prepares Ds
+    instead of Dsw (VTDM_prepb / VTDMb).  Dswfilename / dirichfilename, when given, receive
+    the prepared arrays (of the last tile).  write_blocks writes each tile's video as
+    output + its number (".h5" array files with output_format 1, else ".avi"); output_format
+    0 / 1 writes the whole video as output + ".avi" / ".h5".  Returns the video (T, Py, Px)
+    float64."""
+    enc = _encoded(spikefile)
+    end_time = enc.duration if end_time is None else end_time
     src = _SpikeSource(enc.spikes)
     kw = dict(Mt=Mt, Wx=Wx, Wy=Wy, Wt=Wt, lamb=lamb, dtype=dtype, rnn=rnn, alpha=alpha, steps=steps,
               stitching=stitching, stitch_interval=stitch_interval, source=src)
+    prep, dec = (VTDM_prep, VTDM) if precompute else (VTDM_prepb, VTDMb)
     dom = _full_domain(enc, domain)
     if not spatial_stitching:
-        Dsw, dirich, sel = VTDM_prep(enc, None, None, Mx, My, dom, Wx, Wy, scale=scale, mask=mask)
-        return VTDM(enc, Dsw, dirich, start_time, end_time, dt, Mx, My, domain=dom, neurons=sel, **kw)
+        Dsw, dirich, sel = prep(enc, Dswfilename, dirichfilename, Mx, My, dom, Wx, Wy, scale=scale, mask=mask)
+        u = dec(enc, Dsw, dirich, start_time, end_time, dt, Mx, My, domain=dom, neurons=sel, output_format=2,
+                keep_device=True, **kw)
+        if output_format in (0, 1):
+            _write_output(u, output_format, output + (".avi" if output_format == 0 else ".h5"))
+        return u.cpu().numpy()
     dx, dy = enc.dx, enc.dy
     decPx, decPy = int(round((dom[1] - dom[0]) / dx)), int(round((dom[3] - dom[2]) / dy))
     sx, sy = min(spatial_interval[0], decPx), min(spatial_interval[1], decPy)
@@ -285,21 +373,27 @@ def decode_video(spikefile, Dswfilename=None, dirichfilename=None, start_time=0.
     start_y = max(dom[3] - sy * dy, dom[2])
     end_y = dom[3]
     y0i, y1i = 0, sy
+    tile = 0
     while True:
         start_x = max(dom[1] - sx * dx, dom[0])
         end_x = dom[1]
         x0i, x1i = decPx - sx, decPx
         while True:
             cur = [start_x, end_x, start_y, end_y]
-            Dsw, dirich, sel = VTDM_prep(enc, None, None, Mx, My, cur, Wx, Wy, scale=scale, mask=mask)
+            Dsw, dirich, sel = prep(enc, Dswfilename, dirichfilename, Mx, My, cur, Wx, Wy, scale=scale,
+                                    mask=mask)
             left = 0 if x0i == 0 else ox
             right = 0 if x1i == decPx else ox
             top = 0 if y0i == 0 else oy
             bottom = 0 if y1i == decPy else oy
             xw = np.concatenate([_hann_ramp(left), np.ones(x1i - x0i - left - right), 1 - _hann_ramp(right)])
             yw = np.concatenate([_hann_ramp(top), np.ones(y1i - y0i - top - bottom), 1 - _hann_ramp(bottom)])
-            blk = VTDM(enc, Dsw, dirich, start_time, end_time, dt, Mx, My, domain=cur, neurons=sel,
-                       keep_device=True, **kw)
+            blk = dec(enc, Dsw, dirich, start_time, end_time, dt, Mx, My, domain=cur, neurons=sel,
+                      output_format=2, keep_device=True, **kw)
+            if write_blocks:
+                fmt = 1 if output_format == 1 else 0
+                _write_output(blk, fmt, f"{output}{tile}" + (".h5" if fmt == 1 else ".avi"))
+            tile += 1
             accumulate(u, [(blk, np.ones(blk.shape[0]), yw, xw, 0, y0i, x0i)])
             if start_x == dom[0]:
                 break
@@ -313,4 +407,6 @@ def decode_video(spikefile, Dswfilename=None, dirichfilename=None, start_time=0.
         end_y = end_y - sy * dy + oy * dy
         y0i = y1i - oy
         y1i = min(y0i + sy, decPy)
+    if output_format in (0, 1):
+        _write_output(u, output_format, output + (".avi" if output_format == 0 else ".h5"))
     return u.cpu().numpy()
