@@ -303,9 +303,12 @@ def test_gram_f64(torch_mod, oracle, cfg):
     assert err < 1e-13, err
 
 
-@pytest.mark.parametrize("D", [1, 63, 64, 65, 200, 1089])
+@pytest.mark.parametrize("D", [1, 63, 64, 65, 200, 256, 257, 1089, 2100])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_potrf_potrs(torch_mod, D, prec):
+    """Blocked Cholesky (64-column steps inside 256-column outer panels, one trailing
+    update per outer panel) and the triangular solves, against LAPACK's factor and solve;
+    sizes on both sides of the 64- and 256-column boundaries."""
     torch = torch_mod
     from vtem_amd.block import mat_descs
     from vtem_amd._lib import lib, ptr, stream_ptr
@@ -342,12 +345,13 @@ def test_potrf_potrs(torch_mod, D, prec):
 
 
 def test_potrf_batched_ragged(torch_mod):
-    """One launch sequence factors matrices of different sizes (serving form)."""
+    """One launch sequence factors matrices of different sizes (serving form), some ending
+    inside an outer panel whose trailing update the others still receive."""
     torch = torch_mod
     from vtem_amd.block import mat_descs
     from vtem_amd._lib import lib, ptr, stream_ptr
     _banner("potrf batched ragged")
-    sizes = [45, 130, 64, 300, 7]
+    sizes = [45, 130, 64, 300, 7, 513, 700, 256]
     rng = np.random.default_rng(5)
     mats, As, bs = [], [], []
     maxD = max(sizes)

@@ -24,6 +24,9 @@ ARCH = os.environ.get("VTEM_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["vt_encode.hip", "vt_gemm.hip", "vt_decode.hip", "vt_gram.hip", "vt_chol.hip",
            "vt_kpcg.hip", "vt_batch.hip", "vt_dirichlet.hip", "vt_iaf_par.hip", "vt_svd.hip"]
 LIBS = []
+# per-source flags: keep the MFMA accumulators of the Cholesky's trailing update in
+# ordinary VGPRs (otherwise the compiler copies them between AGPRs and VGPRs every k step)
+EXTRA = {"vt_chol.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
          "-Xclang", "-packed-fp32-ops", "-fvisibility=hidden", "-Wno-unused-result",
@@ -39,7 +42,7 @@ def hipcc() -> str:
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *FLAGS, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *FLAGS, *EXTRA.get(src, []), "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

@@ -38,45 +38,39 @@ template <> struct Mfma16<float> {
 };
 
 // ------------------------------------------------------------------------------------
-// diag: factor + invert the 64x64 diagonal block.  256 threads; thread (r, q) keeps the
-// 16 entries [16q, 16q+16) of row r in registers (static indices after unrolling).  At
-// column step c the threads owning column c publish it to LDS, everybody reads the
-// pivot and the scaled column back as broadcasts, and each thread applies at most 16
-// register FMAs:  L[r][rp] -= L[r][c] L[rp][c]  for c < rp <= r.  The inverse
-// X = L^-1 uses the same layout, right-looking over rows:
-//   X[k,:] /= L[k][k];   X[r,:] -= L[r][k] X[k,:]  (r > k).
+// diag: factor + invert the 64x64 diagonal block, one wavefront per matrix.  Lane r keeps
+// row r in registers (static indices after unrolling).  At column step c every lane
+// publishes its A[r][c] to LDS, reads the pivot and the column back as broadcasts and
+// applies  A[r][rp] -= L[r][c] L[rp][c]  for c < rp <= r.  With a single wavefront the
+// workgroup barriers only order the LDS traffic.  The inverse
+// X = L^-1 is column-parallel: lane c runs the forward substitution of column c,
+//   X[k][c] = (delta_kc - sum_{j<k} L[k][j] X[j][c]) / L[k][k],
+// reading the rows of L as LDS broadcasts, with no communication between lanes.
 // ------------------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
 potrf_diag_kernel(const VtMat<T> *__restrict__ mats, int kb, T *__restrict__ inv_ws,
                   long long inv_stride, int *__restrict__ info)
 {
     __shared__ T Ls[NB][NB + 1];
     __shared__ T col[NB];
-    __shared__ T xrow[NB];
     const VtMat<T> m = mats[blockIdx.x];
     if (kb >= m.D) return;
     const int nb = min(NB, m.D - kb);
-    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6, c0 = q * 16;
+    const int r = threadIdx.x;
     T *Ad = m.A + (long long)kb * m.lda + kb;
-    T Lr[16];
+    T Lr[NB];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int c = c0 + i;
+    for (int i = 0; i < NB; ++i) {
         T v = (T)0;
-        if (r < nb && c <= r) v = Ad[(long long)r * m.lda + c];
-        if (r >= nb && c == r) v = (T)1;      // identity rows pad a partial last block
+        if (r < nb && i <= r) v = Ad[(long long)r * m.lda + i];
+        if (r >= nb && i == r) v = (T)1;      // identity rows pad a partial last block
         Lr[i] = v;
     }
     int bad = 0;
-    for (int c = 0; c < NB; ++c) {
-        const int cq = c >> 4, ci = c & 15;
-        if (q == cq) {
-            T v = (T)0;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) if (i == ci) v = Lr[i];
-            col[r] = v;                         // A[r][c] after the previous updates
-        }
+    for (int c = 0; c < NB; ++c) {
+        col[r] = Lr[c];                         // A[r][c] after the previous updates
         __syncthreads();
         T d = col[c];
         if (!(d > (T)0)) {
@@ -85,53 +79,33 @@ potrf_diag_kernel(const VtMat<T> *__restrict__ mats, int kb, T *__restrict__ inv
         }
         const T sd = sqrt(d);
         const T inv = (T)1 / sd;
-        const T lrc = (r > c) ? col[r] * inv : (T)0;   // L[r][c]
-        if (q == cq) {
+        const T lrc = (r > c) ? Lr[c] * inv : (T)0;   // L[r][c]
+        Lr[c] = (r == c) ? sd : ((r > c) ? lrc : Lr[c]);
 #pragma unroll
-            for (int i = 0; i < 16; ++i)
-                if (i == ci) Lr[i] = (r == c) ? sd : ((r > c) ? lrc : Lr[i]);
-        }
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int rp = c0 + i;
-            if (rp > c && rp <= r) Lr[i] -= lrc * (col[rp] * inv);
-        }
+        for (int rp = c + 1; rp < NB; ++rp)
+            if (rp <= r) Lr[rp] -= lrc * (col[rp] * inv);
         __syncthreads();
     }
-    if (tid == 0 && bad && info && info[blockIdx.x] == 0) info[blockIdx.x] = bad;
+    if (r == 0 && bad && info && info[blockIdx.x] == 0) info[blockIdx.x] = bad;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) Ls[r][c0 + i] = (c0 + i <= r) ? Lr[i] : (T)0;
+    for (int i = 0; i < NB; ++i) Ls[r][i] = (i <= r) ? Lr[i] : (T)0;
     __syncthreads();
-    for (int e = tid; e < NB * NB; e += 256) {
+    for (int e = r; e < NB * NB; e += 64) {
         const int rr = e >> 6, cc = e & 63;
         if (rr < nb && cc <= rr) Ad[(long long)rr * m.lda + cc] = Ls[rr][cc];
     }
-    // inverse: X[r][c0..c0+15] in registers (reuse Lr), starts as the identity
+    // inverse, column c = r (reuse Lr for X[:, c])
+    const int c = r;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) Lr[i] = (r == c0 + i) ? (T)1 : (T)0;
     for (int k = 0; k < NB; ++k) {
-        if (r == k) {
-            const T dk = (T)1 / Ls[k][k];
+        T v = (k == c) ? (T)1 : (T)0;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                Lr[i] *= dk;
-                xrow[c0 + i] = Lr[i];
-            }
-        }
-        __syncthreads();
-        if (r > k) {
-            const T lrk = Ls[r][k];
-#pragma unroll
-            for (int i = 0; i < 16; ++i) Lr[i] -= lrk * xrow[c0 + i];
-        }
-        __syncthreads();
+        for (int j = 0; j < k; ++j) v -= Ls[k][j] * Lr[j];
+        Lr[k] = v * ((T)1 / Ls[k][k]);
     }
     T *W = inv_ws + (long long)blockIdx.x * inv_stride + (long long)(kb / NB) * NB * NB;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int c = c0 + i;
-        W[r * NB + c] = (r < nb && c < nb) ? Lr[i] : (T)0;
-    }
+    for (int k = 0; k < NB; ++k) W[k * NB + c] = (k < nb && c < nb) ? Lr[k] : (T)0;
 }
 
 // ------------------------------------------------------------------------------------
@@ -180,11 +154,13 @@ potrf_panel_kernel(const VtMat<T> *__restrict__ mats, int kb, const T *__restric
 }
 
 // ------------------------------------------------------------------------------------
-// update: A[I,J] -= X_I X_J^T for lower tiles I >= J of the trailing matrix
+// panel update: A[I,J] -= X_I X_J^T for the lower 64x64 tiles I >= J whose columns lie in
+// [kb + 64, jend) -- inside the current 256-column outer panel.  grid (ntg * njg, batch):
+// tile pid <-> (bi, bj) = (pid % ntg, pid / ntg); tiles outside a matrix exit.
 // ------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256)
-potrf_update_kernel(const VtMat<T> *__restrict__ mats, int kb)
+potrf_update_kernel(const VtMat<T> *__restrict__ mats, int kb, int jend, int ntg)
 {
     typedef typename Mfma16<T>::acc_t acc_t;
     __shared__ T Xi[NB][NB + 1];
@@ -193,12 +169,9 @@ potrf_update_kernel(const VtMat<T> *__restrict__ mats, int kb)
     const int t0 = kb + NB;
     if (t0 >= m.D) return;
     const int nt = (m.D - t0 + NB - 1) / NB;
-    const int pid = blockIdx.x;
-    if (pid >= nt * (nt + 1) / 2) return;
-    int bi = (int)((sqrt(8.0 * pid + 1.0) - 1.0) * 0.5);
-    while ((bi + 1) * (bi + 2) / 2 <= pid) ++bi;
-    while (bi * (bi + 1) / 2 > pid) --bi;
-    const int bj = pid - bi * (bi + 1) / 2;
+    const int nj = (min(jend, m.D) - t0 + NB - 1) / NB;
+    const int bi = blockIdx.x % ntg, bj = blockIdx.x / ntg;
+    if (bi >= nt || bj >= nj || bi < bj) return;
     const int ri = t0 + bi * NB, rj = t0 + bj * NB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int e = tid; e < NB * NB; e += 256) {
@@ -229,19 +202,134 @@ potrf_update_kernel(const VtMat<T> *__restrict__ mats, int kb)
             }
 }
 
+// ------------------------------------------------------------------------------------
+// trailing update of an outer panel: A[I,J] -= sum_{k in [K, K+kw)} L[I,k] L[J,k] for the
+// lower 128x128 tiles I >= J of A[K+kw:, K+kw:].  Four waves, each a 64x64 quadrant (4x4
+// MFMA tiles); the panel's rows stream through LDS 16 columns at a time.  grid
+// (nt (nt+1) / 2, batch) for the largest matrix; tiles outside a matrix exit.
+// ------------------------------------------------------------------------------------
+#define TB 128
+#define KC 16
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+potrf_trail_kernel(const VtMat<T> *__restrict__ mats, int K, int kw)
+{
+    typedef typename Mfma16<T>::acc_t acc_t;
+    __shared__ T Xi[TB][KC + 1];
+    __shared__ T Xj[TB][KC + 1];
+    const VtMat<T> m = mats[blockIdx.y];
+    const int t0 = K + kw;
+    if (t0 >= m.D) return;
+    const int nt = (m.D - t0 + TB - 1) / TB;
+    const int pid = blockIdx.x;
+    if (pid >= nt * (nt + 1) / 2) return;
+    int bi = (int)((sqrt(8.0 * pid + 1.0) - 1.0) * 0.5);
+    while ((bi + 1) * (bi + 2) / 2 <= pid) ++bi;
+    while (bi * (bi + 1) / 2 > pid) --bi;
+    const int bj = pid - bi * (bi + 1) / 2;
+    const int ri = t0 + bi * TB, rj = t0 + bj * TB;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    acc_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (acc_t){0, 0, 0, 0};
+    // the panel's rows: lane pair (2 fr, 2 fr + 1) loads the two 8-column halves of row fr
+    // of both tiles (KC = 16 columns); the next chunk is loaded into registers while this
+    // one is used
+    static_assert(TB * KC == 256 * 8, "one row half per thread");
+    typedef T t2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+    typedef const t2 __attribute__((address_space(1))) *gt2;
+    const int fr = tid >> 1, fh = (tid & 1) * 8;
+    // rows past the matrix read its last row and are zeroed: no branch around the loads
+    const bool in_i = ri + fr < m.D, in_j = rj + fr < m.D;
+    const T *pa = m.A + (long long)min(ri + fr, m.D - 1) * m.lda + K + fh;
+    const T *pb = m.A + (long long)min(rj + fr, m.D - 1) * m.lda + K + fh;
+    const t2 zero2 = {(T)0, (T)0};
+    t2 pi[4], pj[4];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const t2 x = *(gt2)(pa + k0 + 2 * i), y = *(gt2)(pb + k0 + 2 * i);
+            pi[i] = in_i ? x : zero2;
+            pj[i] = in_j ? y : zero2;
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            Xi[fr][fh + 2 * i] = pi[i][0];
+            Xi[fr][fh + 2 * i + 1] = pi[i][1];
+            Xj[fr][fh + 2 * i] = pj[i][0];
+            Xj[fr][fh + 2 * i + 1] = pj[i][1];
+        }
+    };
+    fetch(0);
+    stash();
+    __syncthreads();
+    for (int k0 = 0; k0 < kw; k0 += KC) {
+        // the last chunk fetches itself again: the loop body has no branch
+        fetch(k0 + KC < kw ? k0 + KC : k0);
+#pragma unroll
+        for (int ks = 0; ks < KC; ks += 4) {
+            const int k = ks + (lane >> 4);
+            T a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = Xi[wm + 16 * i + (lane & 15)][k];
+                b[i] = Xj[wn + 16 * i + (lane & 15)][k];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma16<T>::op(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+        stash();
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = ri + wm + 16 * i + acc_row<T>(lane, q);
+                const int c = rj + wn + 16 * j + mfma_col(lane);
+                if (r < m.D && c <= r) m.A[(long long)r * m.lda + c] -= acc[i][j][q];
+            }
+}
+
+// Two-level blocking: 256-column outer panels, each factored in 64-column steps (diag,
+// panel solve of every row below, update of the tiles inside the outer panel); the rest
+// of the matrix receives the outer panel's 256 columns in one trailing update, so the
+// trailing matrix is read and written once per 256 columns instead of once per 64.
+#define KW 256
+
 template <typename T>
 static int potrf_run(const void *mats_dev, int batch, int maxD, T *inv_ws, long long inv_stride,
                      int *info, hipStream_t s)
 {
-    VT_CHECK_ARG(mats_dev && batch >= 1 && maxD >= 1 && inv_ws && inv_stride >= 0);
+    VT_CHECK_ARG(mats_dev && batch >= 1 && batch <= 65535 && maxD >= 1 && inv_ws && inv_stride >= 0);
     const VtMat<T> *mats = (const VtMat<T> *)mats_dev;
-    for (int kb = 0; kb < maxD; kb += NB) {
-        potrf_diag_kernel<T><<<batch, 256, 0, s>>>(mats, kb, inv_ws, inv_stride, info);
-        const int rows = maxD - kb - NB;
-        if (rows > 0) {
-            const int nch = (rows + NB - 1) / NB;
-            potrf_panel_kernel<T><<<dim3(nch, batch), 256, 0, s>>>(mats, kb, inv_ws, inv_stride);
-            potrf_update_kernel<T><<<dim3(nch * (nch + 1) / 2, batch), 256, 0, s>>>(mats, kb);
+    for (int K = 0; K < maxD; K += KW) {
+        const int kend = min(K + KW, maxD);
+        for (int kb = K; kb < kend; kb += NB) {
+            potrf_diag_kernel<T><<<batch, 64, 0, s>>>(mats, kb, inv_ws, inv_stride, info);
+            const int rows = maxD - kb - NB;
+            if (rows > 0) {
+                const int nch = (rows + NB - 1) / NB;
+                potrf_panel_kernel<T><<<dim3(nch, batch), 256, 0, s>>>(mats, kb, inv_ws, inv_stride);
+                const int njg = (kend - kb - NB + NB - 1) / NB;     // tile columns left in the panel
+                if (njg > 0)
+                    potrf_update_kernel<T><<<dim3(nch * njg, batch), 256, 0, s>>>(mats, kb, K + KW, nch);
+            }
+        }
+        if (K + KW < maxD) {
+            const int nt = (maxD - K - KW + TB - 1) / TB;
+            potrf_trail_kernel<T><<<dim3(nt * (nt + 1) / 2, batch), 256, 0, s>>>(mats, K, KW);
         }
     }
     VT_LAUNCH_CHECK();
