@@ -103,6 +103,8 @@ def parse():
     p.add_argument("--c3-only", action="store_true", help="only the 512x512 stitched batching form")
     p.add_argument("--roofline-only", action="store_true",
                    help="only time the dominant kernel at the workload's shapes (for rocprofv3 runs)")
+    p.add_argument("--cholesky-only", action="store_true",
+                   help="only time the batched FP64 Cholesky of 16 SPD matrices of D = 4913")
     return p.parse_args()
 
 
@@ -405,6 +407,38 @@ def kernel_source_sha16() -> str:
     return h.hexdigest()[:16]
 
 
+def cholesky_timing(batch=16, D=4913, reps=3, seed=0):
+    """The batched FP64 Cholesky (vt_potrf_f64) of `batch` SPD matrices G G^T / D + I of
+    order D (the plain FP64 path's factorisation at D = 17^3): seconds per call and FP64
+    TFLOP/s at D^3 / 3 flops per matrix (best of `reps` calls on fresh copies)."""
+    import torch
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    from vtem_amd.block import mat_descs
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    base = []
+    for _ in range(batch):
+        G = torch.randn(D, D, dtype=torch.float64, device="cuda", generator=g)
+        base.append(G @ G.T / D + torch.eye(D, dtype=torch.float64, device="cuda"))
+    nblk = (D + 63) // 64
+    inv = torch.zeros(batch * nblk * 4096, dtype=torch.float64, device="cuda")
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    best = float("inf")
+    for _ in range(reps):
+        mats = [x.clone() for x in base]
+        desc = mat_descs([(m.data_ptr(), D, D) for m in mats], "cuda")
+        info.zero_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        lib.vt_potrf_f64(ptr(desc), batch, D, ptr(inv), nblk * 4096, ptr(info), stream_ptr())
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+        if int(info.max()) != 0:
+            raise RuntimeError("Cholesky of an SPD test matrix failed")
+    return {"batch": batch, "D": D, "ms": round(best * 1e3, 3),
+            "tflops": round(batch * D ** 3 / 3 / best / 1e12, 2), "frac_fp64_peak": round(batch * D ** 3 / 3 / best / 78.6e12, 4)}
+
+
 def copy_calibration(nbytes=512 << 20, reps=5):
     """A plain device copy of known size (the scale reference for HBM byte counters)."""
     import torch
@@ -483,6 +517,9 @@ def main():
         return
     if args.c3_only:
         print(json.dumps({"stitched_c3": stitched_c3(args.seed)}), flush=True)
+        return
+    if args.cholesky_only:
+        print(json.dumps({"cholesky_f64": cholesky_timing(seed=args.seed)}), flush=True)
         return
     items = prepare(args.requests, args.seed)
     if args.roofline_only:

@@ -401,7 +401,7 @@ def test_gram_lowprec(torch_mod, oracle, mode, tol):
     dec = Decoder(geom, dev(S), np.ones(N), np.ones(N), np.ones(N))
     Td = dev(Tm)
     A32 = dec.gram_lp(Td, mode)
-    got = np.tril(A32.cpu().numpy().astype(np.float64))
+    got = np.tril(A32.cpu().numpy()).astype(np.float64)      # the upper triangle is never written
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     print(f"gram mode {mode} rel Frobenius error {err:.3e}")
     assert err < tol, err
