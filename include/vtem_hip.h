@@ -378,10 +378,13 @@ int vt_kpcg_place_diag(double *X, const long long *xoff, const int *dim, int bat
 int vt_kpcg_init(const double *r, const double *z, double *p, double *x, const long long *seg,
                  int batch, double *st, double *hist, int hist_stride, void *stream);
 /* it = -1: each block takes its iteration index from its own count st[4b+3] (equal to
- * the loop index while it is active), so every iteration has the same arguments. */
+ * the loop index while it is active), so every iteration has the same arguments.
+ * prune > 0 (>= 10): from iteration `prune` on, every 10 iterations, a block whose last
+ * 10 iterations' residual decay, continued geometrically, would not reach the stop by
+ * iteration `cap` is stopped too (its result then misses the caller's guard); 0: off. */
 int vt_kpcg_step_a(double *x, double *r, const double *p, double *q, const double *lam,
                    const long long *seg, int batch, double *st, double *hist, int hist_stride,
-                   int it, double stop_rel2, void *stream);
+                   int it, double stop_rel2, int prune, int cap, void *stream);
 int vt_kpcg_step_b(const double *r, const double *z, double *p, const long long *seg, int batch,
                    double *st, void *stream);
 
@@ -397,7 +400,7 @@ int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int 
                     int pre3_M, int pre3_K, int pre3_Nc, int batch, int max_Nc, double *x,
                     double *r, double *p, double *q, double *z, const double *lam,
                     const long long *seg, double *st, double *hist, int hist_stride, int iters,
-                    double stop_rel2, int graph_chunk, void *stream);
+                    double stop_rel2, int prune, int graph_chunk, void *stream);
 /* Kronecker-sum preconditioner (S^T C S) (x) Tbar + lam I for blocks whose decode period
  * is longer than their window: c_j as vt_kpcg_tbar, and per block Tbar's eigenvalues theta
  * (batch x Dt) and eigenvectors U (eigenvector k in column k of the Dt x Dt block; UT its

@@ -769,6 +769,40 @@ def test_kpcg_retry_guard(torch_mod, oracle):
                                       err_msg=f"block {i} {specs[i]}")
 
 
+def test_kpcg_prune(torch_mod, oracle):
+    """Pruning: with a cap of 18 iterations and the decay test from iteration 10, blocks
+    whose last 10 iterations predict no stop by iteration 18 end at iteration 10 and are
+    redone in FP64 (equal to the direct FP64 decode, bit for bit); the others are those
+    of the unpruned solve, bit for bit; a block decoded alone is pruned the same way."""
+    torch = torch_mod
+    from vtem_amd import synthetic as syn
+    from vtem_amd.batch import BatchDecoder, BlockJob
+    _banner("kpcg prune")
+    specs = [(32, 0.5), (48, 1.0), (32, 1.0), (48, 2.0), (64, 0.5)]
+    jobs = []
+    for i, (sd, du) in enumerate(specs):
+        blk = syn.serve_block(sd, du, seed=500 + i)
+        geom, st, dec = _encode_block(torch, blk)
+        jobs.append(BlockJob(dec, st))
+    pruned = BatchDecoder(precision="kpcg", cg_iters=18, retry_iters=0, prune=10)
+    res = pruned.decode(jobs)
+    plain = BatchDecoder(precision="kpcg", cg_iters=18, retry_iters=0).decode(jobs)
+    f64 = BatchDecoder(precision="fp64").decode(jobs)
+    n_pruned = 0
+    for i, (r, q, f) in enumerate(zip(res, plain, f64)):
+        if r.fallback:
+            n_pruned += 1
+            np.testing.assert_array_equal(r.frames.cpu().numpy(), f.frames.cpu().numpy())
+        else:
+            np.testing.assert_array_equal(r.frames.cpu().numpy(), q.frames.cpu().numpy())
+        alone = pruned.decode([jobs[i]])[0]
+        assert alone.fallback == r.fallback
+    its = [int(r.state.cpu().numpy()[3]) for r in BatchDecoder(precision="kpcg", cg_iters=18, retry_iters=0,
+                                                              prune=10, verify=False).decode(jobs)]
+    print(f"iterations {its}, pruned {n_pruned} of {len(jobs)}")
+    assert n_pruned >= 1 and 10 in its
+
+
 def test_rnn_batch(torch_mod, oracle):
     """Batched gradient-flow decoder (blocks of different temporal orders in one joint
     launch sequence) against the per-block path and the oracle's coefficient-space RNN

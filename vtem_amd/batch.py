@@ -200,7 +200,7 @@ class BatchDecoder:
     def __init__(self, device="cuda", precision: str = "bf16x3", lam_rel: float = 1e-6,
                  refine_iters: int = 4, tol: float = 1e-10, verify: bool = True,
                  cg_iters: int = 32, cg_stop: float = 1e-12, retry_iters: Optional[int] = None,
-                 graph_chunk: int = 25, preconditioner: str = "auto"):
+                 graph_chunk: int = 25, preconditioner: str = "auto", prune: int = 0):
         if precision not in PRECISIONS:
             raise ValueError(f"batched precision must be one of {PRECISIONS}")
         self.device = device
@@ -222,6 +222,12 @@ class BatchDecoder:
         if preconditioner not in ("auto", "kron", "kron_sum"):
             raise ValueError("preconditioner must be auto, kron or kron_sum")
         self.preconditioner = preconditioner
+        # prune > 0: a kpcg block whose residual decay predicts it cannot reach the stop
+        # within cg_iters is stopped from iteration `prune` on (checked every 10), so it
+        # reaches the FP64 redo without spending the rest of the iterations
+        if prune and prune < 10:
+            raise ValueError("prune must be 0 or at least 10")
+        self.prune = int(prune)
         self._pre_used = None                  # the last KPCG solve's preconditioner
         self.last_stats: Dict[str, Any] = {}
         self.timers: Optional[Dict[str, list]] = None     # stage -> [(ev0, ev1, work)]
@@ -881,7 +887,7 @@ class BatchDecoder:
         pargs = [pr.args5() for pr in pres] + [(0, 0, 0, 0, 0)] * (3 - len(pres))
         lib.vt_kpcg_iterate(*fp.args(), *bp.args(), *pargs[0], *pargs[1], *pargs[2], Bt, maxNc, ptr(X), ptr(R),
                             ptr(Pv), ptr(Q), ptr(Zp), ptr(lam), ptr(seg), ptr(st), ptr(hist), hs, iters,
-                            float(self.cg_stop) ** 2, self.graph_chunk, s)
+                            float(self.cg_stop) ** 2, self.prune, self.graph_chunk, s)
         if X0 is not None:
             lib.vt_add_inplace(ptr(X), ptr(X0), Dtot, s)
         # true FP64 residual of the final iterate (the guard's input)

@@ -422,12 +422,18 @@ VT_EXPORT int vt_kpcg_init(const double *r, const double *z, double *p, double *
 }
 
 // q <- q + lam p; alpha = rz / <p, q>; x += alpha p; r -= alpha q; record |r|^2 and stop
-// the block once |r|^2 <= stop_rel2 * rr0.
+// the block once |r|^2 <= stop_rel2 * rr0.  prune > 0: from iteration `prune` on, every
+// KP_PRUNE_SPAN iterations, a block whose residual decay over the last KP_PRUNE_SPAN
+// iterations, continued geometrically, would not reach the stop within `cap` iterations
+// is stopped as well (it then fails the caller's residual guard and is solved another
+// way); the test reads only the block's own history, so it does not depend on the batch.
+#define KP_PRUNE_SPAN 10
+
 __global__ void __launch_bounds__(256)
 kpcg_step_a_kernel(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
                    double *__restrict__ q, const double *__restrict__ lam,
                    const long long *__restrict__ seg, double *__restrict__ st,
-                   double *__restrict__ hist, int hs, int it, double stop_rel2)
+                   double *__restrict__ hist, int hs, int it, double stop_rel2, int prune, int cap)
 {
     __shared__ double red[256];
     double *s = st + 4 * blockIdx.x;
@@ -462,18 +468,31 @@ kpcg_step_a_kernel(double *__restrict__ x, double *__restrict__ r, const double 
         const int itx = it >= 0 ? it : (int)s[3];
         if (itx + 1 < hs) hist[(long long)blockIdx.x * hs + itx + 1] = rr;
         s[3] = itx + 1;
-        if (rr <= stop_rel2 * s[1]) s[2] = 0.0;
+        if (rr <= stop_rel2 * s[1]) {
+            s[2] = 0.0;
+        } else if (prune > 0 && itx + 1 >= prune && (itx + 1 - prune) % KP_PRUNE_SPAN == 0 &&
+                   itx + 1 < hs) {
+            const double old = hist[(long long)blockIdx.x * hs + itx + 1 - KP_PRUNE_SPAN];
+            const double rate = old > 0.0 ? rr / old : 1.0;     // squared-norm ratio per span
+            bool hopeless = !(rate < 1.0);
+            if (!hopeless) {
+                const double spans = log(stop_rel2 * s[1] / rr) / log(rate);
+                hopeless = itx + 1 + KP_PRUNE_SPAN * spans > (double)cap;
+            }
+            if (hopeless) s[2] = 0.0;
+        }
     }
 }
 
 VT_EXPORT int vt_kpcg_step_a(double *x, double *r, const double *p, double *q, const double *lam,
                              const long long *seg, int batch, double *st, double *hist,
-                             int hist_stride, int it, double stop_rel2, void *stream)
+                             int hist_stride, int it, double stop_rel2, int prune, int cap,
+                             void *stream)
 {
     VT_CHECK_ARG(x && r && p && q && lam && seg && st && hist && batch >= 1 && it >= -1 &&
-                 it + 1 < hist_stride);
+                 it + 1 < hist_stride && (prune == 0 || prune >= KP_PRUNE_SPAN));
     kpcg_step_a_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(x, r, p, q, lam, seg, st, hist,
-                                                               hist_stride, it, stop_rel2);
+                                                               hist_stride, it, stop_rel2, prune, cap);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
@@ -543,7 +562,7 @@ VT_EXPORT int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void 
                               int batch, int max_Nc, double *x, double *r, double *p, double *q,
                               double *z, const double *lam, const long long *seg, double *st,
                               double *hist, int hist_stride, int iters, double stop_rel2,
-                              int graph_chunk, void *stream)
+                              int prune, int graph_chunk, void *stream)
 {
     VT_CHECK_ARG(fwd && bwd && pre && iters >= 0 && iters < hist_stride && graph_chunk >= 0);
     const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
@@ -554,7 +573,8 @@ VT_EXPORT int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void 
     auto step = [&](hipStream_t sq) -> int {
         VT_TRY(skinny_run(F, batch, max_Nc, sq));
         VT_TRY(skinny_run(Bk, batch, max_Nc, sq));
-        VT_TRY(vt_kpcg_step_a(x, r, p, q, lam, seg, batch, st, hist, hist_stride, -1, stop_rel2, sq));
+        VT_TRY(vt_kpcg_step_a(x, r, p, q, lam, seg, batch, st, hist, hist_stride, -1, stop_rel2, prune,
+                              iters, sq));
         for (int i = 0; i < 3; ++i)
             if (P[i].d) VT_TRY(vt_gemm_skinny(P[i].d, P[i].n, P[i].M, P[i].K, P[i].Nc, sq));
         VT_TRY(vt_kpcg_step_b(r, z, p, seg, batch, st, sq));

@@ -251,15 +251,17 @@ class TiledCodec:
         return out
 
     def decode(self, spikes, decs: Sequence[Decoder], precision: str = "kpcg",
-               max_batch: int = 1024, cg_iters: int = 200, retry_iters: int = 0,
+               max_batch: int = 1024, cg_iters: int = 300, retry_iters: int = 0, prune: int = 40,
                **solver) -> torch.Tensor:
         """The blended video (frames, Py, Px) from the tiles' spike trains.  cg_iters,
-        retry_iters and `solver` go to the BatchDecoder (tol, preconditioner, ...).  A
-        window of a tile whose neurons fall silent for part of it (contrast gain above
+        retry_iters, prune and `solver` go to the BatchDecoder (tol, preconditioner, ...).
+        A window of a tile whose neurons fall silent for part of it (contrast gain above
         about 2) is far from any separable model of its operator: CG needs hundreds to
         thousands of iterations there, while the batched FP64 Cholesky that the guard falls
-        back to costs about as much as 400 iterations, so the defaults cap CG at 200
-        iterations with no retry and let the guard redo the rest in FP64.
+        back to costs about as much as 400 iterations.  So the defaults cap CG at 300
+        iterations, stop a block from iteration 40 on once its residual decay predicts it
+        will not finish by then (prune), retry nothing, and let the guard redo those blocks
+        in FP64.
         The blocks go through the decoder max_batch at a time: one joint solve of many blocks
         shares each iteration's launches (about 10 MB of device memory per 64 x 64 x 1 s
         block).  last_stats counts the blocks the guard solved again (kpcg_retries) and
@@ -271,7 +273,8 @@ class TiledCodec:
             for it, f0 in enumerate(p.f0):
                 jobs.append(BlockJob(decs[k], spikes[k], t0=f0 / self.fps, tag=(iy, ix, it)))
                 where.append((iy, ix, it))
-        bd = BatchDecoder(precision=precision, cg_iters=cg_iters, retry_iters=retry_iters, **solver)
+        bd = BatchDecoder(precision=precision, cg_iters=cg_iters, retry_iters=retry_iters, prune=prune,
+                          **solver)
         frames = []
         fallbacks = retries = 0
         for b0 in range(0, len(jobs), max_batch):
