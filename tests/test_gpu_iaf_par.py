@@ -247,7 +247,8 @@ def test_batched_encode_stored_input(torch_mod):
     """The crossing walk reading the input stored by the first pass gives bit-identical
     counts, spike times and carried state to the walk that evaluates the trigonometric input
     again (Dt 5, 9, 17 and 65 in one launch; a non-power-of-two kappa, so the stored value
-    is a rounded quotient)."""
+    is a rounded quotient); with the first pass on MFMA (the inputs' sums in the MFMA's
+    order) the counts are identical and the times within 1e-10 s."""
     torch = torch_mod
     from vtem_amd import synthetic as syn
     from vtem_amd.batch_encode import BatchEncoder, EncodeJob, IAFDevice
@@ -258,12 +259,15 @@ def test_batched_encode_stored_input(torch_mod):
     jobs[1].iaf.kappa.mul_(0.7)
     jobs[1].iaf.bias.mul_(0.7)
     be = BatchEncoder()
-    a = be.encode(jobs, carry=False, store_input=True)
+    a = be.encode(jobs, carry=False, store_input=True, mfma=False)
     sa = [(v.clone(), t.clone()) for v, t in be.last_states]
     b = be.encode(jobs, carry=False, store_input=False)
-    sb = be.last_states
+    sb = [(v.clone(), t.clone()) for v, t in be.last_states]
+    m = be.encode(jobs, carry=False, store_input=True, mfma=True)
+    sm = be.last_states
     torch.cuda.synchronize()
-    for x, y, (v1, t1), (v2, t2) in zip(a, b, sa, sb):
+    worst = 0.0
+    for x, y, z, (v1, t1), (v2, t2), (v3, t3) in zip(a, b, m, sa, sb, sm):
         c = x.counts.cpu().numpy()
         assert c.sum() > 0
         np.testing.assert_array_equal(c, y.counts.cpu().numpy())
@@ -272,3 +276,11 @@ def test_batched_encode_stored_input(torch_mod):
             np.testing.assert_array_equal(g[:c[j], j], o[:c[j], j])
         torch.testing.assert_close(v1, v2, rtol=0, atol=0)
         torch.testing.assert_close(t1, t2, rtol=0, atol=0)
+        # the first pass on MFMA: the same sums in another order
+        np.testing.assert_array_equal(c, z.counts.cpu().numpy())
+        gm = z.spikes.cpu().numpy()
+        worst = max([worst] + [float(np.abs(gm[:c[j], j] - g[:c[j], j]).max(initial=0)) for j in range(c.size)])
+        torch.testing.assert_close(v3, v1, rtol=0, atol=1e-12)
+        torch.testing.assert_close(t3, t1, rtol=0, atol=0)
+    print(f"MFMA first pass: counts identical, max |dt| {worst:.2e} s")
+    assert worst < 1e-10

@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import List
+from typing import Optional, List
 
 import numpy as np
 import torch
@@ -191,7 +191,8 @@ class BatchEncoder:
         return Ct
 
     def encode(self, jobs: List[EncodeJob], overflow: torch.Tensor = None,
-               carry: bool = True, timers=None, store_input: bool = True) -> List[SpikeTrains]:
+               carry: bool = True, timers=None, store_input: bool = True,
+               mfma: Optional[bool] = None) -> List[SpikeTrains]:
         """Encode every job; returns resident spike trains (no host synchronisation).
         With `timers`, stages "encode_coeffs" and "iaf" bracket their launches (the host
         work of making descriptors and buffers stays outside the events).
@@ -200,7 +201,13 @@ class BatchEncoder:
         and puts the advanced state in self.last_states.  overflow (int32, len(jobs),
         zeroed by the caller) gets 1 where a job's buffer was too small.  store_input keeps
         the scaled input y (8 bytes per neuron and sample) between the two passes, so the
-        crossing walk reads it instead of evaluating the trigonometric input again."""
+        crossing walk reads it instead of evaluating the trigonometric input again; with it,
+        mfma forms the first pass's inputs as FP64 MFMA products of the basis rows and the
+        coefficients (the same sums; tested equal to the other forms' spikes).  None
+        (default) takes MFMA for term counts above 17, where it is the faster form (the
+        stitched tiles' Dt = 49: 15.6 against 21.3 ms per batch of 32 tiles), and the
+        per-lane form below (the serving blocks' Dt <= 17: the MFMA form was slower,
+        6.7 against 5.1 ms per wave)."""
         dvc = self.device
         coefs = self.coefficients(jobs, timers)
         if carry:
@@ -246,8 +253,9 @@ class BatchEncoder:
         if jobs:
             dd = _to_dev(d, dvc)
             e0 = timers._tic() if timers is not None else None
-            lib.vt_iaf_encode_trig_batched(ptr(dd), len(jobs), maxN, maxS, maxDt,
-                                           0 if ybuf is None else 1, stream_ptr())
+            use_mfma = maxDt > 17 if mfma is None else bool(mfma)
+            flags = 0 if ybuf is None else (3 if use_mfma else 1)
+            lib.vt_iaf_encode_trig_batched(ptr(dd), len(jobs), maxN, maxS, maxDt, flags, stream_ptr())
         if e0 is not None:
             # algorithmic work: the input's evaluation, 2 Dt flops per sample interval and neuron
             timers._toc("iaf", e0, float(sum(2.0 * j.encoder.geom.Dt * j.encoder.RF.shape[0] *

@@ -308,6 +308,113 @@ iafp_sampled_kernel(IafParDesc d, int wl)
         iafp_walk_dev(d, ScaledInput<SampledSource<TIn>>{src, d.bias[n], d.kappa[n]}, n, s, S);
 }
 
+// Pass 1 of the batched stored-input form on MFMA.  The input of neuron n at sample i is
+// v = sum_p C[p][n] Theta[i][p]: a product of the basis rows (samples x Dt) and the
+// coefficients (Dt x neurons).  A workgroup takes IAFM_SEGS consecutive segments and 64
+// neurons: the basis rows of its IAFM_SEGS * 64 + 1 sample points are staged in LDS, each
+// wave forms 16 neurons' inputs as FP64 MFMA 16x16x4 tiles (A = basis rows, B = the
+// neurons' coefficients, K = the padded term count), then y = (v + b) / kappa goes to the
+// stored input and to LDS, where one lane per (segment, neuron) takes the segment's steps
+// in order exactly as iafp_partial_dev does.  A sample shared by two workgroups (the last
+// point of one, the first of the next) is the same MFMA element in both, so both see the
+// same value.  The sums run in the MFMA's order, not the VALU form's p = 0..Dt-1, so the
+// inputs differ from the other forms by rounding (the walk reads the stored y, so the
+// pass and the walk agree with each other).
+#define IAFM_SEGS 2
+#define IAFM_PTS (IAFM_SEGS * IAFP_SEG + 1)
+#define IAFM_ROWS (16 * ((IAFM_PTS + 15) / 16))
+
+#define IAFM_YN 65
+
+template <int KS>
+__global__ void __launch_bounds__(256) iafp_trig_mfma_kernel(const IafParDesc *__restrict__ descs)
+{
+    constexpr int DTP = 4 * KS;
+    constexpr int LD = DTP + 1;
+    constexpr int RT = IAFM_ROWS / 16;
+    constexpr int LDS_TH = IAFM_ROWS * LD, LDS_Y = IAFM_PTS * IAFM_YN;
+    // the basis rows, then (after the products) y of the workgroup's points x 64 neurons
+    __shared__ double sh[LDS_TH > LDS_Y ? LDS_TH : LDS_Y];
+    const IafParDesc d = descs[blockIdx.z];
+    const int S = iafp_segments(d.size);
+    const int s0 = blockIdx.x * IAFM_SEGS, n0 = blockIdx.y * 64;
+    if (s0 >= S || n0 >= d.N) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int steps = d.size - 1, Dt = d.Dt;
+    const int i0 = s0 * IAFP_SEG;
+    // basis rows of the sample points i0 .. (rows past the call's samples read its last
+    // row and are not used; terms past Dt are zero)
+    for (int e = tid; e < IAFM_ROWS * DTP; e += 256) {
+        const int row = e / DTP, p = e % DTP;
+        const int i = min(i0 + row, d.size - 1);
+        sh[row * LD + p] = p < Dt ? d.basis[(long long)i * Dt + p] : 0.0;
+    }
+    // this wave's 16 neurons' coefficients in the MFMA B layout: slot g of k-step j holds
+    // term 4 j + g of neuron nb
+    const int nb = n0 + 16 * wave + (lane & 15), g = lane >> 4;
+    const bool nin = nb < d.N;
+    double c[KS];
+#pragma unroll
+    for (int j = 0; j < KS; ++j) {
+        const int p = 4 * j + g;
+        c[j] = (nin && p < Dt) ? d.coef[(long long)p * d.coef_ld + (long long)nb * d.coef_ns] : 0.0;
+    }
+    __syncthreads();
+    v4d acc[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) acc[t] = (v4d){0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < KS; ++j)
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sh[(16 * t + (lane & 15)) * LD + 4 * j + g], c[j],
+                                                          acc[t], 0, 0, 0);
+    __syncthreads();                           // the basis rows are no longer read
+    const double bias = nin ? d.bias[nb] : 0.0, kappa = nin ? d.kappa[nb] : 1.0;
+    const int last = min(i0 + IAFM_SEGS * IAFP_SEG, steps);
+    double *yout = d.ybuf + nb;
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma clang fp contract(off)
+            const int row = 16 * t + mfma_row_f64(lane, q);
+            if (row < IAFM_PTS) {
+                const int i = i0 + row;
+                const double y = (acc[t][q] + bias) / kappa;
+                sh[row * IAFM_YN + 16 * wave + (lane & 15)] = y;
+                // every sample once: a workgroup stores its points after i0 (and point 0)
+                if (nin && i <= last && (i > i0 || i0 == 0)) yout[(long long)i * d.y_ld] = y;
+            }
+        }
+    __syncthreads();
+    if (tid >= 64 * IAFM_SEGS) return;
+    const int u = tid >> 6, col = tid & 63, n = n0 + col, s = s0 + u;
+    if (n >= d.N || s >= S) return;
+    {
+#pragma clang fp contract(off)
+        const int si0 = s * IAFP_SEG, si1 = min(si0 + IAFP_SEG, steps);
+        const double dt = d.dt;
+        double y1 = sh[(si0 - i0) * IAFM_YN + col];
+        double P = 0.0, M = -INFINITY;
+        for (int i = si0; i < si1; ++i) {
+            const double y2 = sh[(i + 1 - i0) * IAFM_YN + col];
+            const double area = (y1 + y2) / 2 * dt;
+            double top;
+            if (y2 >= 0 || y1 <= 0) {
+                top = P + area;
+            } else {
+                const double a = (y2 - y1) / dt;
+                top = P + y1 * y1 / (-2 * a);
+            }
+            M = fmax(M, top);
+            P += area;
+            y1 = y2;
+        }
+        d.ws[(long long)s * d.N + n] = make_double2(P, M);
+    }
+}
+
 // Per neuron: exclusive prefixes of the segment partials, totals, and the carried state
 // the serial encoder would leave (ensemble_encode.py:541-545): v0 <- F_T - L_K with K the
 // spike count (at most max_spike + 1, the reference's overflow stop), time_count advanced
@@ -552,17 +659,48 @@ VT_EXPORT int vt_iaf_encode_par_noalloc(const void *input, int in_dtype, long lo
 // host passes the maxima of N, size and Dt over the descriptors for the grid; workgroups
 // beyond a block's own extent exit at once.  flags bit 0: every descriptor carries ybuf, so
 // pass 1 stores the input and pass 2 reads it instead of evaluating it again.
+template <int KS>
+static void iafp_mfma_launch(const IafParDesc *descs, int nd, int maxN, int maxSize, hipStream_t s)
+{
+    const dim3 g1((unsigned)((iafp_segments(maxSize) + IAFM_SEGS - 1) / IAFM_SEGS),
+                  (unsigned)((maxN + 63) / 64), (unsigned)nd);
+    const IafpGrid g = iafp_grid(maxN, maxSize, nd);
+    IafParDesc d0{};
+    iafp_trig_mfma_kernel<KS><<<g1, 256, 0, s>>>(descs);
+    iafp_scan_kernel<true><<<dim3(vt_blocks(maxN, 256), (unsigned)nd), 256, 0, s>>>(descs, d0);
+    iafp_walk_stored_kernel<true><<<g.g, 64 * IAFP_WAVES, 0, s>>>(descs, d0, g.wl);
+}
+
+// flags: 1 = store the scaled input in pass 1 (every descriptor has ybuf) and walk over
+// it; 2 (with 1) = form pass 1's inputs on MFMA (iafp_trig_mfma_kernel).
 VT_EXPORT int vt_iaf_encode_trig_batched(const void *descs, int nd, int maxN, int maxSize,
                                          int maxDt, int flags, void *stream)
 {
     VT_CHECK_ARG(nd >= 0 && maxN >= 0 && maxSize >= 0 && maxDt >= 1);
     VT_CHECK_ARG(descs || nd == 0);
-    VT_CHECK_ARG((flags & ~1) == 0);
+    VT_CHECK_ARG((flags & ~3) == 0 && flags != 2);
+    VT_CHECK_ARG(nd <= 65535 && (maxN + 63) / 64 <= 65535);
     if (nd == 0 || maxN == 0) return VT_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (flags & 2) {
+        if (maxDt <= 12)
+            iafp_mfma_launch<3>((const IafParDesc *)descs, nd, maxN, maxSize, s);
+        else if (maxDt <= 20)
+            iafp_mfma_launch<5>((const IafParDesc *)descs, nd, maxN, maxSize, s);
+        else if (maxDt <= 36)
+            iafp_mfma_launch<9>((const IafParDesc *)descs, nd, maxN, maxSize, s);
+        else if (maxDt <= 52)
+            iafp_mfma_launch<13>((const IafParDesc *)descs, nd, maxN, maxSize, s);
+        else if (maxDt <= 68)
+            iafp_mfma_launch<17>((const IafParDesc *)descs, nd, maxN, maxSize, s);
+        else
+            return VT_ERANGE;
+        VT_LAUNCH_CHECK();
+        return VT_OK;
+    }
     IafParDesc d0{};
     return iafp_trig_dispatch((const IafParDesc *)descs, d0, maxDt, iafp_grid(maxN, maxSize, nd),
-                              dim3(vt_blocks(maxN, 256), (unsigned)nd), (flags & 1) != 0,
-                              (hipStream_t)stream);
+                              dim3(vt_blocks(maxN, 256), (unsigned)nd), (flags & 1) != 0, s);
 }
 
 VT_EXPORT int vt_iaf_par_desc_size(int *bytes)
