@@ -84,6 +84,9 @@ def parse():
     p.add_argument("--max-batch", type=int, default=32)
     p.add_argument("--streams", type=int, default=2)
     p.add_argument("--rnn-lanes", type=int, default=1, help="streams of the RNN lane")
+    p.add_argument("--rnn-lane-delays-ms", default="",
+                   help="staggered RNN lanes: comma-separated hold per lane (ms); '' = --rnn-lanes "
+                        "lanes splitting the waiting requests")
     p.add_argument("--graph-chunk", type=int, default=25,
                    help="RNN / power iterations per replayed HIP graph (0 = launch each)")
     p.add_argument("--form", default="kpcg", help="solver form of the chosen served path")
@@ -539,7 +542,9 @@ def main():
         server = Server(max_batch=args.max_batch, streams=args.streams, precision=form,
                         rnn_lanes=args.rnn_lanes, graph_chunk=args.graph_chunk,
                         adaptive=form != "kpcg", max_delay=args.max_delay_ms * 1e-3,
-                        rnn_max_delay=args.rnn_max_delay_ms * 1e-3)
+                        rnn_max_delay=args.rnn_max_delay_ms * 1e-3,
+                        rnn_lane_delays=[float(x) * 1e-3 for x in args.rnn_lane_delays_ms.split(",")]
+                        if args.rnn_lane_delays_ms else None)
         for _ in range(args.warmup):           # allocator, code objects, precision policy
             server.serve(make_requests(items, zeros, form))
         torch.cuda.synchronize()
@@ -640,7 +645,8 @@ def main():
                    "max_delay_ms": args.max_delay_ms, "rnn_max_delay_ms": args.rnn_max_delay_ms,
                    "requests_per_step": R, "frames_per_step": frames, "D_min": int(D.min()),
                    "D_max": int(D.max()), "max_batch": args.max_batch, "streams": args.streams,
-                              "rnn_lanes": args.rnn_lanes, "graph_chunk": args.graph_chunk},
+                              "rnn_lanes": args.rnn_lanes, "graph_chunk": args.graph_chunk,
+                   "rnn_lane_delays_ms": args.rnn_lane_delays_ms or None},
         "chosen": {"form": args.form, "frames_per_s": round(value, 2),
                    "p50_ms": round(stats[-1]["p50_ms"], 3), "p99_ms": round(stats[-1]["p99_ms"], 3),
                    "batches": stats[-1]["batches"], "fp64_fallbacks": stats[-1]["fallbacks"],

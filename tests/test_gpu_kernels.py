@@ -933,12 +933,14 @@ def test_server_alone_equals_mix(torch_mod):
         np.testing.assert_array_equal(r.result.frames.cpu().numpy(), alone.frames.cpu().numpy())
 
 
-def test_server_encode_decode_alone_equals_mix(torch_mod):
+@pytest.mark.parametrize("lane_delays", [None, (0.0, 0.004)])
+def test_server_encode_decode_alone_equals_mix(torch_mod, lane_delays):
     """Encode+decode requests (video resident, encoded by the server's batched encoder)
     served with Poisson arrivals on 2 streams: every request's spikes and frames are
     bit-identical to the request encoded and decoded alone (KPCG form, Tikhonov and RNN
-    mixed).  One request gets a 4-row spike buffer: the server detects the overflow and
-    redoes it with an exact buffer, giving the alone result with a large buffer."""
+    mixed; one RNN lane, or two staggered lanes holding 0 and 4 ms).  One request gets a
+    4-row spike buffer: the server detects the overflow and redoes it with an exact
+    buffer, giving the alone result with a large buffer."""
     torch = torch_mod
     import dataclasses
     import numpy as np
@@ -948,7 +950,7 @@ def test_server_encode_decode_alone_equals_mix(torch_mod):
     from vtem_amd.batch_encode import BatchEncoder, EncodeJob, IAFDevice, max_spikes_rate
     from vtem_amd.block import Decoder, Encoder, dev, spatial_coeffs
     from vtem_amd.serve import Request, Server, footprint_bytes, poisson_arrivals
-    _banner("server: encode+decode alone == in the mix, overflow redo")
+    _banner(f"server: encode+decode alone == in the mix, overflow redo, RNN lane holds {lane_delays}")
     specs = [(32, 2.0), (48, 1.0), (32, 1.0), (48, 2.0), (64, 1.0), (32, 0.5), (48, 0.5)]
     reqs, parts = [], []
     arr = poisson_arrivals(len(specs), 200.0, np.random.default_rng(5))
@@ -966,7 +968,7 @@ def test_server_encode_decode_alone_equals_mix(torch_mod):
         parts.append((ej, job, max_spikes_rate(geom, blk.iaf)))
         reqs.append(Request(i, float(a), job, geom.frames, footprint_bytes(job, "kpcg", encode=ej),
                             encode=ej, iaf=blk.iaf))
-    server = Server(max_batch=3, streams=2, precision="kpcg", adaptive=False)
+    server = Server(max_batch=3, streams=2, precision="kpcg", adaptive=False, rnn_lane_delays=lane_delays)
     st = server.serve(reqs)
     print(f"served: batches={st['batches']} overflows={st['overflows']} p50={st['p50_ms']:.2f} ms",
           flush=True)

@@ -138,6 +138,11 @@ def test_batch_formation_and_rnn_lane_routing():
     assert split_cap(13, 64, 1) == 13
     assert split_cap(13, 64, 2) == 7 and split_cap(6, 64, 2) == 3
     assert split_cap(13, 4, 2) == 4 and split_cap(0, 64, 3) == 1
+    # staggered lanes: one lane per hold time
+    from vtem_amd.serve import Server
+    s = Server(rnn_lanes=1, rnn_lane_delays=(0, 0.016))
+    assert s.rnn_lanes == 2 and s.rnn_lane_delays == [0.0, 0.016]
+    assert Server(rnn_lanes=3).rnn_lane_delays is None
 
 
 def test_spike_buffer_rate_rows():

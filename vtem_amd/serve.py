@@ -16,7 +16,11 @@ The server:
   decode -- so batches formed from requests that arrive while others are on the GPU run
   concurrently on the device; the RNN queue feeds lanes of its own (rnn_lanes), whose
   batches split the waiting RNN requests between the free RNN lanes (up to
-  rnn_max_batch each), so RNN requests share a few sequences of iteration launches;
+  rnn_max_batch each), so RNN requests share a few sequences of iteration launches; with
+  staggered lanes (rnn_lane_delays) the first lane takes every waiting RNN request at
+  once and a later lane holds its queue for a set time, gathering the RNN requests that
+  arrive while the first lane's chain runs into one more chain that starts before it
+  ends;
 * retires a batch when its completion event fires: re-runs any block whose spike buffer
   overflowed (with an exact buffer) or whose low-precision solve missed tolerance,
   releases its memory and stamps each request's completion time.
@@ -36,7 +40,7 @@ import collections
 import dataclasses
 import math
 import time
-from typing import Any, Dict, List, Optional
+from typing import Any, Dict, List, Optional, Sequence
 
 import numpy as np
 import torch
@@ -119,7 +123,8 @@ class Server:
     def __init__(self, hbm_budget: int = 64 << 30, max_batch: int = 32, streams: int = 2,
                  precision: str = "bf16x3", verify: bool = True, adaptive: bool = True,
                  rnn_lane: bool = True, rnn_max_batch: int = 64, rnn_lanes: int = 1,
-                 graph_chunk: int = 25, max_delay: float = 0.0, rnn_max_delay: float = 0.0):
+                 graph_chunk: int = 25, max_delay: float = 0.0, rnn_max_delay: float = 0.0,
+                 rnn_lane_delays: Optional[Sequence[float]] = None):
         self.hbm_budget = int(hbm_budget)
         self.max_batch = int(max_batch)
         self.n_streams = int(streams)
@@ -138,6 +143,12 @@ class Server:
         # oldest request has waited this long (seconds), or no further arrivals are due
         self.max_delay = float(max_delay)
         self.rnn_max_delay = float(rnn_max_delay)
+        # staggered RNN lanes: lane k holds its queue until the oldest request has waited
+        # rnn_lane_delays[k] (seconds) and takes every waiting RNN request (no even split),
+        # so a later lane gathers the requests that arrive while the first lane's chain runs
+        self.rnn_lane_delays = None if rnn_lane_delays is None else [float(x) for x in rnn_lane_delays]
+        if self.rnn_lane_delays is not None:
+            self.rnn_lanes = len(self.rnn_lane_delays)
 
     # ----------------------------------------------------------------------------------
     def serve(self, requests: List[Request], mode: str = "served",
@@ -207,9 +218,12 @@ class Server:
             for k, (_, queue, cap) in enumerate(lanes):
                 if busy[k] is not None or not queue:
                     continue
-                if k < nr:
+                if k < nr and self.rnn_lane_delays is None:
                     cap = split_cap(len(queue), cap, sum(1 for i in range(k, nr) if busy[i] is None))
-                delay = self.rnn_max_delay if k < nr else self.max_delay
+                if k < nr:
+                    delay = self.rnn_max_delay if self.rnn_lane_delays is None else self.rnn_lane_delays[k]
+                else:
+                    delay = self.max_delay
                 now = clock() - t0
                 if not should_launch(len(queue), cap, now - queue[0].arrival, delay, bool(pending)):
                     wake = min(wake, queue[0].arrival + delay)
