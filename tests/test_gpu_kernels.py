@@ -67,6 +67,56 @@ def test_gemm(torch_mod, M, N, K, ta, tb, dtype):
     assert err < tol, err
 
 
+def test_gemm_skinny_route_invariance(torch_mod):
+    """A problem's route through the skinny product is fixed by its own column count
+    (<= 16: one MFMA tile; 17: one tile plus the vector-ALU column 16; more: two tiles),
+    so its result is bit-identical whatever the launch's widest problem is; each result
+    also matches float64 torch (row factor applied)."""
+    torch = torch_mod
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    from vtem_amd.batch import SK_DTYPE, _to_dev, skinny_workspace
+    _banner("skinny route invariance (Nc 9 / 17 / 25 launches)")
+    g = torch.Generator().manual_seed(11)
+    probs = {}
+    for name, (M, K, Nc) in {"p9": (300, 625, 9), "p17": (2361, 625, 17), "p17s": (100, 2361, 17),
+                             "p25": (200, 289, 25)}.items():
+        A = torch.randn(M, K, generator=g, dtype=torch.float64)
+        Bm = torch.randn(K, Nc, generator=g, dtype=torch.float64)
+        Tn = torch.randn(M, Nc, Nc, generator=g, dtype=torch.float64)
+        ref = torch.einsum("mab,mb->ma", Tn, A @ Bm)
+        probs[name] = (M, K, Nc, A.cuda(), Bm.cuda(), Tn.contiguous().cuda(), ref)
+
+    def run(names):
+        rows, outs, keep = [], [], []
+        for nm in names:
+            M, K, Nc, Ad, Bd, Td, _ = probs[nm]
+            Cd = torch.full((M, Nc), float("nan"), dtype=torch.float64, device="cuda")
+            wn = skinny_workspace(M, K, Nc)
+            Wd = torch.empty(max(1, wn), dtype=torch.float64, device="cuda")
+            rows.append((Ad.data_ptr(), Bd.data_ptr(), Cd.data_ptr(), Td.data_ptr(), Wd.data_ptr() if wn else 0,
+                         0, K, 1, Nc, Nc, Nc * Nc, M, K, Nc, 0))
+            outs.append(Cd)
+            keep.append(Wd)
+        d = np.zeros(len(rows), SK_DTYPE)
+        for i, r in enumerate(rows):
+            d[i] = r
+        dd = _to_dev(d, "cuda")
+        lib.vt_gemm_skinny(ptr(dd), len(rows), max(probs[n][0] for n in names), max(probs[n][1] for n in names),
+                           max(probs[n][2] for n in names), stream_ptr())
+        torch.cuda.synchronize()
+        return [o.cpu() for o in outs]
+
+    alone = {nm: run([nm])[0] for nm in probs}
+    for nm, out in alone.items():
+        ref = probs[nm][6]
+        assert (out - ref).abs().max().item() < 1e-13 * max(1.0, ref.abs().max().item()), nm
+    mixed = run(["p9", "p17", "p17s", "p25"])
+    for nm, out in zip(["p9", "p17", "p17s", "p25"], mixed):
+        assert torch.equal(out, alone[nm]), nm
+    pair = run(["p9", "p17"])
+    assert torch.equal(pair[0], alone["p9"]) and torch.equal(pair[1], alone["p17"])
+
+
 @pytest.mark.parametrize("tmode", ["none", "row", "shared"])
 def test_gemm_skinny(torch_mod, tmode):
     """Skinny grouped product (both A orientations, ragged problems, Nc 5/17/32) with the

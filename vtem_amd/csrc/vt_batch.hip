@@ -417,7 +417,11 @@ VT_EXPORT int vt_seg_counts(const int *offsets, const int *base, int nseg, int *
 // pass adds the ks partials in order and applies the row factor.
 // ------------------------------------------------------------------------------------
 
-template <int NT, int RT>
+// NT = 1: problems with Nc <= 16, and Nc = 17 with X16; NT = 2: any Nc <= 32, problems
+// with Nc <= 17 taking the NT = 1 routes (every problem's route is fixed by its own Nc:
+// <= 16 one MFMA tile, 17 one tile + the vector column, more two tiles), so a launch's
+// width changes no result.
+template <int NT, int RT, bool X16>
 __global__ void __launch_bounds__(64 * SK_WAVES) skinny_kernel(const SkDesc *__restrict__ descs)
 {
     __shared__ double red[SK_WAVES / 2][16 * RT][33];
@@ -425,7 +429,20 @@ __global__ void __launch_bounds__(64 * SK_WAVES) skinny_kernel(const SkDesc *__r
     const int m0 = blockIdx.x * 16 * RT;
     if (m0 >= d.M || (int)blockIdx.z >= skinny_splits(d.K)) return;
     if (d.active && *d.active == 0.0) return;
-    skinny_tile<NT, RT>(d, m0, blockIdx.z, red);
+    if constexpr (NT == 1) {
+        // the vector column reads B's column 16: only problems that have one take it
+        if (X16 && d.Nc == 17)
+            skinny_tile<1, 1, true>(d, m0, blockIdx.z, red);
+        else
+            skinny_tile<1, RT>(d, m0, blockIdx.z, red);
+    } else {
+        if (d.Nc == 17)
+            skinny_tile<1, 1, true>(d, m0, blockIdx.z, red);
+        else if (d.Nc <= 16)
+            skinny_tile<1, 1>(d, m0, blockIdx.z, red);
+        else
+            skinny_tile<2, RT>(d, m0, blockIdx.z, red);
+    }
 }
 
 // second pass of split problems: C = (sum_z partial_z) (row factor applied)
@@ -468,9 +485,11 @@ int vt_skinny_first_pass(const void *descs, int batch, int max_M, int max_K, int
     // operator, the preconditioner and the encoder's coefficient products
     dim3 grid(vt_blocks(max_M, 16), batch, ks);
     if (max_Nc <= 16)
-        skinny_kernel<1, 1><<<grid, 64 * SK_WAVES, 0, s>>>(d);
+        skinny_kernel<1, 1, false><<<grid, 64 * SK_WAVES, 0, s>>>(d);
+    else if (max_Nc == 17)
+        skinny_kernel<1, 1, true><<<grid, 64 * SK_WAVES, 0, s>>>(d);
     else
-        skinny_kernel<2, 1><<<grid, 64 * SK_WAVES, 0, s>>>(d);
+        skinny_kernel<2, 1, false><<<grid, 64 * SK_WAVES, 0, s>>>(d);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }

@@ -93,14 +93,33 @@ __device__ __forceinline__ double skinny_row_factor(const SkDesc &d, int m, int 
     return v;
 }
 
+// Column 16 of a problem with Nc = 17 (the temporal order of Lt = 8 blocks) is not given a
+// second 16-column MFMA tile, of which it would use one column: each lane adds its four
+// products a[s] * B[k][16] of the chunk with FMAs (k = c*16 + 4g + s in order s = 0..3,
+// chunks in the wave's order), the four lane groups g are added as (g0 + g1) + (g2 + g3),
+// and the waves' sums join the MFMA columns in the tree reduction.
+__device__ __forceinline__ void skinny_load_x16(const SkDesc &d, int c, int c_end, int g, double (&x)[4])
+{
+    const int kb = c * 16 + 4 * g;
+    const bool cin = c < c_end;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int k = kb + s;
+        x[s] = (cin && k < d.K && d.Nc > 16) ? d.B[(long long)k * d.ldb + 16] : 0.0;
+    }
+}
+
 // One workgroup of SK_WAVES waves computes rows m0 .. m0 + 16 RT - 1 of K split z of
 // problem d (RT row tiles of 16: the row tiling does not change any sum's order, so a
 // launch may pick it from the batch size).  The waves split K and are reduced in LDS in a
-// fixed tree order; red is the workgroup's LDS scratch.
-template <int NT, int RT>
+// fixed tree order; red is the workgroup's LDS scratch.  X16: the problem has Nc = 17 and
+// its column 16 is summed on the vector ALU (NT = 1, RT = 1); the route depends only on the
+// problem's own Nc, so a problem's sums do not depend on the batch.
+template <int NT, int RT, bool X16 = false>
 __device__ __forceinline__ void skinny_tile(const SkDesc &d, int m0, int z,
                                             double (&red)[SK_WAVES / 2][16 * RT][33])
 {
+    static_assert(!X16 || (NT == 1 && RT == 1), "the vector column extends one 16-column tile");
     constexpr int R = 16 * RT;
     const int ks = skinny_splits(d.K), Nc = d.Nc;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -113,14 +132,27 @@ __device__ __forceinline__ void skinny_tile(const SkDesc &d, int m0, int z,
     const int nch = (d.K + 15) / 16;
     const int per = (nch + ks - 1) / ks;
     const int c_beg = z * per, c_end = min(nch, c_beg + per);
+    double p16 = 0.0;                      // X16: this lane's part of column 16
     // SK_INFLIGHT chunks in flight per wave: c, c + SK_WAVES, ... (added in that order)
     for (int c = c_beg + wave; c < c_end; c += SK_INFLIGHT * SK_WAVES) {
-        double a[SK_INFLIGHT][RT][4], b[SK_INFLIGHT][NT][4];
+        double a[SK_INFLIGHT][RT][4], b[SK_INFLIGHT][NT][4], x[SK_INFLIGHT][4];
 #pragma unroll
-        for (int u = 0; u < SK_INFLIGHT; ++u)
+        for (int u = 0; u < SK_INFLIGHT; ++u) {
             skinny_load<NT, RT>(d, m0, c + u * SK_WAVES, c_end, r, g, Nc, a[u], b[u]);
+            if constexpr (X16) skinny_load_x16(d, c + u * SK_WAVES, c_end, g, x[u]);
+        }
 #pragma unroll
-        for (int u = 0; u < SK_INFLIGHT; ++u) skinny_mma<NT, RT>(acc, a[u], b[u]);
+        for (int u = 0; u < SK_INFLIGHT; ++u) {
+            skinny_mma<NT, RT>(acc, a[u], b[u]);
+            if constexpr (X16) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) p16 = __builtin_fma(a[u][0][s], x[u][s], p16);
+            }
+        }
+    }
+    if constexpr (X16) {
+        p16 += __shfl_xor(p16, 16);        // (g0 + g1), (g2 + g3) in every lane
+        p16 += __shfl_xor(p16, 32);        // (g0 + g1) + (g2 + g3)
     }
     // cross-wave tree reduction: at each level the upper half adds into the lower half
 #pragma unroll
@@ -133,6 +165,9 @@ __device__ __forceinline__ void skinny_tile(const SkDesc &d, int m0, int z,
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                         red[wave - half][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)] = acc[t][j][i];
+            if constexpr (X16) {
+                if (g == 0) red[wave - half][r][16] = p16;
+            }
         }
         __syncthreads();
         if (wave < half) {
@@ -143,6 +178,7 @@ __device__ __forceinline__ void skinny_tile(const SkDesc &d, int m0, int z,
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                         acc[t][j][i] += red[wave][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)];
+            if constexpr (X16) p16 += red[wave][r][16];
         }
         __syncthreads();
     }
@@ -154,6 +190,9 @@ __device__ __forceinline__ void skinny_tile(const SkDesc &d, int m0, int z,
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
                     red[0][16 * t + mfma_row_f64(lane, i)][16 * j + mfma_col(lane)] = acc[t][j][i];
+        if constexpr (X16) {
+            if (g == 0) red[0][r][16] = p16;
+        }
     }
     __syncthreads();
     const int rows = min(R, d.M - m0);
