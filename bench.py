@@ -90,6 +90,8 @@ def parse():
     p.add_argument("--graph-chunk", type=int, default=25,
                    help="RNN / power iterations per replayed HIP graph (0 = launch each)")
     p.add_argument("--form", default="kpcg", help="solver form of the chosen served path")
+    p.add_argument("--rnn-persistent", action="store_true",
+                   help="RNN iterations in one persistent launch instead of three launches each")
     p.add_argument("--compare", default="bf16x3", help="second served form measured ('' = none)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-sweep", action="store_true", help="skip the latency sweep")
@@ -544,7 +546,7 @@ def main():
                         adaptive=form != "kpcg", max_delay=args.max_delay_ms * 1e-3,
                         rnn_max_delay=args.rnn_max_delay_ms * 1e-3,
                         rnn_lane_delays=[float(x) * 1e-3 for x in args.rnn_lane_delays_ms.split(",")]
-                        if args.rnn_lane_delays_ms else None)
+                        if args.rnn_lane_delays_ms else None, rnn_persistent=args.rnn_persistent)
         for _ in range(args.warmup):           # allocator, code objects, precision policy
             server.serve(make_requests(items, zeros, form))
         torch.cuda.synchronize()
@@ -595,7 +597,8 @@ def main():
         return [Request(i, 0.0, it.job, it.geom.frames, footprint_bytes(it.job, form))
                 for i, it in enumerate(items)]
     dserver = Server(max_batch=args.max_batch, streams=args.streams, precision=args.form,
-                     rnn_lanes=args.rnn_lanes, graph_chunk=args.graph_chunk, adaptive=args.form != "kpcg")
+                     rnn_lanes=args.rnn_lanes, graph_chunk=args.graph_chunk, adaptive=args.form != "kpcg",
+                     rnn_persistent=args.rnn_persistent)
     dserver.serve(decode_requests(args.form))                     # warm
     dwall = sum(dserver.serve(decode_requests(args.form))["wall_s"] for _ in range(args.steps))
     Server().serve(decode_requests("fp64"), mode="plain")         # warm
@@ -607,7 +610,8 @@ def main():
     # roofline of the dominant stage (HIP events on the launch stream, one instrumented wave)
     # one stream, so that a stage's events bracket only its own kernels
     iserver = Server(max_batch=args.max_batch, streams=1, precision=args.form,
-                     graph_chunk=args.graph_chunk, adaptive=args.form != "kpcg")
+                     graph_chunk=args.graph_chunk, adaptive=args.form != "kpcg",
+                     rnn_persistent=args.rnn_persistent)
     iserver.policy = server.policy
     timers = iserver.serve(make_requests(items, zeros), instrument=True)["timers"]
     torch.cuda.synchronize()

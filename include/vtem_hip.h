@@ -432,6 +432,33 @@ int vt_kron_power(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bw
                   int batch, int max_Nc, double *v, double *q, const double *lam,
                   const long long *seg, int iters, double *eta, int graph_chunk, void *stream);
 
+/* The same steps+1 iterations in ONE persistent launch (vtem/neural_network.py:78-92, update
+ * kernel :102-128: the loop the reference drives from the host, one launch per step).  Each
+ * block owns the workgroups [gstart[b], gstart[b+1]) (device array, batch + 1 ints, grid =
+ * gstart[batch] within vt_kron_rnn_persistent_capacity, groups of 1..max_group); a
+ * block's group runs its forward tiles, backward tiles and update with barriers of that
+ * group only.  fwd / bwd: the operator descriptors in block order (forward K <= 1008, so
+ * unsplit); max_Nc their widest column count.  x (n doubles) is zeroed here and holds
+ * X_{steps+1} on return, bit-identical to vt_kron_rnn.  X and Z pass through ring slots
+ * (ring_x: ring slots of slot_x >= n doubles, ring_z: ring slots of slot_z >= the joint Z,
+ * slot sizes multiples of 32); ring >= steps + 1 needs no cache invalidation, a shorter
+ * ring (even, >= 4) invalidates every ring/2 iterations.  bar: barrier workspace of
+ * vt_kron_rnn_persistent_bar_words words, zeroed here; bar[0] != 0 afterwards means a
+ * barrier wait exceeded limit_s (the grid was not co-resident) and x is invalid.  No
+ * counterpart call in the reference (its loop is host-driven); this replaces vt_kron_rnn's
+ * launches, not its arithmetic. */
+int vt_kron_rnn_persistent(const void *fwd, const void *bwd, int batch, int max_Nc, const int *gstart,
+                           int grid, double *x, long long n, const double *r, const double *zbase,
+                           const double *lam, const double *eta, const long long *seg, int steps,
+                           double *ring_x, long long slot_x, double *ring_z, long long slot_z, int ring,
+                           unsigned *bar, double limit_s, void *stream);
+/* Workgroups of the persistent RNN launch that fit the current device at once:
+ * max_wgs[0] for batches of at most 17 columns, max_wgs[1] for wider ones; *max_group the
+ * largest group of one block. */
+int vt_kron_rnn_persistent_capacity(int *max_wgs, int *max_group);
+/* 32-bit words of the persistent launch's barrier workspace for a batch. */
+int vt_kron_rnn_persistent_bar_words(int batch, long long *words);
+
 /* One power step per block (q = A v given): w = q + lam v, eta_b = <v,v>/<v,w>, v = w/|w|. */
 int vt_power_step(double *v, const double *q, const double *lam, const long long *seg, int batch,
                   double *eta, void *stream);

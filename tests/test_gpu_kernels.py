@@ -913,8 +913,8 @@ def test_rnn_graph_replay(torch_mod):
         blk = syn.serve_block(sd, du, seed=720 + i)
         geom, st, dec = _encode_block(torch, blk)
         jobs.append(BlockJob(dec, st, method="rnn", rnn_steps=60))
-    ref = BatchDecoder(precision="kpcg", graph_chunk=0).decode(jobs)
-    bd = BatchDecoder(precision="kpcg", graph_chunk=8)
+    ref = BatchDecoder(precision="kpcg", graph_chunk=0, rnn_persistent=False).decode(jobs)
+    bd = BatchDecoder(precision="kpcg", graph_chunk=8, rnn_persistent=False)
     for rep in range(3):
         res = bd.decode(jobs)
         torch.cuda.synchronize()
@@ -923,6 +923,69 @@ def test_rnn_graph_replay(torch_mod):
             assert torch.equal(a.state, b.state)
         # eager work on the same stream between the graph launches
         torch.ones(1 << 20, dtype=torch.float64, device="cuda").sum().item()
+
+
+def test_rnn_persistent(torch_mod):
+    """The persistent RNN launch (every iteration in one launch, per-block workgroup
+    groups with their own barriers, X and Z through ring slots) gives the launch form's
+    iterates bit for bit: temporal orders 5, 9 and 17 (the vector-column route), backward
+    products that split K, a ring holding every iteration and a 32-slot ring with cache
+    invalidation, groups scaled down to a few workgroups each (many tiles per
+    workgroup), a block alone and in the batch, and a launch whose barrier waits are
+    given no time (it aborts, and the jobs are solved again by the launch form)."""
+    torch = torch_mod
+    from vtem_amd import batch as vb
+    from vtem_amd import synthetic as syn
+    from vtem_amd.batch import BatchDecoder, BlockJob
+    _banner("rnn persistent")
+    specs = [(32, 0.5), (48, 1.0), (64, 2.0), (96, 1.0), (32, 2.0)]
+    steps = 60
+    jobs = []
+    for i, (sd, du) in enumerate(specs):
+        blk = syn.serve_block(sd, du, seed=790 + i)
+        geom, st, dec = _encode_block(torch, blk)
+        jobs.append(BlockJob(dec, st, method="rnn", rnn_steps=steps))
+    ns = [int(jb.decoder.N) for jb in jobs]
+    assert max(ns) > 1008                      # some backward products split K
+    ref = BatchDecoder(precision="kpcg", graph_chunk=0, rnn_persistent=False).decode(jobs)
+
+    def same(res, what):
+        torch.cuda.synchronize()
+        for i, (a, b) in enumerate(zip(ref, res)):
+            assert torch.equal(a.X, b.X), (what, specs[i])
+            assert torch.equal(a.state, b.state), (what, specs[i])
+            assert torch.equal(a.frames, b.frames), (what, specs[i])
+        print(f"{what}: bit-identical on {len(res)} blocks", flush=True)
+
+    bd = BatchDecoder(precision="kpcg", rnn_persistent=True)
+    print("case: full ring", flush=True)
+    same(bd.decode(jobs), "full ring")
+    assert not bd.last_stats.get("rnn_aborted")
+    print("case: 32-slot ring", flush=True)
+    bd.rnn_ring_bytes = 0
+    same(bd.decode(jobs), "32-slot ring")
+    bd.rnn_ring_bytes = 2 << 30
+    print("case: block alone", flush=True)
+    alone = bd.decode([jobs[2]])
+    torch.cuda.synchronize()
+    assert torch.equal(alone[0].X, ref[2].X)
+    saved = vb._rnn_capacity()
+    try:
+        print("case: 8 workgroups in all", flush=True)
+        vb._RNN_CAP = ([8, 8], saved[1])
+        N = np.array(ns, np.int64)
+        Ds = np.array([jb.decoder.geom.Ds for jb in jobs], np.int64)
+        Dt = np.array([jb.decoder.geom.Dt for jb in jobs], np.int64)
+        plan = bd._rnn_plan(N, Ds, Dt)
+        assert plan is not None and plan[1] <= 8
+        same(bd.decode(jobs), "8 workgroups")
+    finally:
+        vb._RNN_CAP = saved
+    print("case: barrier waits given no time", flush=True)
+    bd.rnn_limit_s = 1e-12
+    res = bd.decode(jobs)
+    assert bd.last_stats.get("rnn_aborted") == len(jobs)
+    same(res, "aborted and redone")
 
 
 def test_kpcg_graph_replay(torch_mod):

@@ -148,6 +148,28 @@ def _fill_gemm(arr, A, B, C, sam, sak, sbk, sbn, ldc, M, N, K, alpha):
     arr["beta"] = 0.0
 
 
+class _RnnBarrier:
+    """Barrier workspace of a persistent RNN launch (word 0: the abort word)."""
+
+    def __init__(self, bar):
+        self.bar = bar
+
+
+_RNN_CAP = None
+
+
+def _rnn_capacity():
+    """([workgroups that fit for <= 17 columns, for wider], largest group) of the
+    persistent RNN launch on the current device (queried once)."""
+    global _RNN_CAP
+    if _RNN_CAP is None:
+        w = (ctypes.c_int * 2)()
+        mg = ctypes.c_int(0)
+        lib.vt_kron_rnn_persistent_capacity(w, ctypes.byref(mg))
+        _RNN_CAP = ([int(w[0]), int(w[1])], int(mg.value))
+    return _RNN_CAP
+
+
 @dataclasses.dataclass
 class _Skinny:
     """Launch record of one skinny grouped product."""
@@ -175,6 +197,9 @@ class BatchHandle:
     keep: list
     checks: list
     precisions: list = dataclasses.field(default_factory=list)
+    # (job indices, barrier workspace) of every persistent RNN launch: a nonzero abort word
+    # means its iterates are invalid and the jobs are solved again by the launch form
+    rnn_checks: list = dataclasses.field(default_factory=list)
 
 
 PRECISIONS = ("fp64", "mixed", "bf16x3", "fp16", "kpcg")
@@ -200,7 +225,8 @@ class BatchDecoder:
     def __init__(self, device="cuda", precision: str = "bf16x3", lam_rel: float = 1e-6,
                  refine_iters: int = 4, tol: float = 1e-10, verify: bool = True,
                  cg_iters: int = 32, cg_stop: float = 1e-12, retry_iters: Optional[int] = None,
-                 graph_chunk: int = 25, preconditioner: str = "auto", prune: int = 0):
+                 graph_chunk: int = 25, preconditioner: str = "auto", prune: int = 0,
+                 rnn_persistent: bool = False):
         if precision not in PRECISIONS:
             raise ValueError(f"batched precision must be one of {PRECISIONS}")
         self.device = device
@@ -228,6 +254,13 @@ class BatchDecoder:
         if prune and prune < 10:
             raise ValueError("prune must be 0 or at least 10")
         self.prune = int(prune)
+        # RNN iterations in one persistent launch (vt_kron_rnn_persistent) where the batch
+        # allows it (same iterates; measured no faster than the graph-replayed launch form on
+        # the serving batches, so off by default); otherwise, and to redo an aborted
+        # persistent launch, the launch form (vt_kron_rnn)
+        self.rnn_persistent = bool(rnn_persistent)
+        self.rnn_ring_bytes = 2 << 30          # ring slots for every iteration up to this size
+        self.rnn_limit_s = 2.0                 # longest barrier wait of the persistent launch
         self._pre_used = None                  # the last KPCG solve's preconditioner
         self.last_stats: Dict[str, Any] = {}
         self.timers: Optional[Dict[str, list]] = None     # stage -> [(ev0, ev1, work)]
@@ -343,6 +376,8 @@ class BatchDecoder:
         for steps, parts in sorted(rn.items()):
             gs = [g for _, g in parts]
             X, st, boff, skeep = self._solve_rnn(gs, steps)
+            if skeep and isinstance(skeep[-1], _RnnBarrier):
+                h.rnn_checks.append(([i for idx, _ in parts for i in idx], skeep[-1].bar))
             for (idx, g), (b0, e0) in zip(parts, boff):
                 zero = torch.zeros((g.B, 2), dtype=F64, device=self.device)
                 info = torch.zeros(g.B, dtype=I32, device=self.device)
@@ -381,6 +416,19 @@ class BatchDecoder:
         whose refinement did not reach `tol`, are redone as one batched FP64 group
         (flagged)."""
         redo, failed_fp64, retried = [], 0, []
+        aborted = [i for idx, bar in h.rnn_checks if int(bar[0].item()) != 0 for i in idx]
+        if aborted:
+            # a persistent RNN launch whose grid was not co-resident: the launch form
+            # computes the same iterates
+            base = self.rnn_persistent
+            self.rnn_persistent = False
+            try:
+                h3 = self.launch([h.jobs[i] for i in aborted], [h.precisions[i] for i in aborted])
+            finally:
+                self.rnn_persistent = base
+            for i, rr in zip(aborted, h3.results):
+                h.results[i] = rr
+            h.keep.append(h3.keep)
         if not self.verify:
             for _, (_, _, sync), _ in h.checks:
                 if sync is not None and int(sync[1].item()) != 0:
@@ -412,7 +460,7 @@ class BatchDecoder:
                 rr.fallback = True
                 h.results[i] = rr
             h.keep.append(h2.keep)
-        self.last_stats = {"groups": len(h.checks), "fallbacks": len(redo),
+        self.last_stats = {"groups": len(h.checks), "fallbacks": len(redo), "rnn_aborted": len(aborted),
                            "fp64_not_pd": failed_fp64, "fallback_index": redo,
                            "kpcg_retries": len(retried), "preconditioner": self._pre_used}
         torch.cuda.current_stream().synchronize()
@@ -680,15 +728,17 @@ class BatchDecoder:
         return _Skinny(_to_dev(d, self.device), len(rows), int(R[:, 10].max()), int(R[:, 11].max()),
                        int(R[:, 12].max()), ws)
 
-    def _operator(self, blocks, src, dst, Z, dbj, nzj, active=None):
+    def _operator(self, blocks, src, dst, Z, dbj, nzj, active=None, block_order=False):
         """dst_b = A_b src_b for every block (blocks: (S, Tm_ptr, N, Ds, Dt)) as two skinny
         launches: Z_j = T_j (S src)_j (T fused in the epilogue), dst = S^T Z (from the
         decoder's resident S^T) -- 4 N D flops, S read twice, no Gram.  With `active` (per-block flag pointers) converged
-        blocks skip both products."""
+        blocks skip both products.  block_order: the forward descriptors in block order too
+        (as the persistent RNN launch indexes them)."""
         act = active or [0] * len(blocks)
         # forward descriptors largest-first (their order changes no result; the backward
         # descriptors stay in block order, which the fused RNN update indexes)
-        order = sorted(range(len(blocks)), key=lambda b: -blocks[b][2] * blocks[b][3])
+        order = list(range(len(blocks))) if block_order else \
+            sorted(range(len(blocks)), key=lambda b: -blocks[b][2] * blocks[b][3])
         fwd = self._skinny([(S.data_ptr(), src.data_ptr() + 8 * int(dbj[b]), Z.data_ptr() + 8 * int(nzj[b]),
                              Tp, act[b], S.stride(0), 1, Dt, Dt, Dt * Dt, N, Ds, Dt)
                             for b in order for (S, Tp, N, Ds, Dt) in [blocks[b][:5]]])
@@ -1010,11 +1060,50 @@ class BatchDecoder:
         maxNc = int(Dt.max())
         lib.vt_kron_power(*fv.args(), *bv.args(), Bt, maxNc, ptr(V), ptr(Q), ptr(lam), ptr(seg), POWER_ITERS,
                           ptr(eta), self.graph_chunk, s)
-        lib.vt_kron_rnn(*fx.args(), *bx.args(), Bt, maxNc, ptr(X), ptr(r), ptr(Q), ptr(lam), ptr(eta),
-                        ptr(elem_job), Dtot, steps, self.graph_chunk, s)
+        plan = self._rnn_plan(N, Ds, Dt) if self.rnn_persistent else None
+        extra = []
+        if plan is None:
+            lib.vt_kron_rnn(*fx.args(), *bx.args(), Bt, maxNc, ptr(X), ptr(r), ptr(Q), ptr(lam), ptr(eta),
+                            ptr(elem_job), Dtot, steps, self.graph_chunk, s)
+        else:
+            gstart, grid = plan
+            fpo, bpo = self._operator(blocks, X, Q, Z, dbj, nzj, block_order=True)
+            gs_d = _h2d(gstart.astype(np.int32), dev)
+            slot_x = -(-Dtot // 512) * 512
+            slot_z = -(-int(nzj[-1]) // 512) * 512
+            ring = steps + 1 if 8 * (steps + 1) * (slot_x + slot_z) <= self.rnn_ring_bytes else 32
+            ring_x = torch.empty(ring * slot_x, dtype=F64, device=dev)
+            ring_z = torch.empty(ring * slot_z, dtype=F64, device=dev)
+            words = ctypes.c_longlong(0)
+            lib.vt_kron_rnn_persistent_bar_words(Bt, ctypes.byref(words))
+            bar = torch.empty(int(words.value), dtype=torch.int32, device=dev)
+            lib.vt_kron_rnn_persistent(ptr(fpo.descs), ptr(bpo.descs), Bt, maxNc, ptr(gs_d), grid, ptr(X), Dtot,
+                                       ptr(r), ptr(Z), ptr(lam), ptr(eta), ptr(seg), steps, ptr(ring_x), slot_x,
+                                       ptr(ring_z), slot_z, ring, ptr(bar), float(self.rnn_limit_s), s)
+            extra = [fpo, bpo, gs_d, ring_x, ring_z, _RnnBarrier(bar)]
         sv = st.view(Bt, 4)
         sv[:, 0] = eta
         sv[:, 3] = float(steps + 1)
         nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + 4.0 * D)
         self._toc("rnn_iterate", e0, float(nbytes) * (steps + 1 + POWER_ITERS))
-        return X, st, boff, [r, lam, seg, elem_job, X, V, Q, Z, fv, bv, fx, bx, st, eta]
+        return X, st, boff, [r, lam, seg, elem_job, X, V, Q, Z, fv, bv, fx, bx, st, eta] + extra
+
+    def _rnn_plan(self, N, Ds, Dt):
+        """Workgroup groups of the persistent RNN launch, or None where the launch form
+        runs (forward products that split K, or more blocks than fit).  Each block's group
+        gets one workgroup per tile of its larger phase (forward row tiles, or backward row
+        tiles times K splits), all scaled down together when the grid would exceed what
+        fits on the device at once.  Returns (gstart, grid)."""
+        if np.any(Ds > 1008) or np.any(Dt > 32):
+            return None
+        cap, mg = _rnn_capacity()
+        cap = cap[1 if int(Dt.max()) > 17 else 0]
+        ks = np.clip(((N + 15) // 16) // 32, 1, 8)           # the skinny product's split rule
+        need = np.maximum(-(-N // 16), -(-Ds // 16) * ks).astype(np.int64)
+        g = np.minimum(need, mg)
+        if g.sum() > cap:
+            g = np.maximum(1, np.floor(g * (cap / float(g.sum())))).astype(np.int64)
+        if g.sum() > cap:
+            return None
+        gstart = np.concatenate([[0], np.cumsum(g)]).astype(np.int64)
+        return gstart, int(gstart[-1])

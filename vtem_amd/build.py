@@ -22,7 +22,8 @@ LIB = os.path.join(HERE, "libvtem_hip.so")
 ARCH = os.environ.get("VTEM_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["vt_encode.hip", "vt_gemm.hip", "vt_decode.hip", "vt_gram.hip", "vt_chol.hip",
-           "vt_kpcg.hip", "vt_batch.hip", "vt_dirichlet.hip", "vt_iaf_par.hip", "vt_svd.hip"]
+           "vt_kpcg.hip", "vt_batch.hip", "vt_dirichlet.hip", "vt_iaf_par.hip", "vt_svd.hip",
+           "vt_rnn.hip"]
 LIBS = []
 # per-source flags: keep the MFMA accumulators of the Cholesky's trailing update in
 # ordinary VGPRs (otherwise the compiler copies them between AGPRs and VGPRs every k step)

@@ -32,6 +32,13 @@ enum {
         if (e__ != hipSuccess) return (int)e__;    \
     } while (0)
 
+// return a failing library status from the enclosing entry point
+#define VT_TRY(call)                    \
+    do {                                \
+        int st__ = (call);              \
+        if (st__ != VT_OK) return st__; \
+    } while (0)
+
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));

@@ -545,12 +545,6 @@ static int skinny_run(const VtSkinnyLaunch &l, int batch, int max_Nc, void *stre
     return vt_gemm_skinny(l.descs, batch, l.max_M, l.max_K, max_Nc, stream);
 }
 
-#define VT_TRY(call)                 \
-    do {                             \
-        int st__ = (call);           \
-        if (st__ != VT_OK) return st__; \
-    } while (0)
-
 // `iters` CG iterations (after vt_kpcg_init): launches = {fwd, bwd} as {descs, max_M, max_K}
 // triples for the operator halves (batch problems each, max_Nc columns), and the
 // preconditioner as up to three skinny launches run in order, each {descs, count, max_M,
@@ -640,7 +634,7 @@ __global__ void rnn_reduce_update_kernel(double *__restrict__ x, const double *_
         } else {
             qi = q[i];
         }
-        x[i] += eta[b] * (r[i] - qi - lam[2 * b] * x[i]);
+        x[i] = rnn_step(x[i], r[i], qi, lam[2 * b], eta[b]);
     }
 }
 

@@ -26,6 +26,25 @@ __host__ __device__ __forceinline__ int skinny_splits(int K)
     return ks < 1 ? 1 : (ks > 8 ? 8 : ks);
 }
 
+// Output store of a product tile.  COH: a relaxed agent-scope atomic store (written through
+// to the device coherence point), for tiles that other workgroups of a persistent kernel
+// read back within the same launch; the value stored is the same.
+template <bool COH>
+__device__ __forceinline__ void sk_store(double *p, double v)
+{
+    if constexpr (COH)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *p = v;
+}
+
+// One gradient-flow step of an element, x + eta (r - q - lam x): the launch-per-phase and
+// the persistent RNN loops share it, so both round alike.
+__device__ __forceinline__ double rnn_step(double x, double r, double q, double lam, double eta)
+{
+    return x + eta * (r - q - lam * x);
+}
+
 #ifndef SK_WAVES
 #define SK_WAVES 8
 #endif
@@ -114,8 +133,9 @@ __device__ __forceinline__ void skinny_load_x16(const SkDesc &d, int c, int c_en
 // launch may pick it from the batch size).  The waves split K and are reduced in LDS in a
 // fixed tree order; red is the workgroup's LDS scratch.  X16: the problem has Nc = 17 and
 // its column 16 is summed on the vector ALU (NT = 1, RT = 1); the route depends only on the
-// problem's own Nc, so a problem's sums do not depend on the batch.
-template <int NT, int RT, bool X16 = false>
+// problem's own Nc, so a problem's sums do not depend on the batch.  COH: the outputs are
+// stored agent-coherent (sk_store).
+template <int NT, int RT, bool X16 = false, bool COH = false>
 __device__ __forceinline__ void skinny_tile(const SkDesc &d, int m0, int z,
                                             double (&red)[SK_WAVES / 2][16 * RT][33])
 {
@@ -200,14 +220,14 @@ __device__ __forceinline__ void skinny_tile(const SkDesc &d, int m0, int z,
         double *W = d.ws + (long long)z * d.M * Nc;
         for (int e = tid; e < rows * Nc; e += 64 * SK_WAVES) {
             const int row = e / Nc, col = e % Nc;
-            W[(long long)(m0 + row) * Nc + col] = red[0][row][col];
+            sk_store<COH>(W + (long long)(m0 + row) * Nc + col, red[0][row][col]);
         }
         return;
     }
     for (int e = tid; e < rows * Nc; e += 64 * SK_WAVES) {
         const int row = e / Nc, col = e % Nc;
         const double v = d.T ? skinny_row_factor(d, m0 + row, col, Nc, red[0][row]) : red[0][row][col];
-        d.C[(long long)(m0 + row) * d.ldc + col] = v;
+        sk_store<COH>(d.C + (long long)(m0 + row) * d.ldc + col, v);
     }
 }
 

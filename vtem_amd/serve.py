@@ -124,7 +124,7 @@ class Server:
                  precision: str = "bf16x3", verify: bool = True, adaptive: bool = True,
                  rnn_lane: bool = True, rnn_max_batch: int = 64, rnn_lanes: int = 1,
                  graph_chunk: int = 25, max_delay: float = 0.0, rnn_max_delay: float = 0.0,
-                 rnn_lane_delays: Optional[Sequence[float]] = None):
+                 rnn_lane_delays: Optional[Sequence[float]] = None, rnn_persistent: bool = False):
         self.hbm_budget = int(hbm_budget)
         self.max_batch = int(max_batch)
         self.n_streams = int(streams)
@@ -149,6 +149,9 @@ class Server:
         self.rnn_lane_delays = None if rnn_lane_delays is None else [float(x) for x in rnn_lane_delays]
         if self.rnn_lane_delays is not None:
             self.rnn_lanes = len(self.rnn_lane_delays)
+        # RNN iterations in one persistent launch (default: three launches per iteration,
+        # replayed as HIP graphs)
+        self.rnn_persistent = bool(rnn_persistent)
 
     # ----------------------------------------------------------------------------------
     def serve(self, requests: List[Request], mode: str = "served",
@@ -161,7 +164,8 @@ class Server:
         for r in requests:
             if r.footprint > self.hbm_budget:
                 raise ValueError(f"request {r.rid} needs {r.footprint} B, above the HBM budget")
-        bd = BatchDecoder(precision=self.precision, verify=self.verify, graph_chunk=self.graph_chunk)
+        bd = BatchDecoder(precision=self.precision, verify=self.verify, graph_chunk=self.graph_chunk,
+                          rnn_persistent=self.rnn_persistent)
         bd.instrument(instrument)
         be = BatchEncoder()
         overflows = 0
