@@ -416,15 +416,19 @@ int vt_kpcg_shift_pool(const double *kpool, const long long *kbase, const long l
 /* ----------------------------------------- gradient-flow (RNN) decoder, batched -- */
 /* Coefficient-space image of neural_network.rnn3 (vtem/neural_network.py:15-99, update
  * kernel :102-128): X <- X + eta_b (r - (A + lam_b I) X), steps+1 iterations from X = 0,
- * A applied through the skinny operator launches fwd (reads X) / bwd (writes q).
- * elem_job maps each element of the concatenated X to its block.  graph_chunk >= 2 replays
- * the iterations as a HIP graph of graph_chunk iterations, captured once per call on a
- * private stream of the current device (one host call per replay instead of three
- * launches per iteration); 0 launches them one by one.  Results are identical. */
+ * A applied through the skinny operator launches fwd (reads X) / bwd (writes q, or its
+ * split partials), the backward launch doing the update too: the last of a row tile's K
+ * splits to finish adds the partials in split order and updates those rows.  bwd's
+ * descriptors are in block order; seg (batch + 1) gives each block's range of the
+ * concatenated X; cnt is a workspace of batch * ceil(bwd_M / 16) words (zeroed here).
+ * graph_chunk >= 2 replays the iterations as a HIP graph of graph_chunk iterations,
+ * captured once per call on a private stream of the current device (one host call per
+ * replay instead of two launches per iteration); 0 launches them one by one.  Results
+ * are identical. */
 int vt_kron_rnn(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                 int batch, int max_Nc, double *x, const double *r, double *q, const double *lam,
-                const double *eta, const int *elem_job, long long n, int steps, int graph_chunk,
-                void *stream);
+                const double *eta, const long long *seg, long long n, int steps, int graph_chunk,
+                unsigned *cnt, void *stream);
 /* Step sizes eta_b = 1/rho_b, rho_b the power-iteration estimate (iters steps from v,
  * overwritten) of the largest eigenvalue of A_b + lam_b I: the step rule eta = 1/lam_max
  * the configurations use in place of the reference's dt*alpha (vtem/vtdm.py:21). */

@@ -110,7 +110,7 @@ SIGNATURES = {
     "vt_svd_pinv_apply": [P, L, I, P, D, P, P, P],
     "vt_svd_eig_finish": [P, L, P, L, I, D, P, P, L, P, P],
     "vt_stitch_accumulate": [P, L, L, P, I, L, P],
-    "vt_kron_rnn": [P, I, I, P, I, I, I, I, P, P, P, P, P, P, L, I, I, P],
+    "vt_kron_rnn": [P, I, I, P, I, I, I, I, P, P, P, P, P, P, L, I, I, P, P],
     "vt_kron_rnn_persistent": [P, P, I, I, P, I, P, L, P, P, P, P, P, I, P, L, P, L, I, P, D, P],
     "vt_kron_rnn_persistent_capacity": [P, P],
     "vt_kron_rnn_persistent_bar_words": [I, P],

@@ -1048,7 +1048,6 @@ class BatchDecoder:
         r = torch.cat([g.r for g in gs]) if len(gs) > 1 else gs[0].r
         lam = torch.cat([g.lam for g in gs]) if len(gs) > 1 else gs[0].lam
         seg = _h2d(dbj.astype(np.int64), dev)
-        elem_job = _h2d(np.repeat(np.arange(Bt, dtype=np.int32), D), dev)
         X = torch.empty(Dtot, dtype=F64, device=dev)
         V = torch.ones(Dtot, dtype=F64, device=dev)
         Q = torch.empty(Dtot, dtype=F64, device=dev)
@@ -1063,8 +1062,10 @@ class BatchDecoder:
         plan = self._rnn_plan(N, Ds, Dt) if self.rnn_persistent else None
         extra = []
         if plan is None:
+            cnt = torch.empty(Bt * -(-int(Ds.max()) // 16), dtype=torch.int32, device=dev)
             lib.vt_kron_rnn(*fx.args(), *bx.args(), Bt, maxNc, ptr(X), ptr(r), ptr(Q), ptr(lam), ptr(eta),
-                            ptr(elem_job), Dtot, steps, self.graph_chunk, s)
+                            ptr(seg), Dtot, steps, self.graph_chunk, ptr(cnt), s)
+            extra = [cnt]
         else:
             gstart, grid = plan
             fpo, bpo = self._operator(blocks, X, Q, Z, dbj, nzj, block_order=True)
@@ -1086,7 +1087,7 @@ class BatchDecoder:
         sv[:, 3] = float(steps + 1)
         nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + 4.0 * D)
         self._toc("rnn_iterate", e0, float(nbytes) * (steps + 1 + POWER_ITERS))
-        return X, st, boff, [r, lam, seg, elem_job, X, V, Q, Z, fv, bv, fx, bx, st, eta] + extra
+        return X, st, boff, [r, lam, seg, X, V, Q, Z, fv, bv, fx, bx, st, eta] + extra
 
     def _rnn_plan(self, N, Ds, Dt):
         """Workgroup groups of the persistent RNN launch, or None where the launch form

@@ -612,30 +612,103 @@ __global__ void zero_kernel(double *__restrict__ x, long long n)
         x[i] = 0.0;
 }
 
-// x += eta_b (r - q - lam_b x), with q = S^T Z finished here: blocks whose backward product
-// split its reduction sum their partials (in split order, as the product's own second pass
-// does); the others read the product's output.  One launch instead of two per iteration.
-__global__ void rnn_reduce_update_kernel(double *__restrict__ x, const double *__restrict__ r,
-                                         const double *__restrict__ q, const SkDesc *__restrict__ bwd,
-                                         const double *__restrict__ lam, const double *__restrict__ eta,
-                                         const int *__restrict__ elem_job, long long n)
+// Backward product of the RNN step with the step itself fused: problem b (block order) is
+// S_b^T Z_b, this workgroup its row tile m0 and K split z.  An unsplit problem's workgroup
+// updates its rows at once, x += eta_b (r - q - lam_b x).  Split problems store their
+// partials agent-coherent (the L2 caches of the XCDs are separate); the last of a row
+// tile's ks splits to finish (a counter per row tile, taken after the workgroup's stores
+// have completed) reads the partials back with agent-scope loads, adds them in split
+// order, as the product's own second pass does, updates the rows and resets the counter
+// for the next iteration.  One launch per iteration fewer than a separate update, same
+// values: the counters only pick which workgroup does the update.
+template <int NT, bool X16>
+__device__ __forceinline__ void rnn_bwd_tile(const SkDesc &d, int m0, int z, double *__restrict__ x,
+                                             const double *__restrict__ r, double lam, double eta,
+                                             long long s0, unsigned *cnt, double (&red)[SK_WAVES / 2][16][33])
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const int b = elem_job[i];
-        const SkDesc &d = bwd[b];
-        const int ks = skinny_splits(d.K);
+    __shared__ int last;
+    const int ks = skinny_splits(d.K), Nc = d.Nc;
+    skinny_tile<NT, 1, X16, true>(d, m0, z, red);
+    __builtin_amdgcn_s_waitcnt(0);                // this wave's partial or q stores have completed
+    __syncthreads();
+    if (ks > 1) {
+        if (threadIdx.x == 0) {
+            const unsigned o = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            last = o == (unsigned)ks - 1;
+            if (last) __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        if (!last) return;
+    }
+    const int rows = min(16, d.M - m0);
+    const long long mn = (long long)d.M * Nc;
+    for (int e = threadIdx.x; e < rows * Nc; e += 64 * SK_WAVES) {
+        const long long loc = (long long)(m0 + e / Nc) * Nc + e % Nc;
         double qi;
         if (ks > 1) {
-            const long long loc = i - (long long)(d.C - q);     // index within the block
-            const long long mn = (long long)d.M * d.Nc;
             qi = 0.0;
-            for (int z = 0; z < ks; ++z) qi += d.ws[z * mn + loc];
+            for (int zz = 0; zz < ks; ++zz)
+                qi += __hip_atomic_load(d.ws + zz * mn + loc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            qi = q[i];
+            qi = __hip_atomic_load(d.C + loc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        x[i] = rnn_step(x[i], r[i], qi, lam[2 * b], eta[b]);
+        const long long i = s0 + loc;
+        x[i] = rnn_step(x[i], r[i], qi, lam, eta);
     }
+}
+
+// grid (row tiles, batch, max splits); the route by the problem's own Nc, as skinny_kernel
+template <int NT, bool X16>
+__global__ void __launch_bounds__(64 * SK_WAVES)
+rnn_bwd_update_kernel(const SkDesc *__restrict__ descs, double *__restrict__ x, const double *__restrict__ r,
+                      const double *__restrict__ lam, const double *__restrict__ eta,
+                      const long long *__restrict__ seg, unsigned *__restrict__ cnt)
+{
+    __shared__ double red[SK_WAVES / 2][16][33];
+    const int b = blockIdx.y;
+    const SkDesc d = descs[b];
+    const int m0 = blockIdx.x * 16;
+    if (m0 >= d.M || (int)blockIdx.z >= skinny_splits(d.K)) return;
+    unsigned *c = cnt + (long long)b * gridDim.x + blockIdx.x;
+    const double l = lam[2 * b], et = eta[b];
+    const long long s0 = seg[b];
+    if constexpr (NT == 1) {
+        if (X16 && d.Nc == 17)
+            rnn_bwd_tile<1, true>(d, m0, blockIdx.z, x, r, l, et, s0, c, red);
+        else
+            rnn_bwd_tile<1, false>(d, m0, blockIdx.z, x, r, l, et, s0, c, red);
+    } else {
+        if (d.Nc == 17)
+            rnn_bwd_tile<1, true>(d, m0, blockIdx.z, x, r, l, et, s0, c, red);
+        else if (d.Nc <= 16)
+            rnn_bwd_tile<1, false>(d, m0, blockIdx.z, x, r, l, et, s0, c, red);
+        else
+            rnn_bwd_tile<2, false>(d, m0, blockIdx.z, x, r, l, et, s0, c, red);
+    }
+}
+
+static int rnn_bwd_update(const VtSkinnyLaunch &Bk, int batch, int max_Nc, double *x, const double *r,
+                          const double *lam, const double *eta, const long long *seg, unsigned *cnt,
+                          hipStream_t s)
+{
+    if (Bk.max_M <= 0) return VT_OK;
+    const dim3 grid(vt_blocks(Bk.max_M, 16), batch, skinny_splits(Bk.max_K));
+    const SkDesc *d = (const SkDesc *)Bk.descs;
+    if (max_Nc <= 16)
+        rnn_bwd_update_kernel<1, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, x, r, lam, eta, seg, cnt);
+    else if (max_Nc == 17)
+        rnn_bwd_update_kernel<1, true><<<grid, 64 * SK_WAVES, 0, s>>>(d, x, r, lam, eta, seg, cnt);
+    else
+        rnn_bwd_update_kernel<2, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, x, r, lam, eta, seg, cnt);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+__global__ void zero_words_kernel(unsigned *__restrict__ w, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        w[i] = 0u;
 }
 
 VT_EXPORT int vt_power_step(double *v, const double *q, const double *lam, const long long *seg,
@@ -668,27 +741,28 @@ VT_EXPORT int vt_kron_power(const void *fwd, int fwd_M, int fwd_K, const void *b
     return vt_graph_loop((hipStream_t)stream, iters, graph_chunk, step);
 }
 
-// steps+1 gradient-flow iterations from x = 0 (the operator launches read x, write q);
-// graph_chunk as for vt_kron_power.
+// steps+1 gradient-flow iterations from x = 0: per iteration the forward operator launch
+// (reads x) and the backward launch with the update fused (writes x); graph_chunk as for
+// vt_kron_power.  cnt: batch * ceil(bwd_M / 16) words of row-tile counters (zeroed here).
 VT_EXPORT int vt_kron_rnn(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                           int batch, int max_Nc, double *x, const double *r, double *q,
-                          const double *lam, const double *eta, const int *elem_job, long long n,
-                          int steps, int graph_chunk, void *stream)
+                          const double *lam, const double *eta, const long long *seg, long long n,
+                          int steps, int graph_chunk, unsigned *cnt, void *stream)
 {
-    VT_CHECK_ARG(fwd && bwd && x && r && q && lam && eta && elem_job && n >= 0 && steps >= 0 &&
-                 graph_chunk >= 0);
+    VT_CHECK_ARG(fwd && bwd && x && r && q && lam && eta && seg && cnt && n >= 0 && steps >= 0 &&
+                 graph_chunk >= 0 && batch >= 1 && batch <= 65535 && max_Nc >= 1 && max_Nc <= 32);
     const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
     hipStream_t s = (hipStream_t)stream;
     unsigned g = vt_blocks(n, 256);
     if (g > 4096) g = 4096;
-    zero_kernel<<<g, 256, 0, s>>>(x, n);   // a kernel, not a memset node, if captured
+    zero_kernel<<<g, 256, 0, s>>>(x, n);   // kernels, not memset nodes, if captured
+    const long long words = (long long)batch * vt_blocks(bwd_M > 0 ? bwd_M : 1, 16);
+    const unsigned gw = vt_blocks(words, 256);
+    zero_words_kernel<<<gw < 1024 ? gw : 1024, 256, 0, s>>>(cnt, words);
     VT_LAUNCH_CHECK();
     auto step = [&](hipStream_t st) -> int {
         VT_TRY(skinny_run(F, batch, max_Nc, st));
-        if (Bk.max_M > 0) VT_TRY(vt_skinny_first_pass(Bk.descs, batch, Bk.max_M, Bk.max_K, max_Nc, st));
-        rnn_reduce_update_kernel<<<g, 256, 0, st>>>(x, r, q, (const SkDesc *)Bk.descs, lam, eta,
-                                                    elem_job, n);
-        VT_LAUNCH_CHECK();
+        VT_TRY(rnn_bwd_update(Bk, batch, max_Nc, x, r, lam, eta, seg, cnt, st));
         return VT_OK;
     };
     return vt_graph_loop(s, steps + 1, graph_chunk, step);
