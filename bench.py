@@ -59,6 +59,7 @@ PEAKS = {"gram_lp": (2500.0, "TFLOP/s", "mfma"), "gram_f64": (78.6, "TFLOP/s", "
          "kpcg_mgram": (78.6, "TFLOP/s", "mfma"), "kpcg_potrf": (78.6, "TFLOP/s", "mfma"),
          "kpcg_inverse": (78.6, "TFLOP/s", "mfma"), "prologue": (8000.0, "GB/s", "hbm"),
          "synthesis": (78.6, "TFLOP/s", "mfma"), "rnn_iterate": (8000.0, "GB/s", "hbm"),
+         "rnn_power": (8000.0, "GB/s", "hbm"),
          "kpcg_iterate": (8000.0, "GB/s", "hbm"), "encode_coeffs": (8000.0, "GB/s", "hbm"),
          "iaf": (78.6, "TFLOP/s", "fp64")}
 WORK_NOTES = {"encode_coeffs": "algorithmic HBM bytes 8*(F*P + N*P + 2*(P+N)*Dt) per block (frames, "
@@ -90,6 +91,8 @@ def parse():
     p.add_argument("--graph-chunk", type=int, default=25,
                    help="RNN / power iterations per replayed HIP graph (0 = launch each)")
     p.add_argument("--form", default="kpcg", help="solver form of the chosen served path")
+    p.add_argument("--no-rnn-prep", action="store_true",
+                   help="no preparation stream: RNN requests behind a running chain wait whole")
     p.add_argument("--rnn-persistent", action="store_true",
                    help="RNN iterations in one persistent launch instead of three launches each")
     p.add_argument("--compare", default="bf16x3", help="second served form measured ('' = none)")
@@ -546,7 +549,8 @@ def main():
                         adaptive=form != "kpcg", max_delay=args.max_delay_ms * 1e-3,
                         rnn_max_delay=args.rnn_max_delay_ms * 1e-3,
                         rnn_lane_delays=[float(x) * 1e-3 for x in args.rnn_lane_delays_ms.split(",")]
-                        if args.rnn_lane_delays_ms else None, rnn_persistent=args.rnn_persistent)
+                        if args.rnn_lane_delays_ms else None, rnn_persistent=args.rnn_persistent,
+                        rnn_prep=not args.no_rnn_prep)
         for _ in range(args.warmup):           # allocator, code objects, precision policy
             server.serve(make_requests(items, zeros, form))
         torch.cuda.synchronize()
@@ -598,7 +602,7 @@ def main():
                 for i, it in enumerate(items)]
     dserver = Server(max_batch=args.max_batch, streams=args.streams, precision=args.form,
                      rnn_lanes=args.rnn_lanes, graph_chunk=args.graph_chunk, adaptive=args.form != "kpcg",
-                     rnn_persistent=args.rnn_persistent)
+                     rnn_persistent=args.rnn_persistent, rnn_prep=not args.no_rnn_prep)
     dserver.serve(decode_requests(args.form))                     # warm
     dwall = sum(dserver.serve(decode_requests(args.form))["wall_s"] for _ in range(args.steps))
     Server().serve(decode_requests("fp64"), mode="plain")         # warm
@@ -611,7 +615,7 @@ def main():
     # one stream, so that a stage's events bracket only its own kernels
     iserver = Server(max_batch=args.max_batch, streams=1, precision=args.form,
                      graph_chunk=args.graph_chunk, adaptive=args.form != "kpcg",
-                     rnn_persistent=args.rnn_persistent)
+                     rnn_persistent=args.rnn_persistent, rnn_prep=not args.no_rnn_prep)
     iserver.policy = server.policy
     timers = iserver.serve(make_requests(items, zeros), instrument=True)["timers"]
     torch.cuda.synchronize()
@@ -654,6 +658,7 @@ def main():
         "chosen": {"form": args.form, "frames_per_s": round(value, 2),
                    "p50_ms": round(stats[-1]["p50_ms"], 3), "p99_ms": round(stats[-1]["p99_ms"], 3),
                    "batches": stats[-1]["batches"], "fp64_fallbacks": stats[-1]["fallbacks"],
+                   "rnn_preparations": stats[-1].get("rnn_preps", 0),
                    "host_launch_ms": round(stats[-1]["host_launch_s"] * 1e3, 2),
                    "host_finalize_ms": round(stats[-1]["host_finalize_s"] * 1e3, 2),
                    "wave_ms": round(stats[-1]["wall_s"] * 1e3, 2),

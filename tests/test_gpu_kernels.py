@@ -1046,14 +1046,17 @@ def test_server_alone_equals_mix(torch_mod):
         np.testing.assert_array_equal(r.result.frames.cpu().numpy(), alone.frames.cpu().numpy())
 
 
-@pytest.mark.parametrize("lane_delays", [None, (0.0, 0.004)])
+@pytest.mark.parametrize("lane_delays", [None, (0.0, 0.004), "prep"])
 def test_server_encode_decode_alone_equals_mix(torch_mod, lane_delays):
     """Encode+decode requests (video resident, encoded by the server's batched encoder)
     served with Poisson arrivals on 2 streams: every request's spikes and frames are
     bit-identical to the request encoded and decoded alone (KPCG form, Tikhonov and RNN
     mixed; one RNN lane, or two staggered lanes holding 0 and 4 ms).  One request gets a
     4-row spike buffer: the server detects the overflow and redoes it with an exact
-    buffer, giving the alone result with a large buffer."""
+    buffer, giving the alone result with a large buffer.  "prep": mostly RNN requests
+    arriving faster than a chain runs, so the ones behind a running chain are prepared
+    (encode, prologue, step sizes) on the preparation stream and join the next chain;
+    the overflowing request is one of them."""
     torch = torch_mod
     import dataclasses
     import numpy as np
@@ -1065,27 +1068,32 @@ def test_server_encode_decode_alone_equals_mix(torch_mod, lane_delays):
     from vtem_amd.serve import Request, Server, footprint_bytes, poisson_arrivals
     _banner(f"server: encode+decode alone == in the mix, overflow redo, RNN lane holds {lane_delays}")
     specs = [(32, 2.0), (48, 1.0), (32, 1.0), (48, 2.0), (64, 1.0), (32, 0.5), (48, 0.5)]
+    prep = lane_delays == "prep"
+    if prep:
+        lane_delays = None
     reqs, parts = [], []
-    arr = poisson_arrivals(len(specs), 200.0, np.random.default_rng(5))
+    arr = poisson_arrivals(len(specs), 2000.0 if prep else 200.0, np.random.default_rng(5))
     for i, ((sd, du), a) in enumerate(zip(specs, arr)):
         blk = syn.serve_block(sd, du, seed=700 + i)
         geom = Geometry(sd, sd, blk.Lx, blk.Lx, blk.Lt, du)
         enc = Encoder(geom, blk.bank)
-        ms = 4 if i == 3 else max_spikes_rate(geom, blk.iaf)
+        ms = 4 if i == (5 if prep else 3) else max_spikes_rate(geom, blk.iaf)
         ej = EncodeJob(enc, dev(blk.video.frames.reshape(geom.frames, -1)),
                        IAFDevice.from_params(blk.iaf), ms)
         dec = Decoder(geom, spatial_coeffs(enc.RF, geom), blk.iaf.kappa, blk.iaf.delta,
                       blk.iaf.bias)
-        job = BlockJob(dec, None, tag=i, method="rnn" if i % 3 == 2 else "tikhonov",
-                       rnn_steps=100)
+        rnn = (i != 1) if prep else (i % 3 == 2)
+        job = BlockJob(dec, None, tag=i, method="rnn" if rnn else "tikhonov", rnn_steps=100)
         parts.append((ej, job, max_spikes_rate(geom, blk.iaf)))
         reqs.append(Request(i, float(a), job, geom.frames, footprint_bytes(job, "kpcg", encode=ej),
                             encode=ej, iaf=blk.iaf))
     server = Server(max_batch=3, streams=2, precision="kpcg", adaptive=False, rnn_lane_delays=lane_delays)
     st = server.serve(reqs)
-    print(f"served: batches={st['batches']} overflows={st['overflows']} p50={st['p50_ms']:.2f} ms",
-          flush=True)
+    print(f"served: batches={st['batches']} overflows={st['overflows']} rnn_preps={st['rnn_preps']} "
+          f"p50={st['p50_ms']:.2f} ms", flush=True)
     assert st["overflows"] == 1
+    if prep:
+        assert st["rnn_preps"] >= 1
     be, bd = BatchEncoder(), BatchDecoder(precision="kpcg")
     for r, (ej, job, ms_ok) in zip(reqs, parts):
         ej1 = dataclasses.replace(ej, max_spike=max(ej.max_spike, ms_ok)) if ej.max_spike == 4 else ej

@@ -202,6 +202,16 @@ class BatchHandle:
     rnn_checks: list = dataclasses.field(default_factory=list)
 
 
+@dataclasses.dataclass
+class RnnPrep:
+    """RNN jobs made ready for their iterations (BatchDecoder.prepare_rnn): per step count,
+    the prologue groups [(job indices, prologue)] and the blocks' step sizes eta (joint
+    order), with the buffers to keep; `done` is recorded after the last preparing launch."""
+    jobs: List["BlockJob"]
+    steps: Dict[int, tuple]
+    done: Optional[torch.cuda.Event] = None
+
+
 PRECISIONS = ("fp64", "mixed", "bf16x3", "fp16", "kpcg")
 POWER_ITERS = 50          # power steps for the RNN step size 1/lambda_max
 
@@ -367,15 +377,70 @@ class BatchDecoder:
         return ss[key]
 
     def _launch_rnn(self, h: "BatchHandle", jobs, groups, keys):
-        """Prologue, joint RNN iterations and synthesis of the RNN groups (current stream)."""
+        """Prologue, step sizes, joint RNN iterations and synthesis of the RNN groups
+        (current stream)."""
+        self._iterate_prepared(h, [self._prepare_rnn_groups(jobs, groups, keys)])
+
+    def prepare_rnn(self, jobs: List[BlockJob]) -> RnnPrep:
+        """The first stage of RNN jobs, enqueued on the current stream with no host sync:
+        intervals, moments and right-hand sides (the prologue) and the step sizes (50
+        power steps).  launch_rnn runs their iterations later, on any stream, alone or
+        joined with other prepared jobs; the results are those of launch() (every block's
+        arithmetic is its own)."""
+        groups: Dict[tuple, List[int]] = {}
+        for i, jb in enumerate(jobs):
+            if jb.method != "rnn":
+                raise ValueError("prepare_rnn takes RNN jobs only")
+            groups.setdefault((jb.decoder.geom.Dt, "rnn", jb.rnn_steps), []).append(i)
+        prep = self._prepare_rnn_groups(jobs, groups, sorted(groups))
+        prep.done = torch.cuda.Event()
+        prep.done.record()
+        return prep
+
+    def launch_rnn(self, preps: List[RnnPrep]) -> "BatchHandle":
+        """The iterations and synthesis of prepared RNN jobs on the current stream (after
+        each preparation's event): one joint launch sequence per step count.  The handle's
+        jobs are the preparations' jobs in order; finalize() retires it."""
+        main = torch.cuda.current_stream()
+        jobs = [jb for p in preps for jb in p.jobs]
+        h = BatchHandle(jobs, [None] * len(jobs), [], [])
+        h.precisions = [self.precision] * len(jobs)
+        for p in preps:
+            if p.done is not None:
+                main.wait_event(p.done)
+        self._iterate_prepared(h, preps)
+        h.keep.append(preps)
+        return h
+
+    def _prepare_rnn_groups(self, jobs, groups, keys) -> RnnPrep:
         rn: Dict[int, list] = {}
         for key in keys:
             Dt, _, steps = key
             idx = groups[key]
             rn.setdefault(steps, []).append((idx, self._prologue([jobs[i] for i in idx], Dt)))
+        out = {}
         for steps, parts in sorted(rn.items()):
+            eta, keep = self._rnn_power([g for _, g in parts])
+            out[steps] = (parts, eta, keep)
+        return RnnPrep(list(jobs), out)
+
+    def _iterate_prepared(self, h: "BatchHandle", preps: List[RnnPrep]):
+        """Joint iterations per step count over the preparations' blocks (job indices of
+        preparation k offset by the jobs of the ones before it), then each group's
+        synthesis into h.results."""
+        merged: Dict[int, tuple] = {}
+        off = 0
+        for p in preps:
+            for steps, (parts, eta, keep) in p.steps.items():
+                m = merged.setdefault(steps, ([], [], []))
+                m[0].extend([([i + off for i in idx], g) for idx, g in parts])
+                m[1].append(eta)
+                m[2].append(keep)
+            off += len(p.jobs)
+        for steps, (parts, etas, pkeep) in sorted(merged.items()):
             gs = [g for _, g in parts]
-            X, st, boff, skeep = self._solve_rnn(gs, steps)
+            eta = torch.cat(etas) if len(etas) > 1 else etas[0]
+            X, st, boff, skeep = self._rnn_iterate(gs, eta, steps)
             if skeep and isinstance(skeep[-1], _RnnBarrier):
                 h.rnn_checks.append(([i for idx, _ in parts for i in idx], skeep[-1].bar))
             for (idx, g), (b0, e0) in zip(parts, boff):
@@ -385,7 +450,7 @@ class BatchDecoder:
                 for i, r in zip(idx, res):
                     h.results[i] = r
                 h.keep.append((g.keep, keep, zero, info))
-            h.keep.append(skeep)
+            h.keep.append((skeep, pkeep))
 
     def _guard(self, checks):
         """(job index, precision) of every non-FP64 result that failed the guard (factor
@@ -1030,12 +1095,37 @@ class BatchDecoder:
         Dt = np.array([b[4] for b in blocks], np.int64)
         return blocks, boff, N, Ds, Dt
 
-    def _solve_rnn(self, gs: List[_Prologue], steps: int):
+    def _rnn_power(self, gs: List[_Prologue]):
+        """Step sizes eta_b = 1/lambda_max(A_b + lam_b I) of the groups' blocks (joint order)
+        by 50 power steps through the structured operator.  Returns (eta, keep)."""
+        dev = self.device
+        s = stream_ptr()
+        e0 = self._tic()
+        blocks, boff, N, Ds, Dt = self._joint_blocks(gs)
+        Bt = len(blocks)
+        D = Ds * Dt
+        dbj = np.concatenate([[0], np.cumsum(D)])
+        nzj = np.concatenate([[0], np.cumsum(N * Dt)])
+        Dtot = int(dbj[-1])
+        lam = torch.cat([g.lam for g in gs]) if len(gs) > 1 else gs[0].lam
+        seg = _h2d(dbj.astype(np.int64), dev)
+        V = torch.ones(Dtot, dtype=F64, device=dev)
+        Q = torch.empty(Dtot, dtype=F64, device=dev)
+        Z = torch.empty(int(nzj[-1]), dtype=F64, device=dev)
+        fv, bv = self._operator(blocks, V, Q, Z, dbj, nzj)
+        eta = torch.empty(Bt, dtype=F64, device=dev)
+        lib.vt_kron_power(*fv.args(), *bv.args(), Bt, int(Dt.max()), ptr(V), ptr(Q), ptr(lam), ptr(seg),
+                          POWER_ITERS, ptr(eta), self.graph_chunk, s)
+        nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + 4.0 * D)
+        self._toc("rnn_power", e0, float(nbytes) * POWER_ITERS)
+        return eta, [lam, seg, V, Q, Z, fv, bv, eta]
+
+    def _rnn_iterate(self, gs: List[_Prologue], eta, steps: int):
         """Gradient-flow decoder for every block of the groups, as one joint launch
-        sequence: eta_b = 1/lambda_max(A_b + lam_b I) by 50 power steps, then steps + 1
-        iterations X <- X + eta_b (r - (A_b + lam_b I) X) from 0 through the structured
-        operator.  Returns the joint X, the per-block state [eta, 0, 0, iterations], the
-        per-group offsets and the buffers to keep."""
+        sequence: steps + 1 iterations X <- X + eta_b (r - (A_b + lam_b I) X) from 0 through
+        the structured operator, eta (joint order) from _rnn_power.  Returns the joint X,
+        the per-block state [eta, 0, 0, iterations], the per-group offsets and the buffers
+        to keep."""
         dev = self.device
         s = stream_ptr()
         e0 = self._tic()
@@ -1049,16 +1139,11 @@ class BatchDecoder:
         lam = torch.cat([g.lam for g in gs]) if len(gs) > 1 else gs[0].lam
         seg = _h2d(dbj.astype(np.int64), dev)
         X = torch.empty(Dtot, dtype=F64, device=dev)
-        V = torch.ones(Dtot, dtype=F64, device=dev)
         Q = torch.empty(Dtot, dtype=F64, device=dev)
         Z = torch.empty(int(nzj[-1]), dtype=F64, device=dev)
-        fv, bv = self._operator(blocks, V, Q, Z, dbj, nzj)
         fx, bx = self._operator(blocks, X, Q, Z, dbj, nzj)
         st = torch.zeros(4 * Bt, dtype=F64, device=dev)
-        eta = torch.empty(Bt, dtype=F64, device=dev)
         maxNc = int(Dt.max())
-        lib.vt_kron_power(*fv.args(), *bv.args(), Bt, maxNc, ptr(V), ptr(Q), ptr(lam), ptr(seg), POWER_ITERS,
-                          ptr(eta), self.graph_chunk, s)
         plan = self._rnn_plan(N, Ds, Dt) if self.rnn_persistent else None
         extra = []
         if plan is None:
@@ -1086,8 +1171,8 @@ class BatchDecoder:
         sv[:, 0] = eta
         sv[:, 3] = float(steps + 1)
         nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + 4.0 * D)
-        self._toc("rnn_iterate", e0, float(nbytes) * (steps + 1 + POWER_ITERS))
-        return X, st, boff, [r, lam, seg, X, V, Q, Z, fv, bv, fx, bx, st, eta] + extra
+        self._toc("rnn_iterate", e0, float(nbytes) * (steps + 1))
+        return X, st, boff, [r, lam, seg, X, Q, Z, fx, bx, st, eta] + extra
 
     def _rnn_plan(self, N, Ds, Dt):
         """Workgroup groups of the persistent RNN launch, or None where the launch form

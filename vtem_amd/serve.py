@@ -124,7 +124,8 @@ class Server:
                  precision: str = "bf16x3", verify: bool = True, adaptive: bool = True,
                  rnn_lane: bool = True, rnn_max_batch: int = 64, rnn_lanes: int = 1,
                  graph_chunk: int = 25, max_delay: float = 0.0, rnn_max_delay: float = 0.0,
-                 rnn_lane_delays: Optional[Sequence[float]] = None, rnn_persistent: bool = False):
+                 rnn_lane_delays: Optional[Sequence[float]] = None, rnn_persistent: bool = False,
+                 rnn_prep: bool = True):
         self.hbm_budget = int(hbm_budget)
         self.max_batch = int(max_batch)
         self.n_streams = int(streams)
@@ -149,9 +150,13 @@ class Server:
         self.rnn_lane_delays = None if rnn_lane_delays is None else [float(x) for x in rnn_lane_delays]
         if self.rnn_lane_delays is not None:
             self.rnn_lanes = len(self.rnn_lane_delays)
-        # RNN iterations in one persistent launch (default: three launches per iteration,
+        # RNN iterations in one persistent launch (default: two launches per iteration,
         # replayed as HIP graphs)
         self.rnn_persistent = bool(rnn_persistent)
+        # RNN requests that arrive while the RNN lane runs a chain are prepared at once on a
+        # stream of their own (encode, prologue, step sizes), so the next chain starts with
+        # its iterations (one RNN lane without held queues)
+        self.rnn_prep = bool(rnn_prep)
 
     # ----------------------------------------------------------------------------------
     def serve(self, requests: List[Request], mode: str = "served",
@@ -178,7 +183,11 @@ class Server:
         lanes = [(torch.cuda.Stream(), ready_rnn, self.rnn_max_batch) for _ in range(nr)]
         lanes += [(torch.cuda.Stream(), ready, self.max_batch) for _ in range(self.n_streams)]
         streams = [ln[0] for ln in lanes]
-        busy = [None] * len(lanes)              # (event, handle, reqs, bytes, overflow flags)
+        busy = [None] * len(lanes)              # (event, handle, reqs, bytes, [(reqs, overflow flags)])
+        prep_on = (nr == 1 and self.rnn_prep and self.rnn_lane_delays is None and self.rnn_max_delay == 0.0)
+        prep_stream = torch.cuda.Stream() if prep_on else None
+        prepared: List[tuple] = []              # (reqs, RnnPrep, bytes, [(reqs, overflow flags)])
+        preps_made = 0
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         in_use = 0
         nbatch = 0
@@ -186,7 +195,7 @@ class Server:
         host_launch = host_finalize = 0.0      # host seconds spent enqueueing / retiring
         torch.cuda.synchronize()
         t0 = clock()
-        while pending or ready or ready_rnn or any(b is not None for b in busy):
+        while pending or ready or ready_rnn or prepared or any(b is not None for b in busy):
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 r = pending.popleft()
@@ -194,13 +203,12 @@ class Server:
             # retire finished batches
             for k, b in enumerate(busy):
                 if b is not None and b[0].query():
-                    ev, h, reqs, nbytes, flags = b
+                    ev, h, reqs, nbytes, flag_sets = b
                     tf = clock()
                     with torch.cuda.stream(streams[k]):
                         res = bd.finalize(h)
-                        if flags is not None:
-                            over = [r for r, f in zip([r for r in reqs if r.encode is not None],
-                                                      flags.cpu().numpy()) if f]
+                        for enc, flags in flag_sets:
+                            over = [r for r, f in zip(enc, flags.cpu().numpy()) if f]
                             for r in over:
                                 i = reqs.index(r)
                                 res[i] = self._redo_overflow(r, be, bd)
@@ -216,10 +224,53 @@ class Server:
                         r.result, r.done = rr, tdone
                     in_use -= nbytes
                     busy[k] = None
+            # prepare the RNN requests waiting behind a running chain (one preparation in
+            # flight at a time, so arrivals during it are prepared together)
+            if prep_on and busy[0] is not None and ready_rnn and \
+                    (not prepared or prepared[-1][1].done.query()):
+                batch, nbytes = form_batch(ready_rnn, self.rnn_max_batch, self.hbm_budget - in_use)
+                if batch:
+                    tstart = clock() - t0
+                    with torch.cuda.stream(prep_stream):
+                        flags = self._encode(batch, be, bd)
+                        prep = bd.prepare_rnn([r.job for r in batch])
+                    host_launch += clock() - t0 - tstart
+                    for r in batch:
+                        r.start = tstart
+                    in_use += nbytes
+                    preps_made += 1
+                    prepared.append((batch, prep, nbytes, self._flag_set(batch, flags)))
             # launch on a free lane
             launched = False
             wake = math.inf                     # earliest batching deadline of a held queue
             for k, (_, queue, cap) in enumerate(lanes):
+                if prep_on and k == 0 and busy[0] is None and prepared:
+                    # the RNN lane: the prepared requests (and any waiting ones, prepared on
+                    # the lane's own stream first) go into one chain
+                    tstart = clock() - t0
+                    batch, nbytes = form_batch(queue, self.rnn_max_batch, self.hbm_budget - in_use)
+                    reqs = [r for p in prepared for r in p[0]] + batch
+                    flag_sets = [fs for p in prepared for fs in p[3]]
+                    with torch.cuda.stream(streams[0]):
+                        preps = [p[1] for p in prepared]
+                        if batch:
+                            flags = self._encode(batch, be, bd)
+                            preps.append(bd.prepare_rnn([r.job for r in batch]))
+                            flag_sets += self._flag_set(batch, flags)
+                        h = bd.launch_rnn(preps)
+                        ev = torch.cuda.Event()
+                        ev.record()
+                    host_launch += clock() - t0 - tstart
+                    for r in batch:
+                        r.start = tstart
+                    for r in reqs:
+                        r.batch = nbatch
+                    nbatch += 1
+                    in_use += nbytes
+                    busy[0] = (ev, h, reqs, nbytes + sum(p[2] for p in prepared), flag_sets)
+                    prepared = []
+                    launched = True
+                    continue
                 if busy[k] is not None or not queue:
                     continue
                 if k < nr and self.rnn_lane_delays is None:
@@ -248,7 +299,7 @@ class Server:
                     r.start, r.batch = tstart, nbatch
                 nbatch += 1
                 in_use += nbytes
-                busy[k] = (ev, h, batch, nbytes, flags)
+                busy[k] = (ev, h, batch, nbytes, self._flag_set(batch, flags))
                 launched = True
             if not launched:
                 # nothing to launch: wait for the next arrival, a batching deadline or a
@@ -261,11 +312,16 @@ class Server:
                     time.sleep(min(nxt, 5e-3))
         torch.cuda.synchronize()
         wall = clock() - t0
-        extra = {"batches": nbatch, "fallbacks": fallbacks, "overflows": overflows,
+        extra = {"batches": nbatch, "fallbacks": fallbacks, "overflows": overflows, "rnn_preps": preps_made,
                  "host_launch_s": host_launch, "host_finalize_s": host_finalize}
         if instrument:
             extra["timers"] = bd.timers
         return self._stats(requests, wall, extra)
+
+    @staticmethod
+    def _flag_set(batch: List[Request], flags) -> list:
+        """[(the batch's requests that were encoded, their overflow flags)], or []."""
+        return [] if flags is None else [([r for r in batch if r.encode is not None], flags)]
 
     @staticmethod
     def _encode(batch: List[Request], be: BatchEncoder, bd: BatchDecoder):
