@@ -37,75 +37,122 @@ template <> struct Mfma16<float> {
     }
 };
 
+// 1 / sqrt(d) for d > 0: the hardware estimate and Newton steps r <- r (3 - d r^2) / 2
+// (two for FP64, one for FP32), each error squared: full precision in fewer dependent
+// operations than a correctly rounded square root and a division (the diagonal factor's
+// column steps are a chain of these).
+__device__ __forceinline__ double chol_rsqrt(double d)
+{
+    double r = __builtin_amdgcn_rsq(d);
+    const double h = 0.5 * d;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) r = __builtin_fma(r, __builtin_fma(-h * r, r, 0.5), r);
+    return r;
+}
+__device__ __forceinline__ float chol_rsqrt(float d)
+{
+    float r = __builtin_amdgcn_rsqf(d);
+    const float h = 0.5f * d;
+    return __builtin_fmaf(r, __builtin_fmaf(-h * r, r, 0.5f), r);
+}
+
 // ------------------------------------------------------------------------------------
-// diag: factor + invert the 64x64 diagonal block, one wavefront per matrix.  Lane r keeps
-// row r in registers (static indices after unrolling).  At column step c every lane
-// publishes its A[r][c] to LDS, reads the pivot and the column back as broadcasts and
-// applies  A[r][rp] -= L[r][c] L[rp][c]  for c < rp <= r.  With a single wavefront the
-// workgroup barriers only order the LDS traffic.  The inverse
-// X = L^-1 is column-parallel: lane c runs the forward substitution of column c,
-//   X[k][c] = (delta_kc - sum_{j<k} L[k][j] X[j][c]) / L[k][k],
-// reading the rows of L as LDS broadcasts, with no communication between lanes.
+// diag: factor + invert the 64x64 diagonal block, one 256-thread workgroup per matrix.
+// Thread (i, q) = (tid / 4, tid % 4) keeps elements [16q, 16q + 16) of row i in registers.
+// Column step c: the owner of (i, c) publishes it to LDS (double-buffered, one barrier
+// per step); every thread reads the pivot and the column back and applies
+//   A[i][j] -= L[i][c] L[j][c]   for c < j <= i,   L[i][c] = A[i][c] / sqrt(A[c][c])
+// to its own elements (1/sqrt by chol_rsqrt, L[c][c] = A[c][c] / sqrt(A[c][c])).  The inverse X = L^-1 solves L X = I the same way, right-looking:
+// row c's owners publish X[c][:] = B[c][:] / L[c][c], then every row below subtracts
+// L[i][c] X[c][:], so each X[k][j] is (delta_kj - sum_{j'<k} L[k][j'] X[j'][j]) / L[k][k]
+// summed in j' order.  The column loop is unrolled 16 at a time (static register
+// indices), the 64 steps are not: a short instruction stream.
 // ------------------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
 potrf_diag_kernel(const VtMat<T> *__restrict__ mats, int kb, T *__restrict__ inv_ws,
                   long long inv_stride, int *__restrict__ info)
 {
+    __shared__ T col[2][NB];
     __shared__ T Ls[NB][NB + 1];
-    __shared__ T col[NB];
+    __shared__ T rdiag[NB];
     const VtMat<T> m = mats[blockIdx.x];
     if (kb >= m.D) return;
     const int nb = min(NB, m.D - kb);
-    const int r = threadIdx.x;
+    const int tid = threadIdx.x, i = tid >> 2, q = tid & 3, j0 = 16 * q;
     T *Ad = m.A + (long long)kb * m.lda + kb;
-    T Lr[NB];
+    T a[16];
 #pragma unroll
-    for (int i = 0; i < NB; ++i) {
+    for (int t = 0; t < 16; ++t) {
+        const int j = j0 + t;
         T v = (T)0;
-        if (r < nb && i <= r) v = Ad[(long long)r * m.lda + i];
-        if (r >= nb && i == r) v = (T)1;      // identity rows pad a partial last block
-        Lr[i] = v;
+        if (i < nb && j <= i) v = Ad[(long long)i * m.lda + j];
+        if (i >= nb && j == i) v = (T)1;      // identity rows pad a partial last block
+        a[t] = v;
     }
     int bad = 0;
+    for (int c0 = 0; c0 < NB; c0 += 16) {
+        const bool own = q == (c0 >> 4);     // this thread holds column c of its row
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        col[r] = Lr[c];                         // A[r][c] after the previous updates
-        __syncthreads();
-        T d = col[c];
-        if (!(d > (T)0)) {
-            if (c < nb && !bad) bad = kb + c + 1;
-            d = (T)1e-30;    // keep going without NaNs; info reports the failure
+        for (int cc = 0; cc < 16; ++cc) {
+            const int c = c0 + cc, bf = cc & 1;
+            if (own) col[bf][i] = a[cc];      // A[i][c] after the previous updates
+            __syncthreads();
+            T d = col[bf][c];
+            if (!(d > (T)0)) {
+                if (c < nb && !bad) bad = kb + c + 1;
+                d = (T)1e-30;    // keep going without NaNs; info reports the failure
+            }
+            const T inv = chol_rsqrt(d);
+            const T sd = d * inv;
+            const T lic = (i > c) ? col[bf][i] * inv : (T)0;   // L[i][c]
+            if (own && i >= c) a[cc] = (i == c) ? sd : lic;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int j = j0 + t;
+                if (j > c && j <= i) a[t] -= lic * (col[bf][j] * inv);
+            }
         }
-        const T sd = sqrt(d);
-        const T inv = (T)1 / sd;
-        const T lrc = (r > c) ? Lr[c] * inv : (T)0;   // L[r][c]
-        Lr[c] = (r == c) ? sd : ((r > c) ? lrc : Lr[c]);
-#pragma unroll
-        for (int rp = c + 1; rp < NB; ++rp)
-            if (rp <= r) Lr[rp] -= lrc * (col[rp] * inv);
-        __syncthreads();
     }
-    if (r == 0 && bad && info && info[blockIdx.x] == 0) info[blockIdx.x] = bad;
+    if (tid == 0 && bad && info && info[blockIdx.x] == 0) info[blockIdx.x] = bad;
 #pragma unroll
-    for (int i = 0; i < NB; ++i) Ls[r][i] = (i <= r) ? Lr[i] : (T)0;
+    for (int t = 0; t < 16; ++t) {
+        const int j = j0 + t;
+        Ls[i][j] = (j <= i) ? a[t] : (T)0;
+        if (i < nb && j <= i) Ad[(long long)i * m.lda + j] = a[t];
+        if (j == i) rdiag[i] = (T)1 / a[t];   // the inverse's pivots, off its dependent chain
+    }
     __syncthreads();
-    for (int e = r; e < NB * NB; e += 64) {
-        const int rr = e >> 6, cc = e & 63;
-        if (rr < nb && cc <= rr) Ad[(long long)rr * m.lda + cc] = Ls[rr][cc];
-    }
-    // inverse, column c = r (reuse Lr for X[:, c])
-    const int c = r;
+    // inverse: b = row i of B (the identity, then eliminated), X rows published through col
+    T b[16];
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        T v = (k == c) ? (T)1 : (T)0;
+    for (int t = 0; t < 16; ++t) b[t] = (j0 + t == i) ? (T)1 : (T)0;
+    for (int c0 = 0; c0 < NB; c0 += 16) {
 #pragma unroll
-        for (int j = 0; j < k; ++j) v -= Ls[k][j] * Lr[j];
-        Lr[k] = v * ((T)1 / Ls[k][k]);
+        for (int cc = 0; cc < 16; ++cc) {
+            const int c = c0 + cc, bf = cc & 1;
+            if (i == c) {
+                const T rinv = rdiag[c];
+#pragma unroll
+                for (int t = 0; t < 16; ++t) {
+                    b[t] = b[t] * rinv;       // X[c][j]
+                    col[bf][j0 + t] = b[t];
+                }
+            }
+            __syncthreads();
+            if (i > c) {
+                const T lic = Ls[i][c];
+#pragma unroll
+                for (int t = 0; t < 16; ++t) b[t] -= lic * col[bf][j0 + t];
+            }
+        }
     }
     T *W = inv_ws + (long long)blockIdx.x * inv_stride + (long long)(kb / NB) * NB * NB;
 #pragma unroll
-    for (int k = 0; k < NB; ++k) W[k * NB + c] = (k < nb && c < nb) ? Lr[k] : (T)0;
+    for (int t = 0; t < 16; ++t) {
+        const int j = j0 + t;
+        W[i * NB + j] = (i < nb && j < nb) ? b[t] : (T)0;
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -317,7 +364,7 @@ static int potrf_run(const void *mats_dev, int batch, int maxD, T *inv_ws, long 
     for (int K = 0; K < maxD; K += KW) {
         const int kend = min(K + KW, maxD);
         for (int kb = K; kb < kend; kb += NB) {
-            potrf_diag_kernel<T><<<batch, 64, 0, s>>>(mats, kb, inv_ws, inv_stride, info);
+            potrf_diag_kernel<T><<<batch, 256, 0, s>>>(mats, kb, inv_ws, inv_stride, info);
             const int rows = maxD - kb - NB;
             if (rows > 0) {
                 const int nch = (rows + NB - 1) / NB;
