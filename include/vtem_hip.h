@@ -431,10 +431,14 @@ int vt_kron_rnn(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_
                 unsigned *cnt, void *stream);
 /* Step sizes eta_b = 1/rho_b, rho_b the power-iteration estimate (iters steps from v,
  * overwritten) of the largest eigenvalue of A_b + lam_b I: the step rule eta = 1/lam_max
- * the configurations use in place of the reference's dt*alpha (vtem/vtdm.py:21). */
+ * the configurations use in place of the reference's dt*alpha (vtem/vtdm.py:21).  The
+ * step (w = A v + lam v, rho = <v,w>/<v,v>, v = w/|w|) is fused into the backward launch:
+ * bwd's descriptors in block order; cnt: (batch + 1) * ceil(bwd_M / 16) words (zeroed
+ * here); part: 3 * batch * ceil(bwd_M / 16) doubles. */
 int vt_kron_power(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                   int batch, int max_Nc, double *v, double *q, const double *lam,
-                  const long long *seg, int iters, double *eta, int graph_chunk, void *stream);
+                  const long long *seg, int iters, double *eta, int graph_chunk, unsigned *cnt,
+                  double *part, void *stream);
 
 /* The same steps+1 iterations in ONE persistent launch (vtem/neural_network.py:78-92, update
  * kernel :102-128: the loop the reference drives from the host, one launch per step).  Each

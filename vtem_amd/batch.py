@@ -1114,11 +1114,14 @@ class BatchDecoder:
         Z = torch.empty(int(nzj[-1]), dtype=F64, device=dev)
         fv, bv = self._operator(blocks, V, Q, Z, dbj, nzj)
         eta = torch.empty(Bt, dtype=F64, device=dev)
+        ntile = -(-int(Ds.max()) // 16)
+        cnt = torch.empty((Bt + 1) * ntile, dtype=torch.int32, device=dev)
+        part = torch.empty(3 * Bt * ntile, dtype=F64, device=dev)
         lib.vt_kron_power(*fv.args(), *bv.args(), Bt, int(Dt.max()), ptr(V), ptr(Q), ptr(lam), ptr(seg),
-                          POWER_ITERS, ptr(eta), self.graph_chunk, s)
+                          POWER_ITERS, ptr(eta), self.graph_chunk, ptr(cnt), ptr(part), s)
         nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + 4.0 * D)
         self._toc("rnn_power", e0, float(nbytes) * POWER_ITERS)
-        return eta, [lam, seg, V, Q, Z, fv, bv, eta]
+        return eta, [lam, seg, V, Q, Z, fv, bv, eta, cnt, part]
 
     def _rnn_iterate(self, gs: List[_Prologue], eta, steps: int):
         """Gradient-flow decoder for every block of the groups, as one joint launch

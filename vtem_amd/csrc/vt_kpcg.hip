@@ -704,12 +704,6 @@ static int rnn_bwd_update(const VtSkinnyLaunch &Bk, int batch, int max_Nc, doubl
     return VT_OK;
 }
 
-__global__ void zero_words_kernel(unsigned *__restrict__ w, long long n)
-{
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-        w[i] = 0u;
-}
 
 VT_EXPORT int vt_power_step(double *v, const double *q, const double *lam, const long long *seg,
                             int batch, double *eta, void *stream)
@@ -720,25 +714,187 @@ VT_EXPORT int vt_power_step(double *v, const double *q, const double *lam, const
     return VT_OK;
 }
 
-// eta (batch doubles) from `iters` power steps started at v (overwritten); the operator
-// launches read v and write q.  graph_chunk >= 2: the steps are replayed as a HIP graph of
-// that many steps (vt_graph_loop); 0: launched one by one.  Same results either way.
+__global__ void zero_words_kernel(unsigned *__restrict__ w, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        w[i] = 0u;
+}
+
+// Backward product of a power step with the step itself fused: problem b is S_b^T Z_b,
+// this workgroup its row tile and K split.  The last of a row tile's splits to finish (as
+// in the RNN launch) adds the partials in split order, forms w = q + lam_b v for its rows
+// (stored in q's place) and the tile's sums <v,w>, <v,v>, <w,w> (a fixed-order reduction
+// over its threads).  The last row tile of the block to finish (a counter per block) adds
+// the tiles' sums in tile order and completes the step for the whole block:
+// v = w / |w|, eta_b = <v,v> / <v,w>.  The step of power_step_kernel with one launch and
+// the reduce pass fewer per power step (its sums in another fixed order).
+__device__ __forceinline__ double power_sum_wg(double v, double *red)
+{
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 32 * SK_WAVES; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+template <int NT, bool X16>
+__device__ __forceinline__ void power_bwd_tile(const SkDesc &d, int m0, int z, int b, int tile, int ntile,
+                                               double *__restrict__ v, double l, long long s0, long long s1,
+                                               unsigned *tcnt, unsigned *bcnt, double *part,
+                                               double *__restrict__ eta, double (&red)[SK_WAVES / 2][16][33],
+                                               double *red1)
+{
+    __shared__ int last;
+    __shared__ double inv_s;
+    const int ks = skinny_splits(d.K), Nc = d.Nc;
+    skinny_tile<NT, 1, X16, true>(d, m0, z, red);
+    __builtin_amdgcn_s_waitcnt(0);                // this wave's partial or q stores have completed
+    __syncthreads();
+    if (ks > 1) {
+        if (threadIdx.x == 0) {
+            const unsigned o = __hip_atomic_fetch_add(tcnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            last = o == (unsigned)ks - 1;
+            if (last) __hip_atomic_store(tcnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        if (!last) return;
+    }
+    const int rows = min(16, d.M - m0);
+    const long long mn = (long long)d.M * Nc;
+    double vw = 0.0, vv = 0.0, ww = 0.0;
+    for (int e = threadIdx.x; e < rows * Nc; e += 64 * SK_WAVES) {
+        const long long loc = (long long)(m0 + e / Nc) * Nc + e % Nc;
+        double qi;
+        if (ks > 1) {
+            qi = 0.0;
+            for (int zz = 0; zz < ks; ++zz)
+                qi += __hip_atomic_load(d.ws + zz * mn + loc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            qi = __hip_atomic_load(d.C + loc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const double vi = v[s0 + loc];
+        const double wi = qi + l * vi;
+        __hip_atomic_store(d.C + loc, wi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        vw += vi * wi;
+        vv += vi * vi;
+        ww += wi * wi;
+    }
+    vw = power_sum_wg(vw, red1);
+    vv = power_sum_wg(vv, red1);
+    ww = power_sum_wg(ww, red1);
+    if (threadIdx.x == 0) {
+        double *pt = part + 3 * ((long long)b * ntile + tile);
+        __hip_atomic_store(pt, vw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(pt + 1, vv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(pt + 2, ww, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __builtin_amdgcn_s_waitcnt(0);                // the w and partial-sum stores have completed
+    __syncthreads();
+    const int nt = (d.M + 15) / 16;
+    if (threadIdx.x == 0) {
+        const unsigned o = __hip_atomic_fetch_add(bcnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = o == (unsigned)nt - 1;
+        if (last) {
+            __hip_atomic_store(bcnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            double a = 0.0, c = 0.0, e2 = 0.0;
+            const double *pb = part + 3 * (long long)b * ntile;
+            for (int t = 0; t < nt; ++t) {
+                a += __hip_atomic_load(pb + 3 * t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                c += __hip_atomic_load(pb + 3 * t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                e2 += __hip_atomic_load(pb + 3 * t + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            inv_s = e2 > 0.0 ? 1.0 / sqrt(e2) : 0.0;
+            eta[b] = (a > 0.0 && c > 0.0) ? c / a : 0.0;
+        }
+    }
+    __syncthreads();
+    if (!last) return;
+    const double inv = inv_s;                     // the block's w is at d.C = q + s0
+    for (long long i = s0 + threadIdx.x; i < s1; i += 64 * SK_WAVES)
+        v[i] = __hip_atomic_load(d.C + (i - s0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * inv;
+}
+
+// grid (row tiles, batch, max splits); cnt: row-tile counters (batch * row tiles), then one
+// counter per block; part: 3 doubles per block and row tile
+template <int NT, bool X16>
+__global__ void __launch_bounds__(64 * SK_WAVES)
+power_bwd_kernel(const SkDesc *__restrict__ descs, double *__restrict__ v, const double *__restrict__ lam,
+                 const long long *__restrict__ seg, unsigned *__restrict__ cnt, double *__restrict__ part,
+                 double *__restrict__ eta)
+{
+    __shared__ double red[SK_WAVES / 2][16][33];
+    __shared__ double red1[64 * SK_WAVES];
+    const int b = blockIdx.y;
+    const SkDesc d = descs[b];
+    const int m0 = blockIdx.x * 16;
+    if (m0 >= d.M || (int)blockIdx.z >= skinny_splits(d.K)) return;
+    const int ntile = gridDim.x;
+    unsigned *tc = cnt + (long long)b * ntile + blockIdx.x;
+    unsigned *bc = cnt + (long long)gridDim.y * ntile + b;
+    const double l = lam[2 * b];
+    const long long s0 = seg[b], s1 = seg[b + 1];
+    const int z = blockIdx.z, t = blockIdx.x;
+    if constexpr (NT == 1) {
+        if (X16 && d.Nc == 17)
+            power_bwd_tile<1, true>(d, m0, z, b, t, ntile, v, l, s0, s1, tc, bc, part, eta, red, red1);
+        else
+            power_bwd_tile<1, false>(d, m0, z, b, t, ntile, v, l, s0, s1, tc, bc, part, eta, red, red1);
+    } else {
+        if (d.Nc == 17)
+            power_bwd_tile<1, true>(d, m0, z, b, t, ntile, v, l, s0, s1, tc, bc, part, eta, red, red1);
+        else if (d.Nc <= 16)
+            power_bwd_tile<1, false>(d, m0, z, b, t, ntile, v, l, s0, s1, tc, bc, part, eta, red, red1);
+        else
+            power_bwd_tile<2, false>(d, m0, z, b, t, ntile, v, l, s0, s1, tc, bc, part, eta, red, red1);
+    }
+}
+
+static int power_bwd(const VtSkinnyLaunch &Bk, int batch, int max_Nc, double *v, const double *lam,
+                     const long long *seg, unsigned *cnt, double *part, double *eta, hipStream_t s)
+{
+    const dim3 grid(vt_blocks(Bk.max_M, 16), batch, skinny_splits(Bk.max_K));
+    const SkDesc *d = (const SkDesc *)Bk.descs;
+    if (max_Nc <= 16)
+        power_bwd_kernel<1, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, lam, seg, cnt, part, eta);
+    else if (max_Nc == 17)
+        power_bwd_kernel<1, true><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, lam, seg, cnt, part, eta);
+    else
+        power_bwd_kernel<2, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, lam, seg, cnt, part, eta);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// eta (batch doubles) from `iters` power steps started at v (overwritten): per step the
+// forward operator launch (reads v) and the backward launch with the step fused (writes q
+// and v).  bwd's descriptors in block order; cnt: (batch + 1) * ceil(bwd_M / 16) words of
+// counters (zeroed here), part: 3 * batch * ceil(bwd_M / 16) doubles.  graph_chunk >= 2:
+// the steps are replayed as a HIP graph of that many steps (vt_graph_loop); 0: launched
+// one by one.  Same results either way.
 VT_EXPORT int vt_kron_power(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                             int batch, int max_Nc, double *v, double *q, const double *lam,
                             const long long *seg, int iters, double *eta, int graph_chunk,
-                            void *stream)
+                            unsigned *cnt, double *part, void *stream)
 {
-    VT_CHECK_ARG(fwd && bwd && v && q && lam && seg && eta && batch >= 1 && iters >= 1 &&
-                 graph_chunk >= 0);
+    VT_CHECK_ARG(fwd && bwd && v && q && lam && seg && eta && cnt && part && batch >= 1 &&
+                 batch <= 65535 && iters >= 1 && graph_chunk >= 0 && bwd_M >= 1 && max_Nc >= 1 &&
+                 max_Nc <= 32);
     const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
+    hipStream_t s = (hipStream_t)stream;
+    const long long words = (long long)(batch + 1) * vt_blocks(bwd_M, 16);
+    const unsigned gw = vt_blocks(words, 256);
+    zero_words_kernel<<<gw < 1024 ? gw : 1024, 256, 0, s>>>(cnt, words);
+    VT_LAUNCH_CHECK();
     auto step = [&](hipStream_t st) -> int {
         VT_TRY(skinny_run(F, batch, max_Nc, st));
-        VT_TRY(skinny_run(Bk, batch, max_Nc, st));
-        power_step_kernel<<<batch, 256, 0, st>>>(v, q, lam, seg, eta);
-        VT_LAUNCH_CHECK();
+        VT_TRY(power_bwd(Bk, batch, max_Nc, v, lam, seg, cnt, part, eta, st));
         return VT_OK;
     };
-    return vt_graph_loop((hipStream_t)stream, iters, graph_chunk, step);
+    return vt_graph_loop(s, iters, graph_chunk, step);
 }
 
 // steps+1 gradient-flow iterations from x = 0: per iteration the forward operator launch
