@@ -59,7 +59,7 @@ PEAKS = {"gram_lp": (2500.0, "TFLOP/s", "mfma"), "gram_f64": (78.6, "TFLOP/s", "
          "kpcg_mgram": (78.6, "TFLOP/s", "mfma"), "kpcg_potrf": (78.6, "TFLOP/s", "mfma"),
          "kpcg_inverse": (78.6, "TFLOP/s", "mfma"), "prologue": (8000.0, "GB/s", "hbm"),
          "synthesis": (78.6, "TFLOP/s", "mfma"), "rnn_iterate": (8000.0, "GB/s", "hbm"),
-         "rnn_power": (8000.0, "GB/s", "hbm"),
+         "rnn_lanczos": (8000.0, "GB/s", "hbm"),
          "kpcg_iterate": (8000.0, "GB/s", "hbm"), "encode_coeffs": (8000.0, "GB/s", "hbm"),
          "iaf": (78.6, "TFLOP/s", "fp64")}
 WORK_NOTES = {"encode_coeffs": "algorithmic HBM bytes 8*(F*P + N*P + 2*(P+N)*Dt) per block (frames, "

@@ -429,17 +429,6 @@ int vt_kron_rnn(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_
                 int batch, int max_Nc, double *x, const double *r, double *q, const double *lam,
                 const double *eta, const long long *seg, long long n, int steps, int graph_chunk,
                 unsigned *cnt, void *stream);
-/* Step sizes eta_b = 1/rho_b, rho_b the power-iteration estimate (iters steps from v,
- * overwritten) of the largest eigenvalue of A_b + lam_b I: the step rule eta = 1/lam_max
- * the configurations use in place of the reference's dt*alpha (vtem/vtdm.py:21).  The
- * step (w = A v + lam v, rho = <v,w>/<v,v>, v = w/|w|) is fused into the backward launch:
- * bwd's descriptors in block order; cnt: (batch + 1) * ceil(bwd_M / 16) words (zeroed
- * here); part: 3 * batch * ceil(bwd_M / 16) doubles. */
-int vt_kron_power(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
-                  int batch, int max_Nc, double *v, double *q, const double *lam,
-                  const long long *seg, int iters, double *eta, int graph_chunk, unsigned *cnt,
-                  double *part, void *stream);
-
 /* The same steps+1 iterations in ONE persistent launch (vtem/neural_network.py:78-92, update
  * kernel :102-128: the loop the reference drives from the host, one launch per step).  Each
  * block owns the workgroups [gstart[b], gstart[b+1]) (device array, batch + 1 ints, grid =
@@ -467,9 +456,20 @@ int vt_kron_rnn_persistent_capacity(int *max_wgs, int *max_group);
 /* 32-bit words of the persistent launch's barrier workspace for a batch. */
 int vt_kron_rnn_persistent_bar_words(int batch, long long *words);
 
-/* One power step per block (q = A v given): w = q + lam v, eta_b = <v,v>/<v,w>, v = w/|w|. */
-int vt_power_step(double *v, const double *q, const double *lam, const long long *seg, int batch,
-                  double *eta, void *stream);
+/* Step sizes eta_b = 1/theta_b, theta_b the largest Ritz value of A_b + lam_b I after
+ * `steps` Lanczos steps from the normalised all-ones vector (v, vp: two work vectors of
+ * the joint length; q: the backward's output): u = (A + lam I) v_j, alpha_j = <u, v_j>,
+ * w = u - alpha_j v_j - beta_{j-1} v_{j-1}, beta_j = |w|, v_{j+1} = w / beta_j, fused into
+ * the backward launch (two launches per step), then theta_b by bisection on the Sturm
+ * count of the tridiagonal.  The reference steps its ODE by dt*alpha (vtem/vtdm.py:21,
+ * vtem/neural_network.py:78-92); 1/lambda_max is the configurations' rule.  bwd's
+ * descriptors in block order; cnt: batch * ceil(bwd_M / 16) + 2 batch words (zeroed
+ * here); part: batch * ceil(bwd_M / 16) doubles; tri: 2 * steps * batch doubles. */
+int vt_kron_lanczos(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
+                    int batch, int max_Nc, double *v, double *vp, double *q, const double *lam,
+                    const long long *seg, int steps, double *eta, int graph_chunk, unsigned *cnt,
+                    double *part, double *tri, void *stream);
+
 
 /* ------------------------------------------ spatial Dirichlet side, pseudoinverse -- */
 /* Mode table (P x 2M, row stride ld >= 2M, M = (2Mx+1)(2My+1)): row q*Px + p holds

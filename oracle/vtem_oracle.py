@@ -558,6 +558,33 @@ def rnn_dspace(A: np.ndarray, r: np.ndarray, eta: float, steps: int, lam: float 
     return X
 
 
+def lanczos_eta(A: np.ndarray, lam: float, steps: int = 16) -> float:
+    """The RNN step eta = 1/theta, theta the largest Ritz value of A + lam I after `steps`
+    Lanczos steps from the normalised all-ones vector (no reorthogonalisation; a zero
+    beta ends the recurrence).  The reference steps its ODE by dt*alpha
+    (vtem/neural_network.py:78-92); the step 1/lambda_max is this build's rule for the
+    configurations, and the GPU path computes it the same way."""
+    n = A.shape[0]
+    v = np.ones(n) / np.sqrt(n)
+    vp = np.zeros(n)
+    bprev = 0.0
+    al, be = [], []
+    for _ in range(steps):
+        u = A @ v + lam * v
+        a = float(u @ v)
+        w = u - a * v - bprev * vp
+        b = float(np.sqrt(w @ w))
+        if not (b > 0.0 and np.isfinite(b)):
+            b = 0.0
+        al.append(a)
+        be.append(b)
+        vp, v = v, (w / b if b > 0.0 else np.zeros(n))
+        bprev = b
+    T = np.diag(al) + np.diag(be[:-1], 1) + np.diag(be[:-1], -1)
+    th = float(np.linalg.eigvalsh(T)[-1])
+    return 1.0 / th if th > 0.0 else 0.0
+
+
 def synthesize(X: np.ndarray, Bs: np.ndarray, Theta: np.ndarray) -> np.ndarray:
     """u[t, q, p] = sum_{ps, pt} X[ps, pt] Bs[q, p, ps] Theta[t, pt]."""
     Py, Px, Ds = Bs.shape

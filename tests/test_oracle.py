@@ -465,3 +465,22 @@ def test_select_neurons_strict_bounds(oracle):
     assert ch.tolist() == [1, 4, 5, 7]
     np.testing.assert_array_equal(xs, x0[ch] - (-4.0))
     np.testing.assert_array_equal(ys, y0[ch] - 4.0)
+
+
+def test_lanczos_eta_pinned(oracle):
+    """The RNN step from Lanczos: exact 1/lambda_max (to rounding) once the steps reach the
+    order of a small matrix; on a larger SPD matrix with a clear gap, 16 steps give
+    eta lambda_max in [1, 1 + 1e-6] (a Ritz value never exceeds lambda_max); a rank-one
+    matrix stops the recurrence after one step (beta = 0) with the exact answer."""
+    rng = np.random.default_rng(11)
+    G = rng.standard_normal((12, 12))
+    A = G @ G.T
+    lmax = np.linalg.eigvalsh(A + 0.3 * np.eye(12))[-1]
+    assert abs(oracle.lanczos_eta(A, 0.3, steps=12) * lmax - 1.0) < 1e-10
+    Q, _ = np.linalg.qr(rng.standard_normal((400, 400)))
+    ev = np.concatenate([[10.0, 6.0], rng.uniform(0.01, 5.0, 398)])
+    A = (Q * ev) @ Q.T
+    x = oracle.lanczos_eta(A, 0.0, steps=16) * 10.0
+    assert 1.0 - 1e-12 <= x <= 1.0 + 1e-6
+    u = np.ones(50) / np.sqrt(50)
+    assert abs(oracle.lanczos_eta(4.0 * np.outer(u, u), 0.0, steps=5) * 4.0 - 1.0) < 1e-12

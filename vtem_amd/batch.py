@@ -29,7 +29,7 @@ import torch
 
 from ._lib import VtemError, lib, ptr, stream_ptr
 from .autotune import gram_tuner
-from .block import (F64, I32, MAT_DTYPE, Decoder, DecodeResult, SpikeTrains, mat_descs,
+from .block import (F64, I32, LANCZOS_STEPS, MAT_DTYPE, Decoder, DecodeResult, SpikeTrains, mat_descs,
                     split_T_size)
 
 SPIKE_DTYPE = np.dtype([("spikes", np.uint64), ("counts", np.uint64), ("ld", np.int64),
@@ -213,7 +213,6 @@ class RnnPrep:
 
 
 PRECISIONS = ("fp64", "mixed", "bf16x3", "fp16", "kpcg")
-POWER_ITERS = 50          # power steps for the RNN step size 1/lambda_max
 
 
 class BatchDecoder:
@@ -1096,8 +1095,9 @@ class BatchDecoder:
         return blocks, boff, N, Ds, Dt
 
     def _rnn_power(self, gs: List[_Prologue]):
-        """Step sizes eta_b = 1/lambda_max(A_b + lam_b I) of the groups' blocks (joint order)
-        by 50 power steps through the structured operator.  Returns (eta, keep)."""
+        """Step sizes eta_b = 1/theta_b of the groups' blocks (joint order), theta_b the
+        largest Ritz value of A_b + lam_b I after LANCZOS_STEPS Lanczos steps through the
+        structured operator (vt_kron_lanczos).  Returns (eta, keep)."""
         dev = self.device
         s = stream_ptr()
         e0 = self._tic()
@@ -1109,19 +1109,21 @@ class BatchDecoder:
         Dtot = int(dbj[-1])
         lam = torch.cat([g.lam for g in gs]) if len(gs) > 1 else gs[0].lam
         seg = _h2d(dbj.astype(np.int64), dev)
-        V = torch.ones(Dtot, dtype=F64, device=dev)
+        V = torch.empty(Dtot, dtype=F64, device=dev)
+        Vp = torch.empty(Dtot, dtype=F64, device=dev)
         Q = torch.empty(Dtot, dtype=F64, device=dev)
         Z = torch.empty(int(nzj[-1]), dtype=F64, device=dev)
         fv, bv = self._operator(blocks, V, Q, Z, dbj, nzj)
         eta = torch.empty(Bt, dtype=F64, device=dev)
         ntile = -(-int(Ds.max()) // 16)
-        cnt = torch.empty((Bt + 1) * ntile, dtype=torch.int32, device=dev)
-        part = torch.empty(3 * Bt * ntile, dtype=F64, device=dev)
-        lib.vt_kron_power(*fv.args(), *bv.args(), Bt, int(Dt.max()), ptr(V), ptr(Q), ptr(lam), ptr(seg),
-                          POWER_ITERS, ptr(eta), self.graph_chunk, ptr(cnt), ptr(part), s)
-        nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + 4.0 * D)
-        self._toc("rnn_power", e0, float(nbytes) * POWER_ITERS)
-        return eta, [lam, seg, V, Q, Z, fv, bv, eta, cnt, part]
+        cnt = torch.empty(Bt * ntile + 2 * Bt, dtype=torch.int32, device=dev)
+        part = torch.empty(Bt * ntile, dtype=F64, device=dev)
+        tri = torch.empty(2 * LANCZOS_STEPS * Bt, dtype=F64, device=dev)
+        lib.vt_kron_lanczos(*fv.args(), *bv.args(), Bt, int(Dt.max()), ptr(V), ptr(Vp), ptr(Q), ptr(lam),
+                            ptr(seg), LANCZOS_STEPS, ptr(eta), self.graph_chunk, ptr(cnt), ptr(part), ptr(tri), s)
+        nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + 6.0 * D)
+        self._toc("rnn_lanczos", e0, float(nbytes) * LANCZOS_STEPS)
+        return eta, [lam, seg, V, Vp, Q, Z, fv, bv, eta, cnt, part, tri]
 
     def _rnn_iterate(self, gs: List[_Prologue], eta, steps: int):
         """Gradient-flow decoder for every block of the groups, as one joint launch

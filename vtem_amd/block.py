@@ -32,6 +32,8 @@ from .autotune import gram_tuner
 from .basis import Geometry
 from .synthetic import Bank, IAFParams
 
+LANCZOS_STEPS = 16    # Lanczos steps for the RNN step size 1/lambda_max
+
 F64 = torch.float64
 I32 = torch.int32
 
@@ -456,18 +458,32 @@ class Decoder:
         self._keep_solve = (A32, inv, mats, res, AX)
         return X, lam, info, nrm
 
-    def power_eta(self, Tm, lam, iters: int = 50) -> float:
-        """1/rho, rho the power-iteration estimate (from the all-ones vector) of the largest
-        eigenvalue of A + lam I: the RNN step 1/lambda_max."""
+    def power_eta(self, Tm, lam, steps: int = LANCZOS_STEPS) -> float:
+        """The RNN step 1/theta, theta the largest Ritz value of A + lam I after `steps`
+        Lanczos steps from the normalised all-ones vector (u = (A + lam I) v_j,
+        alpha_j = <u, v_j>, w = u - alpha_j v_j - beta_{j-1} v_{j-1}, beta_j = |w|,
+        v_{j+1} = w / beta_j; a zero beta ends the recurrence), the batched decoder's rule."""
         g = self.geom
-        v = torch.ones(g.D, dtype=F64, device=self.device)
+        v = torch.full((g.D,), 1.0 / math.sqrt(g.D), dtype=F64, device=self.device)
+        vp = torch.zeros_like(v)
         q = torch.empty_like(v)
-        seg = torch.tensor([0, g.D], dtype=torch.int64, device=self.device)
-        eta = torch.empty(1, dtype=F64, device=self.device)
-        for _ in range(iters):
+        ab = torch.zeros((steps, 2), dtype=F64, device=self.device)
+        bprev = torch.zeros((), dtype=F64, device=self.device)
+        for j in range(steps):
             self.apply_A(Tm, v, out=q)
-            lib.vt_power_step(ptr(v), ptr(q), ptr(lam), ptr(seg), 1, ptr(eta), stream_ptr())
-        return float(eta.item())
+            u = q + lam[0] * v
+            a = torch.dot(u, v)
+            w = u - a * v - bprev * vp
+            b = torch.linalg.vector_norm(w)
+            b = torch.where(torch.isfinite(b) & (b > 0), b, torch.zeros_like(b))
+            ab[j, 0], ab[j, 1] = a, b
+            vp = v
+            v = torch.where(b > 0, w / torch.where(b > 0, b, torch.ones_like(b)), torch.zeros_like(w))
+            bprev = b
+        h = ab.cpu().numpy()
+        T = np.diag(h[:, 0]) + np.diag(h[:-1, 1], 1) + np.diag(h[:-1, 1], -1)
+        th = float(np.linalg.eigvalsh(T)[-1])
+        return 1.0 / th if th > 0.0 else 0.0
 
     def solve_rnn(self, Tm, r, lam, steps: int, eta: float):
         """X <- X + eta (r - (A + lam I) X), steps+1 iterations from 0 (the D-space image of

@@ -579,32 +579,9 @@ VT_EXPORT int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void 
 
 // ------------------------------------------------------------------------------------
 // Gradient-flow (RNN) decoder in coefficient space: X <- X + eta_b (r - (A + lam I) X),
-// the image under u = Phi^T c of the reference's c <- c + eta (q - G c).  eta_b = 1/rho_b
-// with rho_b the power-iteration estimate of the largest eigenvalue of A + lam I.
+// the image under u = Phi^T c of the reference's c <- c + eta (q - G c).  eta_b = 1/theta_b
+// with theta_b the Lanczos estimate of the largest eigenvalue of A + lam I.
 // ------------------------------------------------------------------------------------
-// power step per block: w = q + lam v; rho = <v,w>/<v,v>; v = w/|w|; eta = 1/rho
-__global__ void __launch_bounds__(256)
-power_step_kernel(double *__restrict__ v, const double *__restrict__ q, const double *__restrict__ lam,
-                  const long long *__restrict__ seg, double *__restrict__ eta)
-{
-    __shared__ double red[256];
-    const double l = lam[2 * blockIdx.x];
-    const long long a = seg[blockIdx.x], e = seg[blockIdx.x + 1];
-    double vw = 0.0, vv = 0.0, ww = 0.0;
-    for (long long i = a + threadIdx.x; i < e; i += 256) {
-        const double w = q[i] + l * v[i];
-        vw += v[i] * w;
-        vv += v[i] * v[i];
-        ww += w * w;
-    }
-    vw = block_sum(vw, red);
-    vv = block_sum(vv, red);
-    ww = block_sum(ww, red);
-    const double inv = ww > 0.0 ? 1.0 / sqrt(ww) : 0.0;
-    for (long long i = a + threadIdx.x; i < e; i += 256) v[i] = (q[i] + l * v[i]) * inv;
-    if (threadIdx.x == 0) eta[blockIdx.x] = (vw > 0.0 && vv > 0.0) ? vv / vw : 0.0;
-}
-
 __global__ void zero_kernel(double *__restrict__ x, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -705,14 +682,6 @@ static int rnn_bwd_update(const VtSkinnyLaunch &Bk, int batch, int max_Nc, doubl
 }
 
 
-VT_EXPORT int vt_power_step(double *v, const double *q, const double *lam, const long long *seg,
-                            int batch, double *eta, void *stream)
-{
-    VT_CHECK_ARG(v && q && lam && seg && eta && batch >= 1 && batch <= 65535);
-    power_step_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(v, q, lam, seg, eta);
-    VT_LAUNCH_CHECK();
-    return VT_OK;
-}
 
 __global__ void zero_words_kernel(unsigned *__restrict__ w, long long n)
 {
@@ -721,15 +690,8 @@ __global__ void zero_words_kernel(unsigned *__restrict__ w, long long n)
         w[i] = 0u;
 }
 
-// Backward product of a power step with the step itself fused: problem b is S_b^T Z_b,
-// this workgroup its row tile and K split.  The last of a row tile's splits to finish (as
-// in the RNN launch) adds the partials in split order, forms w = q + lam_b v for its rows
-// (stored in q's place) and the tile's sums <v,w>, <v,v>, <w,w> (a fixed-order reduction
-// over its threads).  The last row tile of the block to finish (a counter per block) adds
-// the tiles' sums in tile order and completes the step for the whole block:
-// v = w / |w|, eta_b = <v,v> / <v,w>.  The step of power_step_kernel with one launch and
-// the reduce pass fewer per power step (its sums in another fixed order).
-__device__ __forceinline__ double power_sum_wg(double v, double *red)
+// fixed-order sum over the 64 * SK_WAVES threads of a workgroup (red: as many doubles)
+__device__ __forceinline__ double wg_sum(double v, double *red)
 {
     red[threadIdx.x] = v;
     __syncthreads();
@@ -742,15 +704,27 @@ __device__ __forceinline__ double power_sum_wg(double v, double *red)
     return r;
 }
 
+// Lanczos estimate of lambda_max(A_b + lam_b I), the RNN step eta_b = 1/lambda_max.  Per
+// step j, v_j of unit norm (v_0 = ones / sqrt(D_b)):
+//   u = (A + lam I) v_j,  alpha_j = <u, v_j>,  w = u - alpha_j v_j - beta_{j-1} v_{j-1},
+//   beta_j = |w|,  v_{j+1} = w / beta_j,
+// and eta_b = 1/theta with theta the largest eigenvalue of the tridiagonal matrix of the
+// alphas and betas.  Fused into the backward launch as the RNN step is: the last split of
+// a row tile forms u for its rows (in q's place) and the tile's sum <u, v>; the last row
+// tile of the block adds the tiles' sums in tile order (alpha), then makes two passes over
+// the block: |w|^2 (a fixed-order reduction), then v_{j-1} <- v_j, v_j <- w / beta_j.  A
+// zero or non-finite beta (an invariant subspace) stops the block's recurrence (v = 0).
+// The step index comes from each block's own counter, so the steps are identical launches
+// (graph replay).
 template <int NT, bool X16>
-__device__ __forceinline__ void power_bwd_tile(const SkDesc &d, int m0, int z, int b, int tile, int ntile,
-                                               double *__restrict__ v, double l, long long s0, long long s1,
-                                               unsigned *tcnt, unsigned *bcnt, double *part,
-                                               double *__restrict__ eta, double (&red)[SK_WAVES / 2][16][33],
-                                               double *red1)
+__device__ __forceinline__ void lanczos_bwd_tile(const SkDesc &d, int m0, int z, int b, int tile, int ntile,
+                                                 double *__restrict__ v, double *__restrict__ vp, double l,
+                                                 long long s0, long long s1, unsigned *tcnt, unsigned *bcnt,
+                                                 unsigned *jstep, double *part, double *__restrict__ tri,
+                                                 int tri_ld, double (&red)[SK_WAVES / 2][16][33], double *red1)
 {
     __shared__ int last;
-    __shared__ double inv_s;
+    __shared__ double ab[3];
     const int ks = skinny_splits(d.K), Nc = d.Nc;
     skinny_tile<NT, 1, X16, true>(d, m0, z, red);
     __builtin_amdgcn_s_waitcnt(0);                // this wave's partial or q stores have completed
@@ -766,7 +740,7 @@ __device__ __forceinline__ void power_bwd_tile(const SkDesc &d, int m0, int z, i
     }
     const int rows = min(16, d.M - m0);
     const long long mn = (long long)d.M * Nc;
-    double vw = 0.0, vv = 0.0, ww = 0.0;
+    double uv = 0.0;
     for (int e = threadIdx.x; e < rows * Nc; e += 64 * SK_WAVES) {
         const long long loc = (long long)(m0 + e / Nc) * Nc + e % Nc;
         double qi;
@@ -778,22 +752,14 @@ __device__ __forceinline__ void power_bwd_tile(const SkDesc &d, int m0, int z, i
             qi = __hip_atomic_load(d.C + loc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         const double vi = v[s0 + loc];
-        const double wi = qi + l * vi;
-        __hip_atomic_store(d.C + loc, wi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        vw += vi * wi;
-        vv += vi * vi;
-        ww += wi * wi;
+        const double ui = qi + l * vi;
+        __hip_atomic_store(d.C + loc, ui, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uv += ui * vi;
     }
-    vw = power_sum_wg(vw, red1);
-    vv = power_sum_wg(vv, red1);
-    ww = power_sum_wg(ww, red1);
-    if (threadIdx.x == 0) {
-        double *pt = part + 3 * ((long long)b * ntile + tile);
-        __hip_atomic_store(pt, vw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(pt + 1, vv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(pt + 2, ww, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __builtin_amdgcn_s_waitcnt(0);                // the w and partial-sum stores have completed
+    uv = wg_sum(uv, red1);
+    if (threadIdx.x == 0)
+        __hip_atomic_store(part + (long long)b * ntile + tile, uv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);                // the u and partial-sum stores have completed
     __syncthreads();
     const int nt = (d.M + 15) / 16;
     if (threadIdx.x == 0) {
@@ -801,31 +767,50 @@ __device__ __forceinline__ void power_bwd_tile(const SkDesc &d, int m0, int z, i
         last = o == (unsigned)nt - 1;
         if (last) {
             __hip_atomic_store(bcnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            double a = 0.0, c = 0.0, e2 = 0.0;
-            const double *pb = part + 3 * (long long)b * ntile;
-            for (int t = 0; t < nt; ++t) {
-                a += __hip_atomic_load(pb + 3 * t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                c += __hip_atomic_load(pb + 3 * t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                e2 += __hip_atomic_load(pb + 3 * t + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            inv_s = e2 > 0.0 ? 1.0 / sqrt(e2) : 0.0;
-            eta[b] = (a > 0.0 && c > 0.0) ? c / a : 0.0;
+            double a = 0.0;
+            for (int t = 0; t < nt; ++t)
+                a += __hip_atomic_load(part + (long long)b * ntile + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned j = *jstep;
+            ab[0] = a;
+            ab[1] = j > 0 ? tri[(long long)b * tri_ld + 2 * (j - 1) + 1] : 0.0;
         }
     }
     __syncthreads();
     if (!last) return;
-    const double inv = inv_s;                     // the block's w is at d.C = q + s0
-    for (long long i = s0 + threadIdx.x; i < s1; i += 64 * SK_WAVES)
-        v[i] = __hip_atomic_load(d.C + (i - s0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) * inv;
+    const double alpha = ab[0], bprev = ab[1];
+    const double *u = d.C;                        // the block's u, at q + s0
+    double ww = 0.0;
+    for (long long i = s0 + threadIdx.x; i < s1; i += 64 * SK_WAVES) {
+        const double w = __hip_atomic_load(u + (i - s0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+                         alpha * v[i] - bprev * vp[i];
+        ww += w * w;
+    }
+    ww = wg_sum(ww, red1);
+    double beta = sqrt(ww);
+    if (!(beta > 0.0) || !(beta < INFINITY)) beta = 0.0;
+    const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
+    for (long long i = s0 + threadIdx.x; i < s1; i += 64 * SK_WAVES) {
+        const double vi = v[i];
+        const double w = __hip_atomic_load(u + (i - s0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+                         alpha * vi - bprev * vp[i];
+        vp[i] = vi;
+        v[i] = w * inv;
+    }
+    if (threadIdx.x == 0) {
+        const unsigned j = *jstep;
+        tri[(long long)b * tri_ld + 2 * j] = alpha;
+        tri[(long long)b * tri_ld + 2 * j + 1] = beta;
+        *jstep = j + 1;
+    }
 }
 
-// grid (row tiles, batch, max splits); cnt: row-tile counters (batch * row tiles), then one
-// counter per block; part: 3 doubles per block and row tile
+// grid (row tiles, batch, max splits); cnt: row-tile counters (batch * row tiles), a
+// counter per block, then a step index per block; part: a double per block and row tile
 template <int NT, bool X16>
 __global__ void __launch_bounds__(64 * SK_WAVES)
-power_bwd_kernel(const SkDesc *__restrict__ descs, double *__restrict__ v, const double *__restrict__ lam,
-                 const long long *__restrict__ seg, unsigned *__restrict__ cnt, double *__restrict__ part,
-                 double *__restrict__ eta)
+lanczos_bwd_kernel(const SkDesc *__restrict__ descs, double *__restrict__ v, double *__restrict__ vp,
+                   const double *__restrict__ lam, const long long *__restrict__ seg, unsigned *__restrict__ cnt,
+                   double *__restrict__ part, double *__restrict__ tri, int tri_ld)
 {
     __shared__ double red[SK_WAVES / 2][16][33];
     __shared__ double red1[64 * SK_WAVES];
@@ -834,72 +819,129 @@ power_bwd_kernel(const SkDesc *__restrict__ descs, double *__restrict__ v, const
     const int m0 = blockIdx.x * 16;
     if (m0 >= d.M || (int)blockIdx.z >= skinny_splits(d.K)) return;
     const int ntile = gridDim.x;
+    const long long B = gridDim.y;
     unsigned *tc = cnt + (long long)b * ntile + blockIdx.x;
-    unsigned *bc = cnt + (long long)gridDim.y * ntile + b;
+    unsigned *bc = cnt + B * ntile + b;
+    unsigned *js = cnt + B * ntile + B + b;
     const double l = lam[2 * b];
     const long long s0 = seg[b], s1 = seg[b + 1];
     const int z = blockIdx.z, t = blockIdx.x;
     if constexpr (NT == 1) {
         if (X16 && d.Nc == 17)
-            power_bwd_tile<1, true>(d, m0, z, b, t, ntile, v, l, s0, s1, tc, bc, part, eta, red, red1);
+            lanczos_bwd_tile<1, true>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, red, red1);
         else
-            power_bwd_tile<1, false>(d, m0, z, b, t, ntile, v, l, s0, s1, tc, bc, part, eta, red, red1);
+            lanczos_bwd_tile<1, false>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, red, red1);
     } else {
         if (d.Nc == 17)
-            power_bwd_tile<1, true>(d, m0, z, b, t, ntile, v, l, s0, s1, tc, bc, part, eta, red, red1);
+            lanczos_bwd_tile<1, true>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, red, red1);
         else if (d.Nc <= 16)
-            power_bwd_tile<1, false>(d, m0, z, b, t, ntile, v, l, s0, s1, tc, bc, part, eta, red, red1);
+            lanczos_bwd_tile<1, false>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, red, red1);
         else
-            power_bwd_tile<2, false>(d, m0, z, b, t, ntile, v, l, s0, s1, tc, bc, part, eta, red, red1);
+            lanczos_bwd_tile<2, false>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, red, red1);
     }
 }
 
-static int power_bwd(const VtSkinnyLaunch &Bk, int batch, int max_Nc, double *v, const double *lam,
-                     const long long *seg, unsigned *cnt, double *part, double *eta, hipStream_t s)
+// v_0 = ones / sqrt(D_b), v_{-1} = 0 per block
+__global__ void __launch_bounds__(256)
+lanczos_init_kernel(double *__restrict__ v, double *__restrict__ vp, const long long *__restrict__ seg)
+{
+    const long long a = seg[blockIdx.x], e = seg[blockIdx.x + 1];
+    const double x = e > a ? 1.0 / sqrt((double)(e - a)) : 0.0;
+    for (long long i = a + threadIdx.x; i < e; i += 256) {
+        v[i] = x;
+        vp[i] = 0.0;
+    }
+}
+
+// Sturm count: eigenvalues of the k x k symmetric tridiagonal (alpha_i = t[2i], off-diagonal
+// beta_i = t[2i+1] between i and i+1) below x
+__device__ int tri_count_below(const double *t, int k, double x)
+{
+    int c = 0;
+    double dd = 1.0;
+    for (int i = 0; i < k; ++i) {
+        const double b2 = i > 0 ? t[2 * (i - 1) + 1] * t[2 * (i - 1) + 1] : 0.0;
+        dd = (t[2 * i] - x) - (i > 0 ? b2 / dd : 0.0);
+        if (dd == 0.0) dd = -1e-300;
+        if (dd < 0.0) ++c;
+    }
+    return c;
+}
+
+// eta_b = 1 / (largest eigenvalue of block b's tridiagonal), by bisection on the Sturm count
+__global__ void lanczos_eta_kernel(const double *__restrict__ tri, int tri_ld, int k, int batch,
+                                   double *__restrict__ eta)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    const double *t = tri + (long long)b * tri_ld;
+    double lo = INFINITY, hi = -INFINITY;
+    for (int i = 0; i < k; ++i) {           // Gershgorin interval
+        const double r = (i > 0 ? fabs(t[2 * (i - 1) + 1]) : 0.0) + (i + 1 < k ? fabs(t[2 * i + 1]) : 0.0);
+        lo = fmin(lo, t[2 * i] - r);
+        hi = fmax(hi, t[2 * i] + r);
+    }
+    for (int it = 0; it < 200 && hi - lo > 1e-15 * fmax(fabs(lo), fabs(hi)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (tri_count_below(t, k, mid) < k) lo = mid;    // the largest eigenvalue is >= mid
+        else hi = mid;
+    }
+    const double th = 0.5 * (lo + hi);
+    eta[b] = th > 0.0 ? 1.0 / th : 0.0;
+}
+
+static int lanczos_bwd(const VtSkinnyLaunch &Bk, int batch, int max_Nc, double *v, double *vp,
+                       const double *lam, const long long *seg, unsigned *cnt, double *part, double *tri,
+                       int tri_ld, hipStream_t s)
 {
     const dim3 grid(vt_blocks(Bk.max_M, 16), batch, skinny_splits(Bk.max_K));
     const SkDesc *d = (const SkDesc *)Bk.descs;
     if (max_Nc <= 16)
-        power_bwd_kernel<1, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, lam, seg, cnt, part, eta);
+        lanczos_bwd_kernel<1, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, vp, lam, seg, cnt, part, tri, tri_ld);
     else if (max_Nc == 17)
-        power_bwd_kernel<1, true><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, lam, seg, cnt, part, eta);
+        lanczos_bwd_kernel<1, true><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, vp, lam, seg, cnt, part, tri, tri_ld);
     else
-        power_bwd_kernel<2, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, lam, seg, cnt, part, eta);
+        lanczos_bwd_kernel<2, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, vp, lam, seg, cnt, part, tri, tri_ld);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
 
-// eta (batch doubles) from `iters` power steps started at v (overwritten): per step the
-// forward operator launch (reads v) and the backward launch with the step fused (writes q
-// and v).  bwd's descriptors in block order; cnt: (batch + 1) * ceil(bwd_M / 16) words of
-// counters (zeroed here), part: 3 * batch * ceil(bwd_M / 16) doubles.  graph_chunk >= 2:
-// the steps are replayed as a HIP graph of that many steps (vt_graph_loop); 0: launched
-// one by one.  Same results either way.
-VT_EXPORT int vt_kron_power(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
-                            int batch, int max_Nc, double *v, double *q, const double *lam,
-                            const long long *seg, int iters, double *eta, int graph_chunk,
-                            unsigned *cnt, double *part, void *stream)
+// eta (batch doubles) = 1/theta_b from `steps` Lanczos steps per block (v, vp: the joint
+// vector's two work copies; q: the backward's output, holds u and w); per step the forward
+// operator launch and the backward launch with the step fused.  bwd's descriptors in block
+// order; cnt: batch * ceil(bwd_M / 16) + 2 batch words (zeroed here); part: batch * ceil(bwd_M / 16)
+// doubles; tri: batch * 2 * steps doubles (alpha_j, beta_j of every block).
+VT_EXPORT int vt_kron_lanczos(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
+                              int batch, int max_Nc, double *v, double *vp, double *q, const double *lam,
+                              const long long *seg, int steps, double *eta, int graph_chunk, unsigned *cnt,
+                              double *part, double *tri, void *stream)
 {
-    VT_CHECK_ARG(fwd && bwd && v && q && lam && seg && eta && cnt && part && batch >= 1 &&
-                 batch <= 65535 && iters >= 1 && graph_chunk >= 0 && bwd_M >= 1 && max_Nc >= 1 &&
+    VT_CHECK_ARG(fwd && bwd && v && vp && q && lam && seg && eta && cnt && part && tri && batch >= 1 &&
+                 batch <= 65535 && steps >= 1 && graph_chunk >= 0 && bwd_M >= 1 && max_Nc >= 1 &&
                  max_Nc <= 32);
     const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
     hipStream_t s = (hipStream_t)stream;
-    const long long words = (long long)(batch + 1) * vt_blocks(bwd_M, 16);
+    const long long ntile = vt_blocks(bwd_M, 16);
+    const long long words = (long long)batch * ntile + 2LL * batch;
     const unsigned gw = vt_blocks(words, 256);
     zero_words_kernel<<<gw < 1024 ? gw : 1024, 256, 0, s>>>(cnt, words);
+    lanczos_init_kernel<<<batch, 256, 0, s>>>(v, vp, seg);
     VT_LAUNCH_CHECK();
+    const int tri_ld = 2 * steps;
     auto step = [&](hipStream_t st) -> int {
         VT_TRY(skinny_run(F, batch, max_Nc, st));
-        VT_TRY(power_bwd(Bk, batch, max_Nc, v, lam, seg, cnt, part, eta, st));
+        VT_TRY(lanczos_bwd(Bk, batch, max_Nc, v, vp, lam, seg, cnt, part, tri, tri_ld, st));
         return VT_OK;
     };
-    return vt_graph_loop(s, iters, graph_chunk, step);
+    VT_TRY(vt_graph_loop(s, steps, graph_chunk, step));
+    lanczos_eta_kernel<<<vt_blocks(batch, 64), 64, 0, s>>>(tri, tri_ld, steps, batch, eta);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
 }
 
 // steps+1 gradient-flow iterations from x = 0: per iteration the forward operator launch
-// (reads x) and the backward launch with the update fused (writes x); graph_chunk as for
-// vt_kron_power.  cnt: batch * ceil(bwd_M / 16) words of row-tile counters (zeroed here).
+// (reads x) and the backward launch with the update fused (writes x); graph_chunk >= 2
+// replays them as HIP graphs of that many iterations (vt_graph_loop), 0 launches each.  cnt: batch * ceil(bwd_M / 16) words of row-tile counters (zeroed here).
 VT_EXPORT int vt_kron_rnn(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                           int batch, int max_Nc, double *x, const double *r, double *q,
                           const double *lam, const double *eta, const long long *seg, long long n,
