@@ -403,10 +403,14 @@ VT_EXPORT int vt_potrf_f32(const void *mats_dev, int batch, int maxD, float *inv
 // the preconditioner of the mixed-precision path).
 //
 // One workgroup per 64-row block and matrix, one launch per direction.  Workgroup b of
-// the forward sweep folds in L[b, j] y_j for j < b as soon as block j publishes y_j
-// (flag = epoch, release store at agent scope after a fence), then applies the stored
-// diagonal-block inverse and publishes y_b; the backward sweep mirrors it with the
-// bottom block first.  A workgroup only ever waits on lower workgroup IDs, which the
+// the forward sweep folds in L[b, j] y_j for j < b as soon as block j publishes y_j, then
+// applies the stored diagonal-block inverse and publishes y_b; the backward sweep mirrors
+// it with the bottom block first.  The XCDs' L2 caches are not coherent with each other,
+// and a fence or acquire at agent scope writes back or invalidates a whole L2 (every
+// spinning poll would): so the solution blocks are stored and read with relaxed
+// agent-scope atomics (written through to, and read from, the device coherence point),
+// and a block's flag (= epoch, relaxed, agent scope) is stored after the workgroup's
+// stores have completed and polled with relaxed loads; L is only read.  A workgroup only ever waits on lower workgroup IDs, which the
 // in-order dispatcher has already started, so the chain always drains; each wait is
 // additionally bounded (VT_SPIN_LIMIT polls) and sets the error word instead of hanging.
 //
@@ -419,7 +423,7 @@ VT_EXPORT int vt_potrf_f32(const void *mats_dev, int batch, int maxD, float *inv
 __device__ __forceinline__ bool wait_flag(const int *flag, int want, int *err)
 {
     long it = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != want) {
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
         if (++it > VT_SPIN_LIMIT) {
             atomicExch(err, 1);
             return false;
@@ -482,9 +486,9 @@ trsv_fwd_kernel(const VtMat<T> *__restrict__ mats, const T *__restrict__ inv_ws,
         double y = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
         __hip_atomic_store(xv + kb + tid, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __threadfence();
+    __builtin_amdgcn_s_waitcnt(0);                // this wave's block stores have completed
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) __hip_atomic_store(flags + b, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T>
@@ -541,9 +545,9 @@ trsv_bwd_kernel(const VtMat<T> *__restrict__ mats, const T *__restrict__ inv_ws,
         double v = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
         __hip_atomic_store(xv + kb + tid, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __threadfence();
+    __builtin_amdgcn_s_waitcnt(0);                // this wave's block stores have completed
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) __hip_atomic_store(flags + b, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void epoch_bump_kernel(int *sync) { sync[0] += 1; }
