@@ -389,18 +389,20 @@ int vt_kpcg_step_b(const double *r, const double *z, double *p, const long long 
                    double *st, void *stream);
 
 /* `iters` CG iterations after vt_kpcg_init, enqueued by one call: per iteration the
- * operator (skinny products fwd then bwd, q = A p), step_a, the preconditioner (skinny
- * product pre, z = Mi r Ti) and step_b.  Each product is given as its descriptors and
- * its max_M / max_K; max_Nc bounds every problem's column count.  graph_chunk >= 2
- * replays the iterations as a HIP graph of up to graph_chunk (at most iters/2)
- * iterations, as vt_kron_rnn does; 0 launches each.  Same results either way. */
+ * operator (skinny products fwd then bwd, q = A p) with step_a fused into the backward
+ * launch (the last row tile of a block to finish runs it), the preconditioner (skinny
+ * products pre, z = Mi r Ti) and step_b.  Each product is given as its descriptors and
+ * its max_M / max_K; max_Nc bounds every problem's column count; bwd's descriptors are
+ * in block order.  cnt: (batch + 1) * ceil(bwd_M / 16) words of counters (zeroed here).
+ * graph_chunk >= 2 replays the iterations as a HIP graph of up to graph_chunk (at most
+ * iters/2) iterations, as vt_kron_rnn does; 0 launches each.  Same results either way. */
 int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                     const void *pre, int pre_n, int pre_M, int pre_K, int pre_Nc, const void *pre2,
                     int pre2_n, int pre2_M, int pre2_K, int pre2_Nc, const void *pre3, int pre3_n,
                     int pre3_M, int pre3_K, int pre3_Nc, int batch, int max_Nc, double *x,
                     double *r, double *p, double *q, double *z, const double *lam,
                     const long long *seg, double *st, double *hist, int hist_stride, int iters,
-                    double stop_rel2, int prune, int graph_chunk, void *stream);
+                    double stop_rel2, int prune, int graph_chunk, unsigned *cnt, void *stream);
 /* Kronecker-sum preconditioner (S^T C S) (x) Tbar + lam I for blocks whose decode period
  * is longer than their window: c_j as vt_kpcg_tbar, and per block Tbar's eigenvalues theta
  * (batch x Dt) and eigenvectors U (eigenvector k in column k of the Dt x Dt block; UT its

@@ -999,9 +999,10 @@ class BatchDecoder:
             pr.run(s)
         lib.vt_kpcg_init(ptr(R), ptr(Zp), ptr(Pv), ptr(X), ptr(seg), Bt, ptr(st), ptr(hist), hs, s)
         pargs = [pr.args5() for pr in pres] + [(0, 0, 0, 0, 0)] * (3 - len(pres))
+        kcnt = torch.empty((Bt + 1) * -(-int(Ds.max()) // 16), dtype=torch.int32, device=dev)
         lib.vt_kpcg_iterate(*fp.args(), *bp.args(), *pargs[0], *pargs[1], *pargs[2], Bt, maxNc, ptr(X), ptr(R),
                             ptr(Pv), ptr(Q), ptr(Zp), ptr(lam), ptr(seg), ptr(st), ptr(hist), hs, iters,
-                            float(self.cg_stop) ** 2, self.prune, self.graph_chunk, s)
+                            float(self.cg_stop) ** 2, self.prune, self.graph_chunk, ptr(kcnt), s)
         if X0 is not None:
             lib.vt_add_inplace(ptr(X), ptr(X0), Dtot, s)
         # true FP64 residual of the final iterate (the guard's input)
@@ -1018,7 +1019,7 @@ class BatchDecoder:
         per = 8.0 * (2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + (Dt if kron_sum else 1) * Ds * Ds + 12.0 * D)
         self._toc("kpcg_iterate", e_cg,
                   lambda: float(np.sum(per * st.view(Bt, 4)[:, 3].cpu().numpy())))
-        keep += [r, lam, seg, elem_job, Mi, mkeep, R, Zp, pres, X, Pv, Q, Z, fp, bp, fx, bx, st, hist, res, X0]
+        keep += [r, lam, seg, elem_job, Mi, mkeep, R, Zp, pres, X, Pv, Q, Z, fp, bp, fx, bx, st, hist, res, X0, kcnt]
         return X, nrm, info, st, boff, keep
 
     def _kron_sum(self, gs) -> bool:

@@ -429,6 +429,81 @@ VT_EXPORT int vt_kpcg_init(const double *r, const double *z, double *p, double *
 // way); the test reads only the block's own history, so it does not depend on the batch.
 #define KP_PRUNE_SPAN 10
 
+__global__ void zero_words_kernel(unsigned *__restrict__ w, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        w[i] = 0u;
+}
+
+// fixed-order sum over the NTH threads of a workgroup (red: NTH doubles)
+template <int NTH>
+__device__ __forceinline__ double block_sum_n(double v, double *red)
+{
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = NTH / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double s = red[0];
+    __syncthreads();
+    return s;
+}
+
+// step_a of block b by a workgroup of NTH threads; COHQ: q was written inside the same
+// launch by other workgroups (read with agent-scope loads)
+template <int NTH, bool COHQ>
+__device__ void kpcg_step_a_block(int b, double *__restrict__ x, double *__restrict__ r,
+                                  const double *__restrict__ p, double *__restrict__ q, double l, long long a,
+                                  long long e, double *__restrict__ s, double *__restrict__ hist, int hs, int it,
+                                  double stop_rel2, int prune, int cap, double *red)
+{
+    double pq = 0.0;
+    for (long long i = a + threadIdx.x; i < e; i += NTH) {
+        const double q0 = COHQ ? __hip_atomic_load(q + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : q[i];
+        const double qi = q0 + l * p[i];
+        q[i] = qi;
+        pq += p[i] * qi;
+    }
+    pq = block_sum_n<NTH>(pq, red);
+    const double rz = s[0];
+    if (!(pq > 0.0)) {
+        if (threadIdx.x == 0) s[2] = 0.0;
+        return;
+    }
+    const double alpha = rz / pq;
+    double rr = 0.0;
+    for (long long i = a + threadIdx.x; i < e; i += NTH) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        rr += ri * ri;
+    }
+    rr = block_sum_n<NTH>(rr, red);
+    if (threadIdx.x == 0) {
+        // it < 0: the block's own count of completed iterations (s[3], which equals the
+        // loop index while the block is active), so every iteration launches the same
+        // kernel arguments and a loop of them can be replayed as a graph
+        const int itx = it >= 0 ? it : (int)s[3];
+        if (itx + 1 < hs) hist[(long long)b * hs + itx + 1] = rr;
+        s[3] = itx + 1;
+        if (rr <= stop_rel2 * s[1]) {
+            s[2] = 0.0;
+        } else if (prune > 0 && itx + 1 >= prune && (itx + 1 - prune) % KP_PRUNE_SPAN == 0 &&
+                   itx + 1 < hs) {
+            const double old = hist[(long long)b * hs + itx + 1 - KP_PRUNE_SPAN];
+            const double rate = old > 0.0 ? rr / old : 1.0;     // squared-norm ratio per span
+            bool hopeless = !(rate < 1.0);
+            if (!hopeless) {
+                const double spans = log(stop_rel2 * s[1] / rr) / log(rate);
+                hopeless = itx + 1 + KP_PRUNE_SPAN * spans > (double)cap;
+            }
+            if (hopeless) s[2] = 0.0;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256)
 kpcg_step_a_kernel(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
                    double *__restrict__ q, const double *__restrict__ lam,
@@ -438,49 +513,96 @@ kpcg_step_a_kernel(double *__restrict__ x, double *__restrict__ r, const double 
     __shared__ double red[256];
     double *s = st + 4 * blockIdx.x;
     if (s[2] == 0.0) return;
-    const double l = lam[2 * blockIdx.x];
-    const long long a = seg[blockIdx.x], e = seg[blockIdx.x + 1];
-    double pq = 0.0;
-    for (long long i = a + threadIdx.x; i < e; i += 256) {
-        const double qi = q[i] + l * p[i];
-        q[i] = qi;
-        pq += p[i] * qi;
-    }
-    pq = block_sum(pq, red);
-    const double rz = s[0];
-    if (!(pq > 0.0)) {
-        if (threadIdx.x == 0) s[2] = 0.0;
-        return;
-    }
-    const double alpha = rz / pq;
-    double rr = 0.0;
-    for (long long i = a + threadIdx.x; i < e; i += 256) {
-        x[i] += alpha * p[i];
-        const double ri = r[i] - alpha * q[i];
-        r[i] = ri;
-        rr += ri * ri;
-    }
-    rr = block_sum(rr, red);
-    if (threadIdx.x == 0) {
-        // it < 0: the block's own count of completed iterations (s[3], which equals the
-        // loop index while the block is active), so every iteration launches the same
-        // kernel arguments and a loop of them can be replayed as a graph
-        const int itx = it >= 0 ? it : (int)s[3];
-        if (itx + 1 < hs) hist[(long long)blockIdx.x * hs + itx + 1] = rr;
-        s[3] = itx + 1;
-        if (rr <= stop_rel2 * s[1]) {
-            s[2] = 0.0;
-        } else if (prune > 0 && itx + 1 >= prune && (itx + 1 - prune) % KP_PRUNE_SPAN == 0 &&
-                   itx + 1 < hs) {
-            const double old = hist[(long long)blockIdx.x * hs + itx + 1 - KP_PRUNE_SPAN];
-            const double rate = old > 0.0 ? rr / old : 1.0;     // squared-norm ratio per span
-            bool hopeless = !(rate < 1.0);
-            if (!hopeless) {
-                const double spans = log(stop_rel2 * s[1] / rr) / log(rate);
-                hopeless = itx + 1 + KP_PRUNE_SPAN * spans > (double)cap;
-            }
-            if (hopeless) s[2] = 0.0;
+    kpcg_step_a_block<256, false>(blockIdx.x, x, r, p, q, lam[2 * blockIdx.x], seg[blockIdx.x],
+                                  seg[blockIdx.x + 1], s, hist, hs, it, stop_rel2, prune, cap, red);
+}
+
+// The CG operator's backward product with step_a fused: problem b (block order, skipped
+// while the block is inactive) is S_b^T Z_b, this workgroup its row tile and K split.
+// The last of a row tile's splits to finish (a counter per row tile, taken after the
+// workgroup's stores have completed) adds the partials in split order into q; the last
+// row tile of the block to finish (a counter per block) runs step_a for the whole block,
+// reading q with agent-scope loads.  The backward's second pass and step_a are gone as
+// launches; their sums are those of a 512-thread step_a.
+template <int NT, bool X16>
+__device__ __forceinline__ void kpcg_bwd_tile(const SkDesc &d, int m0, int z, int b, unsigned *tcnt,
+                                              unsigned *bcnt, double *__restrict__ x, double *__restrict__ r,
+                                              const double *__restrict__ p, double *__restrict__ q, double l,
+                                              long long a, long long e, double *__restrict__ s,
+                                              double *__restrict__ hist, int hs, double stop_rel2, int prune,
+                                              int cap, double (&red)[SK_WAVES / 2][16][33], double *red1)
+{
+    __shared__ int last;
+    const int ks = skinny_splits(d.K), Nc = d.Nc;
+    skinny_tile<NT, 1, X16, true>(d, m0, z, red);
+    __builtin_amdgcn_s_waitcnt(0);                // this wave's partial or q stores have completed
+    __syncthreads();
+    if (ks > 1) {
+        if (threadIdx.x == 0) {
+            const unsigned o = __hip_atomic_fetch_add(tcnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            last = o == (unsigned)ks - 1;
+            if (last) __hip_atomic_store(tcnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        __syncthreads();
+        if (!last) return;
+        const int rows = min(16, d.M - m0);
+        const long long mn = (long long)d.M * Nc;
+        for (int t = threadIdx.x; t < rows * Nc; t += 64 * SK_WAVES) {
+            const long long loc = (long long)(m0 + t / Nc) * Nc + t % Nc;
+            double qi = 0.0;
+            for (int zz = 0; zz < ks; ++zz)
+                qi += __hip_atomic_load(d.ws + zz * mn + loc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(d.C + loc, qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __builtin_amdgcn_s_waitcnt(0);            // the tile's q stores have completed
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const unsigned o = __hip_atomic_fetch_add(bcnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = o == (unsigned)((d.M + 15) / 16) - 1;
+        if (last) __hip_atomic_store(bcnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (!last) return;
+    kpcg_step_a_block<64 * SK_WAVES, true>(b, x, r, p, q, l, a, e, s, hist, hs, -1, stop_rel2, prune, cap, red1);
+}
+
+// grid (row tiles, batch, max splits); cnt: row-tile counters (batch * row tiles), then a
+// counter per block, zero before the first launch and left zero by every launch
+template <int NT, bool X16>
+__global__ void __launch_bounds__(64 * SK_WAVES)
+kpcg_bwd_kernel(const SkDesc *__restrict__ descs, double *__restrict__ x, double *__restrict__ r,
+                const double *__restrict__ p, double *__restrict__ q, const double *__restrict__ lam,
+                const long long *__restrict__ seg, double *__restrict__ st, double *__restrict__ hist, int hs,
+                double stop_rel2, int prune, int cap, unsigned *__restrict__ cnt)
+{
+    __shared__ double red[SK_WAVES / 2][16][33];
+    __shared__ double red1[64 * SK_WAVES];
+    const int b = blockIdx.y;
+    const SkDesc d = descs[b];
+    const int m0 = blockIdx.x * 16;
+    if (m0 >= d.M || (int)blockIdx.z >= skinny_splits(d.K)) return;
+    if (d.active && *d.active == 0.0) return;
+    double *sb = st + 4 * b;
+    if (sb[2] == 0.0) return;                    // the block has stopped (as step_a would)
+    const int ntile = gridDim.x;
+    unsigned *tc = cnt + (long long)b * ntile + blockIdx.x;
+    unsigned *bc = cnt + (long long)gridDim.y * ntile + b;
+    const double l = lam[2 * b];
+    const long long a = seg[b], e = seg[b + 1];
+    const int z = blockIdx.z;
+    if constexpr (NT == 1) {
+        if (X16 && d.Nc == 17)
+            kpcg_bwd_tile<1, true>(d, m0, z, b, tc, bc, x, r, p, q, l, a, e, sb, hist, hs, stop_rel2, prune, cap, red, red1);
+        else
+            kpcg_bwd_tile<1, false>(d, m0, z, b, tc, bc, x, r, p, q, l, a, e, sb, hist, hs, stop_rel2, prune, cap, red, red1);
+    } else {
+        if (d.Nc == 17)
+            kpcg_bwd_tile<1, true>(d, m0, z, b, tc, bc, x, r, p, q, l, a, e, sb, hist, hs, stop_rel2, prune, cap, red, red1);
+        else if (d.Nc <= 16)
+            kpcg_bwd_tile<1, false>(d, m0, z, b, tc, bc, x, r, p, q, l, a, e, sb, hist, hs, stop_rel2, prune, cap, red, red1);
+        else
+            kpcg_bwd_tile<2, false>(d, m0, z, b, tc, bc, x, r, p, q, l, a, e, sb, hist, hs, stop_rel2, prune, cap, red, red1);
     }
 }
 
@@ -556,25 +678,42 @@ VT_EXPORT int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void 
                               int batch, int max_Nc, double *x, double *r, double *p, double *q,
                               double *z, const double *lam, const long long *seg, double *st,
                               double *hist, int hist_stride, int iters, double stop_rel2,
-                              int prune, int graph_chunk, void *stream)
+                              int prune, int graph_chunk, unsigned *cnt, void *stream)
 {
-    VT_CHECK_ARG(fwd && bwd && pre && iters >= 0 && iters < hist_stride && graph_chunk >= 0);
+    VT_CHECK_ARG(fwd && bwd && pre && cnt && iters >= 0 && iters < hist_stride && graph_chunk >= 0 &&
+                 batch >= 1 && batch <= 65535 && bwd_M >= 1 && max_Nc >= 1 && max_Nc <= 32 &&
+                 (prune == 0 || prune >= KP_PRUNE_SPAN));
     const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
     struct PreLaunch { const void *d; int n, M, K, Nc; };
     const PreLaunch P[3] = {{pre, pre_n, pre_M, pre_K, pre_Nc}, {pre2, pre2_n, pre2_M, pre2_K, pre2_Nc},
                             {pre3, pre3_n, pre3_M, pre3_K, pre3_Nc}};
-    // the iteration index comes from each block's count (it = -1): identical iterations
+    hipStream_t s0 = (hipStream_t)stream;
+    const int ntile = (int)vt_blocks(bwd_M, 16);
+    const long long words = (long long)(batch + 1) * ntile;
+    const unsigned gw = vt_blocks(words, 256);
+    zero_words_kernel<<<gw < 1024 ? gw : 1024, 256, 0, s0>>>(cnt, words);   // a kernel, not a memset node
+    VT_LAUNCH_CHECK();
+    const dim3 bgrid(ntile, batch, skinny_splits(bwd_K));
+    const SkDesc *bd = (const SkDesc *)bwd;
+    // the iteration index comes from each block's count: identical iterations
     auto step = [&](hipStream_t sq) -> int {
         VT_TRY(skinny_run(F, batch, max_Nc, sq));
-        VT_TRY(skinny_run(Bk, batch, max_Nc, sq));
-        VT_TRY(vt_kpcg_step_a(x, r, p, q, lam, seg, batch, st, hist, hist_stride, -1, stop_rel2, prune,
-                              iters, sq));
+        if (max_Nc <= 16)
+            kpcg_bwd_kernel<1, false><<<bgrid, 64 * SK_WAVES, 0, sq>>>(bd, x, r, p, q, lam, seg, st, hist, hist_stride,
+                                                                   stop_rel2, prune, iters, cnt);
+        else if (max_Nc == 17)
+            kpcg_bwd_kernel<1, true><<<bgrid, 64 * SK_WAVES, 0, sq>>>(bd, x, r, p, q, lam, seg, st, hist, hist_stride,
+                                                                  stop_rel2, prune, iters, cnt);
+        else
+            kpcg_bwd_kernel<2, false><<<bgrid, 64 * SK_WAVES, 0, sq>>>(bd, x, r, p, q, lam, seg, st, hist, hist_stride,
+                                                                   stop_rel2, prune, iters, cnt);
+        VT_LAUNCH_CHECK();
         for (int i = 0; i < 3; ++i)
             if (P[i].d) VT_TRY(vt_gemm_skinny(P[i].d, P[i].n, P[i].M, P[i].K, P[i].Nc, sq));
         VT_TRY(vt_kpcg_step_b(r, z, p, seg, batch, st, sq));
         return VT_OK;
     };
-    return vt_graph_loop((hipStream_t)stream, iters, graph_chunk, step);
+    return vt_graph_loop(s0, iters, graph_chunk, step);
 }
 
 // ------------------------------------------------------------------------------------
@@ -683,12 +822,6 @@ static int rnn_bwd_update(const VtSkinnyLaunch &Bk, int batch, int max_Nc, doubl
 
 
 
-__global__ void zero_words_kernel(unsigned *__restrict__ w, long long n)
-{
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-        w[i] = 0u;
-}
 
 // fixed-order sum over the 64 * SK_WAVES threads of a workgroup (red: as many doubles)
 __device__ __forceinline__ double wg_sum(double v, double *red)
