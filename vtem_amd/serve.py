@@ -157,6 +157,14 @@ class Server:
         # stream of their own (encode, prologue, step sizes), so the next chain starts with
         # its iterations (one RNN lane without held queues)
         self.rnn_prep = bool(rnn_prep)
+        self._streams: List[torch.cuda.Stream] = []
+
+    def _lane_streams(self, n: int) -> List[torch.cuda.Stream]:
+        """The server's n streams, made once and reused by every serve call, on distinct
+        hardware queues where the device has enough (independent_streams)."""
+        if len(self._streams) < n:
+            self._streams = independent_streams(n)
+        return self._streams[:n]
 
     # ----------------------------------------------------------------------------------
     def serve(self, requests: List[Request], mode: str = "served",
@@ -180,12 +188,13 @@ class Server:
         # The RNN lane comes first: when several lanes are free, its batch (the longest chain
         # of dependent launches) is enqueued before the host spends its time on the others.
         nr = self.rnn_lanes if self.rnn_lane else 0
-        lanes = [(torch.cuda.Stream(), ready_rnn, self.rnn_max_batch) for _ in range(nr)]
-        lanes += [(torch.cuda.Stream(), ready, self.max_batch) for _ in range(self.n_streams)]
+        prep_on = (nr == 1 and self.rnn_prep and self.rnn_lane_delays is None and self.rnn_max_delay == 0.0)
+        all_streams = self._lane_streams(nr + self.n_streams + int(prep_on))
+        lanes = [(all_streams[k], ready_rnn, self.rnn_max_batch) for k in range(nr)]
+        lanes += [(all_streams[nr + k], ready, self.max_batch) for k in range(self.n_streams)]
         streams = [ln[0] for ln in lanes]
         busy = [None] * len(lanes)              # (event, handle, reqs, bytes, [(reqs, overflow flags)])
-        prep_on = (nr == 1 and self.rnn_prep and self.rnn_lane_delays is None and self.rnn_max_delay == 0.0)
-        prep_stream = torch.cuda.Stream() if prep_on else None
+        prep_stream = all_streams[-1] if prep_on else None
         prepared: List[tuple] = []              # (reqs, RnnPrep, bytes, [(reqs, overflow flags)])
         preps_made = 0
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
@@ -393,6 +402,53 @@ class Server:
                "mean_ms": float(lat.mean())}
         out.update(extra)
         return out
+
+
+def streams_overlap(a: torch.cuda.Stream, b: torch.cuda.Stream, spin_ms: float = 2.0) -> bool:
+    """True if work enqueued on b runs while a spin kernel occupies a (and the reverse).
+    Two streams that HIP placed on one hardware queue run in submission order, so the
+    second one's work waits for the spin."""
+    x = torch.zeros(1, device=a.device)
+    torch.cuda.synchronize()
+    cycles = int(spin_ms * 1.6e6)              # the spin counts at about 1.6 GHz
+    for s, t in ((a, b), (b, a)):
+        with torch.cuda.stream(s):
+            torch.cuda._sleep(cycles)
+            e_spin = torch.cuda.Event()
+            e_spin.record()
+        with torch.cuda.stream(t):
+            x.add_(1.0)
+            e_work = torch.cuda.Event()
+            e_work.record()
+        t0 = time.perf_counter()
+        while not e_work.query() and time.perf_counter() - t0 < 0.1:
+            pass
+        ran_beside = e_work.query() and not e_spin.query()
+        torch.cuda.synchronize()
+        if not ran_beside:
+            return False
+    return True
+
+
+def independent_streams(n: int, tries: int = 16) -> List[torch.cuda.Stream]:
+    """n streams on distinct hardware queues where the device has enough.  HIP gives a
+    stream a hardware queue when the stream first runs work, and once every queue is taken
+    further streams share them: two lanes on one queue run one after the other.  Streams
+    are given work in order and tested against each kept stream (streams_overlap); one that
+    waits for a kept stream is replaced by a new one, up to `tries` times, after which it
+    is kept as it is."""
+    kept: List[torch.cuda.Stream] = []
+    spare = tries
+    while len(kept) < n:
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            torch.zeros(1, device=s.device).add_(1.0)   # the stream takes its queue now
+        torch.cuda.synchronize()
+        if spare > 0 and not all(streams_overlap(k, s) for k in kept):
+            spare -= 1
+            continue
+        kept.append(s)
+    return kept
 
 
 def is_rnn(r: Request) -> bool:
