@@ -85,9 +85,11 @@ def parse():
     p.add_argument("--max-batch", type=int, default=32)
     p.add_argument("--streams", type=int, default=2)
     p.add_argument("--rnn-lanes", type=int, default=1, help="streams of the RNN lane")
-    p.add_argument("--rnn-lane-delays-ms", default="",
-                   help="staggered RNN lanes: comma-separated hold per lane (ms); '' = --rnn-lanes "
-                        "lanes splitting the waiting requests")
+    p.add_argument("--rnn-lane-delays-ms", default="0,16",
+                   help="staggered RNN lanes: comma-separated hold per lane (ms; default: a second "
+                        "lane that gathers the RNN requests arriving during the first lane's chain "
+                        "for 16 ms); '' = --rnn-lanes lanes splitting the waiting requests, with "
+                        "the preparation stage")
     p.add_argument("--graph-chunk", type=int, default=25,
                    help="RNN / power iterations per replayed HIP graph (0 = launch each)")
     p.add_argument("--form", default="kpcg", help="solver form of the chosen served path")

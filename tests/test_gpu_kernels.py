@@ -1046,7 +1046,7 @@ def test_server_alone_equals_mix(torch_mod):
         np.testing.assert_array_equal(r.result.frames.cpu().numpy(), alone.frames.cpu().numpy())
 
 
-@pytest.mark.parametrize("lane_delays", [None, (0.0, 0.004), "prep"])
+@pytest.mark.parametrize("lane_delays", [None, (0.0, 0.004), "prep", "prep_staggered"])
 def test_server_encode_decode_alone_equals_mix(torch_mod, lane_delays):
     """Encode+decode requests (video resident, encoded by the server's batched encoder)
     served with Poisson arrivals on 2 streams: every request's spikes and frames are
@@ -1056,7 +1056,8 @@ def test_server_encode_decode_alone_equals_mix(torch_mod, lane_delays):
     buffer, giving the alone result with a large buffer.  "prep": mostly RNN requests
     arriving faster than a chain runs, so the ones behind a running chain are prepared
     (encode, prologue, step sizes) on the preparation stream and join the next chain;
-    the overflowing request is one of them."""
+    the overflowing request is one of them.  "prep_staggered": the same requests with two
+    staggered lanes, the waiting ones prepared on the held lane's stream."""
     torch = torch_mod
     import dataclasses
     import numpy as np
@@ -1068,9 +1069,9 @@ def test_server_encode_decode_alone_equals_mix(torch_mod, lane_delays):
     from vtem_amd.serve import Request, Server, footprint_bytes, poisson_arrivals
     _banner(f"server: encode+decode alone == in the mix, overflow redo, RNN lane holds {lane_delays}")
     specs = [(32, 2.0), (48, 1.0), (32, 1.0), (48, 2.0), (64, 1.0), (32, 0.5), (48, 0.5)]
-    prep = lane_delays == "prep"
+    prep = lane_delays in ("prep", "prep_staggered")
     if prep:
-        lane_delays = None
+        lane_delays = (0.0, 0.004) if lane_delays == "prep_staggered" else None
     reqs, parts = [], []
     arr = poisson_arrivals(len(specs), 2000.0 if prep else 200.0, np.random.default_rng(5))
     for i, ((sd, du), a) in enumerate(zip(specs, arr)):

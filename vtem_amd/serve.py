@@ -153,9 +153,10 @@ class Server:
         # RNN iterations in one persistent launch (default: two launches per iteration,
         # replayed as HIP graphs)
         self.rnn_persistent = bool(rnn_persistent)
-        # RNN requests that arrive while the RNN lane runs a chain are prepared at once on a
-        # stream of their own (encode, prologue, step sizes), so the next chain starts with
-        # its iterations (one RNN lane without held queues)
+        # RNN requests that arrive while the RNN lane runs a chain are prepared at once
+        # (encode, prologue, step sizes), so the next chain starts with its iterations: on a
+        # stream of their own with one RNN lane (without held queues), on the held lane's
+        # stream with staggered lanes
         self.rnn_prep = bool(rnn_prep)
         self._streams: List[torch.cuda.Stream] = []
 
@@ -194,8 +195,14 @@ class Server:
         lanes += [(all_streams[nr + k], ready, self.max_batch) for k in range(self.n_streams)]
         streams = [ln[0] for ln in lanes]
         busy = [None] * len(lanes)              # (event, handle, reqs, bytes, [(reqs, overflow flags)])
-        prep_stream = all_streams[-1] if prep_on else None
-        prepared: List[tuple] = []              # (reqs, RnnPrep, bytes, [(reqs, overflow flags)])
+        # streams that prepare waiting RNN requests for a lane (lane -> stream): one RNN lane
+        # has a stream of its own for it; staggered lanes prepare on a held lane's stream
+        prep_streams: Dict[int, torch.cuda.Stream] = {}
+        if prep_on:
+            prep_streams[0] = all_streams[-1]
+        elif self.rnn_prep and self.rnn_lane_delays is not None and nr > 1:
+            prep_streams = {k: streams[k] for k in range(1, nr) if self.rnn_lane_delays[k] > 0.0}
+        held: Dict[int, List[tuple]] = {k: [] for k in prep_streams}   # (reqs, RnnPrep, bytes, flag sets)
         preps_made = 0
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         in_use = 0
@@ -204,7 +211,7 @@ class Server:
         host_launch = host_finalize = 0.0      # host seconds spent enqueueing / retiring
         torch.cuda.synchronize()
         t0 = clock()
-        while pending or ready or ready_rnn or prepared or any(b is not None for b in busy):
+        while pending or ready or ready_rnn or any(held.values()) or any(b is not None for b in busy):
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 r = pending.popleft()
@@ -234,13 +241,16 @@ class Server:
                     in_use -= nbytes
                     busy[k] = None
             # prepare the RNN requests waiting behind a running chain (one preparation in
-            # flight at a time, so arrivals during it are prepared together)
-            if prep_on and busy[0] is not None and ready_rnn and \
-                    (not prepared or prepared[-1][1].done.query()):
+            # flight per stream, so arrivals during it are prepared together)
+            for k, ps in prep_streams.items():
+                if not ready_rnn or busy[0] is None or (k > 0 and busy[k] is not None):
+                    continue
+                if held[k] and not held[k][-1][1].done.query():
+                    continue
                 batch, nbytes = form_batch(ready_rnn, self.rnn_max_batch, self.hbm_budget - in_use)
                 if batch:
                     tstart = clock() - t0
-                    with torch.cuda.stream(prep_stream):
+                    with torch.cuda.stream(ps):
                         flags = self._encode(batch, be, bd)
                         prep = bd.prepare_rnn([r.job for r in batch])
                     host_launch += clock() - t0 - tstart
@@ -248,20 +258,31 @@ class Server:
                         r.start = tstart
                     in_use += nbytes
                     preps_made += 1
-                    prepared.append((batch, prep, nbytes, self._flag_set(batch, flags)))
+                    held[k].append((batch, prep, nbytes, self._flag_set(batch, flags)))
             # launch on a free lane
             launched = False
             wake = math.inf                     # earliest batching deadline of a held queue
             for k, (_, queue, cap) in enumerate(lanes):
-                if prep_on and k == 0 and busy[0] is None and prepared:
-                    # the RNN lane: the prepared requests (and any waiting ones, prepared on
-                    # the lane's own stream first) go into one chain
+                # an RNN lane with prepared requests: the first lane takes every preparation
+                # at once, a held lane its own once its oldest request has waited its delay;
+                # they go into one chain with any waiting requests (prepared on the lane's
+                # own stream first)
+                take = [] if k >= nr or busy[k] is not None else \
+                    [j for j in held if held[j]] if k == 0 else ([k] if held.get(k) else [])
+                if take:
+                    oldest = min(r.arrival for j in take for p in held[j] for r in p[0])
+                    delay = 0.0 if k == 0 or self.rnn_lane_delays is None else self.rnn_lane_delays[k]
+                    count = sum(len(p[0]) for j in take for p in held[j]) + len(queue)
+                    if not should_launch(count, cap, clock() - t0 - oldest, delay, bool(pending)):
+                        wake = min(wake, oldest + delay)
+                        continue
                     tstart = clock() - t0
                     batch, nbytes = form_batch(queue, self.rnn_max_batch, self.hbm_budget - in_use)
-                    reqs = [r for p in prepared for r in p[0]] + batch
-                    flag_sets = [fs for p in prepared for fs in p[3]]
-                    with torch.cuda.stream(streams[0]):
-                        preps = [p[1] for p in prepared]
+                    ready_preps = [p for j in take for p in held[j]]
+                    reqs = [r for p in ready_preps for r in p[0]] + batch
+                    flag_sets = [fs for p in ready_preps for fs in p[3]]
+                    with torch.cuda.stream(streams[k]):
+                        preps = [p[1] for p in ready_preps]
                         if batch:
                             flags = self._encode(batch, be, bd)
                             preps.append(bd.prepare_rnn([r.job for r in batch]))
@@ -276,8 +297,9 @@ class Server:
                         r.batch = nbatch
                     nbatch += 1
                     in_use += nbytes
-                    busy[0] = (ev, h, reqs, nbytes + sum(p[2] for p in prepared), flag_sets)
-                    prepared = []
+                    busy[k] = (ev, h, reqs, nbytes + sum(p[2] for p in ready_preps), flag_sets)
+                    for j in take:
+                        held[j] = []
                     launched = True
                     continue
                 if busy[k] is not None or not queue:
