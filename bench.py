@@ -96,7 +96,7 @@ def parse():
     p.add_argument("--no-rnn-prep", action="store_true",
                    help="no preparation stream: RNN requests behind a running chain wait whole")
     p.add_argument("--rnn-persistent", action="store_true",
-                   help="RNN iterations in one persistent launch instead of three launches each")
+                   help="RNN iterations in one persistent launch instead of two launches each")
     p.add_argument("--compare", default="bf16x3", help="second served form measured ('' = none)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-sweep", action="store_true", help="skip the latency sweep")
@@ -182,8 +182,9 @@ def snr_db(u, urec):
 def dominant_kernel(items, max_batch, reps=200):
     """The served path's dominant kernel at its real shapes: the forward skinny product of
     the structured operator (Z_j = T_j (S X)_j, skinny_kernel + its split-K reduction)
-    over the RNN requests of one batch, the product the RNN repeats 551 times.  Timed with
-    HIP events on the stream the kernels are launched on.  Algorithmic bytes per launch:
+    over the RNN requests of one batch, the product each of the RNN's 501 iterations runs
+    once, with its operands in the MFMA tile layout as the iterations run it.  Timed with HIP events
+    on the stream the kernels are launched on.  Algorithmic bytes per launch:
     8 (N Ds + Ds Dt + N Dt + N Dt^2) per block (S and X read, Z written, T_j read)."""
     import torch
     from vtem_amd._lib import stream_ptr
@@ -201,8 +202,9 @@ def dominant_kernel(items, max_batch, reps=200):
     nzj = np.concatenate([[0], np.cumsum(N * Dt)])
     X = torch.randn(int(dbj[-1]), dtype=torch.float64, device="cuda")
     Q = torch.empty_like(X)
-    Z = torch.empty(int(nzj[-1]), dtype=torch.float64, device="cuda")
-    fwd, _ = bd._operator(blocks, X, Q, Z, dbj, nzj)
+    tiles = bd.rnn_tiles(N, Ds, Dt)
+    tiles[0].normal_()
+    fwd, _ = bd._operator(blocks, X, Q, tiles[2], dbj, nzj, tiled=tiles)
     s = stream_ptr()
     for _ in range(5):
         fwd.run(s)

@@ -230,6 +230,14 @@ int vt_row_sumsq(const double *S, long long lds, int N, int Ds, double *s2, void
  * as the first does.  No reference counterpart (the reference forms no such operator). */
 int vt_transpose_f64(const double *src, long long lds, int rows, int cols, double *dst,
                      long long ldd, void *stream);
+/* dst = A in the skinny product's MFMA tile layout, A = src (rows x cols, row stride lds)
+ * or, with trans = 1, its transpose: for A of M x K, ceil(M/16) * ceil(K/16) tiles of 256
+ * doubles, tile (m/16, k/16) holding A[m][k] at (k%16 %4 / 2) * 128 + 2 (16 ((k%16)/4) + m%16)
+ * + k%2, zeros beyond M and K.  Each lane of a wave then reads its operands of a 16 x 16
+ * tile as two 16-byte loads that cover 1 KB in one piece.  Makes the decoder's resident S
+ * and S^T operands.  No reference counterpart (the reference forms no such operator). */
+int vt_tile_f64(const double *src, long long lds, int rows, int cols, int trans, double *dst,
+                void *stream);
 /* Exact tr(A) = sum_j s2_j tr(T_j) in fp64; lam_out = {lam_rel tr(A)/n, tr(A)}.  Replaces
  * the trace needed for lambda = 1e-6 tr(G)/n without forming G or A. */
 int vt_kron_trace(const double *s2, const double *Tm, int N, int Dt, const int *n_dev,
@@ -426,11 +434,14 @@ int vt_kpcg_shift_pool(const double *kpool, const long long *kbase, const long l
  * graph_chunk >= 2 replays the iterations as a HIP graph of graph_chunk iterations,
  * captured once per call on a private stream of the current device (one host call per
  * replay instead of two launches per iteration); 0 launches them one by one.  Results
- * are identical. */
+ * are identical.  xt (optional; n_xt doubles, zeroed here): X also kept as the forward
+ * product's tiled B operand (descriptor flag SK_B_TILED: the tiles of X_b^T, block b's at
+ * xt + xtoff[b]), written by each update; null when the forward reads X row-major.  The
+ * layout changes no value or sum order: iterates are identical either way. */
 int vt_kron_rnn(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                 int batch, int max_Nc, double *x, const double *r, double *q, const double *lam,
                 const double *eta, const long long *seg, long long n, int steps, int graph_chunk,
-                unsigned *cnt, void *stream);
+                unsigned *cnt, double *xt, const long long *xtoff, long long n_xt, void *stream);
 /* The same steps+1 iterations in ONE persistent launch (vtem/neural_network.py:78-92, update
  * kernel :102-128: the loop the reference drives from the host, one launch per step).  Each
  * block owns the workgroups [gstart[b], gstart[b+1]) (device array, batch + 1 ints, grid =

@@ -117,6 +117,76 @@ def test_gemm_skinny_route_invariance(torch_mod):
     assert torch.equal(pair[0], alone["p9"]) and torch.equal(pair[1], alone["p17"])
 
 
+def test_gemm_skinny_tiled_layouts(torch_mod):
+    """The MFMA tile layout (vt_tile_f64) of A, B and C changes no value or sum order: the
+    skinny product with A tiled, A and B tiled, and A, B and C tiled equals the row-major
+    one bit for bit, on ragged shapes, Nc = 9 / 17 / 25 (one tile, the vector column, two
+    tiles), a K split and the row factor.  vt_tile_f64 itself matches the layout built
+    entry by entry from its index formula, zeros in the padding, in both orientations."""
+    torch = torch_mod
+    from vtem_amd._lib import lib, ptr, stream_ptr
+    from vtem_amd.batch import SK_A_TILED, SK_B_TILED, SK_C_TILED, SK_DTYPE, _to_dev, skinny_workspace
+    from vtem_amd.block import tile_doubles
+    _banner("skinny product: tiled operand layouts == row-major")
+    g = torch.Generator().manual_seed(12)
+
+    def tile_index(m, k, kc):
+        return ((m >> 4) * kc + (k >> 4)) * 256 + ((k & 3) >> 1) * 128 + (16 * ((k & 15) >> 2) + (m & 15)) * 2 + (k & 1)
+
+    def tiled(X, trans=0):
+        rows, cols = X.shape
+        M, K = (cols, rows) if trans else (rows, cols)
+        out = torch.full((tile_doubles(M, K),), float("nan"), dtype=torch.float64, device="cuda")
+        lib.vt_tile_f64(ptr(X), X.stride(0), rows, cols, trans, ptr(out), stream_ptr())
+        return out
+
+    A = torch.randn(37, 53, generator=g, dtype=torch.float64).cuda()
+    m = torch.arange(37).view(-1, 1).expand(37, 53)
+    k = torch.arange(53).view(1, -1).expand(37, 53)
+    idx = tile_index(m, k, -(-53 // 16))
+    T = tiled(A).cpu()
+    assert torch.equal(T[idx], A.cpu())
+    pad = torch.ones(T.numel(), dtype=torch.bool)
+    pad[idx.reshape(-1)] = False
+    assert bool((T[pad] == 0).all())
+    assert torch.equal(tiled(A, trans=1), tiled(A.t().contiguous()))
+
+    def product(M, K, Nc, Ad, Bd, Td, lay):
+        wn = skinny_workspace(M, K, Nc)
+        Wd = torch.empty(max(1, wn), dtype=torch.float64, device="cuda")
+        kc, mc = -(-K // 16), -(-M // 16)
+        A_, sam = (tiled(Ad), kc) if lay & SK_A_TILED else (Ad, K)
+        B_, ldb = (tiled(Bd, trans=1), kc) if lay & SK_B_TILED else (Bd, Nc)
+        if lay & SK_C_TILED:
+            C_, ldc = torch.zeros(tile_doubles(Nc, M), dtype=torch.float64, device="cuda"), mc
+        else:
+            C_, ldc = torch.full((M, Nc), float("nan"), dtype=torch.float64, device="cuda"), Nc
+        d = np.zeros(1, SK_DTYPE)
+        d[0] = (A_.data_ptr(), B_.data_ptr(), C_.data_ptr(), Td.data_ptr() if Td is not None else 0,
+                Wd.data_ptr() if wn else 0, 0, sam, 1, ldb, ldc, Nc * Nc if Td is not None else 0, M, K, Nc, lay)
+        dd = _to_dev(d, "cuda")
+        lib.vt_gemm_skinny(ptr(dd), 1, M, K, Nc, stream_ptr())
+        torch.cuda.synchronize()
+        C = C_.cpu()
+        if lay & SK_C_TILED:
+            mm = torch.arange(M).view(-1, 1).expand(M, Nc)
+            nn = torch.arange(Nc).view(1, -1).expand(M, Nc)
+            C = C[tile_index(nn, mm, mc)]
+        return C
+
+    for M, K, Nc, with_t in ((300, 625, 9, True), (2361, 625, 17, True), (100, 2361, 17, False),
+                             (200, 289, 25, True), (37, 53, 17, False)):
+        Ad = torch.randn(M, K, generator=g, dtype=torch.float64).cuda()
+        Bd = torch.randn(K, Nc, generator=g, dtype=torch.float64).cuda()
+        Tn = torch.randn(M, Nc, Nc, generator=g, dtype=torch.float64)
+        ref = torch.einsum("mab,mb->ma", Tn, Ad.cpu() @ Bd.cpu()) if with_t else Ad.cpu() @ Bd.cpu()
+        Td = Tn.contiguous().cuda() if with_t else None
+        row_major = product(M, K, Nc, Ad, Bd, Td, 0)
+        assert (row_major - ref).abs().max().item() < 1e-13 * max(1.0, ref.abs().max().item()), (M, K, Nc)
+        for lay in (SK_A_TILED, SK_A_TILED | SK_B_TILED, SK_A_TILED | SK_B_TILED | SK_C_TILED):
+            assert torch.equal(product(M, K, Nc, Ad, Bd, Td, lay), row_major), (M, K, Nc, lay)
+
+
 @pytest.mark.parametrize("tmode", ["none", "row", "shared"])
 def test_gemm_skinny(torch_mod, tmode):
     """Skinny grouped product (both A orientations, ragged problems, Nc 5/17/32) with the

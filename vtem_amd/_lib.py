@@ -71,6 +71,7 @@ SIGNATURES = {
     "vt_kron_trace_batched": [P, P, P, P, P, I, P, I, D, P, I, P],
     "vt_row_sumsq": [P, L, I, I, P, P],
     "vt_transpose_f64": [P, L, I, I, P, L, P],
+    "vt_tile_f64": [P, L, I, I, I, P, P],
     "vt_symmetrize": [P, I, L, I, P],
     "vt_potrf_f64": [P, I, I, P, L, P, P],
     "vt_potrf_f32": [P, I, I, P, L, P, P],
@@ -109,7 +110,7 @@ SIGNATURES = {
     "vt_svd_pinv_apply": [P, L, I, P, D, P, P, P],
     "vt_svd_eig_finish": [P, L, P, L, I, D, P, P, L, P, P],
     "vt_stitch_accumulate": [P, L, L, P, I, L, P],
-    "vt_kron_rnn": [P, I, I, P, I, I, I, I, P, P, P, P, P, P, L, I, I, P, P],
+    "vt_kron_rnn": [P, I, I, P, I, I, I, I, P, P, P, P, P, P, L, I, I, P, P, P, L, P],
     "vt_kron_rnn_persistent": [P, P, I, I, P, I, P, L, P, P, P, P, P, I, P, L, P, L, I, P, D, P],
     "vt_kron_rnn_persistent_capacity": [P, P],
     "vt_kron_rnn_persistent_bar_words": [I, P],

@@ -30,7 +30,7 @@ import torch
 from ._lib import VtemError, lib, ptr, stream_ptr
 from .autotune import gram_tuner
 from .block import (F64, I32, LANCZOS_STEPS, MAT_DTYPE, Decoder, DecodeResult, SpikeTrains, mat_descs,
-                    split_T_size)
+                    split_T_size, tile_doubles)
 
 SPIKE_DTYPE = np.dtype([("spikes", np.uint64), ("counts", np.uint64), ("ld", np.int64),
                         ("max_rows", np.int32), ("neuron_base", np.int32), ("t0", np.float64),
@@ -63,6 +63,8 @@ assert SPIKE_DTYPE.itemsize == 48 and TJ_DTYPE.itemsize == 40 and GEMM_DTYPE.ite
 assert TN_DTYPE.itemsize == 64 and DIAG_DTYPE.itemsize == 24
 assert GRAM_LP_DTYPE.itemsize == 72 and GRAM64_DTYPE.itemsize == 56 and KPM_DTYPE.itemsize == 48
 assert SK_DTYPE.itemsize == 104
+# SK_DTYPE "pad" bits: operands stored in the MFMA tile layout (vt_tile_f64)
+SK_A_TILED, SK_B_TILED, SK_C_TILED = 1, 2, 4
 
 
 def _to_dev(arr: np.ndarray, device) -> torch.Tensor:
@@ -776,9 +778,11 @@ class BatchDecoder:
 
     def _skinny(self, rows):
         """Skinny grouped product: rows of (A, B, C, T, active, sam, sak, ldb, ldc, tstride,
-        M, K, Nc) with pointers as ints (T = 0: no row factor; active = 0: always run).
-        Returns the launch record (descriptors, max_M, max_K, max_Nc, split-K workspace)."""
-        R = np.array(rows, dtype=np.int64).reshape(len(rows), 13)
+        M, K, Nc[, layout]) with pointers as ints (T = 0: no row factor; active = 0: always
+        run; layout: the SK_*_TILED bits, default 0).  Returns the launch record
+        (descriptors, max_M, max_K, max_Nc, split-K workspace)."""
+        rows = [tuple(r) + (0,) * (14 - len(r)) for r in rows]
+        R = np.array(rows, dtype=np.int64).reshape(len(rows), 14)
         wsz = skinny_workspace_array(R[:, 10], R[:, 11], R[:, 12])
         ws = torch.empty(max(1, int(wsz.sum())), dtype=F64, device=self.device)
         woff = np.concatenate([[0], np.cumsum(wsz)[:-1]])
@@ -787,34 +791,47 @@ class BatchDecoder:
             d[name] = R[:, j].astype(np.uint64)
         d["ws"] = np.where(wsz > 0, ws.data_ptr() + 8 * woff, 0).astype(np.uint64)
         d["active"] = R[:, 4].astype(np.uint64)
-        for j, name in zip(range(5, 13), ("sam", "sak", "ldb", "ldc", "tstride", "M", "K", "Nc")):
+        for j, name in zip(range(5, 14), ("sam", "sak", "ldb", "ldc", "tstride", "M", "K", "Nc", "pad")):
             d[name] = R[:, j]
         return _Skinny(_to_dev(d, self.device), len(rows), int(R[:, 10].max()), int(R[:, 11].max()),
                        int(R[:, 12].max()), ws)
 
-    def _operator(self, blocks, src, dst, Z, dbj, nzj, active=None, block_order=False):
-        """dst_b = A_b src_b for every block (blocks: (S, Tm_ptr, N, Ds, Dt)) as two skinny
-        launches: Z_j = T_j (S src)_j (T fused in the epilogue), dst = S^T Z (from the
-        decoder's resident S^T) -- 4 N D flops, S read twice, no Gram.  With `active` (per-block flag pointers) converged
-        blocks skip both products.  block_order: the forward descriptors in block order too
-        (as the persistent RNN launch indexes them)."""
+    def _operator(self, blocks, src, dst, Z, dbj, nzj, active=None, block_order=False, tiled=None):
+        """dst_b = A_b src_b for every block (blocks: (S, Tm_ptr, N, Ds, Dt, decoder)) as two
+        skinny launches: Z_j = T_j (S src)_j (T fused in the epilogue), dst = S^T Z -- 4 N D
+        flops, S read twice, no Gram.  Both products read the decoder's resident S and S^T
+        in the MFMA tile layout.  With `active` (per-block flag pointers) converged blocks
+        skip both products.  block_order: the forward descriptors in block order too (as the
+        persistent RNN launch indexes them).  tiled = (xt, xt offsets, zt, zt offsets): src
+        is read from its tiled copy xt (kept by the RNN update) and Z is written and read
+        tiled in zt (zeroed by the caller), each block's at its offset (doubles)."""
         act = active or [0] * len(blocks)
         # forward descriptors largest-first (their order changes no result; the backward
         # descriptors stay in block order, which the fused RNN update indexes)
         order = list(range(len(blocks))) if block_order else \
             sorted(range(len(blocks)), key=lambda b: -blocks[b][2] * blocks[b][3])
-        fwd = self._skinny([(S.data_ptr(), src.data_ptr() + 8 * int(dbj[b]), Z.data_ptr() + 8 * int(nzj[b]),
-                             Tp, act[b], S.stride(0), 1, Dt, Dt, Dt * Dt, N, Ds, Dt)
-                            for b in order for (S, Tp, N, Ds, Dt) in [blocks[b][:5]]])
-
+        rows = []
+        for b in order:
+            S, Tp, N, Ds, Dt, dec = blocks[b][:6]
+            if tiled is None:
+                B, ldb, C, ldc, lay = src.data_ptr() + 8 * int(dbj[b]), Dt, Z.data_ptr() + 8 * int(nzj[b]), Dt, 0
+            else:
+                xt, xoff, zt, zoff = tiled
+                B, ldb = xt.data_ptr() + 8 * int(xoff[b]), -(-Ds // 16)
+                C, ldc = zt.data_ptr() + 8 * int(zoff[b]), -(-N // 16)
+                lay = SK_B_TILED | SK_C_TILED
+            rows.append((dec.S_tiled.data_ptr(), B, C, Tp, act[b], -(-Ds // 16), 1, ldb, ldc, Dt * Dt, N, Ds, Dt,
+                         SK_A_TILED | lay))
+        fwd = self._skinny(rows)
         rows = []
         for b, blk in enumerate(blocks):
-            S, Tp, N, Ds, Dt = blk[:5]
-            St = blk[5] if len(blk) > 5 else None
-            # the resident S^T streams rows like the forward product; else S read by columns
-            A, sam, sak = (St.data_ptr(), St.stride(0), 1) if St is not None else (S.data_ptr(), 1, S.stride(0))
-            rows.append((A, Z.data_ptr() + 8 * int(nzj[b]), dst.data_ptr() + 8 * int(dbj[b]), 0, act[b],
-                         sam, sak, Dt, Dt, 0, Ds, N, Dt))
+            S, Tp, N, Ds, Dt, dec = blk[:6]
+            if tiled is None:
+                B, ldb, lay = Z.data_ptr() + 8 * int(nzj[b]), Dt, 0
+            else:
+                B, ldb, lay = tiled[2].data_ptr() + 8 * int(tiled[3][b]), -(-N // 16), SK_B_TILED
+            rows.append((dec.St_tiled.data_ptr(), B, dst.data_ptr() + 8 * int(dbj[b]), 0, act[b],
+                         -(-N // 16), 1, ldb, Dt, 0, Ds, N, Dt, SK_A_TILED | lay))
         bwd = self._skinny(rows)
         return fwd, bwd
 
@@ -936,7 +953,7 @@ class BatchDecoder:
                 keep += [c, TiT, tws]
             for b, d in enumerate(g.decs):
                 blocks.append((d.S, g.Tm.data_ptr() + 8 * int(g.nbase[b]) * E, int(g.N[b]), int(g.Ds[b]), g.Dt,
-                               d.St))
+                               d))
                 c_ptrs.append(c.data_ptr() + 8 * int(g.nbase[b]))
                 if not kron_sum:
                     TiT_ptrs.append(TiT.data_ptr() + 8 * b * E)
@@ -1086,7 +1103,7 @@ class BatchDecoder:
             E = g.Dt * g.Dt
             for b, d in enumerate(g.decs):
                 blocks.append((d.S, g.Tm.data_ptr() + 8 * int(g.nbase[b]) * E, int(g.N[b]), int(g.Ds[b]), g.Dt,
-                               d.St))
+                               d))
             boff.append((b0, e0))
             b0 += g.B
             e0 += g.Dtot
@@ -1146,18 +1163,23 @@ class BatchDecoder:
         seg = _h2d(dbj.astype(np.int64), dev)
         X = torch.empty(Dtot, dtype=F64, device=dev)
         Q = torch.empty(Dtot, dtype=F64, device=dev)
-        Z = torch.empty(int(nzj[-1]), dtype=F64, device=dev)
-        fx, bx = self._operator(blocks, X, Q, Z, dbj, nzj)
         st = torch.zeros(4 * Bt, dtype=F64, device=dev)
         maxNc = int(Dt.max())
         plan = self._rnn_plan(N, Ds, Dt) if self.rnn_persistent else None
         extra = []
         if plan is None:
+            tiles = self.rnn_tiles(N, Ds, Dt)
+            Xt, xoff, Z, _ = tiles
+            xoff_d = _h2d(xoff[:-1].astype(np.int64), dev)
+            fx, bx = self._operator(blocks, X, Q, Z, dbj, nzj, tiled=tiles)
             cnt = torch.empty(Bt * -(-int(Ds.max()) // 16), dtype=torch.int32, device=dev)
             lib.vt_kron_rnn(*fx.args(), *bx.args(), Bt, maxNc, ptr(X), ptr(r), ptr(Q), ptr(lam), ptr(eta),
-                            ptr(seg), Dtot, steps, self.graph_chunk, ptr(cnt), s)
-            extra = [cnt]
+                            ptr(seg), Dtot, steps, self.graph_chunk, ptr(cnt), ptr(Xt), ptr(xoff_d),
+                            int(xoff[-1]), s)
+            extra = [cnt, Xt, xoff_d]
         else:
+            Z = torch.empty(int(nzj[-1]), dtype=F64, device=dev)
+            fx, bx = self._operator(blocks, X, Q, Z, dbj, nzj)
             gstart, grid = plan
             fpo, bpo = self._operator(blocks, X, Q, Z, dbj, nzj, block_order=True)
             gs_d = _h2d(gstart.astype(np.int32), dev)
@@ -1179,6 +1201,19 @@ class BatchDecoder:
         nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + 4.0 * D)
         self._toc("rnn_iterate", e0, float(nbytes) * (steps + 1))
         return X, st, boff, [r, lam, seg, X, Q, Z, fx, bx, st, eta] + extra
+
+    def rnn_tiles(self, N, Ds, Dt):
+        """The RNN launch form's iterate and Z in the MFMA tile layout, the forward's and the
+        backward's B operands: per block the tiles of X_b^T (Dt x Ds) and of Z_b^T (Dt x N).
+        Returns (Xt, X offsets, Zt, Z offsets) with the offsets in doubles (one more entry:
+        the total); Zt is zeroed, as its padding is never written."""
+        xsz = np.array([tile_doubles(int(t), int(d)) for t, d in zip(Dt, Ds)], np.int64)
+        zsz = np.array([tile_doubles(int(t), int(n)) for t, n in zip(Dt, N)], np.int64)
+        xoff = np.concatenate([[0], np.cumsum(xsz)])
+        zoff = np.concatenate([[0], np.cumsum(zsz)])
+        Xt = torch.empty(int(xoff[-1]), dtype=F64, device=self.device)
+        Zt = torch.zeros(int(zoff[-1]), dtype=F64, device=self.device)
+        return Xt, xoff, Zt, zoff
 
     def _rnn_plan(self, N, Ds, Dt):
         """Workgroup groups of the persistent RNN launch, or None where the launch form

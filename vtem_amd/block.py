@@ -238,6 +238,12 @@ class DecodeResult:
     state: Optional[torch.Tensor] = None   # iterative solver state (CG: rz, |r0|^2, active, iterations)
 
 
+def tile_doubles(M: int, K: int) -> int:
+    """Doubles of an M x K matrix in the skinny product's MFMA tile layout (vt_tile_f64):
+    ceil(M/16) * ceil(K/16) tiles of 256."""
+    return -(-M // 16) * -(-K // 16) * 256
+
+
 class Decoder:
     """Coefficient-space decoder for one block geometry and neuron set.
 
@@ -261,13 +267,15 @@ class Decoder:
         self._tn_ws = torch.empty(16 * geom.Ds * geom.Dt, dtype=F64, device=device)
         self.s2 = torch.empty(N, dtype=F64, device=device)
         lib.vt_row_sumsq(ptr(self.S), self.S.stride(0), N, geom.Ds, ptr(self.s2), stream_ptr())
-        # S^T (Ds x N), resident beside S: the operator's product Q = S^T Z reads it by rows
-        self.St = torch.empty((geom.Ds, N), dtype=F64, device=device)
-        lib.vt_transpose_f64(ptr(self.S), self.S.stride(0), N, geom.Ds, ptr(self.St), N,
-                             stream_ptr())
+        # S and S^T resident in the skinny product's MFMA tile layout (vt_tile_f64): the
+        # operator's products Z = T_j (S X)_j and Q = S^T Z take them as their A operands
+        self.S_tiled = torch.empty(tile_doubles(N, geom.Ds), dtype=F64, device=device)
+        lib.vt_tile_f64(ptr(self.S), self.S.stride(0), N, geom.Ds, 0, ptr(self.S_tiled), stream_ptr())
+        self.St_tiled = torch.empty(tile_doubles(geom.Ds, N), dtype=F64, device=device)
+        lib.vt_tile_f64(ptr(self.S), self.S.stride(0), N, geom.Ds, 1, ptr(self.St_tiled), stream_ptr())
 
     def with_iaf(self, kappa, delta, bias, gain: float = 1.0) -> "Decoder":
-        """The same decoder (shared S, S^T and tables) for a population with other IAF
+        """The same decoder (shared S, its tiled copies and tables) for a population with other IAF
         parameters.  gain c: the neurons integrated c u + b, which decodes as u with
         kappa / c and b / c (q_k = kappa delta - b (t_k+1 - t_k) scales by 1 / c)."""
         d = copy.copy(self)

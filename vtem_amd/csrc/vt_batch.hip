@@ -463,7 +463,7 @@ __global__ void __launch_bounds__(256) skinny_reduce_kernel(const SkDesc *__rest
     __syncthreads();
     if (m < d.M && col < Nc) {
         const double o = d.T ? skinny_row_factor(d, m, col, Nc, rowbuf[rr]) : v;
-        d.C[(long long)m * d.ldc + col] = o;
+        d.C[sk_c_offset(d, m, col)] = o;
     }
 }
 

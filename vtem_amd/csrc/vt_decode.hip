@@ -9,6 +9,7 @@
 // diag_add :334-350, reconstruct :354-420) and the RNN update of
 // vtem/neural_network.py:102-128.
 #include "vt_common.hpp"
+#include "vt_skinny.hpp"
 
 #define SQRT2 1.4142135623730950488
 
@@ -568,6 +569,37 @@ VT_EXPORT int vt_transpose_f64(const double *src, long long lds, int rows, int c
     VT_CHECK_ARG(vt_blocks(rows, 32) <= 65535);
     dim3 g(vt_blocks(cols, 32), vt_blocks(rows, 32));
     transpose_kernel<<<g, 256, 0, (hipStream_t)stream>>>(src, lds, rows, cols, dst, ldd);
+    VT_LAUNCH_CHECK();
+    return VT_OK;
+}
+
+// dst = A in the skinny product's MFMA tile layout (sk_tile_index), A = src (rows x cols,
+// row stride lds) or its transpose (trans): ceil(M / 16) * ceil(K / 16) tiles of 256 doubles
+// for A of M x K, entries beyond M or K zero.  One thread per destination element.
+__global__ void __launch_bounds__(256)
+tile_kernel(const double *__restrict__ src, long long lds, int M, int K, int trans,
+            double *__restrict__ dst, long long total)
+{
+    const long long kc = (K + 15) / 16;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long tile = i >> 8;
+        const int w = (int)(i & 255), half = w >> 7, lane = (w & 127) >> 1, s = 2 * half + (w & 1);
+        const long long m = 16 * (tile / kc) + (lane & 15), k = 16 * (tile % kc) + 4 * (lane >> 4) + s;
+        double v = 0.0;
+        if (m < M && k < K) v = trans ? src[k * lds + m] : src[m * lds + k];
+        dst[i] = v;
+    }
+}
+
+VT_EXPORT int vt_tile_f64(const double *src, long long lds, int rows, int cols, int trans, double *dst,
+                          void *stream)
+{
+    VT_CHECK_ARG(src && dst && rows >= 1 && cols >= 1 && lds >= cols && (trans == 0 || trans == 1));
+    const int M = trans ? cols : rows, K = trans ? rows : cols;
+    const long long total = (long long)vt_blocks(M, 16) * vt_blocks(K, 16) * 256;
+    const long long g = vt_blocks(total, 256);
+    tile_kernel<<<g > 8192 ? 8192 : g, 256, 0, (hipStream_t)stream>>>(src, lds, M, K, trans, dst, total);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }

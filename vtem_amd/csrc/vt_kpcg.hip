@@ -737,10 +737,14 @@ __global__ void zero_kernel(double *__restrict__ x, long long n)
 // order, as the product's own second pass does, updates the rows and resets the counter
 // for the next iteration.  One launch per iteration fewer than a separate update, same
 // values: the counters only pick which workgroup does the update.
+// xt (optional): the block's iterate also stored as the forward product's tiled B (the
+// tiles of x^T, SK_B_TILED, ceil(M / 16) tiles per row), each element where the update
+// writes it.
 template <int NT, bool X16>
 __device__ __forceinline__ void rnn_bwd_tile(const SkDesc &d, int m0, int z, double *__restrict__ x,
                                              const double *__restrict__ r, double lam, double eta,
-                                             long long s0, unsigned *cnt, double (&red)[SK_WAVES / 2][16][33])
+                                             long long s0, unsigned *cnt, double *__restrict__ xt,
+                                             double (&red)[SK_WAVES / 2][16][33])
 {
     __shared__ int last;
     const int ks = skinny_splits(d.K), Nc = d.Nc;
@@ -769,7 +773,9 @@ __device__ __forceinline__ void rnn_bwd_tile(const SkDesc &d, int m0, int z, dou
             qi = __hip_atomic_load(d.C + loc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         const long long i = s0 + loc;
-        x[i] = rnn_step(x[i], r[i], qi, lam, eta);
+        const double xn = rnn_step(x[i], r[i], qi, lam, eta);
+        x[i] = xn;
+        if (xt) xt[sk_tile_index(e % Nc, m0 + e / Nc, (d.M + 15) / 16)] = xn;
     }
 }
 
@@ -778,7 +784,8 @@ template <int NT, bool X16>
 __global__ void __launch_bounds__(64 * SK_WAVES)
 rnn_bwd_update_kernel(const SkDesc *__restrict__ descs, double *__restrict__ x, const double *__restrict__ r,
                       const double *__restrict__ lam, const double *__restrict__ eta,
-                      const long long *__restrict__ seg, unsigned *__restrict__ cnt)
+                      const long long *__restrict__ seg, unsigned *__restrict__ cnt,
+                      double *__restrict__ xt, const long long *__restrict__ xtoff)
 {
     __shared__ double red[SK_WAVES / 2][16][33];
     const int b = blockIdx.y;
@@ -788,34 +795,35 @@ rnn_bwd_update_kernel(const SkDesc *__restrict__ descs, double *__restrict__ x, 
     unsigned *c = cnt + (long long)b * gridDim.x + blockIdx.x;
     const double l = lam[2 * b], et = eta[b];
     const long long s0 = seg[b];
+    double *xb = xt ? xt + xtoff[b] : nullptr;
     if constexpr (NT == 1) {
         if (X16 && d.Nc == 17)
-            rnn_bwd_tile<1, true>(d, m0, blockIdx.z, x, r, l, et, s0, c, red);
+            rnn_bwd_tile<1, true>(d, m0, blockIdx.z, x, r, l, et, s0, c, xb, red);
         else
-            rnn_bwd_tile<1, false>(d, m0, blockIdx.z, x, r, l, et, s0, c, red);
+            rnn_bwd_tile<1, false>(d, m0, blockIdx.z, x, r, l, et, s0, c, xb, red);
     } else {
         if (d.Nc == 17)
-            rnn_bwd_tile<1, true>(d, m0, blockIdx.z, x, r, l, et, s0, c, red);
+            rnn_bwd_tile<1, true>(d, m0, blockIdx.z, x, r, l, et, s0, c, xb, red);
         else if (d.Nc <= 16)
-            rnn_bwd_tile<1, false>(d, m0, blockIdx.z, x, r, l, et, s0, c, red);
+            rnn_bwd_tile<1, false>(d, m0, blockIdx.z, x, r, l, et, s0, c, xb, red);
         else
-            rnn_bwd_tile<2, false>(d, m0, blockIdx.z, x, r, l, et, s0, c, red);
+            rnn_bwd_tile<2, false>(d, m0, blockIdx.z, x, r, l, et, s0, c, xb, red);
     }
 }
 
 static int rnn_bwd_update(const VtSkinnyLaunch &Bk, int batch, int max_Nc, double *x, const double *r,
                           const double *lam, const double *eta, const long long *seg, unsigned *cnt,
-                          hipStream_t s)
+                          double *xt, const long long *xtoff, hipStream_t s)
 {
     if (Bk.max_M <= 0) return VT_OK;
     const dim3 grid(vt_blocks(Bk.max_M, 16), batch, skinny_splits(Bk.max_K));
     const SkDesc *d = (const SkDesc *)Bk.descs;
     if (max_Nc <= 16)
-        rnn_bwd_update_kernel<1, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, x, r, lam, eta, seg, cnt);
+        rnn_bwd_update_kernel<1, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, x, r, lam, eta, seg, cnt, xt, xtoff);
     else if (max_Nc == 17)
-        rnn_bwd_update_kernel<1, true><<<grid, 64 * SK_WAVES, 0, s>>>(d, x, r, lam, eta, seg, cnt);
+        rnn_bwd_update_kernel<1, true><<<grid, 64 * SK_WAVES, 0, s>>>(d, x, r, lam, eta, seg, cnt, xt, xtoff);
     else
-        rnn_bwd_update_kernel<2, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, x, r, lam, eta, seg, cnt);
+        rnn_bwd_update_kernel<2, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, x, r, lam, eta, seg, cnt, xt, xtoff);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
@@ -1075,25 +1083,33 @@ VT_EXPORT int vt_kron_lanczos(const void *fwd, int fwd_M, int fwd_K, const void 
 // steps+1 gradient-flow iterations from x = 0: per iteration the forward operator launch
 // (reads x) and the backward launch with the update fused (writes x); graph_chunk >= 2
 // replays them as HIP graphs of that many iterations (vt_graph_loop), 0 launches each.  cnt: batch * ceil(bwd_M / 16) words of row-tile counters (zeroed here).
+// xt (optional, n_xt doubles, zeroed here): the iterate's tiled copy that forward
+// descriptors with SK_B_TILED read, block b's at xt + xtoff[b]; null: x row-major.
 VT_EXPORT int vt_kron_rnn(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                           int batch, int max_Nc, double *x, const double *r, double *q,
                           const double *lam, const double *eta, const long long *seg, long long n,
-                          int steps, int graph_chunk, unsigned *cnt, void *stream)
+                          int steps, int graph_chunk, unsigned *cnt, double *xt, const long long *xtoff,
+                          long long n_xt, void *stream)
 {
     VT_CHECK_ARG(fwd && bwd && x && r && q && lam && eta && seg && cnt && n >= 0 && steps >= 0 &&
-                 graph_chunk >= 0 && batch >= 1 && batch <= 65535 && max_Nc >= 1 && max_Nc <= 32);
+                 graph_chunk >= 0 && batch >= 1 && batch <= 65535 && max_Nc >= 1 && max_Nc <= 32 &&
+                 (!xt || (xtoff && n_xt > 0)));
     const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
     hipStream_t s = (hipStream_t)stream;
     unsigned g = vt_blocks(n, 256);
     if (g > 4096) g = 4096;
     zero_kernel<<<g, 256, 0, s>>>(x, n);   // kernels, not memset nodes, if captured
+    if (xt) {
+        unsigned gx = vt_blocks(n_xt, 256);
+        zero_kernel<<<gx > 4096 ? 4096 : gx, 256, 0, s>>>(xt, n_xt);
+    }
     const long long words = (long long)batch * vt_blocks(bwd_M > 0 ? bwd_M : 1, 16);
     const unsigned gw = vt_blocks(words, 256);
     zero_words_kernel<<<gw < 1024 ? gw : 1024, 256, 0, s>>>(cnt, words);
     VT_LAUNCH_CHECK();
     auto step = [&](hipStream_t st) -> int {
         VT_TRY(skinny_run(F, batch, max_Nc, st));
-        VT_TRY(rnn_bwd_update(Bk, batch, max_Nc, x, r, lam, eta, seg, cnt, st));
+        VT_TRY(rnn_bwd_update(Bk, batch, max_Nc, x, r, lam, eta, seg, cnt, xt, xtoff, st));
         return VT_OK;
     };
     return vt_graph_loop(s, steps + 1, graph_chunk, step);

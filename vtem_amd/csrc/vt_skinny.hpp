@@ -52,6 +52,32 @@ __device__ __forceinline__ double rnn_step(double x, double r, double q, double 
 #define SK_INFLIGHT 2      // chunks each wave has in flight
 #endif
 
+// SkDesc.pad bits.  SK_A_TILED: A is stored in MFMA tiles (sk_tile_index), sam = the tiles
+// per row of tiles (ceil(K / 16)); rows and columns beyond M and K are zeros.  SK_B_TILED: B
+// is stored as the tiles of its transpose (row n of the tiles is column n of B), ldb = the
+// tiles per row (ceil(K / 16)), with two rows of tiles when Nc > 16.  SK_C_TILED: C is
+// stored the same way (the B layout of a product that takes C as its B), ldc = ceil(M / 16);
+// only its entries are written, so the caller zeroes the padding once.
+#define SK_A_TILED 1
+#define SK_B_TILED 2
+#define SK_C_TILED 4
+
+// Position of A[m][k] in the tiled layout: tile (m / 16, k / 16) holds 256 doubles, and lane
+// L = r + 16 g of a wave (row r, k group g) finds its four values of the tile (k = 4g + s)
+// as two pairs, s = 0, 1 at 2L and s = 2, 3 at 128 + 2L: each of a wave's two 16-byte loads
+// reads 1 KB in one piece.
+__host__ __device__ __forceinline__ long long sk_tile_index(long long m, long long k, long long tiles_per_row)
+{
+    const long long r = m & 15, kk = k & 15, g = kk >> 2, s = kk & 3;
+    return ((m >> 4) * tiles_per_row + (k >> 4)) * 256 + (s >> 1) * 128 + (16 * g + r) * 2 + (s & 1);
+}
+
+// offset of C[m][col] in the problem's C layout
+__device__ __forceinline__ long long sk_c_offset(const SkDesc &d, int m, int col)
+{
+    return (d.pad & SK_C_TILED) ? sk_tile_index(col, m, d.ldc) : (long long)m * d.ldc + col;
+}
+
 // operands of one 16-wide K chunk for lane (r, g); chunks at or beyond c_end load zeros
 template <int NT, int RT>
 __device__ __forceinline__ void skinny_load(const SkDesc &d, int m0, int c, int c_end, int r, int g,
@@ -59,25 +85,59 @@ __device__ __forceinline__ void skinny_load(const SkDesc &d, int m0, int c, int 
 {
     const int kb = c * 16 + 4 * g;
     const bool cin = c < c_end;
+    typedef double d2a16 __attribute__((ext_vector_type(2)));
+    if (d.pad & SK_A_TILED) {
 #pragma unroll
-    for (int t = 0; t < RT; ++t) {
-        const int m = m0 + 16 * t + r;
-        if (d.sak == 1 && cin && m < d.M && kb + 3 < d.K) {
-            // row-major A: the lane's 4 consecutive k values as two 16-byte loads
-            typedef double d2a8 __attribute__((ext_vector_type(2), aligned(8)));
-            const double *p = d.A + (long long)m * d.sam + kb;
-            const d2a8 x = *(const d2a8 *)p, y = *(const d2a8 *)(p + 2);
-            a[t][0] = x[0];
-            a[t][1] = x[1];
-            a[t][2] = y[0];
-            a[t][3] = y[1];
-            continue;
+        for (int t = 0; t < RT; ++t) {
+            const int mt = (m0 >> 4) + t;
+            if (cin && 16 * mt < d.M) {
+                const double *p = d.A + ((long long)mt * d.sam + c) * 256 + 2 * (16 * g + r);
+                const d2a16 x = *(const d2a16 *)p, y = *(const d2a16 *)(p + 128);
+                a[t][0] = x[0];
+                a[t][1] = x[1];
+                a[t][2] = y[0];
+                a[t][3] = y[1];
+            } else {
+                a[t][0] = a[t][1] = a[t][2] = a[t][3] = 0.0;
+            }
         }
+    } else {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int k = kb + s;
-            a[t][s] = (cin && m < d.M && k < d.K) ? d.A[(long long)m * d.sam + (long long)k * d.sak] : 0.0;
+        for (int t = 0; t < RT; ++t) {
+            const int m = m0 + 16 * t + r;
+            if (d.sak == 1 && cin && m < d.M && kb + 3 < d.K) {
+                // row-major A: the lane's 4 consecutive k values as two 16-byte loads
+                typedef double d2a8 __attribute__((ext_vector_type(2), aligned(8)));
+                const double *p = d.A + (long long)m * d.sam + kb;
+                const d2a8 x = *(const d2a8 *)p, y = *(const d2a8 *)(p + 2);
+                a[t][0] = x[0];
+                a[t][1] = x[1];
+                a[t][2] = y[0];
+                a[t][3] = y[1];
+                continue;
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int k = kb + s;
+                a[t][s] = (cin && m < d.M && k < d.K) ? d.A[(long long)m * d.sam + (long long)k * d.sak] : 0.0;
+            }
         }
+    }
+    if (d.pad & SK_B_TILED) {
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            if (cin && 16 * j < Nc) {
+                const double *p = d.B + ((long long)j * d.ldb + c) * 256 + 2 * (16 * g + r);
+                const d2a16 x = *(const d2a16 *)p, y = *(const d2a16 *)(p + 128);
+                bv[j][0] = x[0];
+                bv[j][1] = x[1];
+                bv[j][2] = y[0];
+                bv[j][3] = y[1];
+            } else {
+                bv[j][0] = bv[j][1] = bv[j][2] = bv[j][3] = 0.0;
+            }
+        }
+        return;
     }
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
@@ -121,6 +181,21 @@ __device__ __forceinline__ void skinny_load_x16(const SkDesc &d, int c, int c_en
 {
     const int kb = c * 16 + 4 * g;
     const bool cin = c < c_end;
+    if (d.pad & SK_B_TILED) {
+        // column 16 is row 0 of the second row of tiles: lane group g's pairs at 2 (16 g)
+        typedef double d2a16 __attribute__((ext_vector_type(2)));
+        if (cin && d.Nc > 16) {
+            const double *p = d.B + ((long long)d.ldb + c) * 256 + 32 * g;
+            const d2a16 u = *(const d2a16 *)p, v = *(const d2a16 *)(p + 128);
+            x[0] = u[0];
+            x[1] = u[1];
+            x[2] = v[0];
+            x[3] = v[1];
+        } else {
+            x[0] = x[1] = x[2] = x[3] = 0.0;
+        }
+        return;
+    }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const int k = kb + s;
@@ -227,7 +302,7 @@ __device__ __forceinline__ void skinny_tile(const SkDesc &d, int m0, int z,
     for (int e = tid; e < rows * Nc; e += 64 * SK_WAVES) {
         const int row = e / Nc, col = e % Nc;
         const double v = d.T ? skinny_row_factor(d, m0 + row, col, Nc, red[0][row]) : red[0][row][col];
-        sk_store<COH>(d.C + (long long)(m0 + row) * d.ldc + col, v);
+        sk_store<COH>(d.C + sk_c_offset(d, m0 + row, col), v);
     }
 }
 
