@@ -401,7 +401,7 @@ int vt_kpcg_step_b(const double *r, const double *z, double *p, const long long 
  * launch (the last row tile of a block to finish runs it), the preconditioner (skinny
  * products pre, z = Mi r Ti) and step_b.  Each product is given as its descriptors and
  * its max_M / max_K; max_Nc bounds every problem's column count; bwd's descriptors are
- * in block order.  cnt: (batch + 1) * ceil(bwd_M / 16) words of counters (zeroed here).
+ * in block order.  cnt: batch * (ceil(bwd_M / 16) + 1) words of counters (zeroed here).
  * graph_chunk >= 2 replays the iterations as a HIP graph of up to graph_chunk (at most
  * iters/2) iterations, as vt_kron_rnn does; 0 launches each.  Same results either way. */
 int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,

@@ -848,6 +848,30 @@ def test_kpcg_batch(torch_mod, oracle, pre):
                                       err_msg=f"block {i} {specs[i]}")
 
 
+def test_kpcg_more_blocks_than_row_tiles(torch_mod):
+    """The fused CG iteration's counters (one per row tile of every block, then one per
+    block) fill batch * (row tiles + 1) words: with more blocks (12) than row tiles (6 for
+    Ds = 81) and the allocator's small blocks left holding all-ones words, every block
+    still iterates and reaches the 1e-10 relative residual."""
+    torch = torch_mod
+    from vtem_amd import synthetic as syn
+    from vtem_amd.batch import BatchDecoder, BlockJob
+    _banner("kpcg: 12 blocks of 6 row tiles on reused memory")
+    jobs = []
+    for i in range(12):
+        geom, st, dec = _encode_block(torch, syn.serve_block(32, 0.5, seed=800 + i))
+        jobs.append(BlockJob(dec, st))
+    junk = [torch.full((64 * 1024,), -1, dtype=torch.int32, device="cuda") for _ in range(256)]
+    torch.cuda.synchronize()
+    del junk
+    res = BatchDecoder(precision="kpcg", verify=False, preconditioner="kron").decode(jobs)
+    for i, r in enumerate(res):
+        its = float(r.state.cpu().numpy()[3])
+        h = r.resid.cpu().numpy()
+        rel = float(np.sqrt(h[1] / h[0]))
+        assert its > 0 and rel < 1e-10, (i, its, rel)
+
+
 def test_kpcg_retry_guard(torch_mod, oracle):
     """With too few CG iterations every block misses the 1e-10 guard: the guard gives each
     a correction solve of retry_iters iterations from its iterate (no FP64 redo), which

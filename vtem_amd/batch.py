@@ -1016,7 +1016,7 @@ class BatchDecoder:
             pr.run(s)
         lib.vt_kpcg_init(ptr(R), ptr(Zp), ptr(Pv), ptr(X), ptr(seg), Bt, ptr(st), ptr(hist), hs, s)
         pargs = [pr.args5() for pr in pres] + [(0, 0, 0, 0, 0)] * (3 - len(pres))
-        kcnt = torch.empty((Bt + 1) * -(-int(Ds.max()) // 16), dtype=torch.int32, device=dev)
+        kcnt = torch.empty(Bt * (-(-int(Ds.max()) // 16) + 1), dtype=torch.int32, device=dev)
         lib.vt_kpcg_iterate(*fp.args(), *bp.args(), *pargs[0], *pargs[1], *pargs[2], Bt, maxNc, ptr(X), ptr(R),
                             ptr(Pv), ptr(Q), ptr(Zp), ptr(lam), ptr(seg), ptr(st), ptr(hist), hs, iters,
                             float(self.cg_stop) ** 2, self.prune, self.graph_chunk, ptr(kcnt), s)

@@ -689,7 +689,7 @@ VT_EXPORT int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void 
                             {pre3, pre3_n, pre3_M, pre3_K, pre3_Nc}};
     hipStream_t s0 = (hipStream_t)stream;
     const int ntile = (int)vt_blocks(bwd_M, 16);
-    const long long words = (long long)(batch + 1) * ntile;
+    const long long words = (long long)batch * (ntile + 1);   // row-tile counters, then one per block
     const unsigned gw = vt_blocks(words, 256);
     zero_words_kernel<<<gw < 1024 ? gw : 1024, 256, 0, s0>>>(cnt, words);   // a kernel, not a memset node
     VT_LAUNCH_CHECK();
