@@ -498,12 +498,7 @@ def cpu_baseline(items, budget_s):
         lam = O.tikhonov_lambda(A, tk1.size)
         r = O.rhs_kron(S, tv)
         if job.method == "rnn":
-            v = np.ones(r.size)                   # 50 power steps for eta = 1/lambda_max
-            for _ in range(50):
-                w = A @ v + lam * v
-                rho = float(v @ w) / float(v @ v)
-                v = w / np.linalg.norm(w)
-            X = O.rnn_dspace(A, r, 1.0 / rho, job.rnn_steps, lam)
+            X = O.rnn_dspace(A, r, O.lanczos_eta(A, lam), job.rnn_steps, lam)
         else:
             X = O.solve_tikhonov(A, r, lam)
         O.synthesize(X, B, O.temporal_basis(geom.frame_times(), geom.Lt, geom.W))
