@@ -353,6 +353,11 @@ int vt_diag_add_batched(const void *descs, int batch, int dtype, const double *l
  * reports; max_K bounds every problem's K.  The structured operator S^T (T_j (S X)) of the coefficient-space
  * Gram (vtem/dirichlet.py:62-85 builds that Gram in spike space) is two such products. */
 int vt_gemm_skinny(const void *descs, int batch, int max_M, int max_K, int max_Nc, void *stream);
+/* The skinny product's K-split rule, ks = clamp(ceil(K/16) / chunks_per_split, 1,
+ * max_splits), for a host that sizes the split-K workspaces of many problems at once
+ * (vt_gemm_skinny_workspace_size gives one problem's).  Host only.  No reference
+ * counterpart (the reference forms no such operator). */
+int vt_skinny_split_rule(int *chunks_per_split_host, int *max_splits_host);
 /* Doubles of ws one skinny problem (M x K, Nc columns) needs, written to the HOST pointer
  * doubles_host (0 when its reduction is not split). */
 int vt_gemm_skinny_workspace_size(int M, int K, int Nc, long long *doubles_host);

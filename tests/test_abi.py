@@ -53,3 +53,19 @@ def test_error_mapping():
         check(-4, "x")
     with pytest.raises(VtemError):
         check(700, "x")
+
+
+def test_split_workspace_sizes_follow_the_library():
+    """The host sizes many skinny problems' split-K workspaces at once from the rule the
+    library reports; for every K up to 20000 the sizes equal the library's own
+    per-problem answer (host-only calls, no GPU)."""
+    import ctypes
+    import numpy as np
+    from vtem_amd.batch import skinny_workspace_array
+    from vtem_amd._lib import lib
+    K = np.arange(1, 20001, 7)
+    got = skinny_workspace_array(np.full(K.size, 37), K, np.full(K.size, 17))
+    out = ctypes.c_longlong(0)
+    for k, g in zip(K, got):
+        lib.vt_gemm_skinny_workspace_size(37, int(k), 17, ctypes.addressof(out))
+        assert int(out.value) == int(g), int(k)

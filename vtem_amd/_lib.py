@@ -93,6 +93,7 @@ SIGNATURES = {
     "vt_kpcg_place_diag": [P, P, P, I, I, P, L, P],
     "vt_gemm_skinny": [P, I, I, I, I, P],
     "vt_gemm_skinny_workspace_size": [I, I, I, P],
+    "vt_skinny_split_rule": [P, P],
     "vt_kpcg_init": [P, P, P, P, P, I, P, P, I, P],
     "vt_kpcg_step_a": [P, P, P, P, P, P, I, P, P, I, I, D, I, I, P],
     "vt_kpcg_step_b": [P, P, P, P, I, P, P],

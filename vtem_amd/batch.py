@@ -87,11 +87,26 @@ def skinny_workspace(M: int, K: int, Nc: int) -> int:
     return int(out.value)
 
 
+_SPLIT_RULE = None
+
+
+def skinny_splits_array(K) -> np.ndarray:
+    """K splits of skinny problems, by the rule the library reports (vt_skinny_split_rule:
+    one split per chunks_per_split chunks of 16 along K, between 1 and max_splits)."""
+    global _SPLIT_RULE
+    if _SPLIT_RULE is None:
+        c, m = ctypes.c_int(0), ctypes.c_int(0)
+        lib.vt_skinny_split_rule(ctypes.addressof(c), ctypes.addressof(m))
+        _SPLIT_RULE = (int(c.value), int(m.value))
+    cps, mx = _SPLIT_RULE
+    return np.clip(((np.asarray(K, np.int64) + 15) // 16) // cps, 1, mx)
+
+
 def skinny_workspace_array(M, K, Nc) -> np.ndarray:
-    """skinny_workspace for arrays of problems: the library's split rule (one split per 32
-    chunks of 16 along K, between 1 and 8; a workspace of ks M Nc doubles when ks > 1)."""
+    """skinny_workspace for arrays of problems: ks M Nc doubles when a problem's K is split
+    in ks > 1 parts."""
     M, K, Nc = (np.asarray(v, np.int64) for v in (M, K, Nc))
-    ks = np.clip(((K + 15) // 16) // 32, 1, 8)
+    ks = skinny_splits_array(K)
     return np.where(ks > 1, ks * M * Nc, 0)
 
 
@@ -1225,7 +1240,7 @@ class BatchDecoder:
             return None
         cap, mg = _rnn_capacity()
         cap = cap[1 if int(Dt.max()) > 17 else 0]
-        ks = np.clip(((N + 15) // 16) // 32, 1, 8)           # the skinny product's split rule
+        ks = skinny_splits_array(N)                           # the skinny product's split rule
         need = np.maximum(-(-N // 16), -(-Ds // 16) * ks).astype(np.int64)
         g = np.minimum(need, mg)
         if g.sum() > cap:

@@ -467,6 +467,16 @@ __global__ void __launch_bounds__(256) skinny_reduce_kernel(const SkDesc *__rest
     }
 }
 
+// the K-split rule of the skinny product: ks = clamp(ceil(K / 16) / chunks_per_split, 1,
+// max_splits), for hosts that size many problems' workspaces at once
+VT_EXPORT int vt_skinny_split_rule(int *chunks_per_split_host, int *max_splits_host)
+{
+    VT_CHECK_ARG(chunks_per_split_host && max_splits_host);
+    *chunks_per_split_host = VT_SKINNY_CHUNKS_PER_SPLIT;
+    *max_splits_host = 8;
+    return VT_OK;
+}
+
 // workspace (doubles) one problem of the skinny product needs: 0 unless its K is split
 VT_EXPORT int vt_gemm_skinny_workspace_size(int M, int K, int Nc, long long *doubles_host)
 {
