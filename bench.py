@@ -202,7 +202,7 @@ def dominant_kernel(items, max_batch, reps=200):
     nzj = np.concatenate([[0], np.cumsum(N * Dt)])
     X = torch.randn(int(dbj[-1]), dtype=torch.float64, device="cuda")
     Q = torch.empty_like(X)
-    tiles = bd.rnn_tiles(N, Ds, Dt)
+    tiles = bd.operand_tiles(N, Ds, Dt)
     tiles[0].normal_()
     fwd, _ = bd._operator(blocks, X, Q, tiles[2], dbj, nzj, tiled=tiles)
     s = stream_ptr()

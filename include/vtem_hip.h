@@ -387,9 +387,13 @@ int vt_kpcg_place_diag(double *X, const long long *xoff, const int *dim, int bat
  * st: 4 doubles per block {rz, |r0|^2, active, iterations}; hist[b*hist_stride + k] =
  * |r_k|^2.  init: x = 0, p = z, rz = <r,z>.  step_a: q += lam p, alpha = rz/<p,q>,
  * x += alpha p, r -= alpha q, stop when |r|^2 <= stop_rel2 |r0|^2.  step_b: beta =
- * <r,z>/rz, p = z + beta p.  Stopped blocks are left unchanged by later steps. */
+ * <r,z>/rz, p = z + beta p.  Stopped blocks are left unchanged by later steps.  pt
+ * (optional, with ptab): p also kept in the forward product's tiled B layout (SK_B_TILED,
+ * the tiles of block b's P^T, Dt x Ds) at pt + ptab[2b], ptab[2b+1] = block b's Dt; init
+ * and step_b write each element there too (the padding is the caller's zeros). */
 int vt_kpcg_init(const double *r, const double *z, double *p, double *x, const long long *seg,
-                 int batch, double *st, double *hist, int hist_stride, void *stream);
+                 int batch, double *st, double *hist, int hist_stride, double *pt,
+                 const long long *ptab, void *stream);
 /* it = -1: each block takes its iteration index from its own count st[4b+3] (equal to
  * the loop index while it is active), so every iteration has the same arguments.
  * prune > 0 (>= 10): from iteration `prune` on, every 10 iterations, a block whose last
@@ -399,7 +403,7 @@ int vt_kpcg_step_a(double *x, double *r, const double *p, double *q, const doubl
                    const long long *seg, int batch, double *st, double *hist, int hist_stride,
                    int it, double stop_rel2, int prune, int cap, void *stream);
 int vt_kpcg_step_b(const double *r, const double *z, double *p, const long long *seg, int batch,
-                   double *st, void *stream);
+                   double *st, double *pt, const long long *ptab, void *stream);
 
 /* `iters` CG iterations after vt_kpcg_init, enqueued by one call: per iteration the
  * operator (skinny products fwd then bwd, q = A p) with step_a fused into the backward
@@ -408,14 +412,16 @@ int vt_kpcg_step_b(const double *r, const double *z, double *p, const long long 
  * its max_M / max_K; max_Nc bounds every problem's column count; bwd's descriptors are
  * in block order.  cnt: batch * (ceil(bwd_M / 16) + 1) words of counters (zeroed here).
  * graph_chunk >= 2 replays the iterations as a HIP graph of up to graph_chunk (at most
- * iters/2) iterations, as vt_kron_rnn does; 0 launches each.  Same results either way. */
+ * iters/2) iterations, as vt_kron_rnn does; 0 launches each.  Same results either way.
+ * pt, ptab: p's tiled copy as in vt_kpcg_init, for fwd descriptors that read it. */
 int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                     const void *pre, int pre_n, int pre_M, int pre_K, int pre_Nc, const void *pre2,
                     int pre2_n, int pre2_M, int pre2_K, int pre2_Nc, const void *pre3, int pre3_n,
                     int pre3_M, int pre3_K, int pre3_Nc, int batch, int max_Nc, double *x,
                     double *r, double *p, double *q, double *z, const double *lam,
                     const long long *seg, double *st, double *hist, int hist_stride, int iters,
-                    double stop_rel2, int prune, int graph_chunk, unsigned *cnt, void *stream);
+                    double stop_rel2, int prune, int graph_chunk, unsigned *cnt, double *pt,
+                    const long long *ptab, void *stream);
 /* Kronecker-sum preconditioner (S^T C S) (x) Tbar + lam I for blocks whose decode period
  * is longer than their window: c_j as vt_kpcg_tbar, and per block Tbar's eigenvalues theta
  * (batch x Dt) and eigenvectors U (eigenvector k in column k of the Dt x Dt block; UT its

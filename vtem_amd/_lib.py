@@ -94,11 +94,11 @@ SIGNATURES = {
     "vt_gemm_skinny": [P, I, I, I, I, P],
     "vt_gemm_skinny_workspace_size": [I, I, I, P],
     "vt_skinny_split_rule": [P, P],
-    "vt_kpcg_init": [P, P, P, P, P, I, P, P, I, P],
+    "vt_kpcg_init": [P, P, P, P, P, I, P, P, I, P, P, P],
     "vt_kpcg_step_a": [P, P, P, P, P, P, I, P, P, I, I, D, I, I, P],
-    "vt_kpcg_step_b": [P, P, P, P, I, P, P],
+    "vt_kpcg_step_b": [P, P, P, P, I, P, P, P, P],
     "vt_kpcg_iterate": [P, I, I, P, I, I, P, I, I, I, I, P, I, I, I, I, P, I, I, I, I, I, I, P, P, P, P, P, P,
-                        P, P, P, I, I, D, I, I, P, P],
+                        P, P, P, I, I, D, I, I, P, P, P, P],
     "vt_kpcg_tbar_eig": [P, P, I, I, P, P, P, P, P, P],
     "vt_kpcg_shift_pool": [P, P, P, P, P, P, I, I, I, P, P],
     "vt_kron_lanczos": [P, I, I, P, I, I, I, I, P, P, P, P, P, I, P, I, P, P, P, P],

@@ -1006,7 +1006,10 @@ class BatchDecoder:
         Pv = torch.empty(Dtot, dtype=F64, device=dev)
         Q = torch.empty(Dtot, dtype=F64, device=dev)
         Z = torch.empty(int(nzj[-1]), dtype=F64, device=dev)
-        fp, bp = self._operator(blocks, Pv, Q, Z, dbj, nzj, act)
+        # the CG direction and Z also in the MFMA tile layout (p's copy kept by the CG steps)
+        tiles = self.operand_tiles(N, Ds, Dt)
+        ptab = _h2d(np.stack([tiles[1][:-1], Dt]).T.reshape(-1).astype(np.int64), dev)
+        fp, bp = self._operator(blocks, Pv, Q, tiles[2], dbj, nzj, act, tiled=tiles)
         fx, bx = self._operator(blocks, X, Q, Z, dbj, nzj)
         iters = int(self.cg_iters)
         hs = iters + 1
@@ -1029,12 +1032,14 @@ class BatchDecoder:
         e_cg = self._tic()                      # the solve's launches only (host work before)
         for pr in pres:
             pr.run(s)
-        lib.vt_kpcg_init(ptr(R), ptr(Zp), ptr(Pv), ptr(X), ptr(seg), Bt, ptr(st), ptr(hist), hs, s)
+        lib.vt_kpcg_init(ptr(R), ptr(Zp), ptr(Pv), ptr(X), ptr(seg), Bt, ptr(st), ptr(hist), hs, ptr(tiles[0]),
+                         ptr(ptab), s)
         pargs = [pr.args5() for pr in pres] + [(0, 0, 0, 0, 0)] * (3 - len(pres))
         kcnt = torch.empty(Bt * (-(-int(Ds.max()) // 16) + 1), dtype=torch.int32, device=dev)
         lib.vt_kpcg_iterate(*fp.args(), *bp.args(), *pargs[0], *pargs[1], *pargs[2], Bt, maxNc, ptr(X), ptr(R),
                             ptr(Pv), ptr(Q), ptr(Zp), ptr(lam), ptr(seg), ptr(st), ptr(hist), hs, iters,
-                            float(self.cg_stop) ** 2, self.prune, self.graph_chunk, ptr(kcnt), s)
+                            float(self.cg_stop) ** 2, self.prune, self.graph_chunk, ptr(kcnt), ptr(tiles[0]),
+                            ptr(ptab), s)
         if X0 is not None:
             lib.vt_add_inplace(ptr(X), ptr(X0), Dtot, s)
         # true FP64 residual of the final iterate (the guard's input)
@@ -1051,7 +1056,8 @@ class BatchDecoder:
         per = 8.0 * (2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + (Dt if kron_sum else 1) * Ds * Ds + 12.0 * D)
         self._toc("kpcg_iterate", e_cg,
                   lambda: float(np.sum(per * st.view(Bt, 4)[:, 3].cpu().numpy())))
-        keep += [r, lam, seg, elem_job, Mi, mkeep, R, Zp, pres, X, Pv, Q, Z, fp, bp, fx, bx, st, hist, res, X0, kcnt]
+        keep += [r, lam, seg, elem_job, Mi, mkeep, R, Zp, pres, X, Pv, Q, Z, fp, bp, fx, bx, st, hist, res, X0, kcnt,
+                 tiles, ptab]
         return X, nrm, info, st, boff, keep
 
     def _kron_sum(self, gs) -> bool:
@@ -1183,7 +1189,7 @@ class BatchDecoder:
         plan = self._rnn_plan(N, Ds, Dt) if self.rnn_persistent else None
         extra = []
         if plan is None:
-            tiles = self.rnn_tiles(N, Ds, Dt)
+            tiles = self.operand_tiles(N, Ds, Dt)
             Xt, xoff, Z, _ = tiles
             xoff_d = _h2d(xoff[:-1].astype(np.int64), dev)
             fx, bx = self._operator(blocks, X, Q, Z, dbj, nzj, tiled=tiles)
@@ -1217,16 +1223,16 @@ class BatchDecoder:
         self._toc("rnn_iterate", e0, float(nbytes) * (steps + 1))
         return X, st, boff, [r, lam, seg, X, Q, Z, fx, bx, st, eta] + extra
 
-    def rnn_tiles(self, N, Ds, Dt):
-        """The RNN launch form's iterate and Z in the MFMA tile layout, the forward's and the
-        backward's B operands: per block the tiles of X_b^T (Dt x Ds) and of Z_b^T (Dt x N).
-        Returns (Xt, X offsets, Zt, Z offsets) with the offsets in doubles (one more entry:
-        the total); Zt is zeroed, as its padding is never written."""
+    def operand_tiles(self, N, Ds, Dt):
+        """Buffers for the operator's B operands in the MFMA tile layout (the RNN's iterate
+        or the CG direction, and Z): per block the tiles of X_b^T (Dt x Ds) and of Z_b^T
+        (Dt x N).  Returns (Xt, X offsets, Zt, Z offsets) with the offsets in doubles (one
+        more entry: the total); both zeroed, as their padding is never written."""
         xsz = np.array([tile_doubles(int(t), int(d)) for t, d in zip(Dt, Ds)], np.int64)
         zsz = np.array([tile_doubles(int(t), int(n)) for t, n in zip(Dt, N)], np.int64)
         xoff = np.concatenate([[0], np.cumsum(xsz)])
         zoff = np.concatenate([[0], np.cumsum(zsz)])
-        Xt = torch.empty(int(xoff[-1]), dtype=F64, device=self.device)
+        Xt = torch.zeros(int(xoff[-1]), dtype=F64, device=self.device)
         Zt = torch.zeros(int(zoff[-1]), dtype=F64, device=self.device)
         return Xt, xoff, Zt, zoff
 

@@ -384,11 +384,21 @@ __device__ __forceinline__ double block_sum(double v, double *red)
     return s;
 }
 
+// p's copy in the forward product's tiled B layout (SK_B_TILED: the tiles of block b's
+// P^T, Dt x Ds), optional: ptab[2b] = block b's offset in pt, ptab[2b+1] = its Dt; loc is
+// the element's place in the block's n = Ds Dt coefficients (row-major Ds x Dt)
+__device__ __forceinline__ void kp_store_tiled(double *pt, const long long *ptab, int b, long long loc,
+                                               long long n, double v)
+{
+    const long long Dt = ptab[2 * b + 1];
+    pt[ptab[2 * b] + sk_tile_index(loc % Dt, loc / Dt, (n / Dt + 15) / 16)] = v;
+}
+
 // x = 0, p = z, rz = <r,z>, rr0 = <r,r>
 __global__ void __launch_bounds__(256)
 kpcg_init_kernel(const double *__restrict__ r, const double *__restrict__ z, double *__restrict__ p,
                  double *__restrict__ x, const long long *__restrict__ seg, double *__restrict__ st,
-                 double *__restrict__ hist, int hs)
+                 double *__restrict__ hist, int hs, double *__restrict__ pt, const long long *__restrict__ ptab)
 {
     __shared__ double red[256];
     const long long a = seg[blockIdx.x], e = seg[blockIdx.x + 1];
@@ -397,6 +407,7 @@ kpcg_init_kernel(const double *__restrict__ r, const double *__restrict__ z, dou
         rz += r[i] * z[i];
         rr += r[i] * r[i];
         p[i] = z[i];
+        if (pt) kp_store_tiled(pt, ptab, blockIdx.x, i - a, e - a, z[i]);
         x[i] = 0.0;
     }
     rz = block_sum(rz, red);
@@ -413,10 +424,10 @@ kpcg_init_kernel(const double *__restrict__ r, const double *__restrict__ z, dou
 
 VT_EXPORT int vt_kpcg_init(const double *r, const double *z, double *p, double *x,
                            const long long *seg, int batch, double *st, double *hist, int hist_stride,
-                           void *stream)
+                           double *pt, const long long *ptab, void *stream)
 {
-    VT_CHECK_ARG(r && z && p && x && seg && st && hist && batch >= 1 && hist_stride >= 1);
-    kpcg_init_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(r, z, p, x, seg, st, hist, hist_stride);
+    VT_CHECK_ARG(r && z && p && x && seg && st && hist && batch >= 1 && hist_stride >= 1 && (!pt || ptab));
+    kpcg_init_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(r, z, p, x, seg, st, hist, hist_stride, pt, ptab);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
@@ -622,7 +633,8 @@ VT_EXPORT int vt_kpcg_step_a(double *x, double *r, const double *p, double *q, c
 // rz' = <r, z>; beta = rz'/rz; p = z + beta p
 __global__ void __launch_bounds__(256)
 kpcg_step_b_kernel(const double *__restrict__ r, const double *__restrict__ z, double *__restrict__ p,
-                   const long long *__restrict__ seg, double *__restrict__ st)
+                   const long long *__restrict__ seg, double *__restrict__ st, double *__restrict__ pt,
+                   const long long *__restrict__ ptab)
 {
     __shared__ double red[256];
     double *s = st + 4 * blockIdx.x;
@@ -632,7 +644,11 @@ kpcg_step_b_kernel(const double *__restrict__ r, const double *__restrict__ z, d
     for (long long i = a + threadIdx.x; i < e; i += 256) rz += r[i] * z[i];
     rz = block_sum(rz, red);
     const double beta = rz / s[0];
-    for (long long i = a + threadIdx.x; i < e; i += 256) p[i] = z[i] + beta * p[i];
+    for (long long i = a + threadIdx.x; i < e; i += 256) {
+        const double pi = z[i] + beta * p[i];
+        p[i] = pi;
+        if (pt) kp_store_tiled(pt, ptab, blockIdx.x, i - a, e - a, pi);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         s[0] = rz;
@@ -641,10 +657,10 @@ kpcg_step_b_kernel(const double *__restrict__ r, const double *__restrict__ z, d
 }
 
 VT_EXPORT int vt_kpcg_step_b(const double *r, const double *z, double *p, const long long *seg,
-                             int batch, double *st, void *stream)
+                             int batch, double *st, double *pt, const long long *ptab, void *stream)
 {
-    VT_CHECK_ARG(r && z && p && seg && st && batch >= 1);
-    kpcg_step_b_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(r, z, p, seg, st);
+    VT_CHECK_ARG(r && z && p && seg && st && batch >= 1 && (!pt || ptab));
+    kpcg_step_b_kernel<<<batch, 256, 0, (hipStream_t)stream>>>(r, z, p, seg, st, pt, ptab);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
@@ -678,7 +694,8 @@ VT_EXPORT int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void 
                               int batch, int max_Nc, double *x, double *r, double *p, double *q,
                               double *z, const double *lam, const long long *seg, double *st,
                               double *hist, int hist_stride, int iters, double stop_rel2,
-                              int prune, int graph_chunk, unsigned *cnt, void *stream)
+                              int prune, int graph_chunk, unsigned *cnt, double *pt, const long long *ptab,
+                              void *stream)
 {
     VT_CHECK_ARG(fwd && bwd && pre && cnt && iters >= 0 && iters < hist_stride && graph_chunk >= 0 &&
                  batch >= 1 && batch <= 65535 && bwd_M >= 1 && max_Nc >= 1 && max_Nc <= 32 &&
@@ -710,7 +727,7 @@ VT_EXPORT int vt_kpcg_iterate(const void *fwd, int fwd_M, int fwd_K, const void 
         VT_LAUNCH_CHECK();
         for (int i = 0; i < 3; ++i)
             if (P[i].d) VT_TRY(vt_gemm_skinny(P[i].d, P[i].n, P[i].M, P[i].K, P[i].Nc, sq));
-        VT_TRY(vt_kpcg_step_b(r, z, p, seg, batch, st, sq));
+        VT_TRY(vt_kpcg_step_b(r, z, p, seg, batch, st, pt, ptab, sq));
         return VT_OK;
     };
     return vt_graph_loop(s0, iters, graph_chunk, step);
