@@ -1112,6 +1112,22 @@ def test_kpcg_graph_replay(torch_mod):
               f"retries {bd.last_stats.get('kpcg_retries')}")
 
 
+def test_server_streams_run_side_by_side(torch_mod):
+    """The server's streams (independent_streams) sit on distinct hardware queues: work on
+    each runs while a spin kernel occupies any other, both ways; a server reuses the same
+    streams on every serve call."""
+    torch = torch_mod
+    from vtem_amd.serve import Server, independent_streams, streams_overlap
+    _banner("server streams on distinct hardware queues")
+    ss = independent_streams(3)
+    for i in range(3):
+        for j in range(i + 1, 3):
+            assert streams_overlap(ss[i], ss[j]), (i, j)
+    srv = Server()
+    a = srv._lane_streams(3)
+    assert srv._lane_streams(3) == a and srv._lane_streams(2) == a[:2]
+
+
 def test_server_alone_equals_mix(torch_mod):
     """Requests served with Poisson arrivals on 2 streams (continuous batching) give the
     same frames as each request decoded alone with the same precision decision."""
