@@ -430,12 +430,15 @@ def streams_overlap(a: torch.cuda.Stream, b: torch.cuda.Stream, spin_ms: float =
     """True if work enqueued on b runs while a spin kernel occupies a (and the reverse).
     Two streams that HIP placed on one hardware queue run in submission order, so the
     second one's work waits for the spin."""
+    spin = getattr(torch.cuda, "_sleep", None)   # torch's spin kernel (a test utility)
+    if spin is None:
+        return True                              # cannot probe: take the streams as they are
     x = torch.zeros(1, device=a.device)
     torch.cuda.synchronize()
     cycles = int(spin_ms * 1.6e6)              # the spin counts at about 1.6 GHz
     for s, t in ((a, b), (b, a)):
         with torch.cuda.stream(s):
-            torch.cuda._sleep(cycles)
+            spin(cycles)
             e_spin = torch.cuda.Event()
             e_spin.record()
         with torch.cuda.stream(t):
