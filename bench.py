@@ -78,7 +78,7 @@ WORK_NOTES = {"encode_coeffs": "algorithmic HBM bytes 8*(F*P + N*P + 2*(P+N)*Dt)
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--requests", type=int, default=64)
     p.add_argument("--seed", type=int, default=0)
