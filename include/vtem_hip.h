@@ -488,11 +488,12 @@ int vt_kron_rnn_persistent_bar_words(int batch, long long *words);
  * count of the tridiagonal.  The reference steps its ODE by dt*alpha (vtem/vtdm.py:21,
  * vtem/neural_network.py:78-92); 1/lambda_max is the configurations' rule.  bwd's
  * descriptors in block order; cnt: batch * ceil(bwd_M / 16) + 2 batch words (zeroed
- * here); part: batch * ceil(bwd_M / 16) doubles; tri: 2 * steps * batch doubles. */
+ * here); part: batch * ceil(bwd_M / 16) doubles; tri: 2 * steps * batch doubles.  * vt, vtab (optional): v also kept as the forward product's tiled B (as p in
+ * vt_kpcg_init: block b's tiles at vt + vtab[2b], vtab[2b+1] its Dt), written with v. */
 int vt_kron_lanczos(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                     int batch, int max_Nc, double *v, double *vp, double *q, const double *lam,
                     const long long *seg, int steps, double *eta, int graph_chunk, unsigned *cnt,
-                    double *part, double *tri, void *stream);
+                    double *part, double *tri, double *vt, const long long *vtab, void *stream);
 
 
 /* ------------------------------------------ spatial Dirichlet side, pseudoinverse -- */

@@ -101,7 +101,7 @@ SIGNATURES = {
                         P, P, P, I, I, D, I, I, P, P, P, P],
     "vt_kpcg_tbar_eig": [P, P, I, I, P, P, P, P, P, P],
     "vt_kpcg_shift_pool": [P, P, P, P, P, P, I, I, I, P, P],
-    "vt_kron_lanczos": [P, I, I, P, I, I, I, I, P, P, P, P, P, I, P, I, P, P, P, P],
+    "vt_kron_lanczos": [P, I, I, P, I, I, I, I, P, P, P, P, P, I, P, I, P, P, P, P, P, P],
     # spatial Dirichlet side, pseudoinverse, stitching
     "vt_dirichlet_table": [P, L, I, I, D, D, D, D, I, I, D, D, P],
     "vt_ds_assemble": [P, P, L, I, I, I, I, D, P],

@@ -1151,18 +1151,22 @@ class BatchDecoder:
         V = torch.empty(Dtot, dtype=F64, device=dev)
         Vp = torch.empty(Dtot, dtype=F64, device=dev)
         Q = torch.empty(Dtot, dtype=F64, device=dev)
-        Z = torch.empty(int(nzj[-1]), dtype=F64, device=dev)
-        fv, bv = self._operator(blocks, V, Q, Z, dbj, nzj)
+        # the Lanczos vector (kept tiled by the steps) and Z in the MFMA tile layout
+        tiles = self.operand_tiles(N, Ds, Dt)
+        Z = tiles[2]
+        vtab = _h2d(np.stack([tiles[1][:-1], Dt]).T.reshape(-1).astype(np.int64), dev)
+        fv, bv = self._operator(blocks, V, Q, Z, dbj, nzj, tiled=tiles)
         eta = torch.empty(Bt, dtype=F64, device=dev)
         ntile = -(-int(Ds.max()) // 16)
         cnt = torch.empty(Bt * ntile + 2 * Bt, dtype=torch.int32, device=dev)
         part = torch.empty(Bt * ntile, dtype=F64, device=dev)
         tri = torch.empty(2 * LANCZOS_STEPS * Bt, dtype=F64, device=dev)
         lib.vt_kron_lanczos(*fv.args(), *bv.args(), Bt, int(Dt.max()), ptr(V), ptr(Vp), ptr(Q), ptr(lam),
-                            ptr(seg), LANCZOS_STEPS, ptr(eta), self.graph_chunk, ptr(cnt), ptr(part), ptr(tri), s)
+                            ptr(seg), LANCZOS_STEPS, ptr(eta), self.graph_chunk, ptr(cnt), ptr(part), ptr(tri),
+                            ptr(tiles[0]), ptr(vtab), s)
         nbytes = 8.0 * np.sum(2.0 * N * Ds + N * Dt * Dt + 2.0 * N * Dt + 6.0 * D)
         self._toc("rnn_lanczos", e0, float(nbytes) * LANCZOS_STEPS)
-        return eta, [lam, seg, V, Vp, Q, Z, fv, bv, eta, cnt, part, tri]
+        return eta, [lam, seg, V, Vp, Q, Z, fv, bv, eta, cnt, part, tri, tiles, vtab]
 
     def _rnn_iterate(self, gs: List[_Prologue], eta, steps: int):
         """Gradient-flow decoder for every block of the groups, as one joint launch

@@ -879,7 +879,8 @@ __device__ __forceinline__ void lanczos_bwd_tile(const SkDesc &d, int m0, int z,
                                                  double *__restrict__ v, double *__restrict__ vp, double l,
                                                  long long s0, long long s1, unsigned *tcnt, unsigned *bcnt,
                                                  unsigned *jstep, double *part, double *__restrict__ tri,
-                                                 int tri_ld, double (&red)[SK_WAVES / 2][16][33], double *red1)
+                                                 int tri_ld, double *__restrict__ vt, const long long *__restrict__ vtab,
+                                                 double (&red)[SK_WAVES / 2][16][33], double *red1)
 {
     __shared__ int last;
     __shared__ double ab[3];
@@ -953,6 +954,7 @@ __device__ __forceinline__ void lanczos_bwd_tile(const SkDesc &d, int m0, int z,
                          alpha * vi - bprev * vp[i];
         vp[i] = vi;
         v[i] = w * inv;
+        if (vt) kp_store_tiled(vt, vtab, b, i - s0, s1 - s0, w * inv);
     }
     if (threadIdx.x == 0) {
         const unsigned j = *jstep;
@@ -968,7 +970,8 @@ template <int NT, bool X16>
 __global__ void __launch_bounds__(64 * SK_WAVES)
 lanczos_bwd_kernel(const SkDesc *__restrict__ descs, double *__restrict__ v, double *__restrict__ vp,
                    const double *__restrict__ lam, const long long *__restrict__ seg, unsigned *__restrict__ cnt,
-                   double *__restrict__ part, double *__restrict__ tri, int tri_ld)
+                   double *__restrict__ part, double *__restrict__ tri, int tri_ld, double *__restrict__ vt,
+                   const long long *__restrict__ vtab)
 {
     __shared__ double red[SK_WAVES / 2][16][33];
     __shared__ double red1[64 * SK_WAVES];
@@ -986,28 +989,30 @@ lanczos_bwd_kernel(const SkDesc *__restrict__ descs, double *__restrict__ v, dou
     const int z = blockIdx.z, t = blockIdx.x;
     if constexpr (NT == 1) {
         if (X16 && d.Nc == 17)
-            lanczos_bwd_tile<1, true>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, red, red1);
+            lanczos_bwd_tile<1, true>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, vt, vtab, red, red1);
         else
-            lanczos_bwd_tile<1, false>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, red, red1);
+            lanczos_bwd_tile<1, false>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, vt, vtab, red, red1);
     } else {
         if (d.Nc == 17)
-            lanczos_bwd_tile<1, true>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, red, red1);
+            lanczos_bwd_tile<1, true>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, vt, vtab, red, red1);
         else if (d.Nc <= 16)
-            lanczos_bwd_tile<1, false>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, red, red1);
+            lanczos_bwd_tile<1, false>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, vt, vtab, red, red1);
         else
-            lanczos_bwd_tile<2, false>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, red, red1);
+            lanczos_bwd_tile<2, false>(d, m0, z, b, t, ntile, v, vp, l, s0, s1, tc, bc, js, part, tri, tri_ld, vt, vtab, red, red1);
     }
 }
 
 // v_0 = ones / sqrt(D_b), v_{-1} = 0 per block
 __global__ void __launch_bounds__(256)
-lanczos_init_kernel(double *__restrict__ v, double *__restrict__ vp, const long long *__restrict__ seg)
+lanczos_init_kernel(double *__restrict__ v, double *__restrict__ vp, const long long *__restrict__ seg,
+                    double *__restrict__ vt, const long long *__restrict__ vtab)
 {
     const long long a = seg[blockIdx.x], e = seg[blockIdx.x + 1];
     const double x = e > a ? 1.0 / sqrt((double)(e - a)) : 0.0;
     for (long long i = a + threadIdx.x; i < e; i += 256) {
         v[i] = x;
         vp[i] = 0.0;
+        if (vt) kp_store_tiled(vt, vtab, blockIdx.x, i - a, e - a, x);
     }
 }
 
@@ -1050,16 +1055,16 @@ __global__ void lanczos_eta_kernel(const double *__restrict__ tri, int tri_ld, i
 
 static int lanczos_bwd(const VtSkinnyLaunch &Bk, int batch, int max_Nc, double *v, double *vp,
                        const double *lam, const long long *seg, unsigned *cnt, double *part, double *tri,
-                       int tri_ld, hipStream_t s)
+                       int tri_ld, double *vt, const long long *vtab, hipStream_t s)
 {
     const dim3 grid(vt_blocks(Bk.max_M, 16), batch, skinny_splits(Bk.max_K));
     const SkDesc *d = (const SkDesc *)Bk.descs;
     if (max_Nc <= 16)
-        lanczos_bwd_kernel<1, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, vp, lam, seg, cnt, part, tri, tri_ld);
+        lanczos_bwd_kernel<1, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, vp, lam, seg, cnt, part, tri, tri_ld, vt, vtab);
     else if (max_Nc == 17)
-        lanczos_bwd_kernel<1, true><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, vp, lam, seg, cnt, part, tri, tri_ld);
+        lanczos_bwd_kernel<1, true><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, vp, lam, seg, cnt, part, tri, tri_ld, vt, vtab);
     else
-        lanczos_bwd_kernel<2, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, vp, lam, seg, cnt, part, tri, tri_ld);
+        lanczos_bwd_kernel<2, false><<<grid, 64 * SK_WAVES, 0, s>>>(d, v, vp, lam, seg, cnt, part, tri, tri_ld, vt, vtab);
     VT_LAUNCH_CHECK();
     return VT_OK;
 }
@@ -1072,23 +1077,23 @@ static int lanczos_bwd(const VtSkinnyLaunch &Bk, int batch, int max_Nc, double *
 VT_EXPORT int vt_kron_lanczos(const void *fwd, int fwd_M, int fwd_K, const void *bwd, int bwd_M, int bwd_K,
                               int batch, int max_Nc, double *v, double *vp, double *q, const double *lam,
                               const long long *seg, int steps, double *eta, int graph_chunk, unsigned *cnt,
-                              double *part, double *tri, void *stream)
+                              double *part, double *tri, double *vt, const long long *vtab, void *stream)
 {
     VT_CHECK_ARG(fwd && bwd && v && vp && q && lam && seg && eta && cnt && part && tri && batch >= 1 &&
                  batch <= 65535 && steps >= 1 && graph_chunk >= 0 && bwd_M >= 1 && max_Nc >= 1 &&
-                 max_Nc <= 32);
+                 max_Nc <= 32 && (!vt || vtab));
     const VtSkinnyLaunch F{fwd, fwd_M, fwd_K}, Bk{bwd, bwd_M, bwd_K};
     hipStream_t s = (hipStream_t)stream;
     const long long ntile = vt_blocks(bwd_M, 16);
     const long long words = (long long)batch * ntile + 2LL * batch;
     const unsigned gw = vt_blocks(words, 256);
     zero_words_kernel<<<gw < 1024 ? gw : 1024, 256, 0, s>>>(cnt, words);
-    lanczos_init_kernel<<<batch, 256, 0, s>>>(v, vp, seg);
+    lanczos_init_kernel<<<batch, 256, 0, s>>>(v, vp, seg, vt, vtab);
     VT_LAUNCH_CHECK();
     const int tri_ld = 2 * steps;
     auto step = [&](hipStream_t st) -> int {
         VT_TRY(skinny_run(F, batch, max_Nc, st));
-        VT_TRY(lanczos_bwd(Bk, batch, max_Nc, v, vp, lam, seg, cnt, part, tri, tri_ld, st));
+        VT_TRY(lanczos_bwd(Bk, batch, max_Nc, v, vp, lam, seg, cnt, part, tri, tri_ld, vt, vtab, st));
         return VT_OK;
     };
     VT_TRY(vt_graph_loop(s, steps, graph_chunk, step));
